@@ -1,0 +1,1058 @@
+// gf_oracle.cpp -- CPU restatement of the GeoFLOW SEM operator path.
+//
+// TEST INFRASTRUCTURE ONLY (see gf_oracle.h).  Compiled with -ffp-contract=off
+// so that every sum and product rounds exactly as written, in the order the
+// reference writes it.  Citations are file:line in the GeoFLOW source tree.
+#include "gf_oracle.h"
+
+#include <algorithm>
+#include <cassert>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <set>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+typedef long double LD;
+
+// ---------------------------------------------------------------------------
+// GLL basis.  Follows GLLBasis<long double,double>::computeJacobi,
+// computeNodes, computeWeights, computeDerivMatrix
+// (src/cdg/sem/gllbasis.ipp:410-501, 514-531, 579-607, 618-667).
+// ---------------------------------------------------------------------------
+void jacobi(int N, LD alpha, LD beta, LD &poly, LD &pder, LD &polym1, LD &pderm1,
+            LD &polym2, LD &pderm2, LD x) {
+  // gllbasis.ipp:618-667
+  LD apb = alpha + beta, rv = 1.0L + alpha;
+  LD polylist = 0, pderlist = 0, psave = 0, pdsave = 0;
+  poly = 1.0L;
+  pder = 0.0L;
+  if (N == 0) return;
+  polylist = poly;
+  pderlist = pder;
+  poly = rv * x;
+  pder = rv;
+  if (N == 1) return;
+  for (int k = 2; k <= N; k++) {
+    LD fk = (LD)k;
+    LD a1 = 2.0L * fk * (fk + apb) * (2.0L * fk + apb - 2.0L);
+    LD a2 = (2.0L * fk + apb - 1.0L) * (alpha * alpha - beta * beta);
+    LD b3 = 2.0L * fk + apb - 2.0L;
+    LD a3 = b3 * (b3 + 1.0L) * (b3 + 2.0L);
+    LD a4 = 2.0L * (fk + alpha - 1.0L) * (fk + beta - 1.0L) * (2.0L * fk + apb);
+    LD polyn = ((a2 + a3 * x) * poly - a4 * polylist) / a1;
+    LD pdern = ((a2 + a3 * x) * pder - a4 * pderlist + a3 * poly) / a1;
+    psave = polylist;
+    pdsave = pderlist;
+    polylist = poly;
+    poly = polyn;
+    pderlist = pder;
+    pder = pdern;
+  }
+  polym1 = polylist;
+  pderm1 = pderlist;
+  polym2 = psave;
+  pderm2 = pdsave;
+}
+
+struct Basis {
+  int p = 0, N = 0;
+  std::vector<double> xi, w, D, DT;  // D(l,j) column-major: D[l + j*N]
+};
+
+bool gll(int Np, Basis &b) {
+  if (Np < 1) return false;
+  const int kstop = 128;
+  const LD eps = 1.0e-8L;
+  const LD ttiny = 100.0L * std::numeric_limits<LD>::epsilon();
+  const LD ximin = -1.0L, ximax = 1.0L;
+  const LD alpha = 0, beta = 0;
+  int np1 = Np + 1;
+  std::vector<LD> x(np1), Pn(np1), dPn(np1), w(np1);
+  LD pnp1p, pdnp1p, pnp, pdnp, pnm1p, pdnm1, pnp1m, pdnp1m, pnm, pdnm, pnm1m;
+  jacobi(np1, alpha, beta, pnp1p, pdnp1p, pnp, pdnp, pnm1p, pdnm1, ximax);
+  jacobi(np1, alpha, beta, pnp1m, pdnp1m, pnm, pdnm, pnm1m, pdnm1, ximin);
+  LD det = pnp * pnm1m - pnm * pnm1p;
+  LD rp = -pnp1p, rm = -pnp1m;
+  LD a = (rp * pnm1m - rm * pnm1p) / det;
+  LD bb = (rm * pnp - rp * pnm) / det;
+  x[0] = ximax;
+  x[Np] = ximin;
+  int nh = (Np + 1) / 2;
+  // gllbasis.ipp:452-456: the reference's literal pi and libm trig seed
+  // the Newton iteration.
+  LD dth = 3.1415926535898L / (2.0L * Np + 1.0L);
+  LD cd = std::cos(2.0L * dth), sd = std::sin(2.0L * dth);
+  LD cs = std::cos(dth), ss = std::sin(dth);
+  for (int j = 1; j < nh + 1; j++) {
+    LD xx = cs;
+    for (int k = 0; k < kstop; k++) {
+      LD pnp1, pdnp1, pn, pdn, pnm1, pdnm1b;
+      jacobi(np1, alpha, beta, pnp1, pdnp1, pn, pdn, pnm1, pdnm1b, xx);
+      LD poly = pnp1 + a * pn + bb * pnm1;
+      LD pder = pdnp1 + a * pdn + bb * pdnm1b;
+      LD recsum = 0.0L;
+      for (int i = 0; i < j; i++) recsum += (1.0L / (xx - x[i]));
+      LD delx = -poly / (pder - recsum * poly);
+      xx += delx;
+      LD error = std::fabs(delx) / std::fabs(xx);
+      if (error < eps) break;
+    }
+    x[j] = xx;
+    LD cssave = cs * cd - ss * sd;
+    ss = cs * sd + ss * cd;
+    cs = cssave;
+  }
+  for (int i = 0; i < nh; i++) x[np1 - i - 1] = -x[i];
+  if (Np == (2 * (Np / 2))) x[nh] = 0.0L;
+  for (int i = 0; i < nh; i++) {
+    LD t = x[Np - i];
+    x[Np - i] = x[i];
+    x[i] = t;
+  }
+  // computeWeights (gllbasis.ipp:514-531)
+  LD fact = 2.0L / (Np * (Np + 1.0L));
+  for (int i = 0; i < Np + 1; i++) {
+    LD ppn, pder, pm1, pdm1, pm2, pdm2;
+    jacobi(Np, alpha, beta, ppn, pder, pm1, pdm1, pm2, pdm2, x[i]);
+    Pn[i] = ppn;
+    dPn[i] = pder;
+    w[i] = (Pn[i] == 0.0L) ? 0.0L : fact / (Pn[i] * Pn[i]);
+  }
+  // computeDerivMatrix (gllbasis.ipp:579-607)
+  std::vector<LD> dphi((size_t)np1 * np1, 0.0L);
+  for (int j = 0; j < Np + 1; j++) {
+    for (int l = 0; l < j; l++) {
+      LD delxi = x[l] - x[j];
+      dphi[l + (size_t)j * np1] = Pn[l] / (Pn[j] * delxi + ttiny);
+    }
+    for (int l = j + 1; l < Np + 1; l++) {
+      LD delxi = x[l] - x[j];
+      dphi[l + (size_t)j * np1] = Pn[l] / (Pn[j] * delxi + ttiny);
+    }
+  }
+  dphi[0] = -0.25L * Np * (Np + 1.0L);
+  dphi[Np + (size_t)Np * np1] = -dphi[0];
+  b.p = Np;
+  b.N = np1;
+  b.xi.resize(np1);
+  b.w.resize(np1);
+  b.D.resize((size_t)np1 * np1);
+  b.DT.resize((size_t)np1 * np1);
+  for (int i = 0; i < np1; i++) {
+    b.xi[i] = (double)x[i];
+    b.w[i] = (double)w[i];
+  }
+  for (int i = 0; i < np1; i++)
+    for (int j = 0; j < np1; j++) {
+      b.D[i + (size_t)j * np1] = (double)dphi[i + (size_t)j * np1];
+      b.DT[i + (size_t)j * np1] = (double)dphi[j + (size_t)i * np1];
+    }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// dmxm: C(nai,nbj) = A(nai,naj) B(nbi,nbj), column-major.  The unrolled
+// dmxmN kernels form A(i,1)*B(1,j) + A(i,2)*B(2,j) + ... left to right
+// (src/cdg/blas/cff_blas.F:846-943, src/cdg/blas/dmxmp.F:19-897).
+// ---------------------------------------------------------------------------
+void dmxm(double *C, const double *A, size_t nai, const double *B, size_t nbi, size_t nbj) {
+  for (size_t j = 0; j < nbj; j++) {
+    for (size_t i = 0; i < nai; i++) {
+      double s = A[i] * B[j * nbi];
+      for (size_t k = 1; k < nbi; k++) s = s + A[i + k * nai] * B[k + j * nbi];
+      C[i + j * nai] = s;
+    }
+  }
+}
+
+int g_threads = 1;
+
+}  // namespace
+
+struct OrGrid;
+void or_embed_G(OrGrid *g, int i, int j, std::vector<double> &G);
+
+// ---------------------------------------------------------------------------
+// Grid
+// ---------------------------------------------------------------------------
+struct OrGrid {
+  int dim = 3;
+  int gtype = 0;  // 0 GE_REGULAR, 2 GE_2DEMBEDDED (src/cdg/include/gtypes.h:171)
+  int N[3] = {1, 1, 1};
+  Basis basis[3];
+  size_t nelems = 0, ndof = 0, nnodes = 0;  // nnodes per element
+  std::vector<double> xn[3];                // node coordinates
+  std::vector<double> xp[3];                // periodized coordinates
+  std::vector<double> dxidx[3][3];          // dXi_i/dX_j per node
+  std::vector<double> jac, mask, mass, imass, imult;
+  std::vector<std::vector<size_t>> group;   // GGFX local coincidence groups
+  double minnodedist = 0, volume = 0;
+  int nxy() const { return gtype == 2 ? dim + 1 : dim; }
+};
+
+namespace {
+
+// GGFX::init local matching (src/cdg/comm/ggfx.hpp:167-245): every node gets
+// the box [x - tol, x + tol]; the group of an unmapped id is the sorted set of
+// all ids whose boxes intersect its box (Box Intersects,
+// src/tbox/spatial/bound/box.hpp:272-276), and that set is assigned to every
+// member.  Candidate search here uses a sort on the first coordinate with a
+// window wider than the predicate; the predicate itself is the reference's.
+void ggfx_match(OrGrid &g, double tol) {
+  size_t n = g.ndof;
+  int nd = 3;
+  std::vector<double> mn[3], mx[3];
+  for (int d = 0; d < nd; d++) {
+    mn[d].resize(n);
+    mx[d].resize(n);
+    for (size_t i = 0; i < n; i++) {
+      double v = d < g.nxy() ? g.xp[d][i] : 0.0;
+      mn[d][i] = v - tol;
+      mx[d][i] = v + tol;
+    }
+  }
+  std::vector<size_t> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  const std::vector<double> &x0 = g.xp[0];
+  std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return x0[a] < x0[b]; });
+  std::vector<double> xs(n);
+  for (size_t i = 0; i < n; i++) xs[i] = x0[ord[i]];
+  g.group.assign(n, std::vector<size_t>());
+  std::vector<char> mapped(n, 0);
+  for (size_t id = 0; id < n; id++) {
+    if (mapped[id]) continue;
+    double lo = x0[id] - 4.0 * tol, hi = x0[id] + 4.0 * tol;
+    size_t a = std::lower_bound(xs.begin(), xs.end(), lo) - xs.begin();
+    std::set<size_t> ids;
+    for (size_t s = a; s < n && xs[s] <= hi; s++) {
+      size_t c = ord[s];
+      bool hit = true;
+      for (int d = 0; d < nd && hit; d++)
+        hit = (mn[d][id] <= mx[d][c]) && (mx[d][id] >= mn[d][c]);
+      if (hit) ids.insert(c);
+    }
+    std::vector<size_t> v(ids.begin(), ids.end());
+    for (size_t m : v) {
+      g.group[m] = v;
+      mapped[m] = 1;
+    }
+  }
+}
+
+// GGrid::find_min_dist (src/cdg/grid/ggrid.ipp:977-1013)
+double find_min_dist(const OrGrid &g) {
+  double tiny = 1000.0 * std::numeric_limits<double>::epsilon();
+  double xmin = std::numeric_limits<double>::max();
+  int nxy = g.nxy();
+  for (size_t j = 0; j + 1 < g.ndof; j++) {
+    double s = 0;
+    double dd[3] = {0, 0, 0};
+    for (int i = 0; i < nxy; i++) dd[i] = g.xn[i][j + 1] - g.xn[i][j];
+    s = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+    double xn = std::sqrt(s);
+    if (xn > tiny) xmin = std::min(xmin, xn);
+  }
+  return xmin;
+}
+
+void finish_grid(OrGrid &g, double tol_fac);
+
+}  // namespace
+
+extern "C" {
+
+int or_gll_basis(int p, double *xi, double *w, double *D) {
+  Basis b;
+  if (!gll(p, b)) return 1;
+  std::copy(b.xi.begin(), b.xi.end(), xi);
+  std::copy(b.w.begin(), b.w.end(), w);
+  std::copy(b.D.begin(), b.D.end(), D);
+  return 0;
+}
+
+void or_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
+void or_dmxm(double *C, const double *A, size_t nai, size_t naj, const double *B, size_t nbi,
+             size_t nbj) {
+  (void)naj;  // dmxm ignores naj (src/cdg/blas/cff_blas.F:880-927)
+  dmxm(C, A, nai, B, nbi, nbj);
+}
+
+// GGridBox element generation: find_rank_subdomain (single rank) and
+// do_elems2d/do_elems3d (src/cdg/grid/ggrid_box.ipp:862-978, 254-463);
+// GElem_base::dogeom2d/3d for GE_REGULAR (src/cdg/sem/gelem_base.cpp:877-1048);
+// boundary detection by coordinate (find_gbdy_ind2d, ggrid_box.ipp:1494-1600)
+// and periodize (ggrid_box.ipp:753-802).
+OrGrid *or_grid_box_create(int dim, const int *p, const int *ne, const double *P0,
+                           const double *P1, const int *bdy, double tol_fac) {
+  if (dim != 2 && dim != 3) return nullptr;
+  OrGrid *gp = new OrGrid();
+  OrGrid &g = *gp;
+  g.dim = dim;
+  g.gtype = 0;
+  for (int d = 0; d < 3; d++) g.N[d] = 1;
+  for (int d = 0; d < dim; d++) {
+    if (!gll(p[d], g.basis[d])) {
+      delete gp;
+      return nullptr;
+    }
+    g.N[d] = p[d] + 1;
+  }
+  g.nnodes = (size_t)g.N[0] * g.N[1] * g.N[2];
+  g.nelems = 1;
+  for (int d = 0; d < dim; d++) g.nelems *= (size_t)ne[d];
+  g.ndof = g.nelems * g.nnodes;
+  double dx[3] = {0, 0, 0}, Lbox[3];
+  for (int d = 0; d < dim; d++) {
+    Lbox[d] = P1[d] - P0[d];
+    dx[d] = Lbox[d] / (double)ne[d];
+  }
+  for (int d = 0; d < 3; d++) {
+    g.xn[d].assign(d < dim ? g.ndof : 0, 0.0);
+    for (int j = 0; j < 3; j++) g.dxidx[d][j].assign(g.ndof, 0.0);
+  }
+  g.jac.assign(g.ndof, 0.0);
+  size_t n = 0;
+  // Vertex sets in the order of hmesh_/qmesh_ (ggrid_box.ipp:887-975).
+  static const int off2[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+  static const int off3[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                 {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  int nk = dim == 3 ? ne[2] : 1;
+  for (long k = 0; k < nk; k++)
+    for (long j = 0; j < ne[1]; j++)
+      for (long i = 0; i < ne[0]; i++) {
+        double v0[3] = {P0[0] + i * dx[0], P0[1] + j * dx[1], dim == 3 ? P0[2] + k * dx[2] : 0.0};
+        int nv = dim == 3 ? 8 : 4;
+        double V[8][3];
+        for (int m = 0; m < nv; m++)
+          for (int l = 0; l < dim; l++) {
+            int o = dim == 3 ? off3[m][l] : off2[m][l];
+            double dv = o ? dx[l] : 0.0;
+            V[m][l] = v0[l] + dv;
+          }
+        size_t ebeg = n;
+        // Linear shape function node coordinates (gshapefcn_linear.ipp:171-348):
+        // x_l = sum_m N_m(xi) * (v_m[l] * 2^-dim), accumulated from zero.
+        double fac = dim == 3 ? 0.125 : 0.25;
+        const std::vector<double> &x0 = g.basis[0].xi, &x1 = g.basis[1].xi;
+        for (int l = 0; l < dim; l++) {
+          size_t nn = ebeg;
+          for (int kk = 0; kk < g.N[2]; kk++)
+            for (int jj = 0; jj < g.N[1]; jj++)
+              for (int ii = 0; ii < g.N[0]; ii++, nn++) {
+                double acc = 0.0;
+                for (int m = 0; m < nv; m++) {
+                  double rm = 1.0 - x0[ii], rp = 1.0 + x0[ii];
+                  double sm = 1.0 - x1[jj], sp = 1.0 + x1[jj];
+                  double Ni;
+                  if (dim == 2) {
+                    switch (m) {
+                      case 0: Ni = rm * sm; break;
+                      case 1: Ni = rp * sm; break;
+                      case 2: Ni = rp * sp; break;
+                      default: Ni = rm * sp; break;
+                    }
+                  } else {
+                    double tm = 1.0 - g.basis[2].xi[kk], tp = 1.0 + g.basis[2].xi[kk];
+                    switch (m) {
+                      case 0: Ni = rm * sm * tm; break;
+                      case 1: Ni = rp * sm * tm; break;
+                      case 2: Ni = rp * sp * tm; break;
+                      case 3: Ni = rm * sp * tm; break;
+                      case 4: Ni = rm * sm * tp; break;
+                      case 5: Ni = rp * sm * tp; break;
+                      case 6: Ni = rp * sp * tp; break;
+                      default: Ni = rm * sp * tp; break;
+                    }
+                  }
+                  acc += Ni * (V[m][l] * fac);
+                }
+                g.xn[l][nn] = acc;
+              }
+        }
+        // dogeom for GE_REGULAR: L from vertices, dXi/dX = 2/L, Jac = prod(L)/2^d
+        double L[3];
+        L[0] = V[1][0] - V[0][0];
+        L[1] = V[2][1] - V[1][1];
+        if (dim == 3) L[2] = V[4][2] - V[0][2];
+        double jv = dim == 3 ? 0.125 * L[0] * L[1] * L[2] : 0.25 * L[0] * L[1];
+        for (size_t q = 0; q < g.nnodes; q++) {
+          for (int d = 0; d < dim; d++) g.dxidx[d][0][ebeg + q] = 2.0 / L[d];
+          g.jac[ebeg + q] = jv;
+        }
+        n += g.nnodes;
+      }
+  g.minnodedist = find_min_dist(g);
+  // Boundary bins and periodize (ggrid_box.ipp:753-802): a node that lies on
+  // any periodic face has every coordinate that sits on P1 moved to P0.
+  double eps = 0.125 * g.minnodedist;
+  auto fuzzyeq = [](double x, double y, double e) { return (y <= (x + e)) && (y >= (x - e)); };
+  g.mask.assign(g.ndof, 1.0);
+  for (int d = 0; d < 3; d++) g.xp[d] = g.xn[d];
+  for (size_t q = 0; q < g.ndof; q++) {
+    bool onper = false, ondir = false;
+    for (int d = 0; d < dim; d++) {
+      for (int s = 0; s < 2; s++) {
+        double P = s ? P1[d] : P0[d];
+        if (fuzzyeq(P, g.xn[d][q], eps)) {
+          int bt = bdy[2 * d + s];
+          if (bt == 6) onper = true;
+          else if (bt != 7) ondir = true;
+        }
+      }
+    }
+    // init_bc_info (src/cdg/grid/ggrid.ipp:1459-1470)
+    if (ondir) g.mask[q] = 0.0;
+    if (onper) {
+      for (int d = 0; d < dim; d++)
+        if (fuzzyeq(P1[d], g.xn[d][q], eps)) g.xp[d][q] = P0[d];
+    }
+  }
+  finish_grid(g, tol_fac);
+  return gp;
+}
+
+void or_grid_destroy(OrGrid *g) { delete g; }
+size_t or_grid_ndof(const OrGrid *g) { return g->ndof; }
+size_t or_grid_nelems(const OrGrid *g) { return g->nelems; }
+int or_grid_dim(const OrGrid *g) { return g->dim; }
+int or_grid_gtype(const OrGrid *g) { return g->gtype; }
+double or_grid_minnodedist(const OrGrid *g) { return g->minnodedist; }
+double or_grid_volume(const OrGrid *g) { return g->volume; }
+
+int or_grid_get(const OrGrid *g, const char *which, double *out) {
+  std::string w(which);
+  const std::vector<double> *src = nullptr;
+  if (w == "x") src = &g->xn[0];
+  else if (w == "y") src = &g->xn[1];
+  else if (w == "z") src = &g->xn[2];
+  else if (w == "xp") src = &g->xp[0];
+  else if (w == "yp") src = &g->xp[1];
+  else if (w == "zp") src = &g->xp[2];
+  else if (w == "mask") src = &g->mask;
+  else if (w == "mass") src = &g->mass;
+  else if (w == "imass") src = &g->imass;
+  else if (w == "jac") src = &g->jac;
+  else if (w == "imult") src = &g->imult;
+  else if (w.size() == 8 && w.compare(0, 6, "dxidx_") == 0) {
+    int i = w[6] - '0', j = w[7] - '0';
+    if (i < 0 || i > 2 || j < 0 || j > 2) return 1;
+    src = &g->dxidx[i][j];
+  }
+  if (!src || src->size() != g->ndof) return 1;
+  std::copy(src->begin(), src->end(), out);
+  return 0;
+}
+
+int or_grid_ggfx(const OrGrid *g, size_t *offs, size_t *members, size_t *nmem) {
+  size_t tot = 0;
+  for (size_t i = 0; i < g->ndof; i++) tot += g->group[i].size();
+  if (nmem) *nmem = tot;
+  if (!members) return 0;
+  size_t c = 0;
+  for (size_t i = 0; i < g->ndof; i++) {
+    offs[i] = c;
+    for (size_t m : g->group[i]) members[c++] = m;
+  }
+  offs[g->ndof] = c;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-space derivatives (GGrid::grefderiv_constp, ggrid.ipp:2213-2265
+// and the GMTK batched forms, gmtk.ipp:1404-1844, 1998-2294).  Directions 2
+// and 3 receive getDerivMatrix(!dotrans) and contract as U * D^T, so every
+// direction computes y(out) = sum_k Dop(out,k) u(k) with k ascending.
+// ---------------------------------------------------------------------------
+void or_grefderiv_raw(int dim, const int *N, size_t ne, const double *const *Dm, int idir,
+                      int dotrans, const double *u, double *du) {
+  int N0 = N[0], N1 = N[1], N2 = dim == 3 ? N[2] : 1;
+  size_t nn = (size_t)N0 * N1 * N2;
+  const double *D = Dm[idir - 1];
+  int Nd = N[idir - 1];
+  // op(o,k): derivative matrix entry used for output index o, input index k.
+  std::vector<double> op((size_t)Nd * Nd);
+  for (int o = 0; o < Nd; o++)
+    for (int k = 0; k < Nd; k++) op[o + (size_t)k * Nd] = dotrans ? D[k + (size_t)o * Nd] : D[o + (size_t)k * Nd];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (long e = 0; e < (long)ne; e++) {
+    const double *ue = u + e * nn;
+    double *de = du + e * nn;
+    if (idir == 1) {
+      // I x I x D1: dmxm(y, D1, N0, u, N0, N1*N2)
+      dmxm(de, op.data(), N0, ue, N0, (size_t)N1 * N2);
+    } else if (idir == 2) {
+      // I x D2 x I: per z-slice dmxm(y, u, N0, D2T, N1, N1) with D2T(k,j)=op(j,k)
+      std::vector<double> BT((size_t)N1 * N1);
+      for (int k = 0; k < N1; k++)
+        for (int j = 0; j < N1; j++) BT[k + (size_t)j * N1] = op[j + (size_t)k * N1];
+      for (int s = 0; s < N2; s++)
+        dmxm(de + (size_t)s * N0 * N1, ue + (size_t)s * N0 * N1, N0, BT.data(), N1, N1);
+    } else {
+      std::vector<double> BT((size_t)N2 * N2);
+      for (int k = 0; k < N2; k++)
+        for (int j = 0; j < N2; j++) BT[k + (size_t)j * N2] = op[j + (size_t)k * N2];
+      dmxm(de, ue, (size_t)N0 * N1, BT.data(), N2, N2);
+    }
+  }
+}
+
+void or_grefderiv(OrGrid *g, const double *u, int idir, int dotrans, double *du) {
+  const double *Dm[3] = {g->basis[0].D.data(), g->basis[1].D.data(), g->basis[2].D.data()};
+  if (g->dim == 2 && idir == 3) {
+    // grefderiv_varp 2-D idir 3 yields zero (ggrid.ipp:2133-2137)
+    std::fill(du, du + g->ndof, 0.0);
+    return;
+  }
+  or_grefderiv_raw(g->dim, g->N, g->nelems, Dm, idir, dotrans, u, du);
+}
+
+// compute_grefdiv (ggrid.ipp:1847-1927): divu = 0; divu += D1 u1; += D2 u2;
+// += D3 u3.  The reference takes the 3-D direction-3 matrix from basis 1
+// (ggrid.ipp:1906); the contraction is only well formed when N2 == N3, in
+// which case both bases are the same GLL basis, so basis 2 is used here.
+void or_grefdiv(OrGrid *g, const double *const *u, int dotrans, double *divu) {
+  size_t n = g->ndof;
+  std::vector<double> et(n);
+  std::fill(divu, divu + n, 0.0);
+  for (int k = 0; k < g->dim; k++) {
+    if (!u[k]) continue;
+    or_grefderiv(g, u[k], k + 1, dotrans, et.data());
+    for (size_t j = 0; j < n; j++) divu[j] += et[j];
+  }
+}
+
+// GGrid::deriv (ggrid.ipp:1163-1191)
+void or_deriv(OrGrid *g, const double *u, int idir, double *du) {
+  size_t n = g->ndof;
+  if (g->gtype == 0) {
+    or_grefderiv(g, u, idir, 0, du);
+    const std::vector<double> &f = g->dxidx[idir - 1][0];
+    for (size_t j = 0; j < n; j++) du[j] *= f[j];
+  } else {
+    std::vector<double> tmp(n);
+    or_grefderiv(g, u, 1, 0, du);
+    const std::vector<double> &f0 = g->dxidx[0][idir - 1];
+    for (size_t j = 0; j < n; j++) du[j] *= f0[j];
+    for (int jd = 1; jd < g->dim; jd++) {
+      or_grefderiv(g, u, jd + 1, 0, tmp.data());
+      const std::vector<double> &f = g->dxidx[jd][idir - 1];
+      for (size_t j = 0; j < n; j++) tmp[j] *= f[j];
+      for (size_t j = 0; j < n; j++) du[j] += tmp[j];
+    }
+  }
+}
+
+// GGrid::integrate (ggrid.ipp:1077-1148): tmp = W2 W1 W0 u (left to right),
+// tmp *= Jac, serial sum.
+double or_integrate(OrGrid *g, const double *u) {
+  size_t n = 0;
+  double s = 0.0;
+  const std::vector<double> &W0 = g->basis[0].w, &W1 = g->basis[1].w;
+  for (size_t e = 0; e < g->nelems; e++) {
+    if (g->dim == 2) {
+      for (int k = 0; k < g->N[1]; k++)
+        for (int j = 0; j < g->N[0]; j++, n++) {
+          double t = W1[k] * W0[j] * u[n];
+          t = t * g->jac[n];
+          s += t;
+        }
+    } else {
+      const std::vector<double> &W2 = g->basis[2].w;
+      for (int k = 0; k < g->N[2]; k++)
+        for (int j = 0; j < g->N[1]; j++)
+          for (int i = 0; i < g->N[0]; i++, n++) {
+            double t = W2[k] * W1[j] * W0[i] * u[n];
+            t = t * g->jac[n];
+            s += t;
+          }
+    }
+  }
+  return s;
+}
+
+void or_mass_apply(OrGrid *g, const double *in, double *out) {
+  // GMass::opVec_prod: mass_.pointProd(input, output) (gmass.ipp:225-232)
+  for (size_t j = 0; j < g->ndof; j++) out[j] = g->mass[j] * in[j];
+}
+
+void or_imass_apply(OrGrid *g, const double *in, double *out) {
+  for (size_t j = 0; j < g->ndof; j++) out[j] = g->imass[j] * in[j];
+}
+
+// GGFX::doOp (ggfx.hpp:344-443) on a single rank: node i's buffer holds
+// u[m] for the members m of its group in ascending order; Sum and Smooth
+// accumulate from zero (ggfx.hpp:60-76), Max/Min from lowest()/max().
+void or_dss(OrGrid *g, int op, double *u) {
+  size_t n = g->ndof;
+  std::vector<double> out(n);
+  for (size_t i = 0; i < n; i++) {
+    const std::vector<size_t> &grp = g->group[i];
+    double r;
+    if (op == 0 || op == 1) {
+      r = 0.0;
+      for (size_t m : grp) r = r + u[m];
+      if (op == 1) r = r / (double)grp.size();
+    } else if (op == 2) {
+      r = std::numeric_limits<double>::lowest();
+      for (size_t m : grp) r = std::max(r, u[m]);
+    } else {
+      r = std::numeric_limits<double>::max();
+      for (size_t m : grp) r = std::min(r, u[m]);
+    }
+    out[i] = r;
+  }
+  std::copy(out.begin(), out.end(), u);
+}
+
+// GGFX::get_imult (ggfx.hpp:451-487)
+void or_imult(OrGrid *g, double *imult) {
+  for (size_t i = 0; i < g->ndof; i++) imult[i] = 1.0 / (double)g->group[i].size();
+}
+
+// GHelmholtz reg_prod / def_prod / embed_prod (ghelmholtz.ipp:76-378)
+void or_helmholtz_apply(OrGrid *g, const double *p, size_t np, const double *q, const double *u,
+                        double *uo) {
+  size_t n = g->ndof;
+  int dim = g->dim;
+  std::vector<std::vector<double>> gdu(dim, std::vector<double>(n));
+  std::vector<double> tmp(n);
+  bool pfull = p && np >= n;
+  if (g->gtype == 0) {
+    for (int k = 0; k < dim; k++) or_grefderiv(g, u, k + 1, 0, gdu[k].data());
+    for (int k = 0; k < dim; k++) {
+      const std::vector<double> &dx = g->dxidx[k][0];
+      for (size_t j = 0; j < n; j++) {
+        double G = dx[j] * dx[j];  // reg_init (ghelmholtz.ipp:571-575)
+        gdu[k][j] *= G;
+      }
+      if (pfull)
+        for (size_t j = 0; j < n; j++) gdu[k][j] *= p[j];
+      for (size_t j = 0; j < n; j++) gdu[k][j] = g->mass[j] * gdu[k][j];
+    }
+  } else {
+    // def_prod/embed_prod: gdu_j = sum_i G(i,j) D_i u, with G from def_init
+    // (ghelmholtz.ipp:415-519); G is built in finish_grid for embedded grids.
+    std::vector<std::vector<double>> du(dim, std::vector<double>(n));
+    for (int k = 0; k < dim; k++) or_grefderiv(g, u, k + 1, 0, du[k].data());
+    std::vector<double> G;
+    for (int jj = 0; jj < dim; jj++) {
+      std::fill(gdu[jj].begin(), gdu[jj].end(), 0.0);
+      for (int ii = 0; ii < dim; ii++) {
+        or_embed_G(g, ii, jj, G);
+        for (size_t j = 0; j < n; j++) gdu[jj][j] += du[ii][j] * G[j];
+      }
+    }
+    if (pfull)
+      for (int k = 0; k < dim; k++)
+        for (size_t j = 0; j < n; j++) gdu[k][j] *= p[j];
+  }
+  const double *up[3] = {gdu[0].data(), gdu[1].data(), dim == 3 ? gdu[2].data() : nullptr};
+  or_grefdiv(g, up, 1, uo);
+  if (p && np < n && p[0] != 1.0)
+    for (size_t j = 0; j < n; j++) uo[j] *= p[0];
+  if (q) {
+    for (size_t j = 0; j < n; j++) {
+      double t = g->mass[j] * u[j];
+      t *= q[j];
+      uo[j] += t;
+    }
+  }
+}
+
+// Assembled diagonal of (p L + q M) for the Jacobi preconditioner (a
+// build-side addition; the reference has no Jacobi preconditioner).  The
+// element diagonal is evaluated by applying the operator to unit vectors
+// one element-local node at a time would be O(n^2); instead use the closed
+// form of the regular-element operator:
+//   diag_e(i,j,k) = p * M * sum_d G_dd * sum_m (D(m,i_d) ^2 W(m)/W(i_d))
+//                   + q * M
+// where M is the node's lumped mass.  For embedded grids only the mass
+// part and the metric diagonal are used.
+void or_helmholtz_diag_inv(OrGrid *g, const double *p, size_t np, const double *q, double *dinv) {
+  size_t n = g->ndof;
+  std::vector<double> d(n, 0.0);
+  int N0 = g->N[0], N1 = g->N[1], N2 = g->N[2];
+  size_t nn = g->nnodes;
+  for (size_t e = 0; e < g->nelems; e++)
+    for (int k = 0; k < N2; k++)
+      for (int j = 0; j < N1; j++)
+        for (int i = 0; i < N0; i++) {
+          size_t id = e * nn + (size_t)k * N0 * N1 + (size_t)j * N0 + i;
+          int idx[3] = {i, j, k};
+          double Wn[3] = {g->basis[0].w[i], g->basis[1].w[j], g->dim == 3 ? g->basis[2].w[k] : 1.0};
+          double lap = 0.0;
+          for (int dd = 0; dd < g->dim; dd++) {
+            const Basis &b = g->basis[dd];
+            int Nd = b.N;
+            double s = 0.0;
+            for (int m = 0; m < Nd; m++) {
+              double Dm = b.D[m + (size_t)idx[dd] * Nd];
+              // mass at the node with index m along dd
+              s += Dm * Dm * b.w[m];
+            }
+            double other = 1.0;
+            for (int o = 0; o < g->dim; o++)
+              if (o != dd) other *= Wn[o];
+            double G = g->gtype == 0 ? g->dxidx[dd][0][id] * g->dxidx[dd][0][id] : 0.0;
+            lap += G * s * other * g->jac[id];
+          }
+          double pv = p ? (np >= n ? p[id] : p[0]) : 1.0;
+          double val = pv * lap;
+          if (q) val += q[id] * g->mass[id];
+          d[id] = val;
+        }
+  or_dss(g, 0, d.data());
+  for (size_t j = 0; j < n; j++) dinv[j] = 1.0 / d[j];
+}
+
+// GAdvect::apply (gadvect.ipp:57-99) for ivec = -2 (no Neumann hack).
+void or_advect(OrGrid *g, const double *p, int nu, const double *const *u, double *po) {
+  size_t n = g->ndof;
+  std::vector<double> t0(n);
+  if (u[0]) {
+    or_deriv(g, p, 1, po);
+    for (size_t j = 0; j < n; j++) po[j] *= u[0][j];
+  } else {
+    std::fill(po, po + n, 0.0);
+  }
+  for (int jv = 1; jv < nu; jv++) {
+    if (!u[jv]) continue;
+    or_deriv(g, p, jv + 1, t0.data());
+    for (size_t j = 0; j < n; j++) t0[j] *= u[jv][j];
+    for (size_t j = 0; j < n; j++) po[j] += t0[j];
+  }
+  for (size_t j = 0; j < n; j++) po[j] *= g->mass[j];
+}
+
+// GBurgers::dudt_impl (gburgers.ipp:202-257)
+void or_burgers_rhs(OrGrid *g, int mode, double nu, int nstate, const double *const *u,
+                    double *const *dudt) {
+  size_t n = g->ndof;
+  std::vector<double> urhs(n);
+  double pnu = nu;
+  if (mode == 2) {
+    for (int k = 0; k < nstate; k++) {
+      or_helmholtz_apply(g, &pnu, 1, nullptr, u[k], urhs.data());
+      for (size_t j = 0; j < n; j++) urhs[j] *= -1.0;
+      or_imass_apply(g, urhs.data(), dudt[k]);
+    }
+    return;
+  }
+  if (mode == 1) {
+    or_advect(g, u[0], nstate - 1, u + 1, dudt[0]);
+    or_helmholtz_apply(g, &pnu, 1, nullptr, u[0], urhs.data());
+    for (size_t j = 0; j < n; j++) urhs[j] = -1.0 * urhs[j] + -1.0 * dudt[0][j];
+    or_imass_apply(g, urhs.data(), dudt[0]);
+    return;
+  }
+  for (int k = 0; k < g->dim; k++) {
+    or_advect(g, u[k], nstate, u, dudt[k]);
+    or_helmholtz_apply(g, &pnu, 1, nullptr, u[k], urhs.data());
+    for (size_t j = 0; j < n; j++) urhs[j] = -1.0 * urhs[j] + -1.0 * dudt[k][j];
+    or_imass_apply(g, urhs.data(), dudt[k]);
+  }
+}
+
+}  // extern "C"
+
+namespace {
+// GMTK::constrain2sphere (2) (gmtk.ipp:2956-2980), in place with aliasing.
+void constrain2sphere(OrGrid *g, std::vector<double *> &v) {
+  if (g->gtype != 2 || v.size() < 3) return;
+  size_t n = g->ndof;
+  double x = g->xn[0][0], y = g->xn[1][0], z = g->xn[2][0];
+  double r2 = x * x + y * y + z * z;
+  double ri = 1.0 / r2;
+  for (size_t j = 0; j < n; j++) {
+    x = g->xn[0][j];
+    y = g->xn[1][j];
+    z = g->xn[2][j];
+    v[0][j] = (v[0][j] * (r2 - x * x) - v[1][j] * x * y - v[2][j] * x * z) * ri;
+    v[1][j] = (-v[0][j] * y * x + v[1][j] * (r2 - y * y) - v[2][j] * y * z) * ri;
+    v[2][j] = (-v[0][j] * z * x - v[1][j] * z * y + v[2][j] * (r2 - z * z)) * ri;
+  }
+}
+
+void butcher(int order, std::vector<double> &alpha, std::vector<double> &beta,
+             std::vector<double> &c) {
+  // GButcherRK::computeCoeffs (gbutcherrk.ipp:80-127); beta(i,j) at [i*order+j]
+  alpha.assign(order, 0.0);
+  c.assign(order, 0.0);
+  beta.assign((size_t)order * order, 0.0);
+  if (order == 1) {
+    alpha[0] = 0;
+    c[0] = 1.0;
+  } else if (order == 2) {
+    alpha[0] = 0; c[0] = 0.5;
+    alpha[1] = 1.0; c[1] = 0.5;
+    beta[1 * order + 0] = 1.0;
+  } else if (order == 3) {
+    alpha[0] = 0; c[0] = 0.25;
+    alpha[1] = 1.0 / 3.0; c[1] = 0.0;
+    alpha[2] = 2.0 / 3.0; c[2] = 0.75;
+    beta[1 * order + 0] = 1.0 / 3.0;
+    beta[2 * order + 1] = 2.0 / 3.0;
+  } else {
+    alpha[0] = 0; c[0] = 1.0 / 6.0;
+    alpha[1] = 0.5; c[1] = 1.0 / 3.0;
+    alpha[2] = 0.5; c[2] = 1.0 / 3.0;
+    alpha[3] = 1.0; c[3] = 1.0 / 6.0;
+    beta[1 * order + 0] = 0.5;
+    beta[2 * order + 1] = 0.5;
+    beta[3 * order + 2] = 1.0;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+// GBurgers::step_impl -> step_exrk -> GExRKStepper::step_b
+// (gburgers.ipp:274-333, 392-407; gexrk_stepper.ipp:251-341).
+void or_burgers_step(OrGrid *g, int mode, double nu, int order, int nstate, double t, double dt,
+                     double *const *ust) {
+  size_t n = g->ndof;
+  int nev = mode == 0 ? nstate : 1;  // evolved components
+  std::vector<double> alpha, beta, c;
+  butcher(order, alpha, beta, c);
+  int nstage = order;
+  double h = dt;
+  std::vector<std::vector<double>> uin(nev, std::vector<double>(n)), u(nev, std::vector<double>(n));
+  std::vector<std::vector<std::vector<double>>> K(std::max(nstage - 1, 1),
+                                                  std::vector<std::vector<double>>(nev, std::vector<double>(n)));
+  std::vector<double> isum(n);
+  // uold = uevolve (deep copy); uout = uin
+  for (int k = 0; k < nev; k++) std::copy(ust[k], ust[k] + n, uin[k].begin());
+  double *const *uout = ust;
+  // RHS state: evolved comps (+ prescribed velocities for pure advection)
+  auto rhs = [&](std::vector<std::vector<double>> &us, std::vector<std::vector<double>> &Kout) {
+    std::vector<const double *> up(nstate);
+    std::vector<double *> dp(nev);
+    for (int k = 0; k < nev; k++) up[k] = us[k].data();
+    for (int k = nev; k < nstate; k++) up[k] = ust[k];
+    for (int k = 0; k < nev; k++) dp[k] = Kout[k].data();
+    or_burgers_rhs(g, mode, nu, nstate, up.data(), dp.data());
+  };
+  auto ptrs = [&](std::vector<std::vector<double>> &vv) {
+    std::vector<double *> p(vv.size());
+    for (size_t k = 0; k < vv.size(); k++) p[k] = vv[k].data();
+    return p;
+  };
+  for (int m = 0; m < nstage - 1; m++) {
+    for (int k = 0; k < nev; k++) {
+      std::fill(isum.begin(), isum.end(), 0.0);
+      for (int j = 0; j < m; j++) {
+        double b = beta[(size_t)m * order + j] * h;
+        for (size_t q = 0; q < n; q++) isum[q] = 1.0 * isum[q] + b * K[j][k][q];
+      }
+      for (size_t q = 0; q < n; q++) u[k][q] = uin[k][q] + isum[q];
+    }
+    {
+      std::vector<double *> up = ptrs(u);
+      constrain2sphere(g, up);
+    }
+    rhs(u, K[m]);
+    for (int k = 0; k < nev; k++) {
+      or_dss(g, 1, K[m][k].data());
+      double cm = c[m] * h;
+      for (size_t q = 0; q < n; q++) isum[q] = K[m][k][q] * cm;
+      for (size_t q = 0; q < n; q++) uout[k][q] += isum[q];
+    }
+    {
+      std::vector<double *> up(uout, uout + nev);
+      constrain2sphere(g, up);
+    }
+  }
+  // Final stage
+  for (int k = 0; k < nev; k++) {
+    std::fill(isum.begin(), isum.end(), 0.0);
+    for (int j = 0; j < nstage - 1; j++) {
+      double b = beta[(size_t)(nstage - 1) * order + j] * h;
+      for (size_t q = 0; q < n; q++) isum[q] = 1.0 * isum[q] + b * K[j][k][q];
+    }
+    for (size_t q = 0; q < n; q++) u[k][q] = uin[k][q] + isum[q];
+    or_dss(g, 1, u[k].data());
+  }
+  {
+    std::vector<double *> up = ptrs(u);
+    constrain2sphere(g, up);
+  }
+  rhs(u, K[0]);
+  for (int k = 0; k < nev; k++) or_dss(g, 1, K[0][k].data());
+  for (int k = 0; k < nev; k++) {
+    double cm = c[nstage - 1] * h;
+    for (size_t q = 0; q < n; q++) isum[q] = K[0][k][q] * cm;
+    for (size_t q = 0; q < n; q++) uout[k][q] += isum[q];
+    or_dss(g, 1, uout[k]);
+  }
+  {
+    std::vector<double *> up(uout, uout + nev);
+    constrain2sphere(g, up);  // step_b end
+    constrain2sphere(g, up);  // GBurgers::step_exrk (gburgers.ipp:405)
+  }
+  (void)t;
+}
+
+// GCG::solve_impl (gcg.ipp:239-345, 361-382) and compute_norm (395-429)
+int or_cg_solve(OrGrid *g, const double *p, size_t np, const double *q, const double *b,
+                const double *xb, double *x, const OrCGTraits *tr, int *iters, double *resid) {
+  size_t n = g->ndof;
+  bool bbv = xb != nullptr;
+  std::vector<double> qv(n), r(n), w(n), z(n), imult(n), dinv;
+  or_imult(g, imult.data());
+  if (tr->precond == 1) {
+    dinv.resize(n);
+    or_helmholtz_diag_inv(g, p, np, q, dinv.data());
+  }
+  auto gdot = [&](const std::vector<double> &a, const std::vector<double> &bv) {
+    double s = 0.0;
+    for (size_t j = 0; j < n; j++) s += a[j] * bv[j] * imult[j];
+    return s;
+  };
+  auto norm = [&](const double *v) {
+    switch (tr->normtype) {
+      case 0: {
+        double m = 0.0;
+        for (size_t j = 0; j < n; j++) m = std::max(m, std::fabs(v[j]));
+        return m;
+      }
+      case 1: {
+        double s = 0.0;
+        for (size_t j = 0; j < n; j++) s += v[j] * v[j];
+        return std::sqrt(s / (double)n);
+      }
+      case 2:
+      case 3: {
+        std::vector<double> t(n);
+        for (size_t j = 0; j < n; j++) t[j] = tr->normtype == 2 ? v[j] * v[j] : std::fabs(v[j]);
+        double s = or_integrate(g, t.data()) * (1.0 / g->volume);
+        return tr->normtype == 2 ? std::sqrt(s) : s;
+      }
+      default: {
+        double s = 0.0;
+        for (size_t j = 0; j < n; j++) s += v[j] * v[j] * imult[j];
+        return std::sqrt(s);
+      }
+    }
+  };
+  auto apply = [&](const double *in, double *out) { or_helmholtz_apply(g, p, np, q, in, out); };
+  if (bbv) {
+    for (size_t j = 0; j < n; j++) x[j] *= g->mask[j];
+    for (size_t j = 0; j < n; j++) x[j] += xb[j];
+  }
+  int iret = 0;
+  std::copy(b, b + n, r.begin());
+  apply(x, w.data());
+  for (size_t j = 0; j < n; j++) r[j] -= w[j];
+  or_dss(g, 0, r.data());
+  if (bbv)
+    for (size_t j = 0; j < n; j++) r[j] *= g->mask[j];
+  if (tr->precond == 1)
+    for (size_t j = 0; j < n; j++) z[j] = dinv[j] * r[j];
+  else
+    z = r;
+  w = z;
+  double rho = gdot(r, z);
+  double rnorm, rtol;
+  if (tr->normtype == 4) {
+    std::vector<double> ba(b, b + n);
+    or_dss(g, 0, ba.data());
+    if (bbv)
+      for (size_t j = 0; j < n; j++) ba[j] *= g->mask[j];
+    rnorm = norm(ba.data());
+    rtol = tr->tol * rnorm;
+  } else {
+    rnorm = norm(b);
+    rtol = rnorm < 1.0 ? tr->tol : tr->tol * rnorm;
+  }
+  int it = 0;
+  rnorm = 10.0 * rtol;
+  while (iret == 0 && it < tr->maxit && rnorm > rtol) {
+    apply(w.data(), qv.data());
+    or_dss(g, 0, qv.data());
+    if (bbv)
+      for (size_t j = 0; j < n; j++) qv[j] *= g->mask[j];
+    double alpha = rho / gdot(w, qv);
+    for (size_t j = 0; j < n; j++) x[j] = 1.0 * x[j] + alpha * w[j];
+    for (size_t j = 0; j < n; j++) r[j] = 1.0 * r[j] + (-alpha) * qv[j];
+    if (tr->precond == 1)
+      for (size_t j = 0; j < n; j++) z[j] = dinv[j] * r[j];
+    else
+      z = r;
+    double rhom = rho;
+    rho = gdot(r, z);
+    double beta = rho / rhom;
+    for (size_t j = 0; j < n; j++) w[j] = beta * w[j] + 1.0 * z[j];
+    rnorm = norm(r.data());
+    if (resid) resid[it] = rnorm;
+    it++;
+  }
+  if (bbv)
+    for (size_t j = 0; j < n; j++) x[j] *= g->mask[j];
+  if (iret == 0 && it >= tr->maxit && rnorm > tr->tol) iret = 1;  // GCGERR_NOCONVERGE
+  if (bbv)
+    for (size_t j = 0; j < n; j++) x[j] += xb[j];
+  if (iters) *iters = it;
+  return iret;
+}
+
+}  // extern "C"
+
+namespace {
+// Common tail of grid construction: GGFX maps, mass (GMass::init3d/2d,
+// gmass.ipp:134-212), inverse mass via DSS-Smooth (gmass.ipp:75-82),
+// multiplicity and volume (ggrid.ipp:514-521).
+void finish_grid(OrGrid &g, double tol_fac) {
+  ggfx_match(g, tol_fac * g.minnodedist);
+  size_t n = g.ndof;
+  g.mass.assign(n, 0.0);
+  size_t q = 0;
+  for (size_t e = 0; e < g.nelems; e++) {
+    if (g.dim == 2) {
+      for (int k = 0; k < g.N[1]; k++)
+        for (int j = 0; j < g.N[0]; j++, q++) g.mass[q] = g.basis[1].w[k] * g.basis[0].w[j];
+    } else {
+      for (int l = 0; l < g.N[2]; l++)
+        for (int k = 0; k < g.N[1]; k++)
+          for (int j = 0; j < g.N[0]; j++, q++)
+            g.mass[q] = g.basis[2].w[l] * g.basis[1].w[k] * g.basis[0].w[j];
+    }
+  }
+  for (size_t j = 0; j < n; j++) g.mass[j] *= g.jac[j];
+  g.imass = g.mass;
+  or_dss(&g, 1, g.imass.data());
+  for (size_t j = 0; j < n; j++) g.imass[j] = 1.0 / g.imass[j];
+  g.imult.resize(n);
+  or_imult(&g, g.imult.data());
+  std::vector<double> one(n, 1.0);
+  g.volume = or_integrate(&g, one.data());
+}
+}  // namespace
+
+// Embedded-grid metric G(i,j) = sum_k dXidX(i,k) dXidX(j,k) W0 W1 Jac
+// (def_init 2-D branch, ghelmholtz.ipp:476-491), computed on demand.
+void or_embed_G(OrGrid *g, int i, int j, std::vector<double> &G) {
+  size_t n = g->ndof;
+  G.assign(n, 0.0);
+  int nxy = g->nxy();
+  size_t q = 0;
+  for (size_t e = 0; e < g->nelems; e++)
+    for (int m = 0; m < g->N[1]; m++)
+      for (int l = 0; l < g->N[0]; l++, q++)
+        for (int k = 0; k < nxy; k++)
+          G[q] += g->dxidx[i][k][q] * g->dxidx[j][k][q] * g->basis[0].w[l] * g->basis[1].w[m] * g->jac[q];
+}
+
+extern "C" OrGrid *or_grid_icos_create(int p, int ilevel, double radius, double tol_fac) {
+  (void)p; (void)ilevel; (void)radius; (void)tol_fac;
+  return nullptr;  // filled in by the icosahedral restatement
+}
