@@ -1,0 +1,216 @@
+#!/usr/bin/env python
+"""Benchmark of the GeoFLOW SEM operator path on one MI355X.
+
+Workload: 3-D box of 32^3 hex elements at GLL order p=7 (16,777,216 local
+DOF), the configuration the headline metric is quoted on:
+  * Burgers RK4: nonlinear viscous Burgers, 3 velocity components, nu=0.02,
+    dt=1e-5, fully periodic box; a "step" is one classical RK4 step of the
+    whole state (GBurgers + GExRKStepper with GGFX Smooth).  `value` is
+    DOF-steps/s = N_local * steps / seconds.
+  * Helmholtz CG: (M + 0.1 L) u = b with homogeneous Dirichlet boundaries,
+    b = DSS_sum(M f), f ~ N(0,1) per unique node; Jacobi PCG to assembled
+    relative residual 1e-10, plain FP64 versus FP32 inner solves in FP64
+    iterative refinement; plus 100 operator applies alone.
+Inputs are synthetic and resident in HBM before timing starts.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "Burgers RK4 DOF-steps/s & CG solve s at rel-res 1e-10, mixed vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def burgers_ic(torch, xyz, seed=12345, nmodes=12):
+    """Smooth periodic velocity: a few random Fourier modes per component."""
+    rng = np.random.default_rng(seed)
+    x, y, z = xyz
+    comps = []
+    for c in range(3):
+        u = torch.zeros_like(x)
+        for _ in range(nmodes):
+            k = rng.integers(-3, 4, size=3)
+            if not k.any():
+                k[0] = 1
+            a = rng.standard_normal() / (1.0 + (k * k).sum())
+            ph = rng.uniform(0, 2 * np.pi)
+            u += a * torch.cos(2 * np.pi * (int(k[0]) * x + int(k[1]) * y + int(k[2]) * z) + ph)
+        comps.append(u.contiguous())
+    return comps
+
+
+def cpu_baseline(seconds=12.0):
+    """Oracle (scalar C++ restatement of the reference, 1 thread) on a bounded
+    sample of the same workload: 8^3 elements, p=7, periodic, 3 components."""
+    import oracle
+    import geoflow_amd as ga
+    oracle.set_threads(1)
+    og = oracle.Grid.box(3, 7, [8, 8, 8])
+    bg = ga.BoxGrid(3, 7, [8, 8, 8])
+    us = [np.cos(2 * np.pi * (bg.xyz[0] + c * bg.xyz[1])) + 0.3 * np.sin(2 * np.pi * bg.xyz[2])
+          for c in range(3)]
+    og.burgers_step(us, 0.02, 1e-5)  # warm-up
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        og.burgers_step(us, 0.02, 1e-5)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": og.ndof * n / dt, "unit": "DOF-steps/s", "cores": 1, "kind": "port",
+            "sample": "oracle/gf_oracle.cpp (-O3 -march=native -ffp-contract=off), 1 thread: "
+                      "%d RK4 Burgers steps of an 8^3-element p=7 periodic box (%d local DOF, "
+                      "3 components) in %.1f s" % (n, og.ndof, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ne", type=int, default=32)
+    ap.add_argument("--p", type=int, default=7)
+    ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(see DESIGN.md for how the path would shard)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import geoflow_amd as ga
+    from geoflow_amd.device import Context, Mesh
+    from geoflow_amd import workloads
+
+    ne, p = args.ne, args.p
+    ctx = Context(0)
+    out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic"}
+
+    # ---------------- Burgers RK4 ------------------------------------------
+    t = time.perf_counter()
+    bg = ga.BoxGrid(3, p, [ne] * 3)
+    log("periodic box grid + GGFX maps: %.1f s, ndof=%d" % (time.perf_counter() - t, bg.ndof))
+    m = Mesh(ctx, bg)
+    ndof = m.ndof
+    xyz = [torch.from_numpy(a).cuda() for a in bg.xyz]
+    us = burgers_ic(torch, xyz)
+    del xyz
+    nu, dt = 0.02, 1e-5
+    for _ in range(args.warmup):
+        m.burgers_step(us, nu, dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.burgers_steps(us, nu, dt, args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ok = all(bool(torch.isfinite(u).all()) for u in us)
+    dofsteps = ndof * args.steps / el
+    log("Burgers RK4: %d steps in %.3f s -> %.3e DOF-steps/s (finite=%s)" % (args.steps, el, dofsteps, ok))
+    stage_ms = m.time_kernel("burgers_stage", reps=20)
+    stage_bytes = 5 * 8 * 3 * ndof  # stage 2-3 form: read uin,kprev,acc; write acc,k (per comp)
+    stage_gbs = stage_bytes / (stage_ms * 1e-3) / 1e9
+    step_bytes = 16 * 8 * 3 * ndof
+    step_gbs = step_bytes / (el / args.steps) / 1e9
+    out.update({"value": dofsteps, "unit": "DOF-steps/s", "ms_per_step": 1e3 * el / args.steps})
+    out["roofline"] = {"kernel": "k_burgers_stage<3,8> (fused RK stage, stages 2-3 form)",
+                       "bound": "hbm", "achieved": stage_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": stage_gbs / HBM_PEAK_GBS, "traffic": None,
+                       "avg_launch_ms": stage_ms, "alg_bytes_per_launch": stage_bytes,
+                       "step_alg_bytes": step_bytes, "step_achieved_gbs": step_gbs,
+                       "step_frac": step_gbs / HBM_PEAK_GBS}
+    del us, m
+    torch.cuda.empty_cache()
+
+    # ---------------- Helmholtz CG -------------------------------------------
+    if not args.no_cg:
+        t = time.perf_counter()
+        cg_grid = ga.BoxGrid(3, p, [ne] * 3, bdy=[ga.GF_BDY_DIRICHLET] * 6)
+        log("Dirichlet box grid + GGFX maps: %.1f s" % (time.perf_counter() - t))
+        m = Mesh(ctx, cg_grid)
+        f = torch.from_numpy(workloads.unique_normal(cg_grid, seed=67890)).cuda()
+        b = m.dss(m.mass_apply(f), "sum")
+        q = torch.ones(ndof, dtype=torch.float64, device="cuda")
+        mask = torch.from_numpy(cg_grid.mask).cuda()
+        zero = torch.zeros_like(b)
+        bnorm = float(torch.sqrt(((b * mask) ** 2 * m.imult()).sum()))
+
+        def true_relres(x):
+            r = b - m.helmholtz(x, 0.1, q)
+            m.dss(r, "sum")
+            r *= mask
+            return float(torch.sqrt((r * r * m.imult()).sum())) / bnorm
+
+        res = {}
+        for name, prec in (("fp64", 0), ("mixed", 1)):
+            x = torch.zeros_like(b)
+            m.cg_solve(b, x=x, xb=zero, p=0.1, q=q, maxit=30, tol=1e-10, normtype=4, precond=1,
+                       precision=prec)  # warm-up (allocations, code objects)
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, info = m.cg_solve(b, x=x, xb=zero, p=0.1, q=q, maxit=20000, tol=1e-10, normtype=4,
+                                 precond=1, precision=prec)
+            torch.cuda.synchronize()
+            el_cg = time.perf_counter() - t0
+            rr = true_relres(x)
+            res[name] = {"solve_s": el_cg, "iters": info["iters"], "outer": info["outer"],
+                         "status": info["status"], "true_relres": rr}
+            log("CG %s: %.3f s, %d iters (outer %d), rel-res %.2e, status %d"
+                % (name, el_cg, info["iters"], info["outer"], rr, info["status"]))
+            if name == "fp64":
+                xref = x.clone()
+            else:
+                res[name]["x_rel_err_vs_fp64"] = float((x - xref).abs().max() / xref.abs().max())
+        u = torch.randn(ndof, dtype=torch.float64, device="cuda")
+        o = torch.empty_like(u)
+        m.helmholtz(u, 0.1, q, out=o)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(100):
+            m.helmholtz(u, 0.1, q, out=o)
+        torch.cuda.synchronize()
+        a100 = time.perf_counter() - t0
+        ap_ms = m.time_kernel("cg_apply", reps=20)
+        ap32_ms = m.time_kernel("cg_apply_f32", reps=20)
+        res["apply100_s"] = a100
+        res["apply_dof_per_s"] = 100 * ndof / a100
+        res["speedup_mixed_vs_fp64"] = res["fp64"]["solve_s"] / res["mixed"]["solve_s"]
+        res["cg_apply_fp64"] = {"avg_launch_ms": ap_ms, "alg_bytes_per_launch": 5 * 8 * ndof,
+                                "achieved_gbs": 5 * 8 * ndof / (ap_ms * 1e-3) / 1e9}
+        res["cg_apply_fp64"]["frac"] = res["cg_apply_fp64"]["achieved_gbs"] / HBM_PEAK_GBS
+        res["cg_apply_fp32"] = {"avg_launch_ms": ap32_ms, "alg_bytes_per_launch": 5 * 4 * ndof,
+                                "achieved_gbs": 5 * 4 * ndof / (ap32_ms * 1e-3) / 1e9}
+        res["cg_apply_fp32"]["frac"] = res["cg_apply_fp32"]["achieved_gbs"] / HBM_PEAK_GBS
+        log("100 applies %.3f s; cg_apply fp64 %.3f ms, fp32 %.3f ms; speedup %.2fx"
+            % (a100, ap_ms, ap32_ms, res["speedup_mixed_vs_fp64"]))
+        out["cg"] = res
+        del m
+
+    out["config"] = {"workload": "3-D box %d^3 hex elements, p=%d (%d local DOF): Burgers RK4 "
+                                 "(3 comps, nu=0.02, dt=1e-5, periodic) and Helmholtz (M+0.1L) "
+                                 "Jacobi-PCG to rel-res 1e-10 (Dirichlet), FP64 vs FP32-inner/FP64-IR"
+                                 % (ne, p, ndof),
+                     "elements": [ne, ne, ne], "p": p, "ndof": ndof, "global_batch": ndof,
+                     "parallelism": "single GPU"}
+    if not args.no_cpu:
+        log("CPU baseline (oracle, 1 thread) ...")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        log("CPU baseline %.3e DOF-steps/s" % out["cpu_baseline"]["value"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,88 @@
+// gf_internal.hpp -- internal structures shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "geoflow_hip.h"
+
+#define GF_MAX_N 12  // largest isotropic node count per direction with fused kernels
+
+struct gf_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+// Reduction slot layout on the device (see gf_reduce.hpp).
+#define GF_RED_MAX_BLOCKS 4096
+
+struct gf_mesh_s {
+  gf_ctx *ctx = nullptr;
+  int dim = 3, gtype = 0;
+  int N[3] = {1, 1, 1};
+  int Niso = 0;  // isotropic N, 0 if anisotropic
+  size_t nelems = 0, nn = 0, ndof = 0;
+  // basis (device)
+  double *D[3] = {nullptr, nullptr, nullptr};
+  double *W[3] = {nullptr, nullptr, nullptr};
+  float *D32 = nullptr, *W32 = nullptr;  // isotropic FP32 copies
+  // geometry (device)
+  double *edx = nullptr;   // nelems*dim
+  double *ejac = nullptr;  // nelems
+  float *edx32 = nullptr, *ejac32 = nullptr;
+  double *ndx[9] = {nullptr};  // embedded: dXi_i/dX_j at [i*3+j]
+  double *njac = nullptr;
+  double *G[3] = {nullptr, nullptr, nullptr};  // embedded metric G00, G01, G11
+  double *xyz[3] = {nullptr, nullptr, nullptr};
+  double *mask = nullptr, *mass = nullptr, *imass = nullptr, *imult = nullptr;
+  float *mask32 = nullptr, *imult32 = nullptr;
+  bool has_mask = false;
+  // DSS: coincidence groups with more than one member (CSR, ascending members)
+  int ngroups = 0;
+  int *goff = nullptr;
+  int *gmem = nullptr;
+  int maxgrp = 1;
+  // reusable device workspace
+  std::vector<void *> ws;
+  std::vector<size_t> ws_cap;
+  double *red = nullptr;    // reduction partials, GF_RED_MAX_BLOCKS*8 doubles
+  double *scal = nullptr;   // solver scalars
+  double *hscal = nullptr;  // pinned host mirror of scal
+};
+
+int gf_set_err(gf_ctx *ctx, int code, const char *msg);
+int gf_check_hip(gf_ctx *ctx, hipError_t e, const char *what);
+
+// Internal (C++ linkage) helpers shared across translation units.
+int gf_dss_t(gf_mesh *m, int op, double *u);
+int gf_dss_f32(gf_mesh *m, int op, float *u);
+int gf_helm_t(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
+              double *uo);
+int gf_grefderiv_t(gf_mesh *m, int idir, int trans, const double *u, double *du,
+                   const double *edx);
+int gf_mul_t(gf_mesh *m, const double *a, const double *b, double *out);
+int gf_dot_t(gf_mesh *m, const double *a, const double *b, size_t n, double *result);
+int gf_axpy_t(gf_mesh *m, double a, const double *x, double *y);
+
+// Workspace: returns a device buffer of at least `bytes` for slot `slot`.
+void *gf_ws(gf_mesh *m, int slot, size_t bytes);
+
+#define GF_HIP(ctx, call)                                   \
+  do {                                                      \
+    hipError_t _e = (call);                                 \
+    if (_e != hipSuccess) return gf_check_hip(ctx, _e, #call); \
+  } while (0)
+
+#define GF_LAUNCH_CHECK(ctx) GF_HIP(ctx, hipGetLastError())
+
+// Workspace slot numbers
+enum {
+  WS_TMP0 = 0, WS_TMP1, WS_TMP2, WS_TMP3, WS_TMP4, WS_TMP5,
+  WS_RK_BEGIN = 8,          // RK: acc[3], K[3][3], u3[3]
+  WS_CG_BEGIN = 32,         // CG vectors
+  WS_SLOTS = 64
+};

@@ -1,0 +1,925 @@
+// gf_ops.hip -- context, mesh and the element-local operators of the SEM path:
+// tensor-product derivatives, gradient, divergence, mass, Helmholtz, DSS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "gf_kernels.hpp"
+
+using gf::kThreads;
+using gf::Shape;
+
+// ===========================================================================
+// Context, errors, workspace
+// ===========================================================================
+int gf_set_err(gf_ctx *ctx, int code, const char *msg) {
+  if (ctx) ctx->err = msg ? msg : "";
+  return code;
+}
+
+int gf_check_hip(gf_ctx *ctx, hipError_t e, const char *what) {
+  if (ctx) {
+    ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+  }
+  return e == hipErrorOutOfMemory ? GF_ERR_ALLOC : GF_ERR_HIP;
+}
+
+void *gf_ws(gf_mesh *m, int slot, size_t bytes) {
+  if ((int)m->ws.size() <= slot) {
+    m->ws.resize(slot + 1, nullptr);
+    m->ws_cap.resize(slot + 1, 0);
+  }
+  if (m->ws[slot] && m->ws_cap[slot] >= bytes) return m->ws[slot];
+  if (m->ws[slot]) (void)hipFree(m->ws[slot]);
+  m->ws[slot] = nullptr;
+  m->ws_cap[slot] = 0;
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  m->ws[slot] = p;
+  m->ws_cap[slot] = bytes;
+  return p;
+}
+
+extern "C" {
+
+const char *gf_version(void) { return "geoflow_amd 0.1 (gfx950)"; }
+
+int gf_ctx_create(int device, void *stream, gf_ctx **out) {
+  if (!out) return GF_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return GF_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return GF_ERR_HIP;
+  gf_ctx *c = new gf_ctx_s();
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  *out = c;
+  return GF_OK;
+}
+
+int gf_ctx_destroy(gf_ctx *ctx) {
+  delete ctx;
+  return GF_OK;
+}
+
+int gf_ctx_set_stream(gf_ctx *ctx, void *stream) {
+  if (!ctx) return GF_ERR_ARG;
+  ctx->stream = (hipStream_t)stream;
+  return GF_OK;
+}
+
+int gf_ctx_synchronize(gf_ctx *ctx) {
+  if (!ctx) return GF_ERR_ARG;
+  GF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GF_OK;
+}
+
+const char *gf_ctx_last_error(gf_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int gf_malloc(gf_ctx *ctx, size_t bytes, void **dptr) {
+  if (!ctx || !dptr) return GF_ERR_ARG;
+  GF_HIP(ctx, hipMalloc(dptr, bytes ? bytes : 1));
+  return GF_OK;
+}
+int gf_free(gf_ctx *ctx, void *dptr) {
+  if (!ctx) return GF_ERR_ARG;
+  GF_HIP(ctx, hipFree(dptr));
+  return GF_OK;
+}
+int gf_memcpy_h2d(gf_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return GF_ERR_ARG;
+  GF_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  GF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GF_OK;
+}
+int gf_memcpy_d2h(gf_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return GF_ERR_ARG;
+  GF_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  GF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GF_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Kernels
+// ===========================================================================
+namespace {
+
+// ---- derivative along one direction (isotropic N) -------------------------
+// du = Dop_dir u, optionally times the per-element physical factor
+// edx[e*DIM+dir] applied after the sum (GGrid::deriv, ggrid.ipp:1175-1177).
+template <int DIM, int N, bool TRANS>
+__global__ __launch_bounds__(kThreads) void k_deriv(const double *__restrict__ u,
+                                                    double *__restrict__ du,
+                                                    const double *__restrict__ D,
+                                                    const double *__restrict__ edx, int dir,
+                                                    size_t nelems) {
+  using S = Shape<DIM, N>;
+  __shared__ double su[S::NODES];
+  __shared__ double sD[N * N];
+  size_t e0 = (size_t)blockIdx.x * S::EPB;
+  int nel = (int)min((size_t)S::EPB, nelems - e0);
+  size_t base = e0 * S::NN;
+  gf::stage_D(sD, D, N);
+  gf::stage(su, u, base, nel * S::NN);
+  __syncthreads();
+  for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x) {
+    int el = q / S::NN, r = q - el * S::NN, i, j, k;
+    gf::node_ijk<DIM, N>(r, i, j, k);
+    double v = gf::contract<DIM, N, TRANS>(su + el * S::NN, sD, dir, i, j, k);
+    if (edx) v = v * edx[(e0 + el) * DIM + dir];
+    du[base + q] = v;
+  }
+}
+
+// ---- all directions at once (compute_grefderivs / grad) -------------------
+template <int DIM, int N>
+__global__ __launch_bounds__(kThreads) void k_grad(const double *__restrict__ u, double *d0,
+                                                   double *d1, double *d2,
+                                                   const double *__restrict__ D,
+                                                   const double *__restrict__ edx, size_t nelems) {
+  using S = Shape<DIM, N>;
+  __shared__ double su[S::NODES];
+  __shared__ double sD[N * N];
+  size_t e0 = (size_t)blockIdx.x * S::EPB;
+  int nel = (int)min((size_t)S::EPB, nelems - e0);
+  size_t base = e0 * S::NN;
+  gf::stage_D(sD, D, N);
+  gf::stage(su, u, base, nel * S::NN);
+  __syncthreads();
+  double *outs[3] = {d0, d1, d2};
+  for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x) {
+    int el = q / S::NN, r = q - el * S::NN, i, j, k;
+    gf::node_ijk<DIM, N>(r, i, j, k);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      double v = gf::contract<DIM, N, false>(su + el * S::NN, sD, d, i, j, k);
+      if (edx) v = v * edx[(e0 + el) * DIM + d];
+      outs[d][base + q] = v;
+    }
+  }
+}
+
+// ---- reference-space divergence (compute_grefdiv) --------------------------
+// divu = ((0 + D1 u1) + D2 u2) + D3 u3 (ggrid.ipp:1870-1925); NULL inputs skipped.
+template <int DIM, int N, bool TRANS>
+__global__ __launch_bounds__(kThreads) void k_div(const double *__restrict__ u0,
+                                                  const double *__restrict__ u1,
+                                                  const double *__restrict__ u2,
+                                                  double *__restrict__ out,
+                                                  const double *__restrict__ D, size_t nelems) {
+  using S = Shape<DIM, N>;
+  __shared__ double su[S::NODES];
+  __shared__ double sD[N * N];
+  size_t e0 = (size_t)blockIdx.x * S::EPB;
+  int nel = (int)min((size_t)S::EPB, nelems - e0);
+  size_t base = e0 * S::NN;
+  gf::stage_D(sD, D, N);
+  const double *ins[3] = {u0, u1, u2};
+  double acc[(S::NODES + kThreads - 1) / kThreads];
+#pragma unroll
+  for (int t = 0; t < (S::NODES + kThreads - 1) / kThreads; t++) acc[t] = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    if (!ins[d]) continue;
+    __syncthreads();
+    gf::stage(su, ins[d], base, nel * S::NN);
+    __syncthreads();
+    int t = 0;
+    for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x, t++) {
+      int el = q / S::NN, r = q - el * S::NN, i, j, k;
+      gf::node_ijk<DIM, N>(r, i, j, k);
+      acc[t] += gf::contract<DIM, N, TRANS>(su + el * S::NN, sD, d, i, j, k);
+    }
+  }
+  int t = 0;
+  for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x, t++) out[base + q] = acc[t];
+}
+
+// ---- regular-element Helmholtz ------------------------------------------
+// uo = p * sum_k D_k^T ( M * (G_kk * (D_k u)) [* p_field] )  +  q * M * u
+// with G_kk = (dXi_k/dX_k)^2 (reg_init, ghelmholtz.ipp:571-575) and the lumped
+// mass computed in-kernel from the weights and the element Jacobian
+// (reg_prod, ghelmholtz.ipp:308-378).
+template <int DIM, int N>
+__global__ __launch_bounds__(kThreads) void k_helm(const double *__restrict__ u,
+                                                   double *__restrict__ uo,
+                                                   const double *__restrict__ D,
+                                                   const double *__restrict__ W,
+                                                   const double *__restrict__ edx,
+                                                   const double *__restrict__ ejac, double pscal,
+                                                   const double *__restrict__ pfield,
+                                                   const double *__restrict__ qfield,
+                                                   size_t nelems) {
+  using S = Shape<DIM, N>;
+  __shared__ double su[S::NODES];
+  __shared__ double sg[DIM][S::NODES];
+  __shared__ double sD[N * N];
+  __shared__ double sW[N];
+  size_t e0 = (size_t)blockIdx.x * S::EPB;
+  int nel = (int)min((size_t)S::EPB, nelems - e0);
+  size_t base = e0 * S::NN;
+  gf::stage_D(sD, D, N);
+  if (threadIdx.x < N) sW[threadIdx.x] = W[threadIdx.x];
+  gf::stage(su, u, base, nel * S::NN);
+  __syncthreads();
+  for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x) {
+    int el = q / S::NN, r = q - el * S::NN, i, j, k;
+    gf::node_ijk<DIM, N>(r, i, j, k);
+    size_t e = e0 + el;
+    double M = gf::node_mass<DIM>(sW, i, j, k, ejac[e]);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      double g = gf::contract<DIM, N, false>(su + el * S::NN, sD, d, i, j, k);
+      double f = edx[e * DIM + d];
+      g = g * (f * f);
+      if (pfield) g = g * pfield[base + q];
+      sg[d][q] = M * g;
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x) {
+    int el = q / S::NN, r = q - el * S::NN, i, j, k;
+    gf::node_ijk<DIM, N>(r, i, j, k);
+    double acc = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) acc += gf::contract<DIM, N, true>(&sg[d][el * S::NN], sD, d, i, j, k);
+    if (!pfield && pscal != 1.0) acc *= pscal;
+    if (qfield) {
+      double M = gf::node_mass<DIM>(sW, i, j, k, ejac[e0 + el]);
+      double t = M * su[q];
+      t *= qfield[base + q];
+      acc += t;
+    }
+    uo[base + q] = acc;
+  }
+}
+
+// ---- 2-D embedded Helmholtz (embed_prod, ghelmholtz.ipp:205-290) -----------
+// gdu_j = sum_i G(i,j) D_i u ; uo = p sum_j D_j^T gdu_j + q M u
+template <int DIM, int N>
+__global__ __launch_bounds__(kThreads) void k_helm_embed(
+    const double *__restrict__ u, double *__restrict__ uo, const double *__restrict__ D,
+    const double *__restrict__ G00, const double *__restrict__ G01, const double *__restrict__ G11,
+    const double *__restrict__ mass, double pscal, const double *__restrict__ pfield,
+    const double *__restrict__ qfield, size_t nelems) {
+  using S = Shape<2, N>;
+  __shared__ double su[S::NODES];
+  __shared__ double sg[2][S::NODES];
+  __shared__ double sD[N * N];
+  size_t e0 = (size_t)blockIdx.x * S::EPB;
+  int nel = (int)min((size_t)S::EPB, nelems - e0);
+  size_t base = e0 * S::NN;
+  gf::stage_D(sD, D, N);
+  gf::stage(su, u, base, nel * S::NN);
+  __syncthreads();
+  for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x) {
+    int el = q / S::NN, r = q - el * S::NN, i, j, k;
+    gf::node_ijk<2, N>(r, i, j, k);
+    double d0 = gf::contract<2, N, false>(su + el * S::NN, sD, 0, i, j, k);
+    double d1 = gf::contract<2, N, false>(su + el * S::NN, sD, 1, i, j, k);
+    size_t g = base + q;
+    double g0 = 0.0 + d0 * G00[g];
+    g0 += d1 * G01[g];
+    double g1 = 0.0 + d0 * G01[g];
+    g1 += d1 * G11[g];
+    if (pfield) {
+      g0 *= pfield[g];
+      g1 *= pfield[g];
+    }
+    sg[0][q] = g0;
+    sg[1][q] = g1;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x) {
+    int el = q / S::NN, r = q - el * S::NN, i, j, k;
+    gf::node_ijk<2, N>(r, i, j, k);
+    double acc = 0.0;
+    acc += gf::contract<2, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
+    acc += gf::contract<2, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
+    if (!pfield && pscal != 1.0) acc *= pscal;
+    if (qfield) {
+      double t = mass[base + q] * su[q];
+      t *= qfield[base + q];
+      acc += t;
+    }
+    uo[base + q] = acc;
+  }
+}
+
+// ---- generic (anisotropic) derivative: one thread per node, operands from
+// global memory through the cache.
+__global__ void k_deriv_generic(const double *__restrict__ u, double *__restrict__ du,
+                                const double *__restrict__ D, int dir, int trans, int N0, int N1,
+                                int N2, size_t ndof, const double *__restrict__ edx, int dim) {
+  size_t nn = (size_t)N0 * N1 * N2;
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < ndof;
+       g += (size_t)gridDim.x * blockDim.x) {
+    size_t e = g / nn;
+    int r = (int)(g - e * nn);
+    int i = r % N0, j = (r / N0) % N1, k = r / (N0 * N1);
+    const double *ue = u + e * nn;
+    double acc = 0.0;
+    if (dir == 0) {
+      for (int m = 0; m < N0; m++) {
+        double d = trans ? D[m + i * N0] : D[i + m * N0];
+        acc = fma(d, ue[(size_t)k * N0 * N1 + j * N0 + m], acc);
+      }
+    } else if (dir == 1) {
+      for (int m = 0; m < N1; m++) {
+        double d = trans ? D[m + j * N1] : D[j + m * N1];
+        acc = fma(d, ue[(size_t)k * N0 * N1 + m * N0 + i], acc);
+      }
+    } else {
+      for (int m = 0; m < N2; m++) {
+        double d = trans ? D[m + k * N2] : D[k + m * N2];
+        acc = fma(d, ue[(size_t)m * N0 * N1 + j * N0 + i], acc);
+      }
+    }
+    if (edx) acc = acc * edx[e * dim + dir];
+    du[g] = acc;
+  }
+}
+
+// ---- mass -----------------------------------------------------------------
+template <int DIM, int N>
+__global__ void k_mass_init(double *__restrict__ mass, const double *__restrict__ W,
+                            const double *__restrict__ ejac, size_t ndof) {
+  constexpr int NN = Shape<DIM, N>::NN;
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < ndof;
+       g += (size_t)gridDim.x * blockDim.x) {
+    size_t e = g / NN;
+    int r = (int)(g - e * NN), i, j, k;
+    gf::node_ijk<DIM, N>(r, i, j, k);
+    mass[g] = gf::node_mass<DIM>(W, i, j, k, ejac[e]);
+  }
+}
+
+__global__ void k_mass_init_generic(double *__restrict__ mass, const double *__restrict__ W0,
+                                    const double *__restrict__ W1, const double *__restrict__ W2,
+                                    const double *__restrict__ jac, int pernode, int N0, int N1,
+                                    int N2, int dim, size_t ndof) {
+  size_t nn = (size_t)N0 * N1 * N2;
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < ndof;
+       g += (size_t)gridDim.x * blockDim.x) {
+    size_t e = g / nn;
+    int r = (int)(g - e * nn);
+    int i = r % N0, j = (r / N0) % N1, k = r / (N0 * N1);
+    double w = dim == 3 ? W2[k] * W1[j] * W0[i] : W1[j] * W0[i];
+    mass[g] = w * (pernode ? jac[g] : jac[e]);
+  }
+}
+
+__global__ void k_mul(const double *__restrict__ a, const double *__restrict__ b,
+                      double *__restrict__ out, size_t n) {
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < n;
+       g += (size_t)gridDim.x * blockDim.x)
+    out[g] = a[g] * b[g];
+}
+
+__global__ void k_recip(double *__restrict__ a, size_t n) {
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < n;
+       g += (size_t)gridDim.x * blockDim.x)
+    a[g] = 1.0 / a[g];
+}
+
+// ---- DSS over coincidence groups --------------------------------------------
+// Each thread owns one group with more than one member; members are visited
+// in ascending local id and accumulated from zero, the reference's
+// std::accumulate order (ggfx.hpp:60-76, 401-405), then written back to every
+// member.  Singleton groups are identities and are skipped.
+template <int OP, typename T>
+__global__ void k_dss(T *__restrict__ u, const int *__restrict__ goff,
+                      const int *__restrict__ gmem, int ngroups) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += gridDim.x * blockDim.x) {
+    int b = goff[g], e = goff[g + 1];
+    T r;
+    if (OP == GF_DSS_SUM || OP == GF_DSS_SMOOTH) {
+      r = T(0);
+      for (int s = b; s < e; s++) r = r + u[gmem[s]];
+      if (OP == GF_DSS_SMOOTH) r = r / (T)(e - b);
+    } else if (OP == GF_DSS_MAX) {
+      r = std::numeric_limits<T>::lowest();
+      for (int s = b; s < e; s++) r = fmax(r, u[gmem[s]]);
+    } else {
+      r = std::numeric_limits<T>::max();
+      for (int s = b; s < e; s++) r = fmin(r, u[gmem[s]]);
+    }
+    for (int s = b; s < e; s++) u[gmem[s]] = r;
+  }
+}
+
+// mult/imult from groups: start from 1, overwrite group members.
+__global__ void k_fill(double *__restrict__ a, double v, size_t n) {
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < n;
+       g += (size_t)gridDim.x * blockDim.x)
+    a[g] = v;
+}
+__global__ void k_imult_groups(double *__restrict__ imult, const int *__restrict__ goff,
+                               const int *__restrict__ gmem, int ngroups) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += gridDim.x * blockDim.x) {
+    int b = goff[g], e = goff[g + 1];
+    double v = 1.0 / (double)(e - b);
+    for (int s = b; s < e; s++) imult[gmem[s]] = v;
+  }
+}
+
+// Deterministic two-stage sum: per-block partials then one block in order.
+__global__ void k_partial_dot(const double *__restrict__ a, const double *__restrict__ b,
+                              double *__restrict__ part, size_t n) {
+  __shared__ double sred[kThreads / 64];
+  double v = 0.0;
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < n;
+       g += (size_t)gridDim.x * blockDim.x)
+    v += a[g] * (b ? b[g] : 1.0);
+  double r = gf::block_reduce<false>(v, sred);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+__global__ void k_final_sum(const double *__restrict__ part, int np, double *__restrict__ out) {
+  __shared__ double sred[kThreads / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+  double r = gf::block_reduce<false>(v, sred);
+  if (threadIdx.x == 0) *out = r;
+}
+
+inline int nblocks(size_t n, int per = kThreads) {
+  size_t b = (n + per - 1) / per;
+  if (b > 65536) b = 65536;
+  return (int)std::max<size_t>(b, 1);
+}
+
+}  // namespace
+
+// ===========================================================================
+// Launch wrappers (templated on DIM, N)
+// ===========================================================================
+namespace gfl {
+
+template <int DIM, int N>
+int launch_deriv(gf_mesh *m, int dir, int trans, const double *u, double *du, const double *edx) {
+  using S = Shape<DIM, N>;
+  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  if (trans)
+    hipLaunchKernelGGL((k_deriv<DIM, N, true>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, du,
+                       m->D[dir], edx, dir, m->nelems);
+  else
+    hipLaunchKernelGGL((k_deriv<DIM, N, false>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u,
+                       du, m->D[dir], edx, dir, m->nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int DIM, int N>
+int launch_grad(gf_mesh *m, const double *u, double *d0, double *d1, double *d2,
+                const double *edx) {
+  using S = Shape<DIM, N>;
+  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  hipLaunchKernelGGL((k_grad<DIM, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, d0, d1, d2,
+                     m->D[0], edx, m->nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int DIM, int N>
+int launch_div(gf_mesh *m, int trans, const double *u0, const double *u1, const double *u2,
+               double *out) {
+  using S = Shape<DIM, N>;
+  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  if (trans)
+    hipLaunchKernelGGL((k_div<DIM, N, true>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u0, u1,
+                       u2, out, m->D[0], m->nelems);
+  else
+    hipLaunchKernelGGL((k_div<DIM, N, false>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u0, u1,
+                       u2, out, m->D[0], m->nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int DIM, int N>
+int launch_helm(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
+                double *uo) {
+  using S = Shape<DIM, N>;
+  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  if (m->gtype == GF_GE_REGULAR) {
+    hipLaunchKernelGGL((k_helm<DIM, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, uo,
+                       m->D[0], m->W[0], m->edx, m->ejac, p, pf, qf, m->nelems);
+  } else {
+    if (DIM != 2) return GF_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL((k_helm_embed<2, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, uo,
+                       m->D[0], m->G[0], m->G[1], m->G[2], m->mass, p, pf, qf, m->nelems);
+  }
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int DIM, int N>
+int launch_mass_init(gf_mesh *m) {
+  hipLaunchKernelGGL((k_mass_init<DIM, N>), dim3(nblocks(m->ndof)), dim3(kThreads), 0,
+                     m->ctx->stream, m->mass, m->W[0], m->ejac, m->ndof);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+}  // namespace gfl
+
+// Internal entry points used by other translation units.
+int gf_dss_t(gf_mesh *m, int op, double *u) {
+  if (m->ngroups == 0) return GF_OK;
+  int nb = nblocks((size_t)m->ngroups);
+  hipStream_t s = m->ctx->stream;
+  switch (op) {
+    case GF_DSS_SUM: hipLaunchKernelGGL((k_dss<GF_DSS_SUM, double>), dim3(nb), dim3(kThreads), 0, s, u, m->goff, m->gmem, m->ngroups); break;
+    case GF_DSS_SMOOTH: hipLaunchKernelGGL((k_dss<GF_DSS_SMOOTH, double>), dim3(nb), dim3(kThreads), 0, s, u, m->goff, m->gmem, m->ngroups); break;
+    case GF_DSS_MAX: hipLaunchKernelGGL((k_dss<GF_DSS_MAX, double>), dim3(nb), dim3(kThreads), 0, s, u, m->goff, m->gmem, m->ngroups); break;
+    case GF_DSS_MIN: hipLaunchKernelGGL((k_dss<GF_DSS_MIN, double>), dim3(nb), dim3(kThreads), 0, s, u, m->goff, m->gmem, m->ngroups); break;
+    default: return GF_ERR_ARG;
+  }
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+int gf_dss_f32(gf_mesh *m, int op, float *u) {
+  if (m->ngroups == 0) return GF_OK;
+  int nb = nblocks((size_t)m->ngroups);
+  hipStream_t s = m->ctx->stream;
+  if (op == GF_DSS_SUM)
+    hipLaunchKernelGGL((k_dss<GF_DSS_SUM, float>), dim3(nb), dim3(kThreads), 0, s, u, m->goff, m->gmem, m->ngroups);
+  else if (op == GF_DSS_SMOOTH)
+    hipLaunchKernelGGL((k_dss<GF_DSS_SMOOTH, float>), dim3(nb), dim3(kThreads), 0, s, u, m->goff, m->gmem, m->ngroups);
+  else
+    return GF_ERR_ARG;
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+int gf_helm_t(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
+              double *uo) {
+  if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Helmholtz needs isotropic order");
+  return GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_helm, m, p, pf, qf, u, uo);
+}
+
+int gf_grefderiv_t(gf_mesh *m, int idir, int trans, const double *u, double *du,
+                   const double *edx) {
+  int dir = idir - 1;
+  if (m->Niso) return GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_deriv, m, dir, trans, u, du, edx);
+  hipLaunchKernelGGL(k_deriv_generic, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream,
+                     u, du, m->D[dir], dir, trans, m->N[0], m->N[1], m->dim == 3 ? m->N[2] : 1,
+                     m->ndof, edx, m->dim);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+int gf_mul_t(gf_mesh *m, const double *a, const double *b, double *out) {
+  hipLaunchKernelGGL(k_mul, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, a, b, out,
+                     m->ndof);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+// Deterministic sum of a[i]*b[i] (b may be NULL); synchronous.
+int gf_dot_t(gf_mesh *m, const double *a, const double *b, size_t n, double *result) {
+  int nb = std::min(nblocks(n), 1024);
+  double *part = m->red;
+  hipLaunchKernelGGL(k_partial_dot, dim3(nb), dim3(kThreads), 0, m->ctx->stream, a, b, part, n);
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kThreads), 0, m->ctx->stream, part, nb,
+                     m->scal + 63);
+  GF_LAUNCH_CHECK(m->ctx);
+  GF_HIP(m->ctx, hipMemcpyAsync(result, m->scal + 63, sizeof(double), hipMemcpyDeviceToHost,
+                                m->ctx->stream));
+  GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+  return GF_OK;
+}
+
+// ===========================================================================
+// Mesh
+// ===========================================================================
+namespace {
+template <typename T>
+int upload(gf_ctx *c, T **d, const T *h, size_t n) {
+  *d = nullptr;
+  if (!h) return GF_OK;
+  GF_HIP(c, hipMalloc((void **)d, n * sizeof(T) + 16));
+  GF_HIP(c, hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return GF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
+  if (!ctx || !d || !out) return GF_ERR_ARG;
+  if (d->dim != 2 && d->dim != 3) return gf_set_err(ctx, GF_ERR_ARG, "dim must be 2 or 3");
+  if (d->gtype != GF_GE_REGULAR && d->gtype != GF_GE_2DEMBEDDED)
+    return gf_set_err(ctx, GF_ERR_UNSUPPORTED, "only GE_REGULAR and GE_2DEMBEDDED meshes");
+  if (d->gtype == GF_GE_2DEMBEDDED && d->dim != 2)
+    return gf_set_err(ctx, GF_ERR_ARG, "embedded meshes are 2-D");
+  if (!d->ggfx_offs || !d->ggfx_members) return gf_set_err(ctx, GF_ERR_ARG, "missing GGFX maps");
+  GF_HIP(ctx, hipSetDevice(ctx->device));
+  gf_mesh *m = new gf_mesh_s();
+  m->ctx = ctx;
+  m->dim = d->dim;
+  m->gtype = d->gtype;
+  m->nelems = d->nelems;
+  m->nn = 1;
+  for (int k = 0; k < 3; k++) m->N[k] = (k < d->dim) ? d->N[k] : 1;
+  for (int k = 0; k < d->dim; k++) {
+    if (m->N[k] < 2 || m->N[k] > 64 || !d->D[k] || !d->W[k]) {
+      delete m;
+      return gf_set_err(ctx, GF_ERR_ARG, "bad basis");
+    }
+    m->nn *= (size_t)m->N[k];
+  }
+  m->ndof = m->nelems * m->nn;
+  bool iso = true;
+  for (int k = 1; k < d->dim; k++) iso = iso && m->N[k] == m->N[0];
+  int nmax = d->dim == 3 ? 10 : 12;
+  m->Niso = (iso && m->N[0] <= nmax) ? m->N[0] : 0;
+  int rc = GF_OK;
+#define UP(dst, src, n)                      \
+  do {                                       \
+    rc = upload(ctx, &(dst), (src), (n));    \
+    if (rc) { gf_mesh_destroy(m); return rc; } \
+  } while (0)
+  for (int k = 0; k < d->dim; k++) {
+    UP(m->D[k], d->D[k], (size_t)m->N[k] * m->N[k]);
+    UP(m->W[k], d->W[k], (size_t)m->N[k]);
+  }
+  if (m->Niso) {
+    std::vector<float> Df((size_t)m->Niso * m->Niso), Wf(m->Niso);
+    for (size_t q = 0; q < Df.size(); q++) Df[q] = (float)d->D[0][q];
+    for (int q = 0; q < m->Niso; q++) Wf[q] = (float)d->W[0][q];
+    UP(m->D32, Df.data(), Df.size());
+    UP(m->W32, Wf.data(), Wf.size());
+  }
+  size_t n = m->ndof;
+  if (d->gtype == GF_GE_REGULAR) {
+    if (!d->elem_dxidx || !d->elem_jac) {
+      gf_mesh_destroy(m);
+      return gf_set_err(ctx, GF_ERR_ARG, "regular mesh needs elem_dxidx and elem_jac");
+    }
+    UP(m->edx, d->elem_dxidx, m->nelems * d->dim);
+    UP(m->ejac, d->elem_jac, m->nelems);
+    std::vector<float> a(m->nelems * d->dim), b(m->nelems);
+    for (size_t q = 0; q < a.size(); q++) a[q] = (float)d->elem_dxidx[q];
+    for (size_t q = 0; q < b.size(); q++) b[q] = (float)d->elem_jac[q];
+    UP(m->edx32, a.data(), a.size());
+    UP(m->ejac32, b.data(), b.size());
+  } else {
+    if (!d->node_jac) {
+      gf_mesh_destroy(m);
+      return gf_set_err(ctx, GF_ERR_ARG, "embedded mesh needs node metrics");
+    }
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 3; j++) {
+        if (!d->node_dxidx[i * 3 + j]) {
+          gf_mesh_destroy(m);
+          return gf_set_err(ctx, GF_ERR_ARG, "embedded mesh needs dXi/dX for i<2, j<3");
+        }
+        UP(m->ndx[i * 3 + j], d->node_dxidx[i * 3 + j], n);
+      }
+    UP(m->njac, d->node_jac, n);
+    // Metric G(i,j) = sum_k dXidX(i,k) dXidX(j,k) W0 W1 Jac (def_init 2-D,
+    // src/cdg/sem/ghelmholtz.ipp:476-491), built on the host in that order.
+    std::vector<double> G[3];
+    int Ni = m->N[0], Nj = m->N[1];
+    for (int c = 0; c < 3; c++) G[c].assign(n, 0.0);
+    const int ii[3] = {0, 0, 1}, jj[3] = {0, 1, 1};
+    for (int c = 0; c < 3; c++) {
+      std::fill(G[c].begin(), G[c].end(), 0.0);
+      size_t q = 0;
+      for (size_t e = 0; e < m->nelems; e++)
+        for (int mm = 0; mm < Nj; mm++)
+          for (int l = 0; l < Ni; l++, q++)
+            for (int k = 0; k < 3; k++)
+              G[c][q] += d->node_dxidx[ii[c] * 3 + k][q] * d->node_dxidx[jj[c] * 3 + k][q] *
+                         d->W[0][l] * d->W[1][mm] * d->node_jac[q];
+      UP(m->G[c], G[c].data(), n);
+    }
+  }
+  for (int k = 0; k < 3; k++)
+    if (d->xyz[k]) UP(m->xyz[k], d->xyz[k], n);
+  // mask
+  m->has_mask = d->mask != nullptr;
+  {
+    std::vector<double> mk(n, 1.0);
+    if (d->mask) std::copy(d->mask, d->mask + n, mk.begin());
+    UP(m->mask, mk.data(), n);
+    std::vector<float> mf(n);
+    for (size_t q = 0; q < n; q++) mf[q] = (float)mk[q];
+    UP(m->mask32, mf.data(), n);
+  }
+  // DSS groups with more than one member, each stored once, in order of
+  // their smallest member (which is the node at which GGFX::init formed it).
+  {
+    std::vector<int> goff(1, 0), gmem;
+    int maxg = 1;
+    for (size_t q = 0; q < n; q++) {
+      int64_t b = d->ggfx_offs[q], e = d->ggfx_offs[q + 1];
+      if (e - b < 1 || d->ggfx_members[b] < 0 || (size_t)d->ggfx_members[e - 1] >= n) {
+        gf_mesh_destroy(m);
+        return gf_set_err(ctx, GF_ERR_ARG, "malformed GGFX map");
+      }
+      if (e - b > 1 && (size_t)d->ggfx_members[b] == q) {
+        for (int64_t s = b; s < e; s++) gmem.push_back((int)d->ggfx_members[s]);
+        goff.push_back((int)gmem.size());
+        maxg = std::max<int>(maxg, (int)(e - b));
+      }
+    }
+    m->ngroups = (int)goff.size() - 1;
+    m->maxgrp = maxg;
+    UP(m->goff, goff.data(), goff.size());
+    UP(m->gmem, gmem.data(), std::max<size_t>(gmem.size(), 1));
+  }
+  GF_HIP(ctx, hipMalloc((void **)&m->red, GF_RED_MAX_BLOCKS * 8 * sizeof(double)));
+  GF_HIP(ctx, hipMalloc((void **)&m->scal, 64 * sizeof(double)));
+  GF_HIP(ctx, hipHostMalloc((void **)&m->hscal, 64 * sizeof(double), hipHostMallocDefault));
+  GF_HIP(ctx, hipMalloc((void **)&m->mass, n * sizeof(double)));
+  GF_HIP(ctx, hipMalloc((void **)&m->imass, n * sizeof(double)));
+  GF_HIP(ctx, hipMalloc((void **)&m->imult, n * sizeof(double)));
+  GF_HIP(ctx, hipMalloc((void **)&m->imult32, n * sizeof(float)));
+  hipStream_t s = ctx->stream;
+  // mass (GMass::init), then inverse via DSS-Smooth (gmass.ipp:75-82)
+  if (d->gtype == GF_GE_REGULAR && m->Niso) {
+    rc = GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_mass_init, m);
+  } else {
+    hipLaunchKernelGGL(k_mass_init_generic, dim3(nblocks(n)), dim3(kThreads), 0, s, m->mass,
+                       m->W[0], m->W[1], m->dim == 3 ? m->W[2] : m->W[1],
+                       d->gtype == GF_GE_REGULAR ? m->ejac : m->njac,
+                       d->gtype == GF_GE_REGULAR ? 0 : 1, m->N[0], m->N[1], m->N[2], m->dim, n);
+  }
+  if (rc) { gf_mesh_destroy(m); return rc; }
+  GF_HIP(ctx, hipMemcpyAsync(m->imass, m->mass, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  rc = gf_dss_t(m, GF_DSS_SMOOTH, m->imass);
+  if (rc) { gf_mesh_destroy(m); return rc; }
+  hipLaunchKernelGGL(k_recip, dim3(nblocks(n)), dim3(kThreads), 0, s, m->imass, n);
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kThreads), 0, s, m->imult, 1.0, n);
+  if (m->ngroups)
+    hipLaunchKernelGGL(k_imult_groups, dim3(nblocks((size_t)m->ngroups)), dim3(kThreads), 0, s,
+                       m->imult, m->goff, m->gmem, m->ngroups);
+  GF_HIP(ctx, hipStreamSynchronize(s));
+  {
+    std::vector<double> im(n);
+    GF_HIP(ctx, hipMemcpy(im.data(), m->imult, n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<float> imf(n);
+    for (size_t q = 0; q < n; q++) imf[q] = (float)im[q];
+    GF_HIP(ctx, hipMemcpy(m->imult32, imf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+#undef UP
+  *out = m;
+  return GF_OK;
+}
+
+int gf_mesh_destroy(gf_mesh *m) {
+  if (!m) return GF_OK;
+  auto F = [](void *p) {
+    if (p) (void)hipFree(p);
+  };
+  for (int k = 0; k < 3; k++) {
+    F(m->D[k]);
+    F(m->W[k]);
+    F(m->G[k]);
+    F(m->xyz[k]);
+  }
+  for (int k = 0; k < 9; k++) F(m->ndx[k]);
+  F(m->D32); F(m->W32); F(m->edx); F(m->ejac); F(m->edx32); F(m->ejac32); F(m->njac);
+  F(m->mask); F(m->mass); F(m->imass); F(m->imult); F(m->mask32); F(m->imult32);
+  F(m->goff); F(m->gmem); F(m->red); F(m->scal);
+  if (m->hscal) (void)hipHostFree(m->hscal);
+  for (size_t k = 0; k < m->ws.size(); k++) F(m->ws[k]);
+  delete m;
+  return GF_OK;
+}
+
+size_t gf_mesh_ndof(const gf_mesh *m) { return m ? m->ndof : 0; }
+
+int gf_mesh_get(gf_mesh *m, const char *which, double *host_out) {
+  if (!m || !which || !host_out) return GF_ERR_ARG;
+  std::string w(which);
+  const double *src = nullptr;
+  if (w == "mass") src = m->mass;
+  else if (w == "imass") src = m->imass;
+  else if (w == "imult") src = m->imult;
+  else if (w == "mask") src = m->mask;
+  if (!src) return gf_set_err(m->ctx, GF_ERR_ARG, "unknown mesh array");
+  GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+  GF_HIP(m->ctx, hipMemcpy(host_out, src, m->ndof * sizeof(double), hipMemcpyDeviceToHost));
+  return GF_OK;
+}
+
+// ===========================================================================
+// Operators
+// ===========================================================================
+int gf_grefderiv(gf_mesh *m, int idir, int dotrans, const double *u, double *du) {
+  if (!m || !u || !du) return GF_ERR_ARG;
+  if (idir < 1 || idir > m->dim) {
+    if (m->dim == 2 && idir == 3) {  // grefderiv 2-D idir 3 is zero (ggrid.ipp:2133-2137)
+      GF_HIP(m->ctx, hipMemsetAsync(du, 0, m->ndof * sizeof(double), m->ctx->stream));
+      return GF_OK;
+    }
+    return gf_set_err(m->ctx, GF_ERR_ARG, "idir out of range");
+  }
+  return gf_grefderiv_t(m, idir, dotrans, u, du, nullptr);
+}
+
+int gf_grefderivs(gf_mesh *m, int dotrans, const double *u, double *const *du) {
+  if (!m || !u || !du) return GF_ERR_ARG;
+  if (!dotrans && m->Niso) {
+    for (int k = 0; k < m->dim; k++)
+      if (!du[k]) return GF_ERR_ARG;
+    return GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_grad, m, u, du[0], du[1],
+                         m->dim == 3 ? du[2] : nullptr, (const double *)nullptr);
+  }
+  for (int k = 0; k < m->dim; k++) {
+    int rc = gf_grefderiv(m, k + 1, dotrans, u, du[k]);
+    if (rc) return rc;
+  }
+  return GF_OK;
+}
+
+int gf_grefdiv(gf_mesh *m, int dotrans, const double *const *u, double *divu) {
+  if (!m || !u || !divu) return GF_ERR_ARG;
+  if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "grefdiv needs isotropic order");
+  return GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_div, m, dotrans, u[0], u[1],
+                       m->dim == 3 ? u[2] : nullptr, divu);
+}
+
+int gf_deriv(gf_mesh *m, int idir, const double *u, double *du) {
+  if (!m || !u || !du) return GF_ERR_ARG;
+  if (m->gtype == GF_GE_REGULAR) {
+    if (idir < 1 || idir > m->dim) return gf_set_err(m->ctx, GF_ERR_ARG, "idir out of range");
+    return gf_grefderiv_t(m, idir, 0, u, du, m->edx);
+  }
+  // embedded: du = (D1 u) dXidX(0,idir-1) + (D2 u) dXidX(1,idir-1)
+  if (idir < 1 || idir > 3) return gf_set_err(m->ctx, GF_ERR_ARG, "idir out of range");
+  double *t = (double *)gf_ws(m, WS_TMP0, m->ndof * sizeof(double));
+  double *t1 = (double *)gf_ws(m, WS_TMP1, m->ndof * sizeof(double));
+  if (!t || !t1) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  int rc = gf_grefderiv_t(m, 1, 0, u, du, nullptr);
+  if (rc) return rc;
+  rc = gf_mul_t(m, du, m->ndx[0 * 3 + idir - 1], du);
+  if (rc) return rc;
+  rc = gf_grefderiv_t(m, 2, 0, u, t, nullptr);
+  if (rc) return rc;
+  rc = gf_mul_t(m, t, m->ndx[1 * 3 + idir - 1], t);
+  if (rc) return rc;
+  return gf_axpy_t(m, 1.0, t, du);
+}
+
+int gf_grad(gf_mesh *m, const double *u, double *const *du) {
+  if (!m || !u || !du) return GF_ERR_ARG;
+  if (m->gtype == GF_GE_REGULAR && m->Niso) {
+    for (int k = 0; k < m->dim; k++)
+      if (!du[k]) return GF_ERR_ARG;
+    return GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_grad, m, u, du[0], du[1],
+                         m->dim == 3 ? du[2] : nullptr, (const double *)m->edx);
+  }
+  int nd = m->gtype == GF_GE_2DEMBEDDED ? 3 : m->dim;
+  for (int k = 0; k < nd; k++) {
+    int rc = gf_deriv(m, k + 1, u, du[k]);
+    if (rc) return rc;
+  }
+  return GF_OK;
+}
+
+int gf_mass_apply(gf_mesh *m, const double *in, double *out) {
+  if (!m || !in || !out) return GF_ERR_ARG;
+  return gf_mul_t(m, m->mass, in, out);
+}
+
+int gf_imass_apply(gf_mesh *m, const double *in, double *out) {
+  if (!m || !in || !out) return GF_ERR_ARG;
+  return gf_mul_t(m, m->imass, in, out);
+}
+
+int gf_integrate(gf_mesh *m, const double *u, double *result) {
+  if (!m || !u || !result) return GF_ERR_ARG;
+  return gf_dot_t(m, m->mass, u, m->ndof, result);
+}
+
+int gf_helmholtz_apply(gf_mesh *m, double p_scalar, const double *p_field, const double *q_field,
+                       const double *u, double *uo) {
+  if (!m || !u || !uo) return GF_ERR_ARG;
+  return gf_helm_t(m, p_scalar, p_field, q_field, u, uo);
+}
+
+int gf_dss(gf_mesh *m, int op, double *u) {
+  if (!m || !u) return GF_ERR_ARG;
+  if (op < GF_DSS_SUM || op > GF_DSS_MIN) return gf_set_err(m->ctx, GF_ERR_ARG, "bad DSS op");
+  return gf_dss_t(m, op, u);
+}
+
+int gf_imult(gf_mesh *m, double *imult) {
+  if (!m || !imult) return GF_ERR_ARG;
+  GF_HIP(m->ctx, hipMemcpyAsync(imult, m->imult, m->ndof * sizeof(double),
+                                hipMemcpyDeviceToDevice, m->ctx->stream));
+  return GF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1265 @@
+// gf_solve.hip -- advection, Burgers RHS and explicit RK step, and the CG
+// solver (FP64, and FP32 inner solves inside FP64 iterative refinement).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "gf_kernels.hpp"
+
+using gf::kThreads;
+using gf::Shape;
+
+
+namespace {
+
+inline int nblocks(size_t n, int per = kThreads) {
+  size_t b = (n + per - 1) / per;
+  if (b > 65536) b = 65536;
+  return (int)std::max<size_t>(b, 1);
+}
+
+#define GRID_LOOP(g, n)                                                     \
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < (n); \
+       g += (size_t)gridDim.x * blockDim.x)
+
+// z = a*x + b*y   (GMTK::saxpby (2), gmtk.ipp:1187-1203); y may alias z.
+__global__ void k_axpby(double *__restrict__ z, double a, const double *__restrict__ x, double b,
+                        const double *y, size_t n) {
+  GRID_LOOP(g, n) z[g] = a * x[g] + b * y[g];
+}
+// y += a*x
+__global__ void k_axpy(double *y, double a, const double *__restrict__ x, size_t n) {
+  GRID_LOOP(g, n) y[g] = y[g] + a * x[g];
+}
+__global__ void k_copy(double *__restrict__ y, const double *__restrict__ x, size_t n) {
+  GRID_LOOP(g, n) y[g] = x[g];
+}
+__global__ void k_nonfinite(const double *__restrict__ x, size_t n, double *flag) {
+  bool bad = false;
+  GRID_LOOP(g, n) bad |= !isfinite(x[g]);
+  if (bad) *flag = 1.0;
+}
+
+// GMTK::constrain2sphere (2), in place with the reference's aliasing:
+// the updated v0 feeds v1, and both feed v2 (gmtk.ipp:2956-2980).
+__global__ void k_constrain2sphere(double *v0, double *v1, double *v2,
+                                   const double *__restrict__ X, const double *__restrict__ Y,
+                                   const double *__restrict__ Z, size_t n) {
+  double x = X[0], y = Y[0], z = Z[0];
+  double r2 = x * x + y * y + z * z;
+  double ri = 1.0 / r2;
+  GRID_LOOP(g, n) {
+    x = X[g];
+    y = Y[g];
+    z = Z[g];
+    v0[g] = (v0[g] * (r2 - x * x) - v1[g] * x * y - v2[g] * x * z) * ri;
+    v1[g] = (-v0[g] * y * x + v1[g] * (r2 - y * y) - v2[g] * y * z) * ri;
+    v2[g] = (-v0[g] * z * x - v1[g] * z * y + v2[g] * (r2 - z * z)) * ri;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused Burgers stage for GE_REGULAR meshes (nonlinear mode, DIM components):
+//   u_c   = uin_c + bh * kprev_c                 (stage input, step_b :291-295)
+//   acc_c = accsrc_c + ch * kprev_c              (uout accumulation, :306-307)
+//   K_c   = Minv * ( -nu L u_c - M sum_j u_j (dXi_j/dX_j) D_j u_c )
+// i.e. GBurgers::dudt_impl (gburgers.ipp:247-255) with GAdvect::apply
+// (gadvect.ipp:57-99) and GHelmholtz::reg_prod (ghelmholtz.ipp:308-378).
+// K is written unassembled; the caller applies the GGFX Smooth.
+// ---------------------------------------------------------------------------
+struct StageArgs {
+  const double *uin[3];
+  const double *kprev[3];
+  double bh;
+  double *acc[3];
+  const double *accsrc[3];
+  double ch;
+  double *kout[3];
+  const double *D, *W, *edx, *ejac, *imass;
+  double nu;
+  size_t nelems;
+};
+
+template <int DIM, int N>
+__global__ __launch_bounds__(kThreads) void k_burgers_stage(StageArgs a) {
+  using S = Shape<DIM, N>;
+  constexpr int PT = (S::NODES + kThreads - 1) / kThreads;
+  __shared__ double su[DIM][S::NODES];
+  __shared__ double sg[DIM][S::NODES];
+  __shared__ double sD[N * N];
+  __shared__ double sW[N];
+  size_t e0 = (size_t)blockIdx.x * S::EPB;
+  int nel = (int)min((size_t)S::EPB, a.nelems - e0);
+  size_t base = e0 * S::NN;
+  int cnt = nel * S::NN;
+  gf::stage_D(sD, a.D, N);
+  if (threadIdx.x < N) sW[threadIdx.x] = a.W[threadIdx.x];
+#pragma unroll
+  for (int c = 0; c < DIM; c++) {
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+      size_t g = base + q;
+      double ui = a.uin[c][g];
+      double v = ui;
+      if (a.kprev[c]) {
+        double kp = a.kprev[c][g];
+        v = ui + a.bh * kp;
+        if (a.acc[c]) a.acc[c][g] = a.accsrc[c][g] + kp * a.ch;
+      }
+      su[c][q] = v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < DIM; c++) {
+    double adv[PT];
+    int t = 0;
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x, t++) {
+      int el = q / S::NN, r = q - el * S::NN, i, j, k;
+      gf::node_ijk<DIM, N>(r, i, j, k);
+      size_t e = e0 + el;
+      double M = gf::node_mass<DIM>(sW, i, j, k, a.ejac[e]);
+      double ad = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        double gd = gf::contract<DIM, N, false>(&su[c][el * S::NN], sD, d, i, j, k);
+        double f = a.edx[e * DIM + d];
+        double phys = gd * f;
+        if (d == 0) ad = phys * su[0][q];
+        else ad += phys * su[d][q];
+        sg[d][q] = M * (gd * (f * f));
+      }
+      adv[t] = ad * M;
+    }
+    __syncthreads();
+    t = 0;
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x, t++) {
+      int el = q / S::NN, r = q - el * S::NN, i, j, k;
+      gf::node_ijk<DIM, N>(r, i, j, k);
+      double L = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) L += gf::contract<DIM, N, true>(&sg[d][el * S::NN], sD, d, i, j, k);
+      if (a.nu != 1.0) L *= a.nu;
+      double urhs = -1.0 * L + -1.0 * adv[t];
+      a.kout[c][base + q] = a.imass[base + q] * urhs;
+    }
+    __syncthreads();
+  }
+}
+
+// u3 = uin + bh*k ; acc = accsrc + ch*k   (final-stage input of step_b, :315-320)
+__global__ void k_final_input(double *__restrict__ u3, const double *__restrict__ uin,
+                              const double *__restrict__ k, double bh, double *acc,
+                              const double *accsrc, double ch, size_t n) {
+  GRID_LOOP(g, n) {
+    double kv = k[g];
+    u3[g] = uin[g] + bh * kv;
+    if (acc) acc[g] = accsrc[g] + kv * ch;
+  }
+}
+
+template <int DIM, int N>
+int launch_stage(gf_mesh *m, const StageArgs &a) {
+  using S = Shape<DIM, N>;
+  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  hipLaunchKernelGGL((k_burgers_stage<DIM, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CG kernels (T = double or float).  Scalars live on the device in
+// scal[]: 0 rho, 1 alpha, 2 beta, 3 rnorm, 4 rtol, 5 wq, 6 iter, 7 maxit,
+// 8 done, 9 first, 10 volume, 11 n.
+// ---------------------------------------------------------------------------
+enum { S_RHO = 0, S_ALPHA, S_BETA, S_RNORM, S_RTOL, S_WQ, S_ITER, S_MAXIT, S_DONE, S_FIRST,
+       S_VOL, S_N };
+
+struct CGApplyArgs {
+  const void *r;      // T
+  const void *dinv;   // T or NULL
+  void *w;            // T
+  void *q;            // T
+  const void *D, *W, *edx, *ejac, *qf;  // T
+  double p;
+  double *scal;
+  double *part;
+  size_t nelems;
+};
+
+// w = beta*w + z (z = Dinv r, or w = z on the first iteration), then
+// q = A w on the element (unassembled) and the partial sum of w*q.  Because
+// w is continuous and zero where masked, sum_j w_j (DSS q)_j imult_j equals
+// sum_j w_j q_j over local nodes, so the dot product of gcg.ipp:307-308 is
+// formed here before the DSS.
+template <int DIM, int N, typename T>
+__global__ __launch_bounds__(kThreads) void k_cg_apply(CGApplyArgs a) {
+  using S = Shape<DIM, N>;
+  __shared__ T su[S::NODES];
+  __shared__ T sg[DIM][S::NODES];
+  __shared__ T sD[N * N];
+  __shared__ T sW[N];
+  __shared__ double sred[kThreads / 64];
+  const double *scal = a.scal;
+  if (scal[S_DONE] != 0.0) return;
+  const T *r = (const T *)a.r, *dinv = (const T *)a.dinv;
+  T *w = (T *)a.w, *qv = (T *)a.q;
+  const T *D = (const T *)a.D, *W = (const T *)a.W, *edx = (const T *)a.edx,
+          *ejac = (const T *)a.ejac, *qf = (const T *)a.qf;
+  bool first = scal[S_FIRST] != 0.0;
+  T beta = (T)scal[S_BETA];
+  size_t e0 = (size_t)blockIdx.x * S::EPB;
+  int nel = (int)min((size_t)S::EPB, a.nelems - e0);
+  size_t base = e0 * S::NN;
+  int cnt = nel * S::NN;
+  gf::stage_D(sD, D, N);
+  if (threadIdx.x < N) sW[threadIdx.x] = W[threadIdx.x];
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    size_t g = base + q;
+    T z = dinv ? dinv[g] * r[g] : r[g];
+    T wn = first ? z : beta * w[g] + (T)1 * z;
+    w[g] = wn;
+    su[q] = wn;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
+    gf::node_ijk<DIM, N>(rr, i, j, k);
+    size_t e = e0 + el;
+    T M = gf::node_mass<DIM>(sW, i, j, k, ejac[e]);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      T gd = gf::contract<DIM, N, false>(su + el * S::NN, sD, d, i, j, k);
+      T f = edx[e * DIM + d];
+      sg[d][q] = M * (gd * (f * f));
+    }
+  }
+  __syncthreads();
+  double dot = 0.0;
+  T p = (T)a.p;
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
+    gf::node_ijk<DIM, N>(rr, i, j, k);
+    T acc = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) acc += gf::contract<DIM, N, true>(&sg[d][el * S::NN], sD, d, i, j, k);
+    if (p != (T)1) acc *= p;
+    if (qf) {
+      T M = gf::node_mass<DIM>(sW, i, j, k, ejac[e0 + el]);
+      T t = M * su[q];
+      t *= qf[base + q];
+      acc += t;
+    }
+    qv[base + q] = acc;
+    dot += (double)su[q] * (double)acc;
+  }
+  double bsum = gf::block_reduce<false>(dot, sred);
+  if (threadIdx.x == 0) a.part[blockIdx.x] = bsum;
+}
+
+template <typename T>
+__global__ void k_dss_cg(T *__restrict__ u, const int *__restrict__ goff,
+                         const int *__restrict__ gmem, int ngroups, const double *scal) {
+  if (scal[S_DONE] != 0.0) return;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += gridDim.x * blockDim.x) {
+    int b = goff[g], e = goff[g + 1];
+    T s = T(0);
+    for (int k = b; k < e; k++) s = s + u[gmem[k]];
+    for (int k = b; k < e; k++) u[gmem[k]] = s;
+  }
+}
+
+// alpha = rho / (w.q)
+__global__ void k_cg_alpha(double *scal, const double *__restrict__ part, int np) {
+  __shared__ double sred[kThreads / 64];
+  if (scal[S_DONE] != 0.0) return;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+  double wq = gf::block_reduce<false>(v, sred);
+  if (threadIdx.x == 0) {
+    scal[S_WQ] = wq;
+    scal[S_ALPHA] = scal[S_RHO] / wq;
+  }
+}
+
+// x += alpha w ; r -= alpha mask q ; partials of r.z.imult and of the norm.
+template <typename T, int NORM>
+__global__ void k_cg_update(T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r,
+                            const T *__restrict__ qv, const T *__restrict__ mask,
+                            const T *__restrict__ dinv, const T *__restrict__ imult,
+                            const double *__restrict__ mass, double *scal,
+                            double *__restrict__ part, size_t n) {
+  __shared__ double sred[kThreads / 64];
+  if (scal[S_DONE] != 0.0) return;
+  T alpha = (T)scal[S_ALPHA];
+  double rho = 0.0, nrm = 0.0;
+  GRID_LOOP(g, n) {
+    T xv = (T)1 * x[g] + alpha * w[g];
+    x[g] = xv;
+    T qm = mask ? qv[g] * mask[g] : qv[g];
+    T rv = (T)1 * r[g] + (-alpha) * qm;
+    r[g] = rv;
+    T z = dinv ? dinv[g] * rv : rv;
+    rho += (double)rv * (double)z * (double)imult[g];
+    double rd = (double)rv;
+    if (NORM == GF_NORM_INF) nrm = fmax(nrm, fabs(rd));
+    else if (NORM == GF_NORM_EUC) nrm += rd * rd;
+    else if (NORM == GF_NORM_L2) nrm += mass[g] * rd * rd;
+    else if (NORM == GF_NORM_L1) nrm += mass[g] * fabs(rd);
+    else nrm += rd * rd * (double)imult[g];
+  }
+  double a = gf::block_reduce<false>(rho, sred);
+  double b = NORM == GF_NORM_INF ? gf::block_reduce<true>(nrm, sred)
+                                 : gf::block_reduce<false>(nrm, sred);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+__device__ double finish_norm(int normtype, double v, const double *scal) {
+  switch (normtype) {
+    case GF_NORM_INF: return v;
+    case GF_NORM_EUC: return sqrt(v / scal[S_N]);
+    case GF_NORM_L2: return sqrt(v / scal[S_VOL]);
+    case GF_NORM_L1: return v / scal[S_VOL];
+    default: return sqrt(v);
+  }
+}
+
+// rho_new, beta, rnorm, iteration count and the stopping rule of
+// gcg.ipp:297-335: continue while iter < maxit and rnorm > rtol.
+__global__ void k_cg_beta(double *scal, const double *__restrict__ part, int np, int normtype,
+                          double *resid) {
+  __shared__ double sred[kThreads / 64];
+  if (scal[S_DONE] != 0.0) return;
+  double v = 0.0, w = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    v += part[2 * i];
+    w = normtype == GF_NORM_INF ? fmax(w, part[2 * i + 1]) : w + part[2 * i + 1];
+  }
+  double rho = gf::block_reduce<false>(v, sred);
+  double nr = normtype == GF_NORM_INF ? gf::block_reduce<true>(w, sred)
+                                      : gf::block_reduce<false>(w, sred);
+  if (threadIdx.x == 0) {
+    double rhom = scal[S_RHO];
+    scal[S_RHO] = rho;
+    scal[S_BETA] = rho / rhom;
+    double rn = finish_norm(normtype, nr, scal);
+    scal[S_RNORM] = rn;
+    int it = (int)scal[S_ITER];
+    if (resid) resid[it] = rn;
+    it++;
+    scal[S_ITER] = it;
+    scal[S_FIRST] = 0.0;
+    if (!(it < (int)scal[S_MAXIT] && rn > scal[S_RTOL])) scal[S_DONE] = 1.0;
+  }
+}
+
+// Generic reductions used at solver set-up: partial (sum_i a*b*c) and norm.
+template <typename T>
+__global__ void k_setup_red(const T *__restrict__ r, const T *__restrict__ z,
+                            const T *__restrict__ imult, const double *__restrict__ mass,
+                            int normtype, double *__restrict__ part, size_t n) {
+  __shared__ double sred[kThreads / 64];
+  double rho = 0.0, nrm = 0.0;
+  GRID_LOOP(g, n) {
+    double rv = (double)r[g];
+    if (z) rho += rv * (double)z[g] * (double)imult[g];
+    if (normtype == GF_NORM_INF) nrm = fmax(nrm, fabs(rv));
+    else if (normtype == GF_NORM_EUC) nrm += rv * rv;
+    else if (normtype == GF_NORM_L2) nrm += mass[g] * rv * rv;
+    else if (normtype == GF_NORM_L1) nrm += mass[g] * fabs(rv);
+    else nrm += rv * rv * (double)imult[g];
+  }
+  double a = gf::block_reduce<false>(rho, sred);
+  double b = normtype == GF_NORM_INF ? gf::block_reduce<true>(nrm, sred)
+                                     : gf::block_reduce<false>(nrm, sred);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+__global__ void k_setup_fin(const double *__restrict__ part, int np, int normtype, double *out) {
+  __shared__ double sred[kThreads / 64];
+  double v = 0.0, w = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    v += part[2 * i];
+    w = normtype == GF_NORM_INF ? fmax(w, part[2 * i + 1]) : w + part[2 * i + 1];
+  }
+  double a = gf::block_reduce<false>(v, sred);
+  double b = normtype == GF_NORM_INF ? gf::block_reduce<true>(w, sred)
+                                     : gf::block_reduce<false>(w, sred);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+template <typename T>
+__global__ void k_mul_t(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ out,
+                        size_t n) {
+  GRID_LOOP(g, n) out[g] = a[g] * b[g];
+}
+__global__ void k_to_f32(float *__restrict__ o, const double *__restrict__ i, double s, size_t n) {
+  GRID_LOOP(g, n) o[g] = (float)(i[g] * s);
+}
+__global__ void k_add_f32(double *__restrict__ x, const float *__restrict__ d, double s, size_t n) {
+  GRID_LOOP(g, n) x[g] = x[g] + s * (double)d[g];
+}
+__global__ void k_residual(double *__restrict__ r, const double *__restrict__ b,
+                           const double *__restrict__ ax, size_t n) {
+  GRID_LOOP(g, n) r[g] = b[g] - ax[g];
+}
+
+// Element diagonal of (p L + q M) for regular elements, then assembled by the
+// caller with a DSS Sum:
+//   p * sum_d G_dd Jac prod_{o!=d} W(i_o) sum_m D(m,i_d)^2 W(m)  +  q M
+template <int DIM, int N>
+__global__ void k_helm_diag(double *__restrict__ d, const double *__restrict__ D,
+                            const double *__restrict__ W, const double *__restrict__ edx,
+                            const double *__restrict__ ejac, double p,
+                            const double *__restrict__ qf, size_t ndof) {
+  constexpr int NN = Shape<DIM, N>::NN;
+  GRID_LOOP(g, ndof) {
+    size_t e = g / NN;
+    int r = (int)(g - e * NN), idx[3];
+    gf::node_ijk<DIM, N>(r, idx[0], idx[1], idx[2]);
+    double jac = ejac[e];
+    double lap = 0.0;
+    for (int dd = 0; dd < DIM; dd++) {
+      double s = 0.0;
+      for (int m = 0; m < N; m++) {
+        double Dm = D[m + idx[dd] * N];
+        s += Dm * Dm * W[m];
+      }
+      double other = 1.0;
+      for (int o = 0; o < DIM; o++)
+        if (o != dd) other *= W[idx[o]];
+      double f = edx[e * DIM + dd];
+      double G = f * f;
+      lap += G * s * other * jac;
+    }
+    double val = p * lap;
+    if (qf) val += qf[g] * gf::node_mass<DIM>(W, idx[0], idx[1], idx[2], jac);
+    d[g] = val;
+  }
+}
+__global__ void k_recip2(double *__restrict__ a, float *__restrict__ af, size_t n) {
+  GRID_LOOP(g, n) {
+    a[g] = 1.0 / a[g];
+    if (af) af[g] = (float)a[g];
+  }
+}
+
+template <int DIM, int N>
+int launch_diag(gf_mesh *m, double p, const double *qf, double *d) {
+  hipLaunchKernelGGL((k_helm_diag<DIM, N>), dim3(nblocks(m->ndof)), dim3(kThreads), 0,
+                     m->ctx->stream, d, m->D[0], m->W[0], m->edx, m->ejac, p, qf, m->ndof);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int DIM, int N, typename T>
+int launch_cg_apply(gf_mesh *m, const CGApplyArgs &a) {
+  using S = Shape<DIM, N>;
+  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  hipLaunchKernelGGL((k_cg_apply<DIM, N, T>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+template <int DIM, int N>
+int launch_cg_apply_d(gf_mesh *m, const CGApplyArgs &a) { return launch_cg_apply<DIM, N, double>(m, a); }
+template <int DIM, int N>
+int launch_cg_apply_f(gf_mesh *m, const CGApplyArgs &a) { return launch_cg_apply<DIM, N, float>(m, a); }
+
+// FP32 element Helmholtz (used for the FP32 operator entry point).
+template <int DIM, int N>
+__global__ __launch_bounds__(kThreads) void k_helm_f32(const float *__restrict__ u,
+                                                       float *__restrict__ uo,
+                                                       const float *__restrict__ D,
+                                                       const float *__restrict__ W,
+                                                       const float *__restrict__ edx,
+                                                       const float *__restrict__ ejac, float p,
+                                                       const float *__restrict__ qf,
+                                                       size_t nelems) {
+  using S = Shape<DIM, N>;
+  __shared__ float su[S::NODES];
+  __shared__ float sg[DIM][S::NODES];
+  __shared__ float sD[N * N];
+  __shared__ float sW[N];
+  size_t e0 = (size_t)blockIdx.x * S::EPB;
+  int nel = (int)min((size_t)S::EPB, nelems - e0);
+  size_t base = e0 * S::NN;
+  int cnt = nel * S::NN;
+  gf::stage_D(sD, D, N);
+  if (threadIdx.x < N) sW[threadIdx.x] = W[threadIdx.x];
+  gf::stage(su, u, base, cnt);
+  __syncthreads();
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
+    gf::node_ijk<DIM, N>(rr, i, j, k);
+    size_t e = e0 + el;
+    float M = gf::node_mass<DIM>(sW, i, j, k, ejac[e]);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      float gd = gf::contract<DIM, N, false>(su + el * S::NN, sD, d, i, j, k);
+      float f = edx[e * DIM + d];
+      sg[d][q] = M * (gd * (f * f));
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
+    gf::node_ijk<DIM, N>(rr, i, j, k);
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) acc += gf::contract<DIM, N, true>(&sg[d][el * S::NN], sD, d, i, j, k);
+    if (p != 1.f) acc *= p;
+    if (qf) {
+      float M = gf::node_mass<DIM>(sW, i, j, k, ejac[e0 + el]);
+      float t = M * su[q];
+      t *= qf[base + q];
+      acc += t;
+    }
+    uo[base + q] = acc;
+  }
+}
+template <int DIM, int N>
+int launch_helm_f32(gf_mesh *m, float p, const float *qf, const float *u, float *uo) {
+  using S = Shape<DIM, N>;
+  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  hipLaunchKernelGGL((k_helm_f32<DIM, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, uo,
+                     m->D32, m->W32, m->edx32, m->ejac32, p, qf, m->nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+}  // namespace
+
+int gf_axpy_t(gf_mesh *m, double a, const double *x, double *y) {
+  hipLaunchKernelGGL(k_axpy, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, y, a, x,
+                     m->ndof);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+static int axpby(gf_mesh *m, double *z, double a, const double *x, double b, const double *y) {
+  hipLaunchKernelGGL(k_axpby, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, z, a, x,
+                     b, y, m->ndof);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+static int copy(gf_mesh *m, double *y, const double *x) {
+  GF_HIP(m->ctx, hipMemcpyAsync(y, x, m->ndof * sizeof(double), hipMemcpyDeviceToDevice,
+                                m->ctx->stream));
+  return GF_OK;
+}
+
+#define RC(x)            \
+  do {                   \
+    int _rc = (x);       \
+    if (_rc) return _rc; \
+  } while (0)
+
+extern "C" {
+
+int gf_helmholtz_apply_f32(gf_mesh *m, float p_scalar, const float *q_field, const float *u,
+                           float *uo) {
+  if (!m || !u || !uo) return GF_ERR_ARG;
+  if (m->gtype != GF_GE_REGULAR || !m->Niso)
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "FP32 Helmholtz: regular isotropic meshes");
+  return GF_DISPATCH_N(m->dim, m->Niso, launch_helm_f32, m, p_scalar, q_field, u, uo);
+}
+
+// GAdvect::apply (gadvect.ipp:57-99): po = deriv(p,1)*u0 ; po += deriv(p,j+1)*uj ;
+// po *= mass.
+int gf_advect(gf_mesh *m, const double *p, int nvel, const double *const *u, double *po) {
+  if (!m || !p || !u || !po || nvel < 1) return GF_ERR_ARG;
+  double *t = (double *)gf_ws(m, WS_TMP2, m->ndof * sizeof(double));
+  if (!t) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  if (u[0]) {
+    RC(gf_deriv(m, 1, p, po));
+    RC(gf_mul_t(m, po, u[0], po));
+  } else {
+    GF_HIP(m->ctx, hipMemsetAsync(po, 0, m->ndof * sizeof(double), m->ctx->stream));
+  }
+  for (int j = 1; j < nvel; j++) {
+    if (!u[j]) continue;
+    RC(gf_deriv(m, j + 1, p, t));
+    RC(gf_mul_t(m, t, u[j], t));
+    RC(gf_axpy_t(m, 1.0, t, po));
+  }
+  return gf_mul_t(m, po, m->mass, po);
+}
+
+// GBurgers::dudt_impl (gburgers.ipp:202-257), composed from the operators.
+int gf_burgers_rhs(gf_mesh *m, int mode, double nu, int nstate, const double *const *u,
+                   double *const *dudt) {
+  if (!m || !u || !dudt || nstate < 1 || mode < 0 || mode > 2) return GF_ERR_ARG;
+  for (int k = 0; k < nstate; k++)
+    if (!u[k]) return gf_set_err(m->ctx, GF_ERR_ARG, "NULL state component");
+  for (int k = 0; k < (mode == 1 ? 1 : nstate); k++)
+    if (!dudt[k]) return gf_set_err(m->ctx, GF_ERR_ARG, "NULL output component");
+  double *urhs = (double *)gf_ws(m, WS_TMP3, m->ndof * sizeof(double));
+  if (!urhs) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  if (mode == 2) {
+    for (int k = 0; k < nstate; k++) {
+      RC(gf_helm_t(m, nu, nullptr, nullptr, u[k], urhs));
+      RC(axpby(m, urhs, -1.0, urhs, 0.0, urhs));
+      RC(gf_mul_t(m, m->imass, urhs, dudt[k]));
+    }
+    return GF_OK;
+  }
+  if (mode == 1) {
+    RC(gf_advect(m, u[0], nstate - 1, u + 1, dudt[0]));
+    RC(gf_helm_t(m, nu, nullptr, nullptr, u[0], urhs));
+    RC(axpby(m, urhs, -1.0, urhs, -1.0, dudt[0]));
+    return gf_mul_t(m, m->imass, urhs, dudt[0]);
+  }
+  if (nstate < m->dim) return gf_set_err(m->ctx, GF_ERR_ARG, "nonlinear Burgers needs dim components");
+  for (int k = 0; k < m->dim; k++) {
+    RC(gf_advect(m, u[k], nstate, u, dudt[k]));
+    RC(gf_helm_t(m, nu, nullptr, nullptr, u[k], urhs));
+    RC(axpby(m, urhs, -1.0, urhs, -1.0, dudt[k]));
+    RC(gf_mul_t(m, m->imass, urhs, dudt[k]));
+  }
+  return GF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+void butcher(int order, double *alpha, double *beta /*4x4*/, double *c) {
+  for (int i = 0; i < 4; i++) {
+    alpha[i] = 0;
+    c[i] = 0;
+    for (int j = 0; j < 4; j++) beta[i * 4 + j] = 0;
+  }
+  // GButcherRK::computeCoeffs (gbutcherrk.ipp:80-127)
+  if (order == 1) {
+    c[0] = 1.0;
+  } else if (order == 2) {
+    c[0] = 0.5; alpha[1] = 1.0; c[1] = 0.5; beta[1 * 4 + 0] = 1.0;
+  } else if (order == 3) {
+    c[0] = 0.25; alpha[1] = 1.0 / 3.0; alpha[2] = 2.0 / 3.0; c[2] = 0.75;
+    beta[1 * 4 + 0] = 1.0 / 3.0; beta[2 * 4 + 1] = 2.0 / 3.0;
+  } else {
+    c[0] = 1.0 / 6.0; alpha[1] = 0.5; c[1] = 1.0 / 3.0; alpha[2] = 0.5; c[2] = 1.0 / 3.0;
+    alpha[3] = 1.0; c[3] = 1.0 / 6.0;
+    beta[1 * 4 + 0] = 0.5; beta[2 * 4 + 1] = 0.5; beta[3 * 4 + 2] = 1.0;
+  }
+}
+
+int constrain(gf_mesh *m, double *const *v, int nv) {
+  if (m->gtype != GF_GE_2DEMBEDDED || nv < 3) return GF_OK;
+  if (!m->xyz[0] || !m->xyz[1] || !m->xyz[2])
+    return gf_set_err(m->ctx, GF_ERR_ARG, "constrain2sphere needs node coordinates");
+  hipLaunchKernelGGL(k_constrain2sphere, dim3(nblocks(m->ndof)), dim3(kThreads), 0,
+                     m->ctx->stream, v[0], v[1], v[2], m->xyz[0], m->xyz[1], m->xyz[2], m->ndof);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+// Fused RK step for regular isotropic meshes, nonlinear mode.
+int rk_step_fused(gf_mesh *m, double nu, int order, double dt, double *const *u) {
+  int nc = m->dim;
+  size_t n = m->ndof, bytes = n * sizeof(double);
+  double alpha[4], beta[16], c[4];
+  butcher(order, alpha, beta, c);
+  double *acc[3], *K[4][3], *u3[3];
+  for (int k = 0; k < nc; k++) {
+    acc[k] = (double *)gf_ws(m, WS_RK_BEGIN + k, bytes);
+    u3[k] = (double *)gf_ws(m, WS_RK_BEGIN + 3 + k, bytes);
+    for (int s = 0; s < 2; s++) K[s][k] = (double *)gf_ws(m, WS_RK_BEGIN + 6 + 3 * s + k, bytes);
+    if (!acc[k] || !u3[k] || !K[0][k] || !K[1][k]) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  }
+  double h = dt;
+  int ns = order;
+  StageArgs a;
+  memset(&a, 0, sizeof(a));
+  a.D = m->D[0];
+  a.W = m->W[0];
+  a.edx = m->edx;
+  a.ejac = m->ejac;
+  a.imass = m->imass;
+  a.nu = nu;
+  a.nelems = m->nelems;
+  // K_m lives in ping-pong buffers: K[m%2]
+  for (int s = 0; s < ns - 1; s++) {
+    for (int k = 0; k < nc; k++) {
+      a.uin[k] = u[k];
+      a.kprev[k] = s > 0 ? K[(s - 1) % 2][k] : nullptr;
+      a.acc[k] = s > 0 ? acc[k] : nullptr;
+      a.accsrc[k] = s == 1 ? u[k] : acc[k];
+      a.kout[k] = K[s % 2][k];
+    }
+    a.bh = s > 0 ? beta[s * 4 + (s - 1)] * h : 0.0;
+    a.ch = s > 0 ? c[s - 1] * h : 0.0;
+    RC(GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, a));
+    for (int k = 0; k < nc; k++) RC(gf_dss_t(m, GF_DSS_SMOOTH, K[s % 2][k]));
+  }
+  // final stage input
+  for (int k = 0; k < nc; k++) {
+    if (ns >= 2) {
+      const double *kp = K[(ns - 2) % 2][k];
+      double *accd = acc[k];
+      const double *accs = ns == 2 ? u[k] : acc[k];
+      hipLaunchKernelGGL(k_final_input, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, u3[k],
+                         u[k], kp, beta[(ns - 1) * 4 + (ns - 2)] * h, accd, accs, c[ns - 2] * h, n);
+      GF_LAUNCH_CHECK(m->ctx);
+    } else {
+      RC(copy(m, u3[k], u[k]));
+      RC(copy(m, acc[k], u[k]));
+    }
+    RC(gf_dss_t(m, GF_DSS_SMOOTH, u3[k]));
+  }
+  for (int k = 0; k < nc; k++) {
+    a.uin[k] = u3[k];
+    a.kprev[k] = nullptr;
+    a.acc[k] = nullptr;
+    a.accsrc[k] = nullptr;
+    a.kout[k] = K[(ns - 1) % 2][k];
+  }
+  a.bh = 0.0;
+  a.ch = 0.0;
+  RC(GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, a));
+  double cl = c[ns - 1] * h;
+  for (int k = 0; k < nc; k++) {
+    double *kf = K[(ns - 1) % 2][k];
+    RC(gf_dss_t(m, GF_DSS_SMOOTH, kf));
+    // uout = acc + kf*(c h)
+    hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, u[k], 1.0,
+                       acc[k], cl, kf, n);
+    GF_LAUNCH_CHECK(m->ctx);
+    RC(gf_dss_t(m, GF_DSS_SMOOTH, u[k]));
+  }
+  return GF_OK;
+}
+
+// General RK step composed from the operators, following step_b literally
+// (gexrk_stepper.ipp:251-341).  Used for embedded meshes, pure advection and
+// heat modes, and orders without a fused path.
+int rk_step_general(gf_mesh *m, int mode, double nu, int order, int nstate, double dt,
+                    double *const *ust) {
+  size_t n = m->ndof, bytes = n * sizeof(double);
+  int nev = mode == 0 ? nstate : 1;
+  if (nev > 3) return gf_set_err(m->ctx, GF_ERR_ARG, "at most 3 evolved components");
+  double alpha[4], beta[16], c[4];
+  butcher(order, alpha, beta, c);
+  int ns = order;
+  double h = dt;
+  double *uin[3], *us[3], *K[3][3], *isum;
+  for (int k = 0; k < nev; k++) {
+    uin[k] = (double *)gf_ws(m, WS_RK_BEGIN + k, bytes);
+    us[k] = (double *)gf_ws(m, WS_RK_BEGIN + 3 + k, bytes);
+    for (int s = 0; s < std::max(ns - 1, 1); s++) {
+      K[s][k] = (double *)gf_ws(m, WS_RK_BEGIN + 6 + 3 * s + k, bytes);
+      if (!K[s][k]) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+    }
+    if (!uin[k] || !us[k]) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+    RC(copy(m, uin[k], ust[k]));
+  }
+  isum = (double *)gf_ws(m, WS_RK_BEGIN + 15, bytes);
+  if (!isum) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  std::vector<const double *> rin(nstate);
+  for (int k = nev; k < nstate; k++) rin[k] = ust[k];
+  // heat mode evolves only u[0] (gburgers.ipp:295-300); its RHS sees that
+  // component alone, so K never needs more than nev outputs.
+  const int nrhs = mode == 2 ? nev : nstate;
+  for (int s = 0; s < ns - 1; s++) {
+    for (int k = 0; k < nev; k++) {
+      GF_HIP(m->ctx, hipMemsetAsync(isum, 0, bytes, m->ctx->stream));
+      for (int j = 0; j < s; j++) RC(axpby(m, isum, 1.0, isum, beta[s * 4 + j] * h, K[j][k]));
+      RC(axpby(m, us[k], 1.0, uin[k], 1.0, isum));
+    }
+    RC(constrain(m, us, nev));
+    for (int k = 0; k < nev; k++) rin[k] = us[k];
+    RC(gf_burgers_rhs(m, mode, nu, nrhs, rin.data(), K[s]));
+    for (int k = 0; k < nev; k++) {
+      RC(gf_dss_t(m, GF_DSS_SMOOTH, K[s][k]));
+      RC(axpby(m, ust[k], 1.0, ust[k], c[s] * h, K[s][k]));
+    }
+    RC(constrain(m, ust, nev));
+  }
+  for (int k = 0; k < nev; k++) {
+    GF_HIP(m->ctx, hipMemsetAsync(isum, 0, bytes, m->ctx->stream));
+    for (int j = 0; j < ns - 1; j++) RC(axpby(m, isum, 1.0, isum, beta[(ns - 1) * 4 + j] * h, K[j][k]));
+    RC(axpby(m, us[k], 1.0, uin[k], 1.0, isum));
+    RC(gf_dss_t(m, GF_DSS_SMOOTH, us[k]));
+  }
+  RC(constrain(m, us, nev));
+  for (int k = 0; k < nev; k++) rin[k] = us[k];
+  RC(gf_burgers_rhs(m, mode, nu, nrhs, rin.data(), K[0]));
+  for (int k = 0; k < nev; k++) RC(gf_dss_t(m, GF_DSS_SMOOTH, K[0][k]));
+  for (int k = 0; k < nev; k++) {
+    RC(axpby(m, ust[k], 1.0, ust[k], c[ns - 1] * h, K[0][k]));
+    RC(gf_dss_t(m, GF_DSS_SMOOTH, ust[k]));
+  }
+  RC(constrain(m, ust, nev));
+  RC(constrain(m, ust, nev));  // GBurgers::step_exrk (gburgers.ipp:405)
+  return GF_OK;
+}
+
+int check_finite(gf_mesh *m, double *const *u, int nev) {
+  GF_HIP(m->ctx, hipMemsetAsync(m->scal + 62, 0, sizeof(double), m->ctx->stream));
+  for (int k = 0; k < nev; k++)
+    hipLaunchKernelGGL(k_nonfinite, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream,
+                       u[k], m->ndof, m->scal + 62);
+  GF_LAUNCH_CHECK(m->ctx);
+  double f = 0;
+  GF_HIP(m->ctx, hipMemcpyAsync(&f, m->scal + 62, sizeof(double), hipMemcpyDeviceToHost,
+                                m->ctx->stream));
+  GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+  return f != 0.0 ? gf_set_err(m->ctx, GF_ERR_NONFINITE, "Solution not finite") : GF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gf_burgers_step(gf_mesh *m, int mode, double nu, int order, int nstate, double t, double dt,
+                    double *const *u, int check) {
+  (void)t;
+  if (!m || !u || order < 1 || order > 4 || nstate < 1 || nstate > 4) return GF_ERR_ARG;
+  for (int k = 0; k < nstate; k++)
+    if (!u[k]) return GF_ERR_ARG;
+  bool fused = mode == 0 && m->gtype == GF_GE_REGULAR && m->Niso && nstate == m->dim;
+  int rc = fused ? rk_step_fused(m, nu, order, dt, u) : rk_step_general(m, mode, nu, order, nstate, dt, u);
+  if (rc) return rc;
+  if (check) return check_finite(m, u, mode == 0 ? nstate : 1);
+  return GF_OK;
+}
+
+int gf_burgers_steps(gf_mesh *m, int mode, double nu, int order, int nstate, double t, double dt,
+                     int nsteps, double *const *u) {
+  for (int s = 0; s < nsteps; s++) {
+    int rc = gf_burgers_step(m, mode, nu, order, nstate, t + s * dt, dt, u, 0);
+    if (rc) return rc;
+  }
+  return GF_OK;
+}
+
+int gf_helmholtz_diag_inv(gf_mesh *m, double p, const double *qf, double *dinv) {
+  if (!m || !dinv) return GF_ERR_ARG;
+  if (m->gtype != GF_GE_REGULAR || !m->Niso)
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Jacobi diagonal: regular isotropic meshes");
+  RC(GF_DISPATCH_N(m->dim, m->Niso, launch_diag, m, p, qf, dinv));
+  RC(gf_dss_t(m, GF_DSS_SUM, dinv));
+  hipLaunchKernelGGL(k_recip2, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, dinv,
+                     (float *)nullptr, m->ndof);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// CG driver
+// ===========================================================================
+namespace {
+
+constexpr int kUpdBlocks = 1024;
+constexpr int kBatch = 16;
+
+template <typename T>
+struct CGVecs {
+  T *x, *r, *w, *q;
+  const T *dinv, *mask, *imult, *qf;
+  const void *D, *W, *edx, *ejac;
+  double p;
+};
+
+// Reduce (r.z.imult, norm(r)) at set-up, synchronous; returns both.
+template <typename T>
+int setup_red(gf_mesh *m, const T *r, const T *z, const T *imult, int normtype, double out[2]) {
+  hipLaunchKernelGGL(k_setup_red<T>, dim3(kUpdBlocks), dim3(kThreads), 0, m->ctx->stream, r, z,
+                     imult, m->mass, normtype, m->red, m->ndof);
+  hipLaunchKernelGGL(k_setup_fin, dim3(1), dim3(kThreads), 0, m->ctx->stream, m->red, kUpdBlocks,
+                     normtype, m->scal + 60);
+  GF_LAUNCH_CHECK(m->ctx);
+  GF_HIP(m->ctx, hipMemcpyAsync(out, m->scal + 60, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                m->ctx->stream));
+  GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+  return GF_OK;
+}
+
+double finish_norm_host(int normtype, double v, gf_mesh *m, double volume) {
+  switch (normtype) {
+    case GF_NORM_INF: return v;
+    case GF_NORM_EUC: return std::sqrt(v / (double)m->ndof);
+    case GF_NORM_L2: return std::sqrt(v / volume);
+    case GF_NORM_L1: return v / volume;
+    default: return std::sqrt(v);
+  }
+}
+
+template <typename T, int NORM>
+void launch_update(gf_mesh *m, CGVecs<T> &v, double *part) {
+  hipLaunchKernelGGL((k_cg_update<T, NORM>), dim3(kUpdBlocks), dim3(kThreads), 0, m->ctx->stream,
+                     v.x, v.w, v.r, v.q, m->has_mask ? v.mask : nullptr, v.dinv, v.imult, m->mass,
+                     m->scal, part, m->ndof);
+}
+
+// Runs CG iterations from the state prepared in scal (rho, rtol, first=1,
+// iter=0, done) until done.  r holds the masked assembled residual, x the
+// iterate.  Returns iterations performed.
+template <typename T>
+int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters) {
+  hipStream_t s = m->ctx->stream;
+  using S8 = int;
+  (void)sizeof(S8);
+  int nbA = 0;
+  {
+    int NN = (int)m->nn;
+    int epb = NN >= kThreads ? 1 : kThreads / NN;
+    nbA = (int)((m->nelems + epb - 1) / epb);
+  }
+  double *partA = m->red;
+  double *partU = m->red + (size_t)nbA + 64;
+  CGApplyArgs a;
+  a.r = v.r;
+  a.dinv = v.dinv;
+  a.w = v.w;
+  a.q = v.q;
+  a.D = v.D;
+  a.W = v.W;
+  a.edx = v.edx;
+  a.ejac = v.ejac;
+  a.qf = v.qf;
+  a.p = v.p;
+  a.scal = m->scal;
+  a.part = partA;
+  a.nelems = m->nelems;
+  for (;;) {
+    for (int b = 0; b < kBatch; b++) {
+      int rc = sizeof(T) == 8
+                   ? GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_d, m, a)
+                   : GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_f, m, a);
+      if (rc) return rc;
+      if (m->ngroups)
+        hipLaunchKernelGGL(k_dss_cg<T>, dim3(nblocks((size_t)m->ngroups)), dim3(kThreads), 0, s,
+                           v.q, m->goff, m->gmem, m->ngroups, (const double *)m->scal);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kThreads), 0, s, m->scal, partA, nbA);
+      switch (normtype) {
+        case GF_NORM_INF: launch_update<T, GF_NORM_INF>(m, v, partU); break;
+        case GF_NORM_EUC: launch_update<T, GF_NORM_EUC>(m, v, partU); break;
+        case GF_NORM_L2: launch_update<T, GF_NORM_L2>(m, v, partU); break;
+        case GF_NORM_L1: launch_update<T, GF_NORM_L1>(m, v, partU); break;
+        default: launch_update<T, GF_NORM_ASSEMBLED_L2>(m, v, partU); break;
+      }
+      hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kThreads), 0, s, m->scal, partU, kUpdBlocks,
+                         normtype, d_resid);
+    }
+    GF_LAUNCH_CHECK(m->ctx);
+    GF_HIP(m->ctx, hipMemcpyAsync(m->hscal, m->scal, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
+    GF_HIP(m->ctx, hipStreamSynchronize(s));
+    if (m->hscal[S_DONE] != 0.0) break;
+  }
+  *iters = (int)m->hscal[S_ITER];
+  return GF_OK;
+}
+
+}  // namespace
+
+extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b, const double *xb,
+                           double *x, gf_cg_result *res) {
+  if (!m || !prm || !b || !x || !res) return GF_ERR_ARG;
+  memset(res, 0, sizeof(*res));
+  if (m->gtype != GF_GE_REGULAR || !m->Niso)
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: regular isotropic meshes");
+  if (prm->maxit < 1) return gf_set_err(m->ctx, GF_ERR_BADITERNUM, "maxit < 1");
+  if (prm->normtype < GF_NORM_INF || prm->normtype > GF_NORM_ASSEMBLED_L2) return GF_ERR_ARG;
+  if (prm->p_field) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: p field not supported");
+  hipStream_t s = m->ctx->stream;
+  size_t n = m->ndof, bytes = n * sizeof(double);
+  bool bbv = xb != nullptr;
+  const double *mask = bbv ? m->mask : nullptr;
+  double *r = (double *)gf_ws(m, WS_CG_BEGIN + 0, bytes);
+  double *w = (double *)gf_ws(m, WS_CG_BEGIN + 1, bytes);
+  double *q = (double *)gf_ws(m, WS_CG_BEGIN + 2, bytes);
+  double *dinv = prm->precond ? (double *)gf_ws(m, WS_CG_BEGIN + 3, bytes) : nullptr;
+  double *d_resid = (double *)gf_ws(m, WS_CG_BEGIN + 4, (size_t)(prm->maxit + 8) * sizeof(double));
+  if (!r || !w || !q || (prm->precond && !dinv) || !d_resid)
+    return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  // partial-sum buffer large enough for one partial per apply block
+  {
+    size_t need = (m->nelems + 64) + 2 * kUpdBlocks + 64;
+    if (need > (size_t)GF_RED_MAX_BLOCKS * 8) {
+      double *nr = nullptr;
+      GF_HIP(m->ctx, hipMalloc((void **)&nr, need * sizeof(double)));
+      (void)hipFree(m->red);
+      m->red = nr;
+    }
+  }
+  double volume = 0.0;
+  {
+    double *one = w;
+    hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(kThreads), 0, s, one, 0.0, b, 0.0, b, n);
+    // integrate(1) = sum of mass
+    RC(gf_dot_t(m, m->mass, nullptr, n, &volume));
+  }
+  if (prm->precond) RC(gf_helmholtz_diag_inv(m, prm->p_scalar, prm->q_field, dinv));
+  if (bbv) {  // x = x*mask + xb (gcg.ipp:371-375)
+    RC(gf_mul_t(m, x, m->mask, x));
+    RC(gf_axpy_t(m, 1.0, xb, x));
+  }
+  // stopping reference: norm of b (compute_norm(b), gcg.ipp:290-291)
+  double red2[2];
+  double rnorm0, rtol;
+  int normtype = prm->normtype;
+  if (normtype == GF_NORM_ASSEMBLED_L2) {
+    RC(copy(m, q, b));
+    RC(gf_dss_t(m, GF_DSS_SUM, q));
+    if (bbv) RC(gf_mul_t(m, q, m->mask, q));
+    RC(setup_red<double>(m, q, nullptr, m->imult, normtype, red2));
+    rnorm0 = finish_norm_host(normtype, red2[1], m, volume);
+    rtol = prm->tol * rnorm0;
+  } else {
+    RC(setup_red<double>(m, b, nullptr, m->imult, normtype, red2));
+    rnorm0 = finish_norm_host(normtype, red2[1], m, volume);
+    rtol = rnorm0 < 1.0 ? prm->tol : prm->tol * rnorm0;
+  }
+  res->rnorm0 = rnorm0;
+  int total_iters = 0, outer = 0, status = GF_OK;
+  double rn_final = 0.0;
+  // r = b - A x ; DSS ; mask
+  auto residual = [&](double *rr) -> int {
+    RC(gf_helm_t(m, prm->p_scalar, nullptr, prm->q_field, x, q));
+    hipLaunchKernelGGL(k_residual, dim3(nblocks(n)), dim3(kThreads), 0, s, rr, b, q, n);
+    RC(gf_dss_t(m, GF_DSS_SUM, rr));
+    if (bbv) RC(gf_mul_t(m, rr, m->mask, rr));
+    return GF_OK;
+  };
+  if (prm->precision == 0) {
+    RC(residual(r));
+    // z = P^-1 r ; w = z (set inside the first apply) ; rho = r.z.imult
+    double *z = q;
+    if (dinv) RC(gf_mul_t(m, dinv, r, z));
+    RC(setup_red<double>(m, r, dinv ? z : r, m->imult, normtype, red2));
+    double hs[12] = {0};
+    hs[S_RHO] = red2[0];
+    hs[S_RTOL] = rtol;
+    hs[S_RNORM] = 10.0 * rtol;
+    hs[S_ITER] = 0;
+    hs[S_MAXIT] = prm->maxit;
+    hs[S_DONE] = (0 < prm->maxit && hs[S_RNORM] > rtol) ? 0.0 : 1.0;
+    hs[S_FIRST] = 1.0;
+    hs[S_VOL] = volume;
+    hs[S_N] = (double)n;
+    GF_HIP(m->ctx, hipMemcpyAsync(m->scal, hs, 12 * sizeof(double), hipMemcpyHostToDevice, s));
+    CGVecs<double> v;
+    v.x = x;
+    v.r = r;
+    v.w = w;
+    v.q = q;
+    v.dinv = dinv;
+    v.mask = mask;
+    v.imult = m->imult;
+    v.qf = prm->q_field;
+    v.D = m->D[0];
+    v.W = m->W[0];
+    v.edx = m->edx;
+    v.ejac = m->ejac;
+    v.p = prm->p_scalar;
+    int it = 0;
+    RC(cg_loop<double>(m, v, normtype, d_resid, &it));
+    total_iters = it;
+    rn_final = m->hscal[S_RNORM];
+    if (prm->resid && it > 0)
+      GF_HIP(m->ctx, hipMemcpy(prm->resid, d_resid, it * sizeof(double), hipMemcpyDeviceToHost));
+    if (it >= prm->maxit && rn_final > prm->tol) status = GF_ERR_NOCONVERGE;
+  } else {
+    // FP32 inner CG inside FP64 iterative refinement.
+    float *r32 = (float *)gf_ws(m, WS_CG_BEGIN + 5, n * sizeof(float));
+    float *w32 = (float *)gf_ws(m, WS_CG_BEGIN + 6, n * sizeof(float));
+    float *q32 = (float *)gf_ws(m, WS_CG_BEGIN + 7, n * sizeof(float));
+    float *d32 = (float *)gf_ws(m, WS_CG_BEGIN + 8, n * sizeof(float));
+    float *dinv32 = dinv ? (float *)gf_ws(m, WS_CG_BEGIN + 9, n * sizeof(float)) : nullptr;
+    float *qf32 = prm->q_field ? (float *)gf_ws(m, WS_CG_BEGIN + 10, n * sizeof(float)) : nullptr;
+    if (!r32 || !w32 || !q32 || !d32 || (dinv && !dinv32) || (prm->q_field && !qf32))
+      return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+    if (dinv) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, dinv32, dinv, 1.0, n);
+    if (qf32) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, qf32, prm->q_field, 1.0, n);
+    double inner_tol = prm->inner_tol > 0 ? prm->inner_tol : 1e-5;
+    int hist = 0;
+    for (outer = 0; outer < 100; outer++) {
+      RC(residual(r));
+      RC(setup_red<double>(m, r, nullptr, m->imult, normtype, red2));
+      double rn = finish_norm_host(normtype, red2[1], m, volume);
+      rn_final = rn;
+      if (prm->resid && hist < prm->maxit) prm->resid[hist++] = rn;
+      if (rn <= rtol || total_iters >= prm->maxit) break;
+      // scale the correction problem to O(1): r32 = r / rscale
+      double rscale = 0.0;
+      RC(setup_red<double>(m, r, nullptr, m->imult, GF_NORM_INF, red2));
+      rscale = red2[1] > 0 ? red2[1] : 1.0;
+      hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, r32, r, 1.0 / rscale, n);
+      GF_HIP(m->ctx, hipMemsetAsync(d32, 0, n * sizeof(float), s));
+      float *z32 = q32;
+      if (dinv32) hipLaunchKernelGGL(k_mul_t<float>, dim3(nblocks(n)), dim3(kThreads), 0, s, dinv32, r32, z32, n);
+      RC(setup_red<float>(m, r32, dinv32 ? z32 : r32, m->imult32, normtype, red2));
+      double rn32 = finish_norm_host(normtype, red2[1], m, volume);
+      double hs[12] = {0};
+      hs[S_RHO] = red2[0];
+      hs[S_RTOL] = inner_tol * rn32;
+      hs[S_RNORM] = 10.0 * hs[S_RTOL];
+      hs[S_ITER] = 0;
+      hs[S_MAXIT] = std::max(1, prm->maxit - total_iters);
+      hs[S_DONE] = 0.0;
+      hs[S_FIRST] = 1.0;
+      hs[S_VOL] = volume;
+      hs[S_N] = (double)n;
+      GF_HIP(m->ctx, hipMemcpyAsync(m->scal, hs, 12 * sizeof(double), hipMemcpyHostToDevice, s));
+      CGVecs<float> v;
+      v.x = d32;
+      v.r = r32;
+      v.w = w32;
+      v.q = q32;
+      v.dinv = dinv32;
+      v.mask = m->mask32;
+      v.imult = m->imult32;
+      v.qf = qf32;
+      v.D = m->D32;
+      v.W = m->W32;
+      v.edx = m->edx32;
+      v.ejac = m->ejac32;
+      v.p = prm->p_scalar;
+      if (!bbv) v.mask = nullptr;
+      int it = 0;
+      RC(cg_loop<float>(m, v, normtype, (double *)nullptr, &it));
+      total_iters += it;
+      hipLaunchKernelGGL(k_add_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, x, d32, rscale, n);
+      GF_LAUNCH_CHECK(m->ctx);
+    }
+    if (rn_final > rtol) status = GF_ERR_NOCONVERGE;
+  }
+  if (bbv) {  // x *= mask ; x += xb (gcg.ipp:337, 378)
+    RC(gf_mul_t(m, x, m->mask, x));
+    RC(gf_axpy_t(m, 1.0, xb, x));
+  }
+  GF_HIP(m->ctx, hipStreamSynchronize(s));
+  res->status = status;
+  res->iters = total_iters;
+  res->outer = outer;
+  res->rnorm = rn_final;
+  return status;
+}
+
+// ===========================================================================
+// Diagnostics: time one kernel of the path with HIP events on the context
+// stream.  The kernel is launched `reps` times back to back on scratch data
+// of the mesh's size; the average duration per launch is returned in ms.
+// ===========================================================================
+namespace {
+__global__ void k_fill_pattern(double *a, size_t n, double s) {
+  GRID_LOOP(g, n) a[g] = s * (double)((g * 2654435761u) % 1000u) * 1e-3;
+}
+__global__ void k_fill_pattern_f(float *a, size_t n, float s) {
+  GRID_LOOP(g, n) a[g] = s * (float)((g * 2654435761u) % 1000u) * 1e-3f;
+}
+}  // namespace
+
+extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *avg_ms) {
+  if (!m || !which || reps < 1 || !avg_ms) return GF_ERR_ARG;
+  if (m->gtype != GF_GE_REGULAR || !m->Niso)
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "timing: regular isotropic meshes");
+  std::string w(which);
+  hipStream_t s = m->ctx->stream;
+  size_t n = m->ndof, bytes = n * sizeof(double);
+  const int nbuf = 16;
+  double *buf[nbuf];
+  for (int k = 0; k < nbuf; k++) {
+    buf[k] = (double *)gf_ws(m, WS_SLOTS + k, bytes);
+    if (!buf[k]) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+    hipLaunchKernelGGL(k_fill_pattern, dim3(nblocks(n)), dim3(kThreads), 0, s, buf[k], n,
+                       0.5 + 0.1 * k);
+  }
+  GF_LAUNCH_CHECK(m->ctx);
+  std::function<int()> launch;
+  StageArgs sa;
+  CGApplyArgs ca;
+  memset(&sa, 0, sizeof(sa));
+  memset(&ca, 0, sizeof(ca));
+  if (w == "burgers_stage") {
+    // stages 2-3 form: reads uin, kprev, accsrc; writes acc, kout (5 fields per component)
+    if (m->dim != 3 && m->dim != 2) return GF_ERR_ARG;
+    for (int c = 0; c < m->dim; c++) {
+      sa.uin[c] = buf[c];
+      sa.kprev[c] = buf[3 + c];
+      sa.acc[c] = buf[6 + c];
+      sa.accsrc[c] = buf[6 + c];
+      sa.kout[c] = buf[9 + c];
+    }
+    sa.bh = 0.5e-5;
+    sa.ch = 1e-5 / 3.0;
+    sa.D = m->D[0];
+    sa.W = m->W[0];
+    sa.edx = m->edx;
+    sa.ejac = m->ejac;
+    sa.imass = m->imass;
+    sa.nu = 0.02;
+    sa.nelems = m->nelems;
+    launch = [&]() { return GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, sa); };
+  } else if (w == "cg_apply" || w == "cg_apply_f32") {
+    bool f32 = w == "cg_apply_f32";
+    double hs[12] = {0};
+    hs[S_BETA] = 0.5;
+    hs[S_FIRST] = 0.0;
+    hs[S_DONE] = 0.0;
+    GF_HIP(m->ctx, hipMemcpyAsync(m->scal, hs, sizeof(hs), hipMemcpyHostToDevice, s));
+    size_t need = m->nelems + 64;
+    if (need > (size_t)GF_RED_MAX_BLOCKS * 8) {
+      double *nr = nullptr;
+      GF_HIP(m->ctx, hipMalloc((void **)&nr, (need + 2 * kUpdBlocks + 64) * sizeof(double)));
+      (void)hipFree(m->red);
+      m->red = nr;
+    }
+    if (f32) {
+      for (int k = 0; k < 4; k++)
+        hipLaunchKernelGGL(k_fill_pattern_f, dim3(nblocks(n)), dim3(kThreads), 0, s,
+                           (float *)buf[k], n, 0.5f + 0.1f * k);
+    }
+    ca.r = buf[0];
+    ca.dinv = buf[1];
+    ca.w = buf[2];
+    ca.q = buf[3];
+    ca.D = f32 ? (const void *)m->D32 : (const void *)m->D[0];
+    ca.W = f32 ? (const void *)m->W32 : (const void *)m->W[0];
+    ca.edx = f32 ? (const void *)m->edx32 : (const void *)m->edx;
+    ca.ejac = f32 ? (const void *)m->ejac32 : (const void *)m->ejac;
+    ca.qf = nullptr;
+    ca.p = 0.1;
+    ca.scal = m->scal;
+    ca.part = m->red;
+    ca.nelems = m->nelems;
+    if (f32)
+      launch = [&]() { return GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_f, m, ca); };
+    else
+      launch = [&]() { return GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_d, m, ca); };
+  } else if (w == "helmholtz") {
+    launch = [&]() { return gf_helm_t(m, 0.1, nullptr, nullptr, buf[0], buf[1]); };
+  } else {
+    return gf_set_err(m->ctx, GF_ERR_ARG, "unknown kernel name");
+  }
+  RC(launch());  // warm-up
+  hipEvent_t e0, e1;
+  GF_HIP(m->ctx, hipEventCreate(&e0));
+  GF_HIP(m->ctx, hipEventCreate(&e1));
+  GF_HIP(m->ctx, hipEventRecord(e0, s));
+  for (int r = 0; r < reps; r++) RC(launch());
+  GF_HIP(m->ctx, hipEventRecord(e1, s));
+  GF_HIP(m->ctx, hipEventSynchronize(e1));
+  float ms = 0.f;
+  GF_HIP(m->ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = (double)ms / reps;
+  return GF_OK;
+}

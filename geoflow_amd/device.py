@@ -1,0 +1,255 @@
+"""Device-side mirror of the reference operator API over the C ABI.
+
+Fields are torch float64 CUDA tensors holding element-contiguous local
+vectors; every operator call goes straight to libgeoflow_hip.so on the
+context's stream (torch's current stream).  There is no CPU path.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import (CGParams, CGResult, MeshDesc, GF_GE_REGULAR, GF_GE_2DEMBEDDED, check, lib)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _ptrs(ts):
+    return (_vp * len(ts))(*[(t.data_ptr() if t is not None else None) for t in ts])
+
+
+def _hp(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class Context:
+    """gf_ctx bound to a device and torch's current stream on it."""
+
+    def __init__(self, device=0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("geoflow_amd device operators need a GPU")
+        self.device = device
+        torch.cuda.set_device(device)
+        stream = torch.cuda.current_stream(device).cuda_stream
+        h = _vp()
+        check(lib().gf_ctx_create(device, ctypes.c_void_p(stream), ctypes.byref(h)))
+        self.h = h
+
+    def sync(self):
+        check(lib().gf_ctx_synchronize(self.h), self.h)
+
+    def __del__(self):
+        try:
+            lib().gf_ctx_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Mesh:
+    """Device-resident mesh: GLL basis, metrics, mask and GGFX groups.
+
+    Built from a host grid object exposing dim, N, nelems, basis, and either
+    edx/ejac (regular) or node metrics (embedded), plus ggfx_offs/members.
+    """
+
+    def __init__(self, ctx, grid, gtype=GF_GE_REGULAR, node_dxidx=None, node_jac=None,
+                 xyz=None, use_mask=True):
+        self.ctx = ctx
+        self.grid = grid
+        self.dim = grid.dim
+        self.gtype = gtype
+        d = MeshDesc()
+        d.dim = grid.dim
+        d.gtype = gtype
+        for k in range(3):
+            d.N[k] = grid.N[k] if k < grid.dim else 1
+        d.nelems = grid.nelems
+        keep = []
+
+        def host(a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return a.ctypes.data_as(_dp)
+
+        for k in range(grid.dim):
+            xi, w, D = grid.basis[k]
+            d.D[k] = host(D.reshape(-1, order="F"))
+            d.W[k] = host(w)
+        if gtype == GF_GE_REGULAR:
+            d.elem_dxidx = host(grid.edx)
+            d.elem_jac = host(grid.ejac)
+        else:
+            for i in range(2):
+                for j in range(3):
+                    d.node_dxidx[i * 3 + j] = host(node_dxidx[i][j])
+            d.node_jac = host(node_jac)
+        coords = xyz if xyz is not None else getattr(grid, "xyz", None)
+        if coords is not None:
+            for k in range(len(coords)):
+                d.xyz[k] = host(coords[k])
+        if use_mask and getattr(grid, "mask", None) is not None:
+            d.mask = host(grid.mask)
+        offs = np.ascontiguousarray(grid.ggfx_offs, dtype=np.int64)
+        mem = np.ascontiguousarray(grid.ggfx_members, dtype=np.int64)
+        keep += [offs, mem]
+        d.ggfx_offs = offs.ctypes.data_as(_i64p)
+        d.ggfx_members = mem.ctypes.data_as(_i64p)
+        h = _vp()
+        check(lib().gf_mesh_create(ctx.h, ctypes.byref(d), ctypes.byref(h)), ctx.h)
+        self.h = h
+        self.ndof = lib().gf_mesh_ndof(h)
+
+    def __del__(self):
+        try:
+            lib().gf_mesh_destroy(self.h)
+        except Exception:
+            pass
+
+    # -- helpers -----------------------------------------------------------
+    def _c(self, rc):
+        return check(rc, self.ctx.h)
+
+    def new(self, dtype=torch.float64):
+        return torch.zeros(self.ndof, dtype=dtype, device="cuda")
+
+    def get(self, which):
+        out = np.zeros(self.ndof)
+        self._c(lib().gf_mesh_get(self.h, which.encode(), _hp(out)))
+        return out
+
+    # -- operators (mirror GGrid / GMTK / GMass / GHelmholtz / GGFX) -------
+    def grefderiv(self, u, idir, dotrans=False, out=None):
+        out = self.new() if out is None else out
+        self._c(lib().gf_grefderiv(self.h, idir, int(dotrans), _ptr(u), _ptr(out)))
+        return out
+
+    def grefderivs(self, u, dotrans=False):
+        outs = [self.new() for _ in range(self.dim)]
+        self._c(lib().gf_grefderivs(self.h, int(dotrans), _ptr(u), _ptrs(outs)))
+        return outs
+
+    def grefdiv(self, us, dotrans=True, out=None):
+        out = self.new() if out is None else out
+        us = list(us) + [None] * (3 - len(us))
+        self._c(lib().gf_grefdiv(self.h, int(dotrans), _ptrs(us), _ptr(out)))
+        return out
+
+    def deriv(self, u, idir, out=None):
+        out = self.new() if out is None else out
+        self._c(lib().gf_deriv(self.h, idir, _ptr(u), _ptr(out)))
+        return out
+
+    def grad(self, u):
+        # embedded 2-D meshes return all three Cartesian derivatives
+        ncomp = 3 if self.gtype == GF_GE_2DEMBEDDED else self.dim
+        outs = [self.new() for _ in range(ncomp)]
+        self._c(lib().gf_grad(self.h, _ptr(u), _ptrs(outs)))
+        return outs
+
+    def mass_apply(self, u, out=None):
+        out = self.new() if out is None else out
+        self._c(lib().gf_mass_apply(self.h, _ptr(u), _ptr(out)))
+        return out
+
+    def imass_apply(self, u, out=None):
+        out = self.new() if out is None else out
+        self._c(lib().gf_imass_apply(self.h, _ptr(u), _ptr(out)))
+        return out
+
+    def integrate(self, u):
+        r = ctypes.c_double(0)
+        self._c(lib().gf_integrate(self.h, _ptr(u), ctypes.byref(r)))
+        return r.value
+
+    def helmholtz(self, u, p=1.0, q=None, out=None):
+        out = self.new() if out is None else out
+        pf = p if torch.is_tensor(p) else None
+        ps = 1.0 if pf is not None else float(p)
+        self._c(lib().gf_helmholtz_apply(self.h, ps, _ptr(pf), _ptr(q), _ptr(u), _ptr(out)))
+        return out
+
+    def helmholtz_f32(self, u, p=1.0, q=None, out=None):
+        out = self.new(torch.float32) if out is None else out
+        self._c(lib().gf_helmholtz_apply_f32(self.h, float(p), _ptr(q), _ptr(u), _ptr(out)))
+        return out
+
+    def dss(self, u, op="sum"):
+        ops = {"sum": 0, "smooth": 1, "max": 2, "min": 3}
+        self._c(lib().gf_dss(self.h, ops[op], _ptr(u)))
+        return u
+
+    def imult(self):
+        out = self.new()
+        self._c(lib().gf_imult(self.h, _ptr(out)))
+        return out
+
+    def advect(self, p, us, out=None):
+        out = self.new() if out is None else out
+        self._c(lib().gf_advect(self.h, _ptr(p), len(us), _ptrs(us), _ptr(out)))
+        return out
+
+    def burgers_rhs(self, us, nu, mode=0):
+        # one output per component for the nonlinear (0) and heat (2) modes,
+        # a single output for pure advection (1)
+        nout = 1 if mode == 1 else len(us)
+        outs = [self.new() for _ in range(nout)]
+        self._c(lib().gf_burgers_rhs(self.h, mode, nu, len(us), _ptrs(us), _ptrs(outs)))
+        return outs
+
+    def burgers_step(self, us, nu, dt, t=0.0, mode=0, order=4, check_finite=False):
+        self._c(lib().gf_burgers_step(self.h, mode, nu, order, len(us), t, dt, _ptrs(us),
+                                      int(check_finite)))
+        return us
+
+    def burgers_steps(self, us, nu, dt, nsteps, t=0.0, mode=0, order=4):
+        self._c(lib().gf_burgers_steps(self.h, mode, nu, order, len(us), t, dt, nsteps,
+                                       _ptrs(us)))
+        return us
+
+    def time_kernel(self, which, reps=20):
+        """Average ms per launch of one kernel, timed with HIP events."""
+        ms = ctypes.c_double(0)
+        self._c(lib().gf_time_kernel(self.h, which.encode(), reps, ctypes.byref(ms)))
+        return ms.value
+
+    def helmholtz_diag_inv(self, p=1.0, q=None):
+        out = self.new()
+        self._c(lib().gf_helmholtz_diag_inv(self.h, float(p), _ptr(q), _ptr(out)))
+        return out
+
+    def cg_solve(self, b, x=None, xb=None, p=1.0, q=None, maxit=512, tol=1e-6, normtype=0,
+                 precond=0, precision=0, inner_tol=1e-5, raise_on_error=False):
+        x = self.new() if x is None else x
+        prm = CGParams()
+        prm.maxit = maxit
+        prm.tol = tol
+        prm.normtype = normtype
+        prm.precond = precond
+        prm.precision = precision
+        prm.inner_tol = inner_tol
+        prm.p_scalar = float(p)
+        prm.p_field = None
+        prm.q_field = _ptr(q)
+        hist = np.zeros(max(maxit, 1) + 8)
+        prm.resid = hist.ctypes.data_as(_dp)
+        res = CGResult()
+        rc = lib().gf_cg_solve(self.h, ctypes.byref(prm), _ptr(b), _ptr(xb), _ptr(x),
+                               ctypes.byref(res))
+        if rc not in (0, _capi.GF_ERR_NOCONVERGE) or (rc and raise_on_error):
+            self._c(rc)
+        nh = res.iters if precision == 0 else res.outer + 1
+        return x, dict(status=res.status, iters=res.iters, outer=res.outer, rnorm=res.rnorm,
+                       rnorm0=res.rnorm0, resid=hist[: max(nh, 0)])

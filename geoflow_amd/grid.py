@@ -1,0 +1,91 @@
+"""Host-side grid construction through the C ABI (no device work).
+
+``BoxGrid`` mirrors GGridBox (src/cdg/grid/ggrid_box.ipp) plus the GGFX set-up
+of ``init_ggfx`` (src/apps/geoflow_cdg.cpp:731-774): node coordinates,
+regular-element metrics, boundary mask, periodized coordinates and the
+coincidence groups matched with tolerance 0.001 * minnodedist.
+"""
+import ctypes
+
+import numpy as np
+
+from ._capi import GF_BDY_PERIODIC, GF_BDY_DIRICHLET, check, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def gll_basis(p):
+    """GLL nodes, weights and derivative matrix D (N x N, D[l, j] = dphi_j/dxi(xi_l))."""
+    N = p + 1
+    xi, w, D = np.zeros(N), np.zeros(N), np.zeros(N * N)
+    check(lib().gf_gll_basis(p, _p(xi), _p(w), _p(D)))
+    return xi, w, D.reshape(N, N, order="F")
+
+
+def box_grid(dim, p, ne, P0, P1, bdy):
+    """Run gf_box_grid; returns dict of host arrays."""
+    p = [p] * dim if np.isscalar(p) else list(p)
+    ne = list(ne)
+    nn = int(np.prod([q + 1 for q in p]))
+    nel = int(np.prod(ne))
+    ndof = nn * nel
+    xyz = [np.zeros(ndof) for _ in range(dim)]
+    xyzp = [np.zeros(ndof) for _ in range(dim)]
+    mask = np.zeros(ndof)
+    edx = np.zeros(nel * dim)
+    ejac = np.zeros(nel)
+    md = ctypes.c_double(0)
+    ia = lambda v: (ctypes.c_int * len(v))(*v)
+    P0a = np.ascontiguousarray(P0[:dim], dtype=np.float64)
+    P1a = np.ascontiguousarray(P1[:dim], dtype=np.float64)
+    xp = (_dp * dim)(*[_p(a) for a in xyz])
+    xpp = (_dp * dim)(*[_p(a) for a in xyzp])
+    check(lib().gf_box_grid(dim, ia(p), ia(ne), _p(P0a), _p(P1a), ia(list(bdy)), xp, xpp,
+                            _p(mask), _p(edx), _p(ejac), ctypes.byref(md)))
+    return dict(xyz=xyz, xyzp=xyzp, mask=mask, edx=edx, ejac=ejac, minnodedist=md.value,
+                ndof=ndof, nelems=nel, p=p, ne=ne)
+
+
+def ggfx_build(coords, tol, max_dup=0):
+    """GGFX coincidence groups: (offs[ndof+1], members) as int64 arrays."""
+    coords = [np.ascontiguousarray(c, dtype=np.float64) for c in coords]
+    ndof = coords[0].size
+    cp = (_dp * len(coords))(*[_p(c) for c in coords])
+    nm = ctypes.c_size_t(0)
+    offs = np.zeros(ndof + 1, dtype=np.int64)
+    check(lib().gf_ggfx_build(ndof, len(coords), cp, tol, max_dup, offs.ctypes.data_as(_i64p),
+                              None, ctypes.byref(nm)))
+    mem = np.zeros(max(nm.value, 1), dtype=np.int64)
+    check(lib().gf_ggfx_build(ndof, len(coords), cp, tol, max_dup, offs.ctypes.data_as(_i64p),
+                              mem.ctypes.data_as(_i64p), ctypes.byref(nm)))
+    return offs, mem[: nm.value]
+
+
+class BoxGrid:
+    """GGridBox equivalent: host arrays for a box of regular elements."""
+
+    def __init__(self, dim, p, ne, P0=(0.0, 0.0, 0.0), P1=(1.0, 1.0, 1.0), bdy=None,
+                 tol_fac=0.001):
+        if bdy is None:
+            bdy = [GF_BDY_PERIODIC] * (2 * dim)
+        self.dim = dim
+        g = box_grid(dim, p, ne, np.asarray(P0, dtype=np.float64), np.asarray(P1, dtype=np.float64),
+                     bdy)
+        self.__dict__.update(g)
+        self.bdy = list(bdy)
+        self.P0 = np.asarray(P0, dtype=np.float64)[:dim]
+        self.P1 = np.asarray(P1, dtype=np.float64)[:dim]
+        self.N = [q + 1 for q in self.p]
+        self.basis = [gll_basis(q) for q in self.p]
+        self.tol = tol_fac * self.minnodedist
+        # GGFX::init uses 3 coordinates; missing ones are zero (geoflow_cdg.cpp:745-753)
+        self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyzp, self.tol,
+                                                       max_dup=2 ** dim)

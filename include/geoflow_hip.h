@@ -1,0 +1,246 @@
+/*
+ * geoflow_hip.h -- C ABI of the MI355X-native GeoFLOW SEM operator path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No
+ * exceptions cross this boundary and no torch types appear in it.  All field
+ * pointers passed to operator entry points are DEVICE pointers holding
+ * element-contiguous "local" vectors, exactly the layout of GeoFLOW's
+ * GTVector state components:
+ *     field[e*Nx*Ny*Nz + k*Nx*Ny + j*Nx + i]   (x fastest)
+ * (reference: src/cdg/sem/gmass.ipp:180-212, src/cdg/grid/ggrid_box.ipp:409-457).
+ * Matrices are column-major, D[l + j*N] = dphi_j/dxi at node l
+ * (src/cdg/blas/gtmatrix.hpp:104-107, src/cdg/sem/gllbasis.ipp:589-600).
+ *
+ * Threading: one gf_ctx per host thread.  Operator calls are enqueued on the
+ * context's HIP stream; they return once enqueued unless documented as
+ * synchronous (those that return host scalars).  Hot-path entry points do
+ * not allocate device memory after a mesh's first use of that entry point.
+ */
+#ifndef GEOFLOW_HIP_H
+#define GEOFLOW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------
+ * 0..4 mirror GCG::GCGERR (src/cdg/solvers/gcg.hpp:32). */
+enum {
+  GF_OK = 0,
+  GF_ERR_NOCONVERGE = 1, /* GCGERR_NOCONVERGE */
+  GF_ERR_SOLVE = 2,      /* GCGERR_SOLVE      */
+  GF_ERR_PRECOND = 3,    /* GCGERR_PRECOND    */
+  GF_ERR_BADITERNUM = 4, /* GCGERR_BADITERNUM */
+  GF_ERR_ARG = 10,       /* invalid argument / shape mismatch */
+  GF_ERR_HIP = 11,       /* HIP runtime error */
+  GF_ERR_ALLOC = 12,     /* device allocation failed */
+  GF_ERR_NONFINITE = 13, /* state not finite (GBurgers::step_impl assert, gburgers.ipp:327-331) */
+  GF_ERR_UNSUPPORTED = 14
+};
+
+/* GBdyType (src/cdg/include/gtypes.h:157) */
+enum {
+  GF_BDY_DIRICHLET = 0, GF_BDY_INFLOW = 1, GF_BDY_NOSLIP = 2, GF_BDY_0FLUX = 3,
+  GF_BDY_OUTFLOW = 4, GF_BDY_SPONGE = 5, GF_BDY_PERIODIC = 6, GF_BDY_NONE = 7
+};
+
+/* GElemType (src/cdg/include/gtypes.h:171) */
+enum { GF_GE_REGULAR = 0, GF_GE_DEFORMED = 1, GF_GE_2DEMBEDDED = 2 };
+
+/* GGFX reduction operators (src/cdg/comm/ggfx.hpp:36-76) */
+enum { GF_DSS_SUM = 0, GF_DSS_SMOOTH = 1, GF_DSS_MAX = 2, GF_DSS_MIN = 3 };
+
+/* LinSolverBase norm types (src/pdeint/lin_solver_base.hpp:38-44), plus
+ * GF_NORM_ASSEMBLED_L2: 2-norm over unique (assembled) nodes with a purely
+ * relative stopping rule ||r|| <= tol*||DSS(b)||. */
+enum { GF_NORM_INF = 0, GF_NORM_EUC = 1, GF_NORM_L2 = 2, GF_NORM_L1 = 3, GF_NORM_ASSEMBLED_L2 = 4 };
+
+typedef struct gf_ctx_s gf_ctx;
+typedef struct gf_mesh_s gf_mesh;
+
+/* ---- context ----------------------------------------------------------- */
+/* stream: a hipStream_t (NULL = the legacy default stream). */
+int  gf_ctx_create(int device, void *stream, gf_ctx **out);
+int  gf_ctx_destroy(gf_ctx *ctx);
+int  gf_ctx_set_stream(gf_ctx *ctx, void *stream);
+int  gf_ctx_synchronize(gf_ctx *ctx);
+/* Last error message for this context (static storage, never NULL). */
+const char *gf_ctx_last_error(gf_ctx *ctx);
+/* Library version string. */
+const char *gf_version(void);
+
+/* ---- host-side setup (no device work) ----------------------------------
+ * GLLBasis<long double,double>::init (src/cdg/sem/gllbasis.ipp:714-747):
+ * GLL nodes, weights and the derivative matrix D for order p (N = p+1). */
+int gf_gll_basis(int p, double *xi, double *w, double *D);
+
+/* GGridBox element and node generation for one rank
+ * (src/cdg/grid/ggrid_box.ipp:254-463, 862-978), regular-element geometry
+ * (GElem_base::dogeom2d/3d, src/cdg/sem/gelem_base.cpp:877-1048), boundary
+ * mask (GGrid::init_bc_info, src/cdg/grid/ggrid.ipp:1410-1472) and periodized
+ * coordinates (GGridBox::periodize, src/cdg/grid/ggrid_box.ipp:753-802).
+ *   p[dim], ne[dim], P0[dim], P1[dim]; bdy[2*dim] ordered x0,x1,y0,y1(,z0,z1).
+ * Outputs are caller-allocated (ndof = prod(ne) * prod(p+1)):
+ *   xyz[dim]   node coordinates, each ndof (may be NULL)
+ *   xyzp[dim]  periodized coordinates for DSS matching, each ndof (may be NULL)
+ *   mask       ndof, 0 on Dirichlet boundary nodes (may be NULL)
+ *   elem_dxidx nelems*dim, dXi_d/dX_d = 2/L_d per element (may be NULL)
+ *   elem_jac   nelems, prod(L_d)/2^dim (may be NULL)
+ *   minnodedist  GGrid::find_min_dist (src/cdg/grid/ggrid.ipp:977-1013) */
+int gf_box_grid(int dim, const int *p, const int *ne, const double *P0, const double *P1,
+                const int *bdy, double *const *xyz, double *const *xyzp, double *mask,
+                double *elem_dxidx, double *elem_jac, double *minnodedist);
+
+/* GGFX::init local coincidence matching (src/cdg/comm/ggfx.hpp:167-342) on
+ * host coordinates xyz[ncoord][ndof] with box tolerance tol.  Produces the
+ * group of every node as CSR: members[offs[n] .. offs[n+1]) ascending.
+ * Call with members == NULL to obtain *nmembers; offs needs ndof+1 entries.
+ * Returns GF_ERR_ARG if a group exceeds max_dup (0 = unchecked). */
+int gf_ggfx_build(size_t ndof, int ncoord, const double *const *xyz, double tol,
+                  size_t max_dup, int64_t *offs, int64_t *members, size_t *nmembers);
+
+/* ---- mesh (device-resident operator data) ------------------------------ */
+typedef struct {
+  int dim;               /* 2 or 3 (GDIM is a runtime value here)                  */
+  int gtype;             /* GF_GE_REGULAR or GF_GE_2DEMBEDDED                       */
+  int N[3];              /* nodes per direction, p+1 (N[2] ignored when dim == 2)   */
+  size_t nelems;
+  const double *D[3];    /* host: column-major N x N derivative matrices            */
+  const double *W[3];    /* host: GLL weights                                       */
+  /* GE_REGULAR: per-element constants (host) */
+  const double *elem_dxidx;  /* nelems*dim */
+  const double *elem_jac;    /* nelems     */
+  /* GE_2DEMBEDDED: per-node metrics (host), dXi_i/dX_j, i < dim, j < 3 at
+   * node_dxidx[i*3+j], and node Jacobian. */
+  const double *node_dxidx[9];
+  const double *node_jac;
+  const double *xyz[3];  /* host node coordinates (optional; needed for sphere ops) */
+  const double *mask;    /* host, ndof, or NULL (all ones)                          */
+  /* GGFX groups from gf_ggfx_build (host) */
+  const int64_t *ggfx_offs;
+  const int64_t *ggfx_members;
+} gf_mesh_desc;
+
+int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *desc, gf_mesh **out);
+int gf_mesh_destroy(gf_mesh *mesh);
+size_t gf_mesh_ndof(const gf_mesh *mesh);
+/* Copy a device-resident mesh array to host: "mass", "imass", "imult",
+ * "mask" (ndof doubles each).  Synchronous. */
+int gf_mesh_get(gf_mesh *mesh, const char *which, double *host_out);
+
+/* ---- device memory helpers (optional; any device allocation works) ----- */
+int gf_malloc(gf_ctx *ctx, size_t bytes, void **dptr);
+int gf_free(gf_ctx *ctx, void *dptr);
+int gf_memcpy_h2d(gf_ctx *ctx, void *dst, const void *src, size_t bytes);  /* synchronous */
+int gf_memcpy_d2h(gf_ctx *ctx, void *dst, const void *src, size_t bytes);  /* synchronous */
+
+/* ---- tensor-product derivatives ----------------------------------------
+ * GMTK::I3_X_I2_X_D1 / I3_X_D2_X_I1 / D3_X_I2_X_I1 (and 2-D forms) as
+ * dispatched by GGrid::compute_grefderiv (src/cdg/grid/ggrid.ipp:2046-2265,
+ * src/cdg/blas/gmtk.ipp:1404-2294).  idir in 1..dim; dotrans selects D^T. */
+int gf_grefderiv(gf_mesh *mesh, int idir, int dotrans, const double *u, double *du);
+/* GGrid::compute_grefderivs (src/cdg/grid/ggrid.ipp:1954-2016): du[dim]. */
+int gf_grefderivs(gf_mesh *mesh, int dotrans, const double *u, double *const *du);
+/* GGrid::compute_grefdiv (src/cdg/grid/ggrid.ipp:1847-1927): u[dim] (entries may be NULL). */
+int gf_grefdiv(gf_mesh *mesh, int dotrans, const double *const *u, double *divu);
+/* GGrid::deriv (src/cdg/grid/ggrid.ipp:1163-1191): physical derivative, idir 1..dim(+1 embedded). */
+int gf_deriv(gf_mesh *mesh, int idir, const double *u, double *du);
+/* GMTK::grad (src/cdg/blas/gmtk.ipp:2883-2892): du[k] = d u / d x_k for k < dim (+1 embedded). */
+int gf_grad(gf_mesh *mesh, const double *u, double *const *du);
+
+/* ---- mass -----------------------------------------------------------------
+ * GMass::opVec_prod (src/cdg/sem/gmass.ipp:225-232); the inverse uses the
+ * DSS-Smoothed mass (src/cdg/sem/gmass.ipp:75-82). */
+int gf_mass_apply(gf_mesh *mesh, const double *in, double *out);
+int gf_imass_apply(gf_mesh *mesh, const double *in, double *out);
+/* GGrid::integrate (src/cdg/grid/ggrid.ipp:1077-1148).  Synchronous. */
+int gf_integrate(gf_mesh *mesh, const double *u, double *result);
+
+/* ---- Helmholtz ------------------------------------------------------------
+ * GHelmholtz::opVec_prod -> reg_prod / embed_prod
+ * (src/cdg/sem/ghelmholtz.ipp:76-378): uo = p L u + q M u, unassembled.
+ * p: p_field (device, ndof) if non-NULL else the scalar p_scalar
+ *    (set_Lap_scalar, ghelmholtz.ipp:590-603);
+ * q: q_field (device, ndof) or NULL for the pure Laplacian
+ *    (set_mass_scalar, ghelmholtz.ipp:613-625). */
+int gf_helmholtz_apply(gf_mesh *mesh, double p_scalar, const double *p_field,
+                       const double *q_field, const double *u, double *uo);
+/* Same operator in FP32 (inputs/outputs float) for the mixed-precision solver. */
+int gf_helmholtz_apply_f32(gf_mesh *mesh, float p_scalar, const float *q_field,
+                           const float *u, float *uo);
+
+/* ---- direct stiffness summation --------------------------------------------
+ * GGFX::doOp (src/cdg/comm/ggfx.hpp:344-443), in place.  The sum of each
+ * coincidence group runs over members in ascending local id from 0, as the
+ * reference's std::accumulate does, so results are bit-identical to it. */
+int gf_dss(gf_mesh *mesh, int op, double *u);
+/* GGFX::get_imult (src/cdg/comm/ggfx.hpp:475-487): 1/multiplicity. */
+int gf_imult(gf_mesh *mesh, double *imult);
+
+/* ---- advection, Burgers RHS, RK step ---------------------------------------
+ * GAdvect::apply (src/cdg/sem/gadvect.ipp:57-99): po = M sum_j u_j dp/dx_j. */
+int gf_advect(gf_mesh *mesh, const double *p, int nvel, const double *const *u, double *po);
+/* GBurgers::dudt_impl (src/cdg/pdes/gburgers.ipp:202-257).
+ * mode 0: nonlinear, nstate = dim components;  mode 1: pure advection of
+ * u[0] by u[1..nstate-1];  mode 2: heat. */
+int gf_burgers_rhs(gf_mesh *mesh, int mode, double nu, int nstate, const double *const *u,
+                   double *const *dudt);
+/* GBurgers::step_impl -> GExRKStepper::step_b with Butcher RK of order
+ * 1..4 and GGFX Smooth after each stage (src/cdg/pdes/gburgers.ipp:274-407,
+ * src/cdg/pdes/gexrk_stepper.ipp:251-341, src/cdg/pdes/gbutcherrk.ipp:80-127).
+ * u[nstate] device fields advanced in place.  check_finite != 0 performs the
+ * reference's isfinite check and returns GF_ERR_NONFINITE on failure. */
+int gf_burgers_step(gf_mesh *mesh, int mode, double nu, int order, int nstate, double t,
+                    double dt, double *const *u, int check_finite);
+/* nsteps consecutive steps (one host call; identical arithmetic). */
+int gf_burgers_steps(gf_mesh *mesh, int mode, double nu, int order, int nstate, double t,
+                     double dt, int nsteps, double *const *u);
+
+/* ---- conjugate gradient -----------------------------------------------------
+ * GCG::solve_impl (src/cdg/solvers/gcg.ipp:239-345, 361-382) applied to the
+ * Helmholtz operator above. */
+typedef struct {
+  int    maxit;      /* default 512 */
+  double tol;        /* default 1e-6 */
+  int    normtype;   /* GF_NORM_* */
+  int    precond;    /* 0: identity (reference);  1: Jacobi on the assembled diagonal */
+  int    precision;  /* 0: FP64 CG;  1: FP32 applies in an FP64 iterative refinement */
+  double inner_tol;  /* mixed: relative reduction per inner FP32 solve (default 1e-5) */
+  double p_scalar;   /* Laplacian factor when p_field == NULL */
+  const double *p_field;
+  const double *q_field;
+  double *resid;     /* host, optional, length >= maxit: residual history */
+} gf_cg_params;
+
+typedef struct {
+  int    status;     /* GF_OK or GF_ERR_NOCONVERGE ... */
+  int    iters;      /* CG iterations (mixed: total inner iterations) */
+  int    outer;      /* mixed: refinement steps */
+  double rnorm;      /* final residual norm in the selected norm */
+  double rnorm0;     /* norm of b used for the stopping rule */
+} gf_cg_result;
+
+/* b, xb, x: device fields.  xb != NULL selects solve_impl(A,b,xb,x): the
+ * boundary mask is applied and xb is added back (gcg.ipp:361-382).
+ * Synchronous (reads residual norms on the host each iteration batch). */
+int gf_cg_solve(gf_mesh *mesh, const gf_cg_params *prm, const double *b, const double *xb,
+                double *x, gf_cg_result *res);
+/* Assembled Jacobi diagonal inverse of (p L + q M). */
+int gf_helmholtz_diag_inv(gf_mesh *mesh, double p_scalar, const double *q_field, double *dinv);
+
+/* ---- diagnostics ------------------------------------------------------------
+ * Average duration in ms of `reps` back-to-back launches of one kernel of the
+ * path on scratch data of the mesh's size, timed with HIP events on the
+ * context stream.  which: "burgers_stage" (fused RK stage, the stage 2-3
+ * form), "cg_apply" / "cg_apply_f32" (CG direction update + Helmholtz apply),
+ * "helmholtz".  No reference counterpart (the reference's timers are never
+ * set, src/apps/hackathon_driver.cpp:91,258-259).  Synchronous. */
+int gf_time_kernel(gf_mesh *mesh, const char *which, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEOFLOW_HIP_H */

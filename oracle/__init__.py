@@ -252,7 +252,8 @@ class Grid:
         return o
 
     def burgers_rhs(self, us, nu, mode=0):
-        outs = [np.zeros(self.ndof) for _ in range(len(us) if mode == 0 else 1)]
+        # heat (2) and nonlinear (0) modes produce one output per state component
+        outs = [np.zeros(self.ndof) for _ in range(1 if mode == 1 else len(us))]
         lib().or_burgers_rhs(self.h, mode, nu, len(us), _pp(us), _pp(outs))
         return outs
 

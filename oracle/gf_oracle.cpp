@@ -194,7 +194,8 @@ struct OrGrid {
   std::vector<double> xp[3];                // periodized coordinates
   std::vector<double> dxidx[3][3];          // dXi_i/dX_j per node
   std::vector<double> jac, mask, mass, imass, imult;
-  std::vector<std::vector<size_t>> group;   // GGFX local coincidence groups
+  std::vector<std::vector<size_t>> group;   // GGFX recv_map_[my_rank]: id -> matched set
+  std::vector<std::vector<size_t>> buf;     // doOp reduction buffer sources per node
   double minnodedist = 0, volume = 0;
   int nxy() const { return gtype == 2 ? dim + 1 : dim; }
 };
@@ -589,14 +590,16 @@ void or_imass_apply(OrGrid *g, const double *in, double *out) {
   for (size_t j = 0; j < g->ndof; j++) out[j] = g->imass[j] * in[j];
 }
 
-// GGFX::doOp (ggfx.hpp:344-443) on a single rank: node i's buffer holds
-// u[m] for the members m of its group in ascending order; Sum and Smooth
-// accumulate from zero (ggfx.hpp:60-76), Max/Min from lowest()/max().
+// GGFX::doOp (ggfx.hpp:344-443) on a single rank: for every local_id in
+// ascending order (recv_map_ is a std::map) and every id in its matched set,
+// u[local_id] is appended to id's buffer (ggfx.hpp:401-405).  Sum and Smooth
+// accumulate the buffer from zero (ggfx.hpp:60-76), Max/Min from
+// lowest()/max().
 void or_dss(OrGrid *g, int op, double *u) {
   size_t n = g->ndof;
   std::vector<double> out(n);
   for (size_t i = 0; i < n; i++) {
-    const std::vector<size_t> &grp = g->group[i];
+    const std::vector<size_t> &grp = g->buf[i];
     double r;
     if (op == 0 || op == 1) {
       r = 0.0;
@@ -614,9 +617,9 @@ void or_dss(OrGrid *g, int op, double *u) {
   std::copy(out.begin(), out.end(), u);
 }
 
-// GGFX::get_imult (ggfx.hpp:451-487)
+// GGFX::get_mult/get_imult (ggfx.hpp:451-487): count of buffer sources.
 void or_imult(OrGrid *g, double *imult) {
-  for (size_t i = 0; i < g->ndof; i++) imult[i] = 1.0 / (double)g->group[i].size();
+  for (size_t i = 0; i < g->ndof; i++) imult[i] = 1.0 / (double)g->buf[i].size();
 }
 
 // GHelmholtz reg_prod / def_prod / embed_prod (ghelmholtz.ipp:76-378)
@@ -840,7 +843,9 @@ void or_burgers_step(OrGrid *g, int mode, double nu, int order, int nstate, doub
     for (int k = 0; k < nev; k++) up[k] = us[k].data();
     for (int k = nev; k < nstate; k++) up[k] = ust[k];
     for (int k = 0; k < nev; k++) dp[k] = Kout[k].data();
-    or_burgers_rhs(g, mode, nu, nstate, up.data(), dp.data());
+    // heat mode evolves only uin[0] (gburgers.ipp:295-300): its RHS sees the
+    // evolved component alone
+    or_burgers_rhs(g, mode, nu, mode == 2 ? nev : nstate, up.data(), dp.data());
   };
   auto ptrs = [&](std::vector<std::vector<double>> &vv) {
     std::vector<double *> p(vv.size());
@@ -1014,6 +1019,9 @@ namespace {
 void finish_grid(OrGrid &g, double tol_fac) {
   ggfx_match(g, tol_fac * g.minnodedist);
   size_t n = g.ndof;
+  g.buf.assign(n, std::vector<size_t>());
+  for (size_t lid = 0; lid < n; lid++)
+    for (size_t id : g.group[lid]) g.buf[id].push_back(lid);
   g.mass.assign(n, 0.0);
   size_t q = 0;
   for (size_t e = 0; e < g.nelems; e++) {

@@ -1,0 +1,295 @@
+"""GPU parity: HIP path (through the C ABI) against the CPU oracle.
+
+Tolerance for floating point: inf-norm relative error <= 1e-12 (the plain
+FP64 GPU path against the reference CPU operators).  DSS maps, mass and
+inverse mass are compared bit-for-bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import geoflow_amd as ga
+from geoflow_amd import workloads
+from geoflow_amd.device import Mesh
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+CASES = [
+    dict(dim=3, p=7, ne=[3, 2, 2], bdy=[6] * 6),
+    dict(dim=3, p=4, ne=[2, 3, 2], bdy=[0] * 6),
+    dict(dim=2, p=4, ne=[5, 4], bdy=[6] * 4),
+    dict(dim=2, p=6, ne=[3, 3], bdy=[0] * 4),
+    dict(dim=3, p=2, ne=[2, 2, 3], bdy=[6, 6, 0, 0, 6, 6]),
+]
+
+
+def _ids(c):
+    return "d%d_p%d_%s_%s" % (c["dim"], c["p"], "x".join(map(str, c["ne"])), "".join(map(str, c["bdy"][:2])))
+
+
+def make(ctx, c):
+    og = oracle.Grid.box(c["dim"], c["p"], c["ne"], bdy=c["bdy"])
+    bg = ga.BoxGrid(c["dim"], c["p"], c["ne"], bdy=c["bdy"])
+    m = Mesh(ctx, bg)
+    return og, bg, m
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def rand(n, seed):
+    return np.random.default_rng(seed).standard_normal(n)
+
+
+@pytest.mark.parametrize("c", CASES, ids=_ids)
+def test_grefderiv(ctx, c):
+    og, bg, m = make(ctx, c)
+    u = rand(og.ndof, 1)
+    du = dev(u)
+    for idir in range(1, c["dim"] + 1):
+        for tr in (False, True):
+            got = host(m.grefderiv(du, idir, tr))
+            ref = og.grefderiv(u, idir, tr)
+            assert relerr(got, ref) <= TOL, (idir, tr)
+
+
+@pytest.mark.parametrize("c", CASES, ids=_ids)
+def test_deriv_grad_div(ctx, c):
+    og, bg, m = make(ctx, c)
+    u = rand(og.ndof, 2)
+    du = dev(u)
+    g = m.grad(du)
+    for idir in range(1, c["dim"] + 1):
+        ref = og.deriv(u, idir)
+        assert relerr(host(m.deriv(du, idir)), ref) <= TOL
+        assert relerr(host(g[idir - 1]), ref) <= TOL
+    gs = m.grefderivs(du)
+    for idir in range(1, c["dim"] + 1):
+        assert relerr(host(gs[idir - 1]), og.grefderiv(u, idir)) <= TOL
+    vs = [rand(og.ndof, 10 + k) for k in range(c["dim"])]
+    for tr in (True, False):
+        got = host(m.grefdiv([dev(v) for v in vs], tr))
+        assert relerr(got, og.grefdiv(vs, tr)) <= TOL
+
+
+def test_anisotropic_generic_deriv(ctx):
+    c = dict(dim=3, p=[2, 3, 4], ne=[2, 2, 2], bdy=[6] * 6)
+    og, bg, m = make(ctx, c)
+    u = rand(og.ndof, 3)
+    for idir in (1, 2, 3):
+        for tr in (False, True):
+            assert relerr(host(m.grefderiv(dev(u), idir, tr)), og.grefderiv(u, idir, tr)) <= TOL
+
+
+def test_gmtk_integer_kat(ctx):
+    """cdg_gmtk.cpp:77-268: D(i,j) = 2i + N0 j, u(i,j,k) = 2i + N0 j with
+    N = (np+1, np+2, np+3), np = 4; derivatives checked exactly against an
+    explicit Kronecker product."""
+    N = [5, 6, 7]
+    Ds = []
+    for n in N:
+        i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        Ds.append((2.0 * i + N[0] * j).astype(np.float64))
+
+    class G:  # a one-element host grid with integer "derivative" matrices
+        dim = 3
+        nelems = 1
+        basis = [(np.zeros(n), np.ones(n), D) for n, D in zip(N, Ds)]
+        edx = np.ones(3)
+        ejac = np.ones(1)
+        mask = None
+        xyz = None
+    G.N = N
+    nd = int(np.prod(N))
+    G.ggfx_offs = np.arange(nd + 1, dtype=np.int64)
+    G.ggfx_members = np.arange(nd, dtype=np.int64)
+    m = Mesh(ctx, G)
+    kk, jj, ii = np.meshgrid(np.arange(N[2]), np.arange(N[1]), np.arange(N[0]), indexing="ij")
+    u = (2.0 * ii + N[0] * jj).reshape(-1).astype(np.float64)
+    I = [np.eye(n) for n in N]
+    ops = {1: np.kron(I[2], np.kron(I[1], Ds[0])), 2: np.kron(I[2], np.kron(Ds[1], I[0])),
+           3: np.kron(Ds[2], np.kron(I[1], I[0]))}
+    for idir in (1, 2, 3):
+        got = host(m.grefderiv(dev(u), idir))
+        assert np.array_equal(got, ops[idir] @ u), idir
+        gotT = host(m.grefderiv(dev(u), idir, True))
+        assert np.array_equal(gotT, ops[idir].T @ u), idir
+
+
+@pytest.mark.parametrize("c", CASES, ids=_ids)
+def test_mass_and_dss_bitexact(ctx, c):
+    og, bg, m = make(ctx, c)
+    assert np.array_equal(m.get("mass"), og.get("mass"))
+    assert np.array_equal(m.get("imass"), og.get("imass"))
+    assert np.array_equal(m.get("imult"), og.get("imult"))
+    u = rand(og.ndof, 4)
+    for op in ("sum", "smooth", "max", "min"):
+        got = host(m.dss(dev(u), op))
+        assert np.array_equal(got, og.dss(u, op)), op
+    assert relerr(host(m.mass_apply(dev(u))), og.mass_apply(u)) == 0.0
+    assert relerr(host(m.imass_apply(dev(u))), og.imass_apply(u)) == 0.0
+    assert abs(m.integrate(dev(u)) - og.integrate(u)) <= 1e-12 * np.abs(og.mass_apply(np.abs(u))).sum()
+
+
+@pytest.mark.parametrize("c", CASES, ids=_ids)
+def test_helmholtz(ctx, c):
+    og, bg, m = make(ctx, c)
+    u = rand(og.ndof, 5)
+    q = 1.0 + 0.5 * rand(og.ndof, 6) ** 2
+    pf = 1.0 + 0.25 * rand(og.ndof, 7) ** 2
+    du = dev(u)
+    assert relerr(host(m.helmholtz(du)), og.helmholtz(u)) <= TOL
+    assert relerr(host(m.helmholtz(du, 0.1, dev(q))), og.helmholtz(u, 0.1, q)) <= TOL
+    assert relerr(host(m.helmholtz(du, dev(pf), dev(q))), og.helmholtz(u, pf, q)) <= TOL
+
+
+@pytest.mark.parametrize("c", CASES[:3], ids=_ids)
+def test_advect_and_rhs(ctx, c):
+    og, bg, m = make(ctx, c)
+    us = [rand(og.ndof, 20 + k) for k in range(c["dim"])]
+    dus = [dev(x) for x in us]
+    for k in range(c["dim"]):
+        assert relerr(host(m.advect(dus[k], dus)), og.advect(us[k], us)) <= TOL
+    got = m.burgers_rhs(dus, 0.05)
+    ref = og.burgers_rhs(us, 0.05)
+    for k in range(c["dim"]):
+        assert relerr(host(got[k]), ref[k]) <= TOL
+    # pure advection and heat modes
+    got = m.burgers_rhs(dus, 0.02, mode=1)
+    ref = og.burgers_rhs(us, 0.02, mode=1)
+    assert relerr(host(got[0]), ref[0]) <= TOL
+    got = m.burgers_rhs(dus, 0.3, mode=2)
+    ref = og.burgers_rhs(us, 0.3, mode=2)
+    for k in range(c["dim"]):
+        assert relerr(host(got[k]), ref[k]) <= TOL
+
+
+def test_burgers_2d_rk4_100_steps(ctx):
+    """2-D periodic box 16x16, p=4, nu=0.05, RK4 dt=1e-4, 100 steps; the full
+    state is compared after every step."""
+    c = dict(dim=2, p=4, ne=[16, 16], bdy=[6] * 4)
+    og, bg, m = make(ctx, c)
+    us = workloads.gaussian_bumps_2d(bg.xyz[0], bg.xyz[1], seed=12345)
+    ref = [u.copy() for u in us]
+    dus = [dev(u) for u in us]
+    worst = 0.0
+    for s in range(100):
+        m.burgers_step(dus, 0.05, 1e-4, t=s * 1e-4)
+        og.burgers_step(ref, 0.05, 1e-4, t=s * 1e-4)
+        for k in range(2):
+            worst = max(worst, relerr(host(dus[k]), ref[k]))
+        assert worst <= TOL, (s, worst)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_burgers_3d_rk_orders(ctx, order):
+    c = dict(dim=3, p=5, ne=[3, 3, 2], bdy=[6] * 6)
+    og, bg, m = make(ctx, c)
+    us = workloads.fourier_field_3d(*bg.xyz, seed=7, kmax=3)
+    ref = [u.copy() for u in us]
+    dus = [dev(u) for u in us]
+    for s in range(5):
+        m.burgers_step(dus, 0.02, 5e-4, order=order)
+        og.burgers_step(ref, 0.02, 5e-4, order=order)
+    for k in range(3):
+        assert relerr(host(dus[k]), ref[k]) <= TOL
+
+
+def test_burgers_pure_advection_and_heat(ctx):
+    c = dict(dim=2, p=5, ne=[4, 4], bdy=[6] * 4)
+    og, bg, m = make(ctx, c)
+    x, y = bg.xyz
+    u = np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+    cx = np.ones_like(x) * 0.7
+    cy = np.ones_like(x) * -0.3
+    st = [u.copy(), cx, cy]
+    dst = [dev(a) for a in st]
+    for s in range(4):
+        m.burgers_step(dst, 0.0, 1e-3, mode=1)
+        og.burgers_step(st, 0.0, 1e-3, mode=1)
+    assert relerr(host(dst[0]), st[0]) <= TOL
+    st = [u.copy(), (2 * u).copy()]
+    dst = [dev(a) for a in st]
+    for s in range(4):
+        m.burgers_step(dst, 0.1, 1e-3, mode=2)
+        og.burgers_step(st, 0.1, 1e-3, mode=2)
+    for k in range(2):
+        assert relerr(host(dst[k]), st[k]) <= TOL
+
+
+def test_cg_poisson_cdg_cg(ctx):
+    """cdg_cg.cpp:252-320: 2-D 4x4 box, p=6, Dirichlet; L u = -M f with
+    f = 6xy(1-y) - 2x^3, exact u = y(1-y)x^3; boundary values ub; INF norm,
+    tol 1e-7, maxit 256.  Pass: mean-square error < 1e-12 and no error code."""
+    c = dict(dim=2, p=6, ne=[4, 4], bdy=[0] * 4)
+    og, bg, m = make(ctx, c)
+    x, y = bg.xyz
+    f = 6.0 * x * y * (1.0 - y) - 2.0 * x ** 3
+    ua = y * (1.0 - y) * x ** 3
+    eps = 100 * np.finfo(float).eps
+    ub = np.zeros_like(x)
+    e = np.abs(x - 1.0) <= eps
+    ub[e] = (y * (1.0 - y))[e]
+    b = -og.mass_apply(f)
+    xg, info = m.cg_solve(dev(b), xb=dev(ub), p=1.0, maxit=256, tol=1e-7, normtype=0)
+    xo, ret, it, res = og.cg_solve(b, xb=ub, p=1.0, maxit=256, tol=1e-7, normtype=0)
+    xgh = host(xg)
+    assert info["status"] == 0 and ret == 0
+    err = og.integrate((xgh - ua) ** 2) / og.volume
+    assert err < 1e-12
+    # both stop at the same INF-norm tolerance, so they agree to ~tol
+    assert relerr(xgh, xo) <= 1e-5
+    assert abs(info["iters"] - it) <= 1
+    # parity proper: the same number of iterations on both sides (tol = 0
+    # would run none: the loop starts from rnorm = 10*rtol, gcg.ipp:295).
+    # CG amplifies summation-order differences of the dot products as the
+    # iteration count grows, so the iterate check uses the first 10.
+    xg2, info2 = m.cg_solve(dev(b), xb=dev(ub), p=1.0, maxit=10, tol=1e-300, normtype=0)
+    xo2, ret2, it2, res2 = og.cg_solve(b, xb=ub, p=1.0, maxit=10, tol=1e-300, normtype=0)
+    assert info2["iters"] == it2 == 10
+    assert relerr(host(xg2), xo2) <= 1e-11
+    assert relerr(info2["resid"][:10], res2[:10]) <= 1e-11
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_cg_helmholtz_fixed_iterations(ctx, precond):
+    """(M + 0.1 L) x = b on a small 3-D Dirichlet box: the same number of
+    iterations on both sides, x compared to the oracle."""
+    c = dict(dim=3, p=5, ne=[3, 3, 3], bdy=[0] * 6)
+    og, bg, m = make(ctx, c)
+    f = workloads.unique_normal(bg, seed=67890)
+    q = np.ones(og.ndof)
+    b = og.dss(og.mass_apply(f), "sum")
+    xg, info = m.cg_solve(dev(b), xb=dev(np.zeros(og.ndof)), p=0.1, q=dev(q), maxit=25,
+                          tol=1e-30, normtype=0, precond=precond)
+    xo, ret, it, res = og.cg_solve(b, xb=np.zeros(og.ndof), p=0.1, q=q, maxit=25, tol=1e-30,
+                                   normtype=0, precond=precond)
+    assert info["iters"] == it == 25
+    assert relerr(host(xg), xo) <= 1e-11
+    assert relerr(info["resid"][:25], res[:25]) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_cg_helmholtz_converged(ctx, precision):
+    c = dict(dim=3, p=7, ne=[3, 3, 2], bdy=[0] * 6)
+    og, bg, m = make(ctx, c)
+    f = workloads.unique_normal(bg, seed=67890)
+    q = np.ones(og.ndof)
+    b = og.dss(og.mass_apply(f), "sum")
+    xg, info = m.cg_solve(dev(b), xb=dev(np.zeros(og.ndof)), p=0.1, q=dev(q), maxit=2000,
+                          tol=1e-10, normtype=4, precond=1, precision=precision)
+    xo, ret, it, res = og.cg_solve(b, xb=np.zeros(og.ndof), p=0.1, q=q, maxit=2000, tol=1e-12,
+                                   normtype=4, precond=1)
+    assert info["status"] == 0 and ret == 0
+    assert info["rnorm"] <= 1e-10 * info["rnorm0"]
+    assert relerr(host(xg), xo) <= 1e-8
