@@ -141,11 +141,12 @@ def main():
         log("Dirichlet box grid + GGFX maps: %.1f s" % (time.perf_counter() - t))
         m = Mesh(ctx, cg_grid)
         f = torch.from_numpy(workloads.unique_normal(cg_grid, seed=67890)).cuda()
-        b = m.dss(m.mass_apply(f), "sum")
+        b = m.mass_apply(f)  # local RHS; GCG assembles b - A x (so the system's RHS is DSS_sum(M f))
         q = torch.ones(ndof, dtype=torch.float64, device="cuda")
         mask = torch.from_numpy(cg_grid.mask).cuda()
         zero = torch.zeros_like(b)
-        bnorm = float(torch.sqrt(((b * mask) ** 2 * m.imult()).sum()))
+        bA = m.dss(b.clone(), "sum") * mask
+        bnorm = float(torch.sqrt((bA ** 2 * m.imult()).sum()))
 
         def true_relres(x):
             r = b - m.helmholtz(x, 0.1, q)

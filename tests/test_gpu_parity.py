@@ -269,7 +269,7 @@ def test_cg_helmholtz_fixed_iterations(ctx, precond):
     og, bg, m = make(ctx, c)
     f = workloads.unique_normal(bg, seed=67890)
     q = np.ones(og.ndof)
-    b = og.dss(og.mass_apply(f), "sum")
+    b = og.mass_apply(f)  # local RHS: GCG assembles b - A x
     xg, info = m.cg_solve(dev(b), xb=dev(np.zeros(og.ndof)), p=0.1, q=dev(q), maxit=25,
                           tol=1e-30, normtype=0, precond=precond)
     xo, ret, it, res = og.cg_solve(b, xb=np.zeros(og.ndof), p=0.1, q=q, maxit=25, tol=1e-30,
@@ -285,7 +285,7 @@ def test_cg_helmholtz_converged(ctx, precision):
     og, bg, m = make(ctx, c)
     f = workloads.unique_normal(bg, seed=67890)
     q = np.ones(og.ndof)
-    b = og.dss(og.mass_apply(f), "sum")
+    b = og.mass_apply(f)  # local RHS: GCG assembles b - A x
     xg, info = m.cg_solve(dev(b), xb=dev(np.zeros(og.ndof)), p=0.1, q=dev(q), maxit=2000,
                           tol=1e-10, normtype=4, precond=1, precision=precision)
     xo, ret, it, res = og.cg_solve(b, xb=np.zeros(og.ndof), p=0.1, q=q, maxit=2000, tol=1e-12,
