@@ -30,6 +30,9 @@ struct gf_mesh_s {
   double *D[3] = {nullptr, nullptr, nullptr};
   double *W[3] = {nullptr, nullptr, nullptr};
   float *D32 = nullptr, *W32 = nullptr;  // isotropic FP32 copies
+  // Per-direction row-major / column-major copies of D for the pencil kernels
+  double *pDR[3] = {nullptr, nullptr, nullptr}, *pDC[3] = {nullptr, nullptr, nullptr};
+  float *pDR32[3] = {nullptr, nullptr, nullptr}, *pDC32[3] = {nullptr, nullptr, nullptr};
   // geometry (device)
   double *edx = nullptr;   // nelems*dim
   double *ejac = nullptr;  // nelems
@@ -67,6 +70,8 @@ int gf_grefderiv_t(gf_mesh *m, int idir, int trans, const double *u, double *du,
 int gf_mul_t(gf_mesh *m, const double *a, const double *b, double *out);
 int gf_dot_t(gf_mesh *m, const double *a, const double *b, size_t n, double *result);
 int gf_axpy_t(gf_mesh *m, double a, const double *x, double *y);
+int gf_helm3c(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
+              double *uo);
 
 // Workspace: returns a device buffer of at least `bytes` for slot `slot`.
 void *gf_ws(gf_mesh *m, int slot, size_t bytes);

@@ -562,6 +562,7 @@ int gf_dss_f32(gf_mesh *m, int op, float *u) {
 int gf_helm_t(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
               double *uo) {
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Helmholtz needs isotropic order");
+  if (m->dim == 3 && m->gtype == GF_GE_REGULAR) return gf_helm3c(m, p, pf, qf, u, uo);
   return GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_helm, m, p, pf, qf, u, uo);
 }
 
@@ -657,6 +658,22 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     for (int q = 0; q < m->Niso; q++) Wf[q] = (float)d->W[0][q];
     UP(m->D32, Df.data(), Df.size());
     UP(m->W32, Wf.data(), Wf.size());
+    int Nn = m->Niso;
+    std::vector<double> Rm((size_t)Nn * Nn), Cm((size_t)Nn * Nn);
+    std::vector<float> Rf((size_t)Nn * Nn), Cf((size_t)Nn * Nn);
+    for (int l = 0; l < Nn; l++)
+      for (int q = 0; q < Nn; q++) {
+        Rm[l * Nn + q] = d->D[0][l + q * Nn];  // D(l,q)
+        Cm[l * Nn + q] = d->D[0][q + l * Nn];  // D(q,l)
+        Rf[l * Nn + q] = (float)Rm[l * Nn + q];
+        Cf[l * Nn + q] = (float)Cm[l * Nn + q];
+      }
+    for (int c = 0; c < 3; c++) {
+      UP(m->pDR[c], Rm.data(), Rm.size());
+      UP(m->pDC[c], Cm.data(), Cm.size());
+      UP(m->pDR32[c], Rf.data(), Rf.size());
+      UP(m->pDC32[c], Cf.data(), Cf.size());
+    }
   }
   size_t n = m->ndof;
   if (d->gtype == GF_GE_REGULAR) {
@@ -788,6 +805,12 @@ int gf_mesh_destroy(gf_mesh *m) {
     F(m->xyz[k]);
   }
   for (int k = 0; k < 9; k++) F(m->ndx[k]);
+  for (int k = 0; k < 3; k++) {
+    F(m->pDR[k]);
+    F(m->pDC[k]);
+    F(m->pDR32[k]);
+    F(m->pDC32[k]);
+  }
   F(m->D32); F(m->W32); F(m->edx); F(m->ejac); F(m->edx32); F(m->ejac32); F(m->njac);
   F(m->mask); F(m->mass); F(m->imass); F(m->imult); F(m->mask32); F(m->imult32);
   F(m->goff); F(m->gmem); F(m->red); F(m->scal);

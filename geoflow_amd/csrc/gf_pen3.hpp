@@ -1,0 +1,164 @@
+// gf_pen3.hpp -- pencil-per-thread 3-D element operator building block.
+//
+// A regular element's Laplacian separates by direction,
+//     L u = sum_d D_d^T ( M G_dd (D_d u) ),
+// so each direction can be handled by threads that own whole pencils along
+// it.  An element is worked by N*N threads; thread (a,b) owns
+//   * the z-column (i=a, j=b) -- loaded from HBM, z-part done in registers,
+//   * the x-pencil (j=a, k=b) and the y-pencil (i=a, k=b) -- read once from
+//     the element staged in LDS; gradient, scaling and D^T are all applied
+//     in registers.
+// D(l,m) entries are wave-uniform in every contraction (the loop indices are
+// compile-time), so they come from scalar loads; no thread keeps D in VGPRs.
+// LDS traffic is ~7 accesses per node.  The LDS element uses row stride N+1
+// and slice stride N(N+1) (= 8 mod 16 doubles for N=8), which makes the
+// column writes and both pencil reads (nearly) bank-conflict free.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gf {
+
+// Six copies of D for the pencil passes: R[d] row-major (R[d][l*N+m] = D(l,m)),
+// C[d] column-major (C[d][l*N+m] = D(m,l)), d = x, y, z.
+template <typename T>
+struct PenD {
+  const T *R[3];
+  const T *C[3];
+};
+
+template <int N>
+struct Pen3 {
+  // Each pass reads D through its own copy (rows contiguous for the gradient,
+  // columns contiguous for D^T), so no D value is shared between passes and
+  // only one row or column needs to be live in scalar registers at a time.
+  static constexpr int NN = N * N * N;
+  static constexpr int N2 = N * N;
+  static constexpr int P = N + 1;       // row stride (j)
+  static constexpr int SK = N * (N + 1);  // slice stride (k)
+  static constexpr int SZ = N * SK;       // LDS element size
+  static constexpr int EPB = N2 >= 64 ? 1 : 64 / N2;  // elements per workgroup
+  static constexpr int THREADS = EPB * N2;
+  __device__ static constexpr int at(int i, int j, int k) { return i + j * P + k * SK; }
+};
+
+// Computes, for the column (i=a, j=b) owned by the calling thread,
+//   post(k, Lu_k, M_k),   Lu = ((x-part + y-part) + z-part)
+// where each part is D_d^T (M * ((D_d u) * G_d) [* p]) summed in ascending m,
+// i.e. the order of GHelmholtz::reg_prod / GGrid::compute_grefdiv
+// (ghelmholtz.ipp:308-378, ggrid.ipp:1847-1927), and M_k is the lumped mass
+// ((W_k W_j) W_i) Jac (gmass.ipp:200-210).
+//   ru    the thread's column of u (nodes (a,b,0..N-1))
+//   su,sd two LDS element buffers (Pen3<N>::SZ each) owned by this element
+//   pe    p field restricted to this element, or NULL
+// Optional hooks for advection-type uses:
+//   xcb(i, ux, M) at x-pencil node (i,a,b);  ycb(j, uy, M) at y-pencil node
+//   (a,j,b);  zcb(k, uz, M) at column node (a,b,k)  -- reference derivatives.
+// All threads of the workgroup must call this (it contains barriers).
+template <int N, typename T, typename XCB, typename YCB, typename ZCB, typename POST>
+__device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, const PenD<T> &Dp,
+                                        const T *__restrict__ W, int a, int b, T jac, T g0, T g1,
+                                        T g2, const T *__restrict__ pe, XCB &&xcb, YCB &&ycb,
+                                        ZCB &&zcb, POST &&post) {
+  using S = Pen3<N>;
+  const T *__restrict__ Rx = Dp.R[0], *__restrict__ Ry = Dp.R[1], *__restrict__ Rz = Dp.R[2];
+  const T *__restrict__ Cx = Dp.C[0], *__restrict__ Cy = Dp.C[1], *__restrict__ Cz = Dp.C[2];
+  const T Wa = W[a], Wb = W[b];
+#pragma unroll
+  for (int k = 0; k < N; k++) su[S::at(a, b, k)] = ru[k];
+  // z-part on the column, in registers
+  T dz[N];
+  {
+    T gz[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      T uz = T(0);
+#pragma unroll
+      for (int m = 0; m < N; m++) uz = fma(Rz[k * N + m], ru[m], uz);
+      const T M = W[k] * Wb * Wa * jac;
+      zcb(k, uz, M);
+      T f = uz * g2;
+      if (pe) f *= pe[a + b * N + k * N * N];
+      gz[k] = M * f;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      T c = T(0);
+#pragma unroll
+      for (int m = 0; m < N; m++) c = fma(Cz[k * N + m], gz[m], c);
+      dz[k] = c;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  __syncthreads();
+  // x-part on the x-pencil (j=a, k=b)
+  {
+    T xp[N], f[N];
+#pragma unroll
+    for (int m = 0; m < N; m++) xp[m] = su[S::at(m, a, b)];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      T ux = T(0);
+#pragma unroll
+      for (int m = 0; m < N; m++) ux = fma(Rx[i * N + m], xp[m], ux);
+      const T M = Wb * Wa * W[i] * jac;
+      xcb(i, ux, M);
+      T g = ux * g0;
+      if (pe) g *= pe[i + a * N + b * N * N];
+      f[i] = M * g;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      T s = T(0);
+#pragma unroll
+      for (int m = 0; m < N; m++) s = fma(Cx[i * N + m], f[m], s);
+      sd[S::at(i, a, b)] = s;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // y-part on the y-pencil (i=a, k=b), kept in registers until su is free
+  T dy[N];
+  {
+    T yp[N], f[N];
+#pragma unroll
+    for (int m = 0; m < N; m++) yp[m] = su[S::at(a, m, b)];
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+      T uy = T(0);
+#pragma unroll
+      for (int m = 0; m < N; m++) uy = fma(Ry[j * N + m], yp[m], uy);
+      const T M = Wb * W[j] * Wa * jac;
+      ycb(j, uy, M);
+      T g = uy * g1;
+      if (pe) g *= pe[a + j * N + b * N * N];
+      f[j] = M * g;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+      T s = T(0);
+#pragma unroll
+      for (int m = 0; m < N; m++) s = fma(Cy[j * N + m], f[m], s);
+      dy[j] = s;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < N; j++) su[S::at(a, j, b)] = dy[j];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const T M = W[k] * Wb * Wa * jac;
+    post(k, (sd[S::at(a, b, k)] + su[S::at(a, b, k)]) + dz[k], M);
+  }
+}
+
+struct NoPenCB {
+  template <typename T>
+  __device__ __forceinline__ void operator()(int, T, T) const {}
+};
+
+}  // namespace gf

@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include "gf_col3.hpp"
+#include "gf_pen3.hpp"
 #include "gf_kernels.hpp"
 
 using gf::kThreads;
@@ -23,6 +25,11 @@ inline int nblocks(size_t n, int per = kThreads) {
   if (b > 65536) b = 65536;
   return (int)std::max<size_t>(b, 1);
 }
+
+// Solver scalars on the device, scal[]: 0 rho, 1 alpha, 2 beta, 3 rnorm,
+// 4 rtol, 5 wq, 6 iter, 7 maxit, 8 done, 9 first, 10 volume, 11 n.
+enum { S_RHO = 0, S_ALPHA, S_BETA, S_RNORM, S_RTOL, S_WQ, S_ITER, S_MAXIT, S_DONE, S_FIRST,
+       S_VOL, S_N };
 
 #define GRID_LOOP(g, n)                                                     \
   for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < (n); \
@@ -152,6 +159,374 @@ __global__ __launch_bounds__(kThreads) void k_burgers_stage(StageArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Column-per-thread 3-D kernels (see gf_col3.hpp)
+// ---------------------------------------------------------------------------
+inline gf::PenD<double> pend_d(gf_mesh *m) {
+  gf::PenD<double> p;
+  for (int c = 0; c < 3; c++) {
+    p.R[c] = m->pDR[c];
+    p.C[c] = m->pDC[c];
+  }
+  return p;
+}
+inline gf::PenD<float> pend_f(gf_mesh *m) {
+  gf::PenD<float> p;
+  for (int c = 0; c < 3; c++) {
+    p.R[c] = m->pDR32[c];
+    p.C[c] = m->pDC32[c];
+  }
+  return p;
+}
+
+// Thread/element mapping shared by the column kernels.
+template <int N>
+struct ColIdx {
+  int el, i, j, r;
+  size_t e;
+  bool active;
+  __device__ __forceinline__ ColIdx(size_t nelems) {
+    using C = gf::Col3<N>;
+    el = threadIdx.x / C::N2;
+    r = threadIdx.x - el * C::N2;
+    i = r % N;
+    j = r / N;
+    size_t eg = (size_t)blockIdx.x * C::EPB + el;
+    active = el < C::EPB && eg < nelems;
+    e = active ? eg : (nelems - 1);
+  }
+};
+
+// Generic Helmholtz apply, 3-D regular elements:
+// uo = p * L u + q * M u, with p scalar or a field (reg_prod, ghelmholtz.ipp:308-378).
+template <int N, typename T>
+__global__ __launch_bounds__(gf::Col3<N>::THREADS, gf::ColWaves<T>::value) void k_helm3c(
+    const T *__restrict__ u, T *__restrict__ uo, const T *__restrict__ D, const T *__restrict__ W,
+    const T *__restrict__ edx, const T *__restrict__ ejac, T p, const T *__restrict__ pf,
+    const T *__restrict__ qf, size_t nelems) {
+  using C = gf::Col3<N>;
+  __shared__ T su[C::EPB][C::SL], sgx[C::EPB][C::SL], sgy[C::EPB][C::SL];
+  __shared__ gf::DShared<N, T> ds;
+  ds.stage(D);
+  ColIdx<N> t(nelems);
+  const size_t col = t.e * C::NN + t.r;
+  T ru[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) ru[k] = u[col + k * C::N2];
+  gf::DRegs<N, T> dr;
+  dr.load(D, t.i, t.j);
+  __syncthreads();
+  const T jac = ejac[t.e];
+  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
+  auto post = [&](int k, T Lu, T M) {
+    if (!t.active) return;
+    T acc = Lu;
+    if (!pf && p != T(1)) acc *= p;
+    if (qf) {
+      T tq = M * ru[k];
+      tq *= qf[col + k * C::N2];
+      acc += tq;
+    }
+    uo[col + k * C::N2] = acc;
+  };
+  gf::col_op3<N, T>(ru, su[t.el], sgx[t.el], sgy[t.el], dr, ds, W, t.i, t.j, jac, f0 * f0, f1 * f1,
+                    f2 * f2, pf ? pf + col : (const T *)nullptr, gf::NoZ(), gf::NoXY(), post);
+}
+
+// CG direction update + operator apply (see k_cg_apply for the contract),
+// 3-D column form; the Dirichlet mask is applied to q here (the mask is
+// constant over each coincidence group, so DSS(q) * mask = DSS(q * mask)).
+struct CGApply3Args {
+  const void *r, *dinv;
+  void *w, *q;
+  const void *D, *W, *edx, *ejac, *qf, *mask;
+  const void *DR[3], *DC[3];  // pencil copies of D (same precision as T)
+  double p;
+  double *scal, *part;
+  size_t nelems;
+};
+
+template <int N, typename T>
+__global__ __launch_bounds__(gf::Col3<N>::THREADS, gf::ColWaves<T>::value) void k_cg_apply3c(
+    CGApply3Args a) {
+  using C = gf::Col3<N>;
+  __shared__ T su[C::EPB][C::SL], sgx[C::EPB][C::SL], sgy[C::EPB][C::SL];
+  __shared__ gf::DShared<N, T> ds;
+  __shared__ double sred[4];
+  const double *scal = a.scal;
+  if (scal[S_DONE] != 0.0) return;
+  const T *r = (const T *)a.r, *dinv = (const T *)a.dinv, *D = (const T *)a.D,
+          *W = (const T *)a.W, *edx = (const T *)a.edx, *ejac = (const T *)a.ejac,
+          *qf = (const T *)a.qf, *mask = (const T *)a.mask;
+  T *w = (T *)a.w, *qv = (T *)a.q;
+  const bool first = scal[S_FIRST] != 0.0;
+  const T beta = (T)scal[S_BETA];
+  ColIdx<N> t(a.nelems);
+  const size_t col = t.e * C::NN + t.r;
+  T ru[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const size_t g = col + k * C::N2;
+    const T z = dinv ? dinv[g] * r[g] : r[g];
+    const T wn = first ? z : beta * w[g] + T(1) * z;
+    ru[k] = wn;
+    if (t.active) w[g] = wn;
+  }
+  ds.stage(D);
+  gf::DRegs<N, T> dr;
+  dr.load(D, t.i, t.j);
+  __syncthreads();
+  const T jac = ejac[t.e];
+  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
+  double dot = 0.0;
+  const T p = (T)a.p;
+  auto post = [&](int k, T Lu, T M) {
+    if (!t.active) return;
+    const size_t g = col + k * C::N2;
+    T acc = Lu;
+    if (p != T(1)) acc *= p;
+    if (qf) {
+      T tq = M * ru[k];
+      tq *= qf[g];
+      acc += tq;
+    }
+    if (mask) acc *= mask[g];
+    qv[g] = acc;
+    dot += (double)ru[k] * (double)acc;
+  };
+  gf::col_op3<N, T>(ru, su[t.el], sgx[t.el], sgy[t.el], dr, ds, W, t.i, t.j, jac, f0 * f0, f1 * f1,
+                    f2 * f2, (const T *)nullptr, gf::NoZ(), gf::NoXY(), post);
+  double bs = gf::block_reduce<false>(dot, sred);
+  if (threadIdx.x == 0) a.part[blockIdx.x] = bs;
+}
+
+// Fused Burgers stage, 3-D column form (see k_burgers_stage for the contract).
+template <int N>
+__global__ __launch_bounds__(gf::Col3<N>::THREADS, 3) void k_burgers_stage3c(StageArgs a) {
+  using C = gf::Col3<N>;
+  __shared__ double su[C::EPB][C::SL], sgx[C::EPB][C::SL], sgy[C::EPB][C::SL];
+  __shared__ gf::DShared<N, double> ds;
+  ds.stage(a.D);
+  ColIdx<N> t(a.nelems);
+  const size_t col = t.e * C::NN + t.r;
+  double ru[3][N];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const size_t g = col + k * C::N2;
+      const double ui = a.uin[c][g];
+      double v = ui;
+      if (a.kprev[c]) {
+        const double kp = a.kprev[c][g];
+        v = ui + a.bh * kp;
+        if (a.acc[c] && t.active) a.acc[c][g] = a.accsrc[c][g] + kp * a.ch;
+      }
+      ru[c][k] = v;
+    }
+  }
+  gf::DRegs<N, double> dr;
+  dr.load(a.D, t.i, t.j);
+  __syncthreads();
+  const double jac = a.ejac[t.e];
+  const double rx = a.edx[t.e * 3], ry = a.edx[t.e * 3 + 1], rz = a.edx[t.e * 3 + 2];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double zpart[N];
+    double adv = 0.0;
+    // GAdvect::apply: po = (D1 p rx) u0 ; += (D2 p ry) u1 ; += (D3 p rz) u2 ; *= M
+    auto zcb = [&](int k, double uz, double) { zpart[k] = (uz * rz) * ru[2][k]; };
+    auto xycb = [&](int k, double ux, double uy, double M) {
+      double po = (ux * rx) * ru[0][k];
+      po += (uy * ry) * ru[1][k];
+      po += zpart[k];
+      adv = po * M;
+    };
+    auto post = [&](int k, double L, double) {
+      if (!t.active) return;
+      const size_t g = col + k * C::N2;
+      if (a.nu != 1.0) L *= a.nu;
+      const double urhs = -1.0 * L + -1.0 * adv;
+      a.kout[c][g] = a.imass[g] * urhs;
+    };
+    gf::col_op3<N, double>(ru[c], su[t.el], sgx[t.el], sgy[t.el], dr, ds, a.W, t.i, t.j, jac,
+                           rx * rx, ry * ry, rz * rz, (const double *)nullptr, zcb, xycb, post);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pencil-form 3-D kernels (see gf_pen3.hpp): Helmholtz apply and CG apply.
+// ---------------------------------------------------------------------------
+template <int N, typename T>
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_helm3p(
+    const T *__restrict__ u, T *__restrict__ uo, gf::PenD<T> D, const T *__restrict__ W,
+    const T *__restrict__ edx, const T *__restrict__ ejac, T p, const T *__restrict__ pf,
+    const T *__restrict__ qf, size_t nelems) {
+  using S = gf::Pen3<N>;
+  __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
+  ColIdx<N> t(nelems);
+  const size_t col = t.e * S::NN + t.r;
+  T ru[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) ru[k] = u[col + k * S::N2];
+  const T jac = ejac[t.e];
+  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
+  auto post = [&](int k, T Lu, T M) {
+    if (!t.active) return;
+    T acc = Lu;
+    if (!pf && p != T(1)) acc *= p;
+    if (qf) {
+      T tq = M * ru[k];
+      tq *= qf[col + k * S::N2];
+      acc += tq;
+    }
+    uo[col + k * S::N2] = acc;
+  };
+  gf::pen_op3<N, T>(ru, su[t.el], sd[t.el], D, W, t.i, t.j, jac, f0 * f0, f1 * f1, f2 * f2,
+                    pf ? pf + t.e * S::NN : (const T *)nullptr, gf::NoPenCB(), gf::NoPenCB(),
+                    gf::NoPenCB(), post);
+}
+
+template <int N, typename T>
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
+    const T *__restrict__ r, const T *__restrict__ dinv, T *__restrict__ w, T *__restrict__ qv,
+    gf::PenD<T> Dp, const T *__restrict__ W, const T *__restrict__ edx,
+    const T *__restrict__ ejac, const T *__restrict__ qf, const T *__restrict__ mask, double pd,
+    const double *__restrict__ scal, double *__restrict__ part, size_t nelems) {
+  using S = gf::Pen3<N>;
+  __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
+  __shared__ double sred[4];
+  if (scal[S_DONE] != 0.0) return;
+  const bool first = scal[S_FIRST] != 0.0;
+  const T beta = (T)scal[S_BETA];
+  ColIdx<N> t(nelems);
+  const size_t col = t.e * S::NN + t.r;
+  T ru[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const size_t g = col + k * S::N2;
+    const T z = dinv ? dinv[g] * r[g] : r[g];
+    const T wn = first ? z : beta * w[g] + T(1) * z;
+    ru[k] = wn;
+    if (t.active) w[g] = wn;
+  }
+  const T jac = ejac[t.e];
+  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
+  double dot = 0.0;
+  const T p = (T)pd;
+  auto post = [&](int k, T Lu, T M) {
+    if (!t.active) return;
+    const size_t g = col + k * S::N2;
+    T acc = Lu;
+    if (p != T(1)) acc *= p;
+    if (qf) {
+      T tq = M * ru[k];
+      tq *= qf[g];
+      acc += tq;
+    }
+    if (mask) acc *= mask[g];
+    qv[g] = acc;
+    dot += (double)ru[k] * (double)acc;
+  };
+  gf::pen_op3<N, T>(ru, su[t.el], sd[t.el], Dp, W, t.i, t.j, jac, f0 * f0, f1 * f1, f2 * f2,
+                    (const T *)nullptr, gf::NoPenCB(), gf::NoPenCB(), gf::NoPenCB(), post);
+  double bs = gf::block_reduce<false>(dot, sred);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+template <int N, typename T>
+int launch_helm3p(gf_mesh *m, T p, const T *pf, const T *qf, const T *u, T *uo,
+                  const gf::PenD<T> &D, const T *W, const T *edx, const T *ejac) {
+  using S = gf::Pen3<N>;
+  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  hipLaunchKernelGGL((k_helm3p<N, T>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream, u, uo, D, W,
+                     edx, ejac, p, pf, qf, m->nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int N, typename T>
+int launch_cg_apply3p(gf_mesh *m, const CGApply3Args &a) {
+  using S = gf::Pen3<N>;
+  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  gf::PenD<T> Dp;
+  for (int c = 0; c < 3; c++) {
+    Dp.R[c] = (const T *)a.DR[c];
+    Dp.C[c] = (const T *)a.DC[c];
+  }
+  hipLaunchKernelGGL((k_cg_apply3p<N, T>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream,
+                     (const T *)a.r, (const T *)a.dinv, (T *)a.w, (T *)a.q, Dp, (const T *)a.W,
+                     (const T *)a.edx, (const T *)a.ejac, (const T *)a.qf, (const T *)a.mask, a.p,
+                     (const double *)a.scal, a.part, a.nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int N, typename T>
+int launch_helm3c(gf_mesh *m, T p, const T *pf, const T *qf, const T *u, T *uo, const T *D,
+                  const T *W, const T *edx, const T *ejac) {
+  using C = gf::Col3<N>;
+  int nb = (int)((m->nelems + C::EPB - 1) / C::EPB);
+  hipLaunchKernelGGL((k_helm3c<N, T>), dim3(nb), dim3(C::THREADS), 0, m->ctx->stream, u, uo, D, W,
+                     edx, ejac, p, pf, qf, m->nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int N, typename T>
+int launch_cg_apply3c(gf_mesh *m, const CGApply3Args &a) {
+  using C = gf::Col3<N>;
+  int nb = (int)((m->nelems + C::EPB - 1) / C::EPB);
+  hipLaunchKernelGGL((k_cg_apply3c<N, T>), dim3(nb), dim3(C::THREADS), 0, m->ctx->stream, a);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int N>
+int launch_stage3c(gf_mesh *m, const StageArgs &a) {
+  using C = gf::Col3<N>;
+  int nb = (int)((m->nelems + C::EPB - 1) / C::EPB);
+  hipLaunchKernelGGL((k_burgers_stage3c<N>), dim3(nb), dim3(C::THREADS), 0, m->ctx->stream, a);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+// Number of workgroups (= partial sums) the 3-D column apply launches.
+inline int col_blocks(gf_mesh *m) {
+  int N = m->Niso, N2 = N * N, epb = N2 >= 64 ? 1 : 64 / N2;
+  return (int)((m->nelems + epb - 1) / epb);
+}
+
+#define GF_DISPATCH_N3(NV, FN, ...)                   \
+  [&]() -> int {                                      \
+    switch (NV) {                                     \
+      case 2: return FN<2>(__VA_ARGS__);              \
+      case 3: return FN<3>(__VA_ARGS__);              \
+      case 4: return FN<4>(__VA_ARGS__);              \
+      case 5: return FN<5>(__VA_ARGS__);              \
+      case 6: return FN<6>(__VA_ARGS__);              \
+      case 7: return FN<7>(__VA_ARGS__);              \
+      case 8: return FN<8>(__VA_ARGS__);              \
+      case 9: return FN<9>(__VA_ARGS__);              \
+      case 10: return FN<10>(__VA_ARGS__);            \
+      default: return GF_ERR_UNSUPPORTED;             \
+    }                                                 \
+  }()
+
+template <int N>
+int helm3c_d(gf_mesh *m, double p, const double *pf, const double *qf, const double *u, double *uo) {
+  return launch_helm3p<N, double>(m, p, pf, qf, u, uo, pend_d(m), m->W[0], m->edx, m->ejac);
+}
+template <int N>
+int helm3c_f(gf_mesh *m, float p, const float *qf, const float *u, float *uo) {
+  return launch_helm3p<N, float>(m, p, (const float *)nullptr, qf, u, uo, pend_f(m), m->W32,
+                                 m->edx32, m->ejac32);
+}
+template <int N>
+int cgapply3c_d(gf_mesh *m, const CGApply3Args &a) { return launch_cg_apply3p<N, double>(m, a); }
+template <int N>
+int cgapply3c_f(gf_mesh *m, const CGApply3Args &a) { return launch_cg_apply3p<N, float>(m, a); }
+
 // u3 = uin + bh*k ; acc = accsrc + ch*k   (final-stage input of step_b, :315-320)
 __global__ void k_final_input(double *__restrict__ u3, const double *__restrict__ uin,
                               const double *__restrict__ k, double bh, double *acc,
@@ -177,8 +552,6 @@ int launch_stage(gf_mesh *m, const StageArgs &a) {
 // scal[]: 0 rho, 1 alpha, 2 beta, 3 rnorm, 4 rtol, 5 wq, 6 iter, 7 maxit,
 // 8 done, 9 first, 10 volume, 11 n.
 // ---------------------------------------------------------------------------
-enum { S_RHO = 0, S_ALPHA, S_BETA, S_RNORM, S_RTOL, S_WQ, S_ITER, S_MAXIT, S_DONE, S_FIRST,
-       S_VOL, S_N };
 
 struct CGApplyArgs {
   const void *r;      // T
@@ -274,14 +647,30 @@ __global__ void k_dss_cg(T *__restrict__ u, const int *__restrict__ goff,
   }
 }
 
-// alpha = rho / (w.q)
-__global__ void k_cg_alpha(double *scal, const double *__restrict__ part, int np) {
-  __shared__ double sred[kThreads / 64];
+// alpha = rho / (w.q).  One block of 1024 threads reduces the per-block
+// partials of the apply kernel in a fixed order (deterministic).
+constexpr int kRedThreads = 1024;
+__global__ __launch_bounds__(kRedThreads) void k_cg_alpha(double *scal,
+                                                          const double *__restrict__ part, int np) {
+  __shared__ double sred[kRedThreads / 64];
   if (scal[S_DONE] != 0.0) return;
-  double v = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
-  double wq = gf::block_reduce<false>(v, sred);
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * kRedThreads < np; i += 4 * kRedThreads) {
+    v0 += part[i];
+    v1 += part[i + kRedThreads];
+    v2 += part[i + 2 * kRedThreads];
+    v3 += part[i + 3 * kRedThreads];
+  }
+  for (; i < np; i += kRedThreads) v0 += part[i];
+  double v = (v0 + v1) + (v2 + v3);
+  v = gf::wave_sum(v);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sred[wid] = v;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double wq = sred[0];
+    for (int w = 1; w < kRedThreads / 64; w++) wq += sred[w];
     scal[S_WQ] = wq;
     scal[S_ALPHA] = scal[S_RHO] / wq;
   }
@@ -291,7 +680,7 @@ __global__ void k_cg_alpha(double *scal, const double *__restrict__ part, int np
 template <typename T, int NORM>
 __global__ void k_cg_update(T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r,
                             const T *__restrict__ qv, const T *__restrict__ mask,
-                            const T *__restrict__ dinv, const T *__restrict__ imult,
+                            const T *__restrict__ dinv, const float *__restrict__ imult,
                             const double *__restrict__ mass, double *scal,
                             double *__restrict__ part, size_t n) {
   __shared__ double sred[kThreads / 64];
@@ -364,7 +753,7 @@ __global__ void k_cg_beta(double *scal, const double *__restrict__ part, int np,
 // Generic reductions used at solver set-up: partial (sum_i a*b*c) and norm.
 template <typename T>
 __global__ void k_setup_red(const T *__restrict__ r, const T *__restrict__ z,
-                            const T *__restrict__ imult, const double *__restrict__ mass,
+                            const float *__restrict__ imult, const double *__restrict__ mass,
                             int normtype, double *__restrict__ part, size_t n) {
   __shared__ double sred[kThreads / 64];
   double rho = 0.0, nrm = 0.0;
@@ -542,6 +931,11 @@ int launch_helm_f32(gf_mesh *m, float p, const float *qf, const float *u, float 
 
 }  // namespace
 
+int gf_helm3c(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
+              double *uo) {
+  return GF_DISPATCH_N3(m->Niso, helm3c_d, m, p, pf, qf, u, uo);
+}
+
 int gf_axpy_t(gf_mesh *m, double a, const double *x, double *y) {
   hipLaunchKernelGGL(k_axpy, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, y, a, x,
                      m->ndof);
@@ -574,6 +968,7 @@ int gf_helmholtz_apply_f32(gf_mesh *m, float p_scalar, const float *q_field, con
   if (!m || !u || !uo) return GF_ERR_ARG;
   if (m->gtype != GF_GE_REGULAR || !m->Niso)
     return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "FP32 Helmholtz: regular isotropic meshes");
+  if (m->dim == 3) return GF_DISPATCH_N3(m->Niso, helm3c_f, m, p_scalar, q_field, u, uo);
   return GF_DISPATCH_N(m->dim, m->Niso, launch_helm_f32, m, p_scalar, q_field, u, uo);
 }
 
@@ -702,7 +1097,8 @@ int rk_step_fused(gf_mesh *m, double nu, int order, double dt, double *const *u)
     }
     a.bh = s > 0 ? beta[s * 4 + (s - 1)] * h : 0.0;
     a.ch = s > 0 ? c[s - 1] * h : 0.0;
-    RC(GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, a));
+    RC(m->dim == 3 ? GF_DISPATCH_N3(m->Niso, launch_stage3c, m, a)
+                   : GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, a));
     for (int k = 0; k < nc; k++) RC(gf_dss_t(m, GF_DSS_SMOOTH, K[s % 2][k]));
   }
   // final stage input
@@ -729,7 +1125,8 @@ int rk_step_fused(gf_mesh *m, double nu, int order, double dt, double *const *u)
   }
   a.bh = 0.0;
   a.ch = 0.0;
-  RC(GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, a));
+  RC(m->dim == 3 ? GF_DISPATCH_N3(m->Niso, launch_stage3c, m, a)
+                 : GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, a));
   double cl = c[ns - 1] * h;
   for (int k = 0; k < nc; k++) {
     double *kf = K[(ns - 1) % 2][k];
@@ -871,14 +1268,15 @@ constexpr int kBatch = 16;
 template <typename T>
 struct CGVecs {
   T *x, *r, *w, *q;
-  const T *dinv, *mask, *imult, *qf;
+  const T *dinv, *mask, *qf;
+  const float *imult;
   const void *D, *W, *edx, *ejac;
   double p;
 };
 
 // Reduce (r.z.imult, norm(r)) at set-up, synchronous; returns both.
 template <typename T>
-int setup_red(gf_mesh *m, const T *r, const T *z, const T *imult, int normtype, double out[2]) {
+int setup_red(gf_mesh *m, const T *r, const T *z, const float *imult, int normtype, double out[2]) {
   hipLaunchKernelGGL(k_setup_red<T>, dim3(kUpdBlocks), dim3(kThreads), 0, m->ctx->stream, r, z,
                      imult, m->mass, normtype, m->red, m->ndof);
   hipLaunchKernelGGL(k_setup_fin, dim3(1), dim3(kThreads), 0, m->ctx->stream, m->red, kUpdBlocks,
@@ -901,10 +1299,10 @@ double finish_norm_host(int normtype, double v, gf_mesh *m, double volume) {
 }
 
 template <typename T, int NORM>
-void launch_update(gf_mesh *m, CGVecs<T> &v, double *part) {
+void launch_update(gf_mesh *m, CGVecs<T> &v, double *part, bool mask_in_q) {
+  const T *mk = (m->has_mask && !mask_in_q) ? v.mask : nullptr;
   hipLaunchKernelGGL((k_cg_update<T, NORM>), dim3(kUpdBlocks), dim3(kThreads), 0, m->ctx->stream,
-                     v.x, v.w, v.r, v.q, m->has_mask ? v.mask : nullptr, v.dinv, v.imult, m->mass,
-                     m->scal, part, m->ndof);
+                     v.x, v.w, v.r, v.q, mk, v.dinv, v.imult, m->mass, m->scal, part, m->ndof);
 }
 
 // Runs CG iterations from the state prepared in scal (rho, rtol, first=1,
@@ -913,10 +1311,11 @@ void launch_update(gf_mesh *m, CGVecs<T> &v, double *part) {
 template <typename T>
 int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters) {
   hipStream_t s = m->ctx->stream;
-  using S8 = int;
-  (void)sizeof(S8);
+  const bool col = m->dim == 3;
   int nbA = 0;
-  {
+  if (col) {
+    nbA = col_blocks(m);
+  } else {
     int NN = (int)m->nn;
     int epb = NN >= kThreads ? 1 : kThreads / NN;
     nbA = (int)((m->nelems + epb - 1) / epb);
@@ -937,22 +1336,45 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a.scal = m->scal;
   a.part = partA;
   a.nelems = m->nelems;
+  CGApply3Args a3;
+  a3.r = v.r;
+  a3.dinv = v.dinv;
+  a3.w = v.w;
+  a3.q = v.q;
+  a3.D = v.D;
+  for (int c = 0; c < 3; c++) {
+    a3.DR[c] = sizeof(T) == 8 ? (const void *)m->pDR[c] : (const void *)m->pDR32[c];
+    a3.DC[c] = sizeof(T) == 8 ? (const void *)m->pDC[c] : (const void *)m->pDC32[c];
+  }
+  a3.W = v.W;
+  a3.edx = v.edx;
+  a3.ejac = v.ejac;
+  a3.qf = v.qf;
+  a3.mask = m->has_mask ? v.mask : nullptr;
+  a3.p = v.p;
+  a3.scal = m->scal;
+  a3.part = partA;
+  a3.nelems = m->nelems;
   for (;;) {
     for (int b = 0; b < kBatch; b++) {
-      int rc = sizeof(T) == 8
-                   ? GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_d, m, a)
-                   : GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_f, m, a);
+      int rc;
+      if (col)
+        rc = sizeof(T) == 8 ? GF_DISPATCH_N3(m->Niso, cgapply3c_d, m, a3)
+                            : GF_DISPATCH_N3(m->Niso, cgapply3c_f, m, a3);
+      else
+        rc = sizeof(T) == 8 ? GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_d, m, a)
+                            : GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_f, m, a);
       if (rc) return rc;
       if (m->ngroups)
         hipLaunchKernelGGL(k_dss_cg<T>, dim3(nblocks((size_t)m->ngroups)), dim3(kThreads), 0, s,
                            v.q, m->goff, m->gmem, m->ngroups, (const double *)m->scal);
-      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kThreads), 0, s, m->scal, partA, nbA);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kRedThreads), 0, s, m->scal, partA, nbA);
       switch (normtype) {
-        case GF_NORM_INF: launch_update<T, GF_NORM_INF>(m, v, partU); break;
-        case GF_NORM_EUC: launch_update<T, GF_NORM_EUC>(m, v, partU); break;
-        case GF_NORM_L2: launch_update<T, GF_NORM_L2>(m, v, partU); break;
-        case GF_NORM_L1: launch_update<T, GF_NORM_L1>(m, v, partU); break;
-        default: launch_update<T, GF_NORM_ASSEMBLED_L2>(m, v, partU); break;
+        case GF_NORM_INF: launch_update<T, GF_NORM_INF>(m, v, partU, col); break;
+        case GF_NORM_EUC: launch_update<T, GF_NORM_EUC>(m, v, partU, col); break;
+        case GF_NORM_L2: launch_update<T, GF_NORM_L2>(m, v, partU, col); break;
+        case GF_NORM_L1: launch_update<T, GF_NORM_L1>(m, v, partU, col); break;
+        default: launch_update<T, GF_NORM_ASSEMBLED_L2>(m, v, partU, col); break;
       }
       hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kThreads), 0, s, m->scal, partU, kUpdBlocks,
                          normtype, d_resid);
@@ -990,7 +1412,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
   // partial-sum buffer large enough for one partial per apply block
   {
-    size_t need = (m->nelems + 64) + 2 * kUpdBlocks + 64;
+    size_t need = (m->nelems + 64) + 2 * kUpdBlocks + 64;  // >= apply blocks + update partials
     if (need > (size_t)GF_RED_MAX_BLOCKS * 8) {
       double *nr = nullptr;
       GF_HIP(m->ctx, hipMalloc((void **)&nr, need * sizeof(double)));
@@ -1018,11 +1440,11 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     RC(copy(m, q, b));
     RC(gf_dss_t(m, GF_DSS_SUM, q));
     if (bbv) RC(gf_mul_t(m, q, m->mask, q));
-    RC(setup_red<double>(m, q, nullptr, m->imult, normtype, red2));
+    RC(setup_red<double>(m, q, nullptr, m->imult32, normtype, red2));
     rnorm0 = finish_norm_host(normtype, red2[1], m, volume);
     rtol = prm->tol * rnorm0;
   } else {
-    RC(setup_red<double>(m, b, nullptr, m->imult, normtype, red2));
+    RC(setup_red<double>(m, b, nullptr, m->imult32, normtype, red2));
     rnorm0 = finish_norm_host(normtype, red2[1], m, volume);
     rtol = rnorm0 < 1.0 ? prm->tol : prm->tol * rnorm0;
   }
@@ -1042,7 +1464,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     // z = P^-1 r ; w = z (set inside the first apply) ; rho = r.z.imult
     double *z = q;
     if (dinv) RC(gf_mul_t(m, dinv, r, z));
-    RC(setup_red<double>(m, r, dinv ? z : r, m->imult, normtype, red2));
+    RC(setup_red<double>(m, r, dinv ? z : r, m->imult32, normtype, red2));
     double hs[12] = {0};
     hs[S_RHO] = red2[0];
     hs[S_RTOL] = rtol;
@@ -1061,7 +1483,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     v.q = q;
     v.dinv = dinv;
     v.mask = mask;
-    v.imult = m->imult;
+    v.imult = m->imult32;
     v.qf = prm->q_field;
     v.D = m->D[0];
     v.W = m->W[0];
@@ -1091,14 +1513,14 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     int hist = 0;
     for (outer = 0; outer < 100; outer++) {
       RC(residual(r));
-      RC(setup_red<double>(m, r, nullptr, m->imult, normtype, red2));
+      RC(setup_red<double>(m, r, nullptr, m->imult32, normtype, red2));
       double rn = finish_norm_host(normtype, red2[1], m, volume);
       rn_final = rn;
       if (prm->resid && hist < prm->maxit) prm->resid[hist++] = rn;
       if (rn <= rtol || total_iters >= prm->maxit) break;
       // scale the correction problem to O(1): r32 = r / rscale
       double rscale = 0.0;
-      RC(setup_red<double>(m, r, nullptr, m->imult, GF_NORM_INF, red2));
+      RC(setup_red<double>(m, r, nullptr, m->imult32, GF_NORM_INF, red2));
       rscale = red2[1] > 0 ? red2[1] : 1.0;
       hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, r32, r, 1.0 / rscale, n);
       GF_HIP(m->ctx, hipMemsetAsync(d32, 0, n * sizeof(float), s));
@@ -1187,9 +1609,10 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
   CGApplyArgs ca;
   memset(&sa, 0, sizeof(sa));
   memset(&ca, 0, sizeof(ca));
+  CGApply3Args ca3;
+  memset(&ca3, 0, sizeof(ca3));
   if (w == "burgers_stage") {
     // stages 2-3 form: reads uin, kprev, accsrc; writes acc, kout (5 fields per component)
-    if (m->dim != 3 && m->dim != 2) return GF_ERR_ARG;
     for (int c = 0; c < m->dim; c++) {
       sa.uin[c] = buf[c];
       sa.kprev[c] = buf[3 + c];
@@ -1206,8 +1629,12 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
     sa.imass = m->imass;
     sa.nu = 0.02;
     sa.nelems = m->nelems;
-    launch = [&]() { return GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, sa); };
+    if (m->dim == 3)
+      launch = [&]() { return GF_DISPATCH_N3(m->Niso, launch_stage3c, m, sa); };
+    else
+      launch = [&]() { return GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, sa); };
   } else if (w == "cg_apply" || w == "cg_apply_f32") {
+    if (m->dim != 3) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "cg_apply timing: 3-D meshes");
     bool f32 = w == "cg_apply_f32";
     double hs[12] = {0};
     hs[S_BETA] = 0.5;
@@ -1226,23 +1653,28 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
         hipLaunchKernelGGL(k_fill_pattern_f, dim3(nblocks(n)), dim3(kThreads), 0, s,
                            (float *)buf[k], n, 0.5f + 0.1f * k);
     }
-    ca.r = buf[0];
-    ca.dinv = buf[1];
-    ca.w = buf[2];
-    ca.q = buf[3];
-    ca.D = f32 ? (const void *)m->D32 : (const void *)m->D[0];
-    ca.W = f32 ? (const void *)m->W32 : (const void *)m->W[0];
-    ca.edx = f32 ? (const void *)m->edx32 : (const void *)m->edx;
-    ca.ejac = f32 ? (const void *)m->ejac32 : (const void *)m->ejac;
-    ca.qf = nullptr;
-    ca.p = 0.1;
-    ca.scal = m->scal;
-    ca.part = m->red;
-    ca.nelems = m->nelems;
+    ca3.r = buf[0];
+    ca3.dinv = buf[1];
+    ca3.w = buf[2];
+    ca3.q = buf[3];
+    ca3.D = f32 ? (const void *)m->D32 : (const void *)m->D[0];
+    for (int c = 0; c < 3; c++) {
+      ca3.DR[c] = f32 ? (const void *)m->pDR32[c] : (const void *)m->pDR[c];
+      ca3.DC[c] = f32 ? (const void *)m->pDC32[c] : (const void *)m->pDC[c];
+    }
+    ca3.W = f32 ? (const void *)m->W32 : (const void *)m->W[0];
+    ca3.edx = f32 ? (const void *)m->edx32 : (const void *)m->edx;
+    ca3.ejac = f32 ? (const void *)m->ejac32 : (const void *)m->ejac;
+    ca3.qf = nullptr;
+    ca3.mask = nullptr;
+    ca3.p = 0.1;
+    ca3.scal = m->scal;
+    ca3.part = m->red;
+    ca3.nelems = m->nelems;
     if (f32)
-      launch = [&]() { return GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_f, m, ca); };
+      launch = [&]() { return GF_DISPATCH_N3(m->Niso, cgapply3c_f, m, ca3); };
     else
-      launch = [&]() { return GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_d, m, ca); };
+      launch = [&]() { return GF_DISPATCH_N3(m->Niso, cgapply3c_d, m, ca3); };
   } else if (w == "helmholtz") {
     launch = [&]() { return gf_helm_t(m, 0.1, nullptr, nullptr, buf[0], buf[1]); };
   } else {
