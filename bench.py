@@ -142,7 +142,7 @@ def main():
         m = Mesh(ctx, cg_grid)
         f = torch.from_numpy(workloads.unique_normal(cg_grid, seed=67890)).cuda()
         b = m.mass_apply(f)  # local RHS; GCG assembles b - A x (so the system's RHS is DSS_sum(M f))
-        q = torch.ones(ndof, dtype=torch.float64, device="cuda")
+        q = 1.0  # (M + 0.1 L): a one-entry q of 1, as GHelmholtz::set_mass_scalar allows
         mask = torch.from_numpy(cg_grid.mask).cuda()
         zero = torch.zeros_like(b)
         bA = m.dss(b.clone(), "sum") * mask

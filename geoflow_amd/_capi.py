@@ -28,19 +28,24 @@ class GeoflowError(RuntimeError):
         self.code = code
 
 
+GF_MESH_BOX_GATHER = 1
+GF_MESH_BOX_MASK = 2
+
+
 class MeshDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("gtype", ctypes.c_int), ("N", ctypes.c_int * 3),
                 ("nelems", ctypes.c_size_t), ("D", _dp * 3), ("W", _dp * 3),
                 ("elem_dxidx", _dp), ("elem_jac", _dp), ("node_dxidx", _dp * 9),
                 ("node_jac", _dp), ("xyz", _dp * 3), ("mask", _dp),
-                ("ggfx_offs", _i64p), ("ggfx_members", _i64p)]
+                ("ggfx_offs", _i64p), ("ggfx_members", _i64p),
+                ("box_ne", ctypes.c_int * 3), ("box_periodic", ctypes.c_int * 3)]
 
 
 class CGParams(ctypes.Structure):
     _fields_ = [("maxit", ctypes.c_int), ("tol", ctypes.c_double), ("normtype", ctypes.c_int),
                 ("precond", ctypes.c_int), ("precision", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("p_scalar", ctypes.c_double),
-                ("p_field", _vp), ("q_field", _vp), ("resid", _dp)]
+                ("p_field", _vp), ("q_field", _vp), ("resid", _dp), ("q_scalar", ctypes.c_double)]
 
 
 class CGResult(ctypes.Structure):
@@ -67,6 +72,7 @@ SIGNATURES = {
     "gf_mesh_create": (ctypes.c_int, [_vp, ctypes.POINTER(MeshDesc), ctypes.POINTER(_vp)]),
     "gf_mesh_destroy": (ctypes.c_int, [_vp]),
     "gf_mesh_ndof": (ctypes.c_size_t, [_vp]),
+    "gf_mesh_flags": (ctypes.c_int, [_vp]),
     "gf_mesh_get": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
     "gf_malloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "gf_free": (ctypes.c_int, [_vp, _vp]),
@@ -80,8 +86,9 @@ SIGNATURES = {
     "gf_mass_apply": (ctypes.c_int, [_vp, _vp, _vp]),
     "gf_imass_apply": (ctypes.c_int, [_vp, _vp, _vp]),
     "gf_integrate": (ctypes.c_int, [_vp, _vp, _dp]),
-    "gf_helmholtz_apply": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
-    "gf_helmholtz_apply_f32": (ctypes.c_int, [_vp, ctypes.c_float, _vp, _vp, _vp]),
+    "gf_helmholtz_apply": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp,
+                                          _vp]),
+    "gf_helmholtz_apply_f32": (ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]),
     "gf_dss": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "gf_imult": (ctypes.c_int, [_vp, _vp]),
     "gf_advect": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(_vp), _vp]),
@@ -95,7 +102,7 @@ SIGNATURES = {
                                         ctypes.c_int, ctypes.POINTER(_vp)]),
     "gf_cg_solve": (ctypes.c_int, [_vp, ctypes.POINTER(CGParams), _vp, _vp, _vp,
                                    ctypes.POINTER(CGResult)]),
-    "gf_helmholtz_diag_inv": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp]),
+    "gf_helmholtz_diag_inv": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),
     "gf_time_kernel": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_double)]),
 }

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "geoflow_hip.h"
+#include "gf_boxdss.hpp"
 
 #define GF_MAX_N 12  // largest isotropic node count per direction with fused kernels
 
@@ -49,6 +50,13 @@ struct gf_mesh_s {
   int *goff = nullptr;
   int *gmem = nullptr;
   int maxgrp = 1;
+  // Structured box: when has_box, the groups above are exactly those of
+  // gf::box_group for `box`, and fused kernels may assemble by gather.
+  gf::BoxTopo box = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0, 0, 0, 0}};
+  bool has_box = false;
+  // has_box_mask: the boundary mask equals gf::box_masked for `box`, so
+  // kernels may form it from the node position instead of reading it.
+  bool has_box_mask = false;
   // reusable device workspace
   std::vector<void *> ws;
   std::vector<size_t> ws_cap;
@@ -63,15 +71,15 @@ int gf_check_hip(gf_ctx *ctx, hipError_t e, const char *what);
 // Internal (C++ linkage) helpers shared across translation units.
 int gf_dss_t(gf_mesh *m, int op, double *u);
 int gf_dss_f32(gf_mesh *m, int op, float *u);
-int gf_helm_t(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
-              double *uo);
+int gf_helm_t(gf_mesh *m, double p, const double *pf, const double *qf, double qs,
+              const double *u, double *uo);
 int gf_grefderiv_t(gf_mesh *m, int idir, int trans, const double *u, double *du,
                    const double *edx);
 int gf_mul_t(gf_mesh *m, const double *a, const double *b, double *out);
 int gf_dot_t(gf_mesh *m, const double *a, const double *b, size_t n, double *result);
 int gf_axpy_t(gf_mesh *m, double a, const double *x, double *y);
-int gf_helm3c(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
-              double *uo);
+int gf_helm3c(gf_mesh *m, double p, const double *pf, const double *qf, double qs,
+              const double *u, double *uo);
 
 // Workspace: returns a device buffer of at least `bytes` for slot `slot`.
 void *gf_ws(gf_mesh *m, int slot, size_t bytes);

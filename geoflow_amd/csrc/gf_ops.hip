@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kThreads) void k_helm(const double *__restrict__ u,
                                                    const double *__restrict__ ejac, double pscal,
                                                    const double *__restrict__ pfield,
                                                    const double *__restrict__ qfield,
-                                                   size_t nelems) {
+                                                   double qscal, size_t nelems) {
   using S = Shape<DIM, N>;
   __shared__ double su[S::NODES];
   __shared__ double sg[DIM][S::NODES];
@@ -251,10 +251,11 @@ __global__ __launch_bounds__(kThreads) void k_helm(const double *__restrict__ u,
 #pragma unroll
     for (int d = 0; d < DIM; d++) acc += gf::contract<DIM, N, true>(&sg[d][el * S::NN], sD, d, i, j, k);
     if (!pfield && pscal != 1.0) acc *= pscal;
-    if (qfield) {
+    if (qfield || qscal != 0.0) {
       double M = gf::node_mass<DIM>(sW, i, j, k, ejac[e0 + el]);
       double t = M * su[q];
-      t *= qfield[base + q];
+      if (qfield) t *= qfield[base + q];
+      else if (qscal != 1.0) t *= qscal;
       acc += t;
     }
     uo[base + q] = acc;
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void k_helm_embed(
     const double *__restrict__ u, double *__restrict__ uo, const double *__restrict__ D,
     const double *__restrict__ G00, const double *__restrict__ G01, const double *__restrict__ G11,
     const double *__restrict__ mass, double pscal, const double *__restrict__ pfield,
-    const double *__restrict__ qfield, size_t nelems) {
+    const double *__restrict__ qfield, double qscal, size_t nelems) {
   using S = Shape<2, N>;
   __shared__ double su[S::NODES];
   __shared__ double sg[2][S::NODES];
@@ -304,9 +305,10 @@ __global__ __launch_bounds__(kThreads) void k_helm_embed(
     acc += gf::contract<2, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
     acc += gf::contract<2, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
     if (!pfield && pscal != 1.0) acc *= pscal;
-    if (qfield) {
+    if (qfield || qscal != 0.0) {
       double t = mass[base + q] * su[q];
-      t *= qfield[base + q];
+      if (qfield) t *= qfield[base + q];
+      else if (qscal != 1.0) t *= qscal;
       acc += t;
     }
     uo[base + q] = acc;
@@ -503,17 +505,17 @@ int launch_div(gf_mesh *m, int trans, const double *u0, const double *u1, const 
 }
 
 template <int DIM, int N>
-int launch_helm(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
-                double *uo) {
+int launch_helm(gf_mesh *m, double p, const double *pf, const double *qf, double qs,
+                const double *u, double *uo) {
   using S = Shape<DIM, N>;
   int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
   if (m->gtype == GF_GE_REGULAR) {
     hipLaunchKernelGGL((k_helm<DIM, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, uo,
-                       m->D[0], m->W[0], m->edx, m->ejac, p, pf, qf, m->nelems);
+                       m->D[0], m->W[0], m->edx, m->ejac, p, pf, qf, qs, m->nelems);
   } else {
     if (DIM != 2) return GF_ERR_UNSUPPORTED;
     hipLaunchKernelGGL((k_helm_embed<2, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, uo,
-                       m->D[0], m->G[0], m->G[1], m->G[2], m->mass, p, pf, qf, m->nelems);
+                       m->D[0], m->G[0], m->G[1], m->G[2], m->mass, p, pf, qf, qs, m->nelems);
   }
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
@@ -559,11 +561,12 @@ int gf_dss_f32(gf_mesh *m, int op, float *u) {
   return GF_OK;
 }
 
-int gf_helm_t(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
-              double *uo) {
+int gf_helm_t(gf_mesh *m, double p, const double *pf, const double *qf, double qs,
+              const double *u, double *uo) {
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Helmholtz needs isotropic order");
-  if (m->dim == 3 && m->gtype == GF_GE_REGULAR) return gf_helm3c(m, p, pf, qf, u, uo);
-  return GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_helm, m, p, pf, qf, u, uo);
+  if (qf) qs = 0.0;
+  if (m->dim == 3 && m->gtype == GF_GE_REGULAR) return gf_helm3c(m, p, pf, qf, qs, u, uo);
+  return GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_helm, m, p, pf, qf, qs, u, uo);
 }
 
 int gf_grefderiv_t(gf_mesh *m, int idir, int trans, const double *u, double *du,
@@ -754,6 +757,56 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     UP(m->goff, goff.data(), goff.size());
     UP(m->gmem, gmem.data(), std::max<size_t>(gmem.size(), 1));
   }
+  // Structured box: enable gather assembly only if every node's GGFX group
+  // is exactly the group the box structure implies.
+  if (d->box_ne[0] > 0 && d->dim == 3 && d->gtype == GF_GE_REGULAR && m->Niso) {
+    gf::BoxTopo t;
+    size_t ne_tot = 1;
+    for (int k = 0; k < 3; k++) {
+      t.ne[k] = d->box_ne[k];
+      t.per[k] = d->box_periodic[k] ? 1 : 0;
+      ne_tot *= (size_t)std::max(d->box_ne[k], 0);
+    }
+    for (int f = 0; f < 6; f++) t.fmask[f] = 0;
+    bool ok = ne_tot == m->nelems;
+    const int Nn = m->Niso;
+    // face mask types, read off the mask at the middle of each face (checked
+    // against every node below)
+    if (ok && d->mask) {
+      for (int f = 0; f < 6; f++) {
+        const int dd = f / 2, side = f % 2;
+        int ec[3], nc[3];
+        for (int c = 0; c < 3; c++) {
+          ec[c] = c == dd ? (side ? t.ne[c] - 1 : 0) : t.ne[c] / 2;
+          nc[c] = c == dd ? (side ? Nn - 1 : 0) : Nn / 2;
+        }
+        const size_t e = (size_t)ec[0] + (size_t)t.ne[0] * ((size_t)ec[1] + (size_t)t.ne[1] * ec[2]);
+        const size_t g = e * m->nn + (size_t)nc[2] * Nn * Nn + (size_t)nc[1] * Nn + nc[0];
+        t.fmask[f] = d->mask[g] == 0.0 ? 1 : 0;
+      }
+    }
+    bool mask_ok = ok;
+    for (size_t q = 0; ok && q < n; q++) {
+      const size_t e = q / m->nn, r = q - e * m->nn;
+      const int i = (int)(r % Nn), j = (int)((r / Nn) % Nn), k = (int)(r / ((size_t)Nn * Nn));
+      // the construction the kernels use: per-element partner offsets
+      int64_t g[8];
+      int ec[3];
+      gf::box_elem_coords(t, e, ec);
+      const gf::BoxElemDeltas de = gf::box_elem_deltas(t, Nn, ec);
+      const int c = gf::box_members<int64_t>(de, Nn, (int64_t)q, i, j, k, g);
+      const int64_t b = d->ggfx_offs[q], eo = d->ggfx_offs[q + 1];
+      ok = (eo - b) == c;
+      for (int s2 = 0; ok && s2 < c; s2++) ok = d->ggfx_members[b + s2] == g[s2];
+      if (mask_ok) {
+        const double mv = d->mask ? d->mask[q] : 1.0;
+        mask_ok = (mv == 0.0 || mv == 1.0) && ((mv == 0.0) == gf::box_masked(t, Nn, ec, i, j, k));
+      }
+    }
+    m->box = t;
+    m->has_box = ok;
+    m->has_box_mask = ok && mask_ok;
+  }
   GF_HIP(ctx, hipMalloc((void **)&m->red, GF_RED_MAX_BLOCKS * 8 * sizeof(double)));
   GF_HIP(ctx, hipMalloc((void **)&m->scal, 64 * sizeof(double)));
   GF_HIP(ctx, hipHostMalloc((void **)&m->hscal, 64 * sizeof(double), hipHostMallocDefault));
@@ -791,6 +844,11 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
 #undef UP
   *out = m;
   return GF_OK;
+}
+
+int gf_mesh_flags(const gf_mesh *m) {
+  if (!m) return 0;
+  return (m->has_box ? GF_MESH_BOX_GATHER : 0) | (m->has_box_mask ? GF_MESH_BOX_MASK : 0);
 }
 
 int gf_mesh_destroy(gf_mesh *m) {
@@ -926,10 +984,10 @@ int gf_integrate(gf_mesh *m, const double *u, double *result) {
   return gf_dot_t(m, m->mass, u, m->ndof, result);
 }
 
-int gf_helmholtz_apply(gf_mesh *m, double p_scalar, const double *p_field, const double *q_field,
-                       const double *u, double *uo) {
+int gf_helmholtz_apply(gf_mesh *m, double p_scalar, const double *p_field, double q_scalar,
+                       const double *q_field, const double *u, double *uo) {
   if (!m || !u || !uo) return GF_ERR_ARG;
-  return gf_helm_t(m, p_scalar, p_field, q_field, u, uo);
+  return gf_helm_t(m, p_scalar, p_field, q_field, q_scalar, u, uo);
 }
 
 int gf_dss(gf_mesh *m, int op, double *u) {

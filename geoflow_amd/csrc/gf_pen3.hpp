@@ -27,6 +27,16 @@ struct PenD {
   const T *C[3];
 };
 
+// Returns p unchanged, but hidden from the optimiser: loads through the
+// result cannot be hoisted above this point or merged with earlier loads.
+// Used per row of D, so only one row is live in scalar registers at a time
+// instead of the whole matrix being hoisted (and spilled) ahead of the loops.
+template <typename P>
+__device__ __forceinline__ P opaque_ptr(P p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 template <int N>
 struct Pen3 {
   // Each pass reads D through its own copy (rows contiguous for the gradient,
@@ -43,26 +53,35 @@ struct Pen3 {
 };
 
 // Computes, for the column (i=a, j=b) owned by the calling thread,
-//   post(k, Lu_k, M_k),   Lu = ((x-part + y-part) + z-part)
+//   post(k, Lu_k, M_k, hx_k, hy_k),   Lu = ((x-part + y-part) + z-part)
 // where each part is D_d^T (M * ((D_d u) * G_d) [* p]) summed in ascending m,
 // i.e. the order of GHelmholtz::reg_prod / GGrid::compute_grefdiv
 // (ghelmholtz.ipp:308-378, ggrid.ipp:1847-1927), and M_k is the lumped mass
 // ((W_k W_j) W_i) Jac (gmass.ipp:200-210).
 //   ru    the thread's column of u (nodes (a,b,0..N-1))
 //   su,sd two LDS element buffers (Pen3<N>::SZ each) owned by this element
+//   sx    a third LDS element buffer when HOOK is true, else unused
 //   pe    p field restricted to this element, or NULL
-// Optional hooks for advection-type uses:
-//   xcb(i, ux, M) at x-pencil node (i,a,b);  ycb(j, uy, M) at y-pencil node
-//   (a,j,b);  zcb(k, uz, M) at column node (a,b,k)  -- reference derivatives.
+// Hooks see the reference derivatives where they are formed:
+//   zcb(k, uz, M)  at column node (a,b,k);
+//   xcb(i, ux, M)  at x-pencil node (i,a,b), returns a value hx;
+//   ycb(j, uy, M)  at y-pencil node (a,j,b), returns a value hy;
+// with HOOK true, hx and hy are carried through LDS to the node's column
+// owner and handed to post (advection-type terms); with HOOK false they are
+// ignored and post receives zeros.
 // All threads of the workgroup must call this (it contains barriers).
-template <int N, typename T, typename XCB, typename YCB, typename ZCB, typename POST>
-__device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, const PenD<T> &Dp,
+template <int N, typename T, bool HOOK, typename XCB, typename YCB, typename ZCB, typename POST>
+__device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, const PenD<T> &Dp,
                                         const T *__restrict__ W, int a, int b, T jac, T g0, T g1,
                                         T g2, const T *__restrict__ pe, XCB &&xcb, YCB &&ycb,
                                         ZCB &&zcb, POST &&post) {
   using S = Pen3<N>;
-  const T *__restrict__ Rx = Dp.R[0], *__restrict__ Ry = Dp.R[1], *__restrict__ Rz = Dp.R[2];
-  const T *__restrict__ Cx = Dp.C[0], *__restrict__ Cy = Dp.C[1], *__restrict__ Cz = Dp.C[2];
+  // D is read-only for the whole launch: reading it through the constant
+  // address space lets the compiler use scalar loads even when the caller's
+  // output pointers are not marked restrict.
+  using CP = const __attribute__((address_space(4))) T *;
+  const CP Rx = (CP)Dp.R[0], Ry = (CP)Dp.R[1], Rz = (CP)Dp.R[2];
+  const CP Cx = (CP)Dp.C[0], Cy = (CP)Dp.C[1], Cz = (CP)Dp.C[2];
   const T Wa = W[a], Wb = W[b];
 #pragma unroll
   for (int k = 0; k < N; k++) su[S::at(a, b, k)] = ru[k];
@@ -73,8 +92,9 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, const Pe
 #pragma unroll
     for (int k = 0; k < N; k++) {
       T uz = T(0);
+      const CP row = opaque_ptr(Rz + k * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) uz = fma(Rz[k * N + m], ru[m], uz);
+      for (int m = 0; m < N; m++) uz = fma(row[m], ru[m], uz);
       const T M = W[k] * Wb * Wa * jac;
       zcb(k, uz, M);
       T f = uz * g2;
@@ -85,14 +105,15 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, const Pe
 #pragma unroll
     for (int k = 0; k < N; k++) {
       T c = T(0);
+      const CP row = opaque_ptr(Cz + k * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) c = fma(Cz[k * N + m], gz[m], c);
+      for (int m = 0; m < N; m++) c = fma(row[m], gz[m], c);
       dz[k] = c;
       __builtin_amdgcn_sched_barrier(0);
     }
   }
   __syncthreads();
-  // x-part on the x-pencil (j=a, k=b)
+  // x-part on the x-pencil (j=a, k=b) -> sd
   {
     T xp[N], f[N];
 #pragma unroll
@@ -100,10 +121,12 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, const Pe
 #pragma unroll
     for (int i = 0; i < N; i++) {
       T ux = T(0);
+      const CP row = opaque_ptr(Rx + i * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) ux = fma(Rx[i * N + m], xp[m], ux);
+      for (int m = 0; m < N; m++) ux = fma(row[m], xp[m], ux);
       const T M = Wb * Wa * W[i] * jac;
-      xcb(i, ux, M);
+      const T hx = xcb(i, ux, M);
+      if (HOOK) sx[S::at(i, a, b)] = hx;
       T g = ux * g0;
       if (pe) g *= pe[i + a * N + b * N * N];
       f[i] = M * g;
@@ -112,14 +135,15 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, const Pe
 #pragma unroll
     for (int i = 0; i < N; i++) {
       T s = T(0);
+      const CP row = opaque_ptr(Cx + i * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) s = fma(Cx[i * N + m], f[m], s);
+      for (int m = 0; m < N; m++) s = fma(row[m], f[m], s);
       sd[S::at(i, a, b)] = s;
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  // y-part on the y-pencil (i=a, k=b), kept in registers until su is free
-  T dy[N];
+  // y-part on the y-pencil (i=a, k=b), kept in registers until sd is complete
+  T dy[N], hy[HOOK ? N : 1];
   {
     T yp[N], f[N];
 #pragma unroll
@@ -127,10 +151,12 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, const Pe
 #pragma unroll
     for (int j = 0; j < N; j++) {
       T uy = T(0);
+      const CP row = opaque_ptr(Ry + j * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) uy = fma(Ry[j * N + m], yp[m], uy);
+      for (int m = 0; m < N; m++) uy = fma(row[m], yp[m], uy);
       const T M = Wb * W[j] * Wa * jac;
-      ycb(j, uy, M);
+      const T h = ycb(j, uy, M);
+      if (HOOK) hy[j] = h;
       T g = uy * g1;
       if (pe) g *= pe[a + j * N + b * N * N];
       f[j] = M * g;
@@ -139,26 +165,35 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, const Pe
 #pragma unroll
     for (int j = 0; j < N; j++) {
       T s = T(0);
+      const CP row = opaque_ptr(Cy + j * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) s = fma(Cy[j * N + m], f[m], s);
+      for (int m = 0; m < N; m++) s = fma(row[m], f[m], s);
       dy[j] = s;
       __builtin_amdgcn_sched_barrier(0);
     }
   }
   __syncthreads();
+  // (x-part + y-part) at the y-pencil's nodes; su is free for hy
 #pragma unroll
-  for (int j = 0; j < N; j++) su[S::at(a, j, b)] = dy[j];
+  for (int j = 0; j < N; j++) {
+    const int q = S::at(a, j, b);
+    sd[q] = sd[q] + dy[j];
+    if (HOOK) su[q] = hy[j];
+  }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < N; k++) {
+    const int q = S::at(a, b, k);
     const T M = W[k] * Wb * Wa * jac;
-    post(k, (sd[S::at(a, b, k)] + su[S::at(a, b, k)]) + dz[k], M);
+    post(k, sd[q] + dz[k], M, HOOK ? sx[q] : T(0), HOOK ? su[q] : T(0));
   }
 }
 
 struct NoPenCB {
   template <typename T>
-  __device__ __forceinline__ void operator()(int, T, T) const {}
+  __device__ __forceinline__ T operator()(int, T, T) const {
+    return T(0);
+  }
 };
 
 }  // namespace gf

@@ -10,7 +10,6 @@
 #include <string>
 #include <vector>
 
-#include "gf_col3.hpp"
 #include "gf_pen3.hpp"
 #include "gf_kernels.hpp"
 
@@ -27,9 +26,10 @@ inline int nblocks(size_t n, int per = kThreads) {
 }
 
 // Solver scalars on the device, scal[]: 0 rho, 1 alpha, 2 beta, 3 rnorm,
-// 4 rtol, 5 wq, 6 iter, 7 maxit, 8 done, 9 first, 10 volume, 11 n.
+// 4 rtol, 5 wq, 6 iter, 7 maxit, 8 done, 9 first, 10 volume, 11 n,
+// 12 rho of the previous iteration.
 enum { S_RHO = 0, S_ALPHA, S_BETA, S_RNORM, S_RTOL, S_WQ, S_ITER, S_MAXIT, S_DONE, S_FIRST,
-       S_VOL, S_N };
+       S_VOL, S_N, S_RHOM, S_COUNT };
 
 #define GRID_LOOP(g, n)                                                     \
   for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < (n); \
@@ -160,7 +160,8 @@ __global__ __launch_bounds__(kThreads) void k_burgers_stage(StageArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Column-per-thread 3-D kernels (see gf_col3.hpp)
+// 3-D regular-element kernels: thread (i,j) of an element owns the node
+// column (i,j,0..N-1) and the x/y pencils of gf_pen3.hpp.
 // ---------------------------------------------------------------------------
 inline gf::PenD<double> pend_d(gf_mesh *m) {
   gf::PenD<double> p;
@@ -179,62 +180,38 @@ inline gf::PenD<float> pend_f(gf_mesh *m) {
   return p;
 }
 
-// Thread/element mapping shared by the column kernels.
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Grids of the element kernels are a multiple of kXCD workgroups and
+// are renumbered so that each XCD walks one contiguous range of elements:
+// neighbouring elements, whose shared faces the gather kernels read, then
+// run on the same XCD close together in time.
+constexpr int kXCD = 8;
+inline int xcd_grid(size_t nblocks) { return (int)((nblocks + kXCD - 1) / kXCD * kXCD); }
+__device__ __forceinline__ size_t xcd_block() {
+  const size_t per = gridDim.x / kXCD;
+  return (size_t)(blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+}
+
+// Thread/element mapping shared by the 3-D element kernels.
 template <int N>
 struct ColIdx {
   int el, i, j, r;
   size_t e;
   bool active;
   __device__ __forceinline__ ColIdx(size_t nelems) {
-    using C = gf::Col3<N>;
+    using C = gf::Pen3<N>;
     el = threadIdx.x / C::N2;
     r = threadIdx.x - el * C::N2;
     i = r % N;
     j = r / N;
-    size_t eg = (size_t)blockIdx.x * C::EPB + el;
+    size_t eg = xcd_block() * C::EPB + el;
     active = el < C::EPB && eg < nelems;
     e = active ? eg : (nelems - 1);
   }
 };
 
-// Generic Helmholtz apply, 3-D regular elements:
-// uo = p * L u + q * M u, with p scalar or a field (reg_prod, ghelmholtz.ipp:308-378).
-template <int N, typename T>
-__global__ __launch_bounds__(gf::Col3<N>::THREADS, gf::ColWaves<T>::value) void k_helm3c(
-    const T *__restrict__ u, T *__restrict__ uo, const T *__restrict__ D, const T *__restrict__ W,
-    const T *__restrict__ edx, const T *__restrict__ ejac, T p, const T *__restrict__ pf,
-    const T *__restrict__ qf, size_t nelems) {
-  using C = gf::Col3<N>;
-  __shared__ T su[C::EPB][C::SL], sgx[C::EPB][C::SL], sgy[C::EPB][C::SL];
-  __shared__ gf::DShared<N, T> ds;
-  ds.stage(D);
-  ColIdx<N> t(nelems);
-  const size_t col = t.e * C::NN + t.r;
-  T ru[N];
-#pragma unroll
-  for (int k = 0; k < N; k++) ru[k] = u[col + k * C::N2];
-  gf::DRegs<N, T> dr;
-  dr.load(D, t.i, t.j);
-  __syncthreads();
-  const T jac = ejac[t.e];
-  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
-  auto post = [&](int k, T Lu, T M) {
-    if (!t.active) return;
-    T acc = Lu;
-    if (!pf && p != T(1)) acc *= p;
-    if (qf) {
-      T tq = M * ru[k];
-      tq *= qf[col + k * C::N2];
-      acc += tq;
-    }
-    uo[col + k * C::N2] = acc;
-  };
-  gf::col_op3<N, T>(ru, su[t.el], sgx[t.el], sgy[t.el], dr, ds, W, t.i, t.j, jac, f0 * f0, f1 * f1,
-                    f2 * f2, pf ? pf + col : (const T *)nullptr, gf::NoZ(), gf::NoXY(), post);
-}
-
 // CG direction update + operator apply (see k_cg_apply for the contract),
-// 3-D column form; the Dirichlet mask is applied to q here (the mask is
+// 3-D pencil form; the Dirichlet mask is applied to q here (the mask is
 // constant over each coincidence group, so DSS(q) * mask = DSS(q * mask)).
 struct CGApply3Args {
   const void *r, *dinv;
@@ -242,79 +219,133 @@ struct CGApply3Args {
   const void *D, *W, *edx, *ejac, *qf, *mask;
   const void *DR[3], *DC[3];  // pencil copies of D (same precision as T)
   double p;
+  double qs;  // mass-term scalar when qf is NULL (0: no mass term)
   double *scal, *part;
   size_t nelems;
+  gf::BoxTopo topo;  // with tmask: mask formed from the box faces (mask NULL)
+  int tmask;
 };
 
+// ---------------------------------------------------------------------------
+// Pencil-form 3-D kernels (see gf_pen3.hpp): Helmholtz apply and CG apply.
+// ---------------------------------------------------------------------------
 template <int N, typename T>
-__global__ __launch_bounds__(gf::Col3<N>::THREADS, gf::ColWaves<T>::value) void k_cg_apply3c(
-    CGApply3Args a) {
-  using C = gf::Col3<N>;
-  __shared__ T su[C::EPB][C::SL], sgx[C::EPB][C::SL], sgy[C::EPB][C::SL];
-  __shared__ gf::DShared<N, T> ds;
-  __shared__ double sred[4];
-  const double *scal = a.scal;
-  if (scal[S_DONE] != 0.0) return;
-  const T *r = (const T *)a.r, *dinv = (const T *)a.dinv, *D = (const T *)a.D,
-          *W = (const T *)a.W, *edx = (const T *)a.edx, *ejac = (const T *)a.ejac,
-          *qf = (const T *)a.qf, *mask = (const T *)a.mask;
-  T *w = (T *)a.w, *qv = (T *)a.q;
-  const bool first = scal[S_FIRST] != 0.0;
-  const T beta = (T)scal[S_BETA];
-  ColIdx<N> t(a.nelems);
-  const size_t col = t.e * C::NN + t.r;
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_helm3p(
+    const T *__restrict__ u, T *__restrict__ uo, gf::PenD<T> D, const T *__restrict__ W,
+    const T *__restrict__ edx, const T *__restrict__ ejac, T p, const T *__restrict__ pf,
+    const T *__restrict__ qf, T qs, size_t nelems) {
+  using S = gf::Pen3<N>;
+  __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
+  ColIdx<N> t(nelems);
+  const size_t col = t.e * S::NN + t.r;
   T ru[N];
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    const size_t g = col + k * C::N2;
-    const T z = dinv ? dinv[g] * r[g] : r[g];
-    const T wn = first ? z : beta * w[g] + T(1) * z;
-    ru[k] = wn;
-    if (t.active) w[g] = wn;
-  }
-  ds.stage(D);
-  gf::DRegs<N, T> dr;
-  dr.load(D, t.i, t.j);
-  __syncthreads();
+  for (int k = 0; k < N; k++) ru[k] = u[col + k * S::N2];
   const T jac = ejac[t.e];
   const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
-  double dot = 0.0;
-  const T p = (T)a.p;
-  auto post = [&](int k, T Lu, T M) {
+  auto post = [&](int k, T Lu, T M, T, T) {
     if (!t.active) return;
-    const size_t g = col + k * C::N2;
+    T acc = Lu;
+    if (!pf && p != T(1)) acc *= p;
+    if (qf) {
+      T tq = M * ru[k];
+      tq *= qf[col + k * S::N2];
+      acc += tq;
+    } else if (qs != T(0)) {
+      T tq = M * ru[k];
+      if (qs != T(1)) tq *= qs;
+      acc += tq;
+    }
+    uo[col + k * S::N2] = acc;
+  };
+  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, D, W, t.i, t.j, jac, f0 * f0,
+                           f1 * f1, f2 * f2, pf ? pf + t.e * S::NN : (const T *)nullptr,
+                           gf::NoPenCB(), gf::NoPenCB(), gf::NoPenCB(), post);
+}
+
+template <int N, typename T>
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
+    const T *__restrict__ r, const T *__restrict__ dinv, T *__restrict__ w, T *__restrict__ qv,
+    gf::PenD<T> Dp, const T *__restrict__ W, const T *__restrict__ edx,
+    const T *__restrict__ ejac, const T *__restrict__ qf, T qs, const T *__restrict__ mask, double pd,
+    const double *__restrict__ scal, double *__restrict__ part, size_t nelems, gf::BoxTopo topo,
+    int tmask) {
+  using S = gf::Pen3<N>;
+  __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
+  __shared__ double sred[4];
+  if (scal[S_DONE] != 0.0) return;
+  const bool first = scal[S_FIRST] != 0.0;
+  const T beta = (T)scal[S_BETA];
+  ColIdx<N> t(nelems);
+  const size_t col = t.e * S::NN + t.r;
+  const T jac = ejac[t.e];
+  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
+  // all loads of the column first, then the stores of w
+  T ru[N], dv[N], wv[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const size_t g = col + k * S::N2;
+    ru[k] = r[g];
+    dv[k] = dinv ? dinv[g] : T(1);
+    wv[k] = first ? T(0) : w[g];
+  }
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const T z = dinv ? dv[k] * ru[k] : ru[k];
+    ru[k] = first ? z : beta * wv[k] + T(1) * z;
+  }
+  if (t.active) {
+#pragma unroll
+    for (int k = 0; k < N; k++) w[col + k * S::N2] = ru[k];
+  }
+  double dot = 0.0;
+  const T p = (T)pd;
+  int ec[3] = {0, 0, 0};
+  if (tmask) gf::box_elem_coords(topo, t.e, ec);
+  auto post = [&](int k, T Lu, T M, T, T) {
+    if (!t.active) return;
+    const size_t g = col + k * S::N2;
     T acc = Lu;
     if (p != T(1)) acc *= p;
     if (qf) {
       T tq = M * ru[k];
       tq *= qf[g];
       acc += tq;
+    } else if (qs != T(0)) {
+      T tq = M * ru[k];
+      if (qs != T(1)) tq *= qs;
+      acc += tq;
     }
     if (mask) acc *= mask[g];
+    else if (tmask && gf::box_masked(topo, N, ec, t.i, t.j, k)) acc *= T(0);
     qv[g] = acc;
     dot += (double)ru[k] * (double)acc;
   };
-  gf::col_op3<N, T>(ru, su[t.el], sgx[t.el], sgy[t.el], dr, ds, W, t.i, t.j, jac, f0 * f0, f1 * f1,
-                    f2 * f2, (const T *)nullptr, gf::NoZ(), gf::NoXY(), post);
+  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, Dp, W, t.i, t.j, jac, f0 * f0,
+                           f1 * f1, f2 * f2, (const T *)nullptr, gf::NoPenCB(), gf::NoPenCB(),
+                           gf::NoPenCB(), post);
   double bs = gf::block_reduce<false>(dot, sred);
-  if (threadIdx.x == 0) a.part[blockIdx.x] = bs;
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
-// Fused Burgers stage, 3-D column form (see k_burgers_stage for the contract).
+// Fused Burgers stage, 3-D pencil form (see k_burgers_stage for the contract).
+// The x- and y-derivative factors of the advection term are carried to the
+// column owner through LDS, so every node's advection is summed in the
+// reference order ((ux rx) u0 + (uy ry) u1) + (uz rz) u2 (gadvect.ipp:58-100).
 template <int N>
-__global__ __launch_bounds__(gf::Col3<N>::THREADS, 3) void k_burgers_stage3c(StageArgs a) {
-  using C = gf::Col3<N>;
-  __shared__ double su[C::EPB][C::SL], sgx[C::EPB][C::SL], sgy[C::EPB][C::SL];
-  __shared__ gf::DShared<N, double> ds;
-  ds.stage(a.D);
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+void k_burgers_stage3p(
+    StageArgs a, gf::PenD<double> Dp) {
+  using S = gf::Pen3<N>;
+  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZ];
   ColIdx<N> t(a.nelems);
-  const size_t col = t.e * C::NN + t.r;
+  const size_t col = t.e * S::NN + t.r;
   double ru[3][N];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      const size_t g = col + k * C::N2;
+      const size_t g = col + k * S::N2;
       const double ui = a.uin[c][g];
       double v = ui;
       if (a.kprev[c]) {
@@ -325,122 +356,269 @@ __global__ __launch_bounds__(gf::Col3<N>::THREADS, 3) void k_burgers_stage3c(Sta
       ru[c][k] = v;
     }
   }
-  gf::DRegs<N, double> dr;
-  dr.load(a.D, t.i, t.j);
-  __syncthreads();
   const double jac = a.ejac[t.e];
   const double rx = a.edx[t.e * 3], ry = a.edx[t.e * 3 + 1], rz = a.edx[t.e * 3 + 2];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    double zpart[N];
-    double adv = 0.0;
-    // GAdvect::apply: po = (D1 p rx) u0 ; += (D2 p ry) u1 ; += (D3 p rz) u2 ; *= M
-    auto zcb = [&](int k, double uz, double) { zpart[k] = (uz * rz) * ru[2][k]; };
-    auto xycb = [&](int k, double ux, double uy, double M) {
-      double po = (ux * rx) * ru[0][k];
-      po += (uy * ry) * ru[1][k];
-      po += zpart[k];
-      adv = po * M;
-    };
-    auto post = [&](int k, double L, double) {
+    auto zcb = [&](int, double, double) {};
+    auto xcb = [&](int, double ux, double) { return ux * rx; };
+    auto ycb = [&](int, double uy, double) { return uy * ry; };
+    double *kout = a.kout[c];
+    auto post = [&](int k, double L, double M, double hx, double hy) {
       if (!t.active) return;
-      const size_t g = col + k * C::N2;
+      const size_t g = col + k * S::N2;
+      // z-derivative again (same operands and order as in the z pass)
+      using CP = const __attribute__((address_space(4))) double *;
+      const CP row = gf::opaque_ptr((CP)Dp.R[2] + k * N);
+      double uz = 0.0;
+#pragma unroll
+      for (int m2 = 0; m2 < N; m2++) uz = fma(row[m2], ru[c][m2], uz);
+      double po = hx * ru[0][k];
+      po += hy * ru[1][k];
+      po += (uz * rz) * ru[2][k];
+      const double adv = po * M;
       if (a.nu != 1.0) L *= a.nu;
       const double urhs = -1.0 * L + -1.0 * adv;
-      a.kout[c][g] = a.imass[g] * urhs;
+      kout[g] = a.imass[g] * urhs;
     };
-    gf::col_op3<N, double>(ru[c], su[t.el], sgx[t.el], sgy[t.el], dr, ds, a.W, t.i, t.j, jac,
-                           rx * rx, ry * ry, rz * rz, (const double *)nullptr, zcb, xycb, post);
+    gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, a.W, t.i, t.j, jac,
+                                 rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
+                                 post);
+    __syncthreads();  // the next component reuses the LDS buffers
   }
 }
 
-// ---------------------------------------------------------------------------
-// Pencil-form 3-D kernels (see gf_pen3.hpp): Helmholtz apply and CG apply.
-// ---------------------------------------------------------------------------
-template <int N, typename T>
-__global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_helm3p(
-    const T *__restrict__ u, T *__restrict__ uo, gf::PenD<T> D, const T *__restrict__ W,
-    const T *__restrict__ edx, const T *__restrict__ ejac, T p, const T *__restrict__ pf,
-    const T *__restrict__ qf, size_t nelems) {
+template <int N>
+int launch_stage3p(gf_mesh *m, const StageArgs &a) {
   using S = gf::Pen3<N>;
-  __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
-  ColIdx<N> t(nelems);
-  const size_t col = t.e * S::NN + t.r;
-  T ru[N];
-#pragma unroll
-  for (int k = 0; k < N; k++) ru[k] = u[col + k * S::N2];
-  const T jac = ejac[t.e];
-  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
-  auto post = [&](int k, T Lu, T M) {
-    if (!t.active) return;
-    T acc = Lu;
-    if (!pf && p != T(1)) acc *= p;
-    if (qf) {
-      T tq = M * ru[k];
-      tq *= qf[col + k * S::N2];
-      acc += tq;
-    }
-    uo[col + k * S::N2] = acc;
-  };
-  gf::pen_op3<N, T>(ru, su[t.el], sd[t.el], D, W, t.i, t.j, jac, f0 * f0, f1 * f1, f2 * f2,
-                    pf ? pf + t.e * S::NN : (const T *)nullptr, gf::NoPenCB(), gf::NoPenCB(),
-                    gf::NoPenCB(), post);
+  int nb = xcd_grid((m->nelems + S::EPB - 1) / S::EPB);
+  hipLaunchKernelGGL((k_burgers_stage3p<N>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream, a,
+                     pend_d(m));
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
 }
 
-template <int N, typename T>
-__global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
-    const T *__restrict__ r, const T *__restrict__ dinv, T *__restrict__ w, T *__restrict__ qv,
-    gf::PenD<T> Dp, const T *__restrict__ W, const T *__restrict__ edx,
-    const T *__restrict__ ejac, const T *__restrict__ qf, const T *__restrict__ mask, double pd,
-    const double *__restrict__ scal, double *__restrict__ part, size_t nelems) {
+// ---------------------------------------------------------------------------
+// Burgers RK on a structured box (mesh->has_box): the GGFX Smooth of each
+// stage derivative, of the final-stage input and of the step output is formed
+// by gather in the kernels that consume them (gf_boxdss.hpp), in the GGFX
+// summation order, instead of separate DSS passes.  Stage derivatives are
+// stored unsmoothed.
+//   kp       previous stage derivative K_{s-1} (unsmoothed) or NULL (stage 1)
+//   acc_mode 0: no accumulator; 1: acc = uin + Ks*ch; 2: acc += Ks*ch
+//   smooth_v the stage input is the final-stage input of step_b, which is
+//            smoothed: v = Smooth(uin + bh*Ks) (gexrk_stepper.ipp:300-318)
+// ---------------------------------------------------------------------------
+template <int N, bool KP, int ACC, bool SMV>
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict__ ui1,
+                       const double *__restrict__ ui2, const double *__restrict__ kp0,
+                       const double *__restrict__ kp1, const double *__restrict__ kp2,
+                       double *__restrict__ ac0, double *__restrict__ ac1, double *__restrict__ ac2,
+                       double *__restrict__ ko0, double *__restrict__ ko1, double *__restrict__ ko2,
+                       const double *__restrict__ imass, const double *__restrict__ W,
+                       const double *__restrict__ edx, const double *__restrict__ ejac,
+                       gf::PenD<double> Dp, double bh, double ch, double nu, gf::BoxTopo topo,
+                       size_t nelems) {
   using S = gf::Pen3<N>;
-  __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
-  __shared__ double sred[4];
-  if (scal[S_DONE] != 0.0) return;
-  const bool first = scal[S_FIRST] != 0.0;
-  const T beta = (T)scal[S_BETA];
+  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZ];
+  const double *uin[3] = {ui0, ui1, ui2};
+  const double *kp[3] = {kp0, kp1, kp2};
+  double *acc[3] = {ac0, ac1, ac2};
+  double *kout[3] = {ko0, ko1, ko2};
   ColIdx<N> t(nelems);
   const size_t col = t.e * S::NN + t.r;
-  T ru[N];
+  gf::BoxElemDeltas de = {{0, 0, 0}, {0, 0, 0}};
+  if (KP) {
+    int ec[3];
+    gf::box_elem_coords(topo, t.e, ec);
+    de = gf::box_elem_deltas(topo, N, ec);
+  }
+  // Stage input (and accumulator) per node; all loads are issued before any
+  // store, the accumulator stores follow the loop.
+  double ru[3][N];
+  double av[ACC ? 3 : 1][ACC ? N : 1];
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    const size_t g = col + k * S::N2;
-    const T z = dinv ? dinv[g] * r[g] : r[g];
-    const T wn = first ? z : beta * w[g] + T(1) * z;
-    ru[k] = wn;
-    if (t.active) w[g] = wn;
-  }
-  const T jac = ejac[t.e];
-  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
-  double dot = 0.0;
-  const T p = (T)pd;
-  auto post = [&](int k, T Lu, T M) {
-    if (!t.active) return;
-    const size_t g = col + k * S::N2;
-    T acc = Lu;
-    if (p != T(1)) acc *= p;
-    if (qf) {
-      T tq = M * ru[k];
-      tq *= qf[g];
-      acc += tq;
+    const size_t g = col + (size_t)k * S::N2;
+    if (!KP) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) ru[c][k] = uin[c][g];
+      continue;
     }
-    if (mask) acc *= mask[g];
-    qv[g] = acc;
-    dot += (double)ru[k] * (double)acc;
-  };
-  gf::pen_op3<N, T>(ru, su[t.el], sd[t.el], Dp, W, t.i, t.j, jac, f0 * f0, f1 * f1, f2 * f2,
-                    (const T *)nullptr, gf::NoPenCB(), gf::NoPenCB(), gf::NoPenCB(), post);
-  double bs = gf::block_reduce<false>(dot, sred);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+    size_t mem[8];
+    const int cnt = gf::box_members<size_t>(de, N, g, t.i, t.j, k, mem);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double kv[8];
+      gf::box_load8<size_t, double>(kp[c], mem, cnt, g, kv);
+      const double ui = uin[c][g];
+      const double ao = ACC == 2 ? acc[c][g] : 0.0;
+      double uv[8];
+      if (SMV) gf::box_load8<size_t, double>(uin[c], mem, cnt, g, uv);
+      // GGFX Smooth: Sum / count (an exact multiply for counts 1, 2, 4, 8)
+      const double ks = cnt == 1 ? kv[0] : gf::box_sum8<double>(kv, cnt) * gf::box_inv(cnt);
+      double v = ui + bh * ks;
+      if (ACC) av[c][k] = (ACC == 1 ? ui : ao) + ks * ch;
+      if (SMV) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) uv[q] = uv[q] + bh * ks;
+        if (cnt > 1) v = gf::box_sum8<double>(uv, cnt) * gf::box_inv(cnt);
+      }
+      ru[c][k] = v;
+    }
+  }
+  if (ACC && t.active) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int k = 0; k < N; k++) acc[c][col + (size_t)k * S::N2] = av[ACC ? c : 0][ACC ? k : 0];
+  }
+  const double jac = ejac[t.e];
+  const double rx = edx[t.e * 3], ry = edx[t.e * 3 + 1], rz = edx[t.e * 3 + 2];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    auto zcb = [&](int, double, double) {};
+    auto xcb = [&](int, double ux, double) { return ux * rx; };
+    auto ycb = [&](int, double uy, double) { return uy * ry; };
+    double *ko = kout[c];
+    auto post = [&](int k, double L, double M, double hx, double hy) {
+      if (!t.active) return;
+      const size_t g = col + k * S::N2;
+      // z-derivative again (same operands and order as in the z pass)
+      using CP = const __attribute__((address_space(4))) double *;
+      const CP row = gf::opaque_ptr((CP)Dp.R[2] + k * N);
+      double uz = 0.0;
+#pragma unroll
+      for (int m2 = 0; m2 < N; m2++) uz = fma(row[m2], ru[c][m2], uz);
+      double po = hx * ru[0][k];
+      po += hy * ru[1][k];
+      po += (uz * rz) * ru[2][k];
+      const double adv = po * M;
+      if (nu != 1.0) L *= nu;
+      const double urhs = -1.0 * L + -1.0 * adv;
+      ko[g] = imass[g] * urhs;
+    };
+    gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, W, t.i, t.j, jac,
+                                 rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
+                                 post);
+    __syncthreads();  // the next component reuses the LDS buffers
+  }
+}
+
+// Step output on a structured box: u = Smooth(1*acc + cl*Smooth(K)) with both
+// smooths by gather (step_b's last update and GGFX Smooth).
+template <int N, typename I>
+__global__ __launch_bounds__(kThreads) void k_rk_out_box(
+    double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2,
+    const double *__restrict__ a0, const double *__restrict__ a1, const double *__restrict__ a2,
+    const double *__restrict__ k0, const double *__restrict__ k1, const double *__restrict__ k2,
+    double cl, gf::BoxTopo topo, size_t nelems) {
+  constexpr int NN = N * N * N;
+  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;
+  constexpr int CN = CE * NN;
+  double *u[3] = {u0, u1, u2};
+  const double *acc[3] = {a0, a1, a2};
+  const double *kf[3] = {k0, k1, k2};
+  const size_t nchunks = (nelems + CE - 1) / CE;
+  const size_t cpb = (nchunks + gridDim.x - 1) / gridDim.x;
+  const size_t c0 = xcd_block() * cpb, c1 = c0 + cpb < nchunks ? c0 + cpb : nchunks;
+  for (size_t ch = c0; ch < c1; ch++) {
+    int ec0[3];
+    gf::box_elem_coords(topo, ch * CE, ec0);
+    const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);
+#pragma unroll
+    for (int q0 = 0; q0 < CN; q0 += kThreads) {
+      const int q = q0 + (int)threadIdx.x;
+      if (q >= CN) break;
+      const int el = CE == 1 ? 0 : q / NN;
+      const int rr = CE == 1 ? q : q - el * NN;
+      const size_t e = ch * CE + el;
+      if (e >= nelems) break;
+      gf::BoxElemDeltas de = d0;
+      if (CE > 1 && el) {
+        int ec[3];
+        gf::box_elem_coords(topo, e, ec);
+        de = gf::box_elem_deltas(topo, N, ec);
+      }
+      const int i = rr % N, j = (rr / N) % N, k = rr / (N * N);
+      const I g = (I)(e * NN + rr);
+      I mem[8];
+      const int cnt = gf::box_members<I>(de, N, g, i, j, k, mem);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double kv[8], av[8];
+        gf::box_load8<I, double>(kf[c], mem, cnt, g, kv);
+        gf::box_load8<I, double>(acc[c], mem, cnt, g, av);
+        const double ks = cnt == 1 ? kv[0] : gf::box_sum8<double>(kv, cnt) * gf::box_inv(cnt);
+#pragma unroll
+        for (int m = 0; m < 8; m++) av[m] = 1.0 * av[m] + cl * ks;
+        u[c][g] = cnt == 1 ? av[0] : gf::box_sum8<double>(av, cnt) * gf::box_inv(cnt);
+      }
+    }
+  }
+}
+
+template <int N, bool KP, int ACC, bool SMV>
+void launch_stage3b_v(gf_mesh *m, int nb, double *const *uin, double *const *kp,
+                      double *const *acc, double *const *kout, double bh, double ch, double nu) {
+  using S = gf::Pen3<N>;
+  hipLaunchKernelGGL((k_burgers_stage3b<N, KP, ACC, SMV>), dim3(nb), dim3(S::THREADS), 0,
+                     m->ctx->stream, uin[0], uin[1], uin[2], kp ? kp[0] : nullptr,
+                     kp ? kp[1] : nullptr, kp ? kp[2] : nullptr, acc ? acc[0] : nullptr,
+                     acc ? acc[1] : nullptr, acc ? acc[2] : nullptr, kout[0], kout[1], kout[2],
+                     m->imass, m->W[0], m->edx, m->ejac, pend_d(m), bh, ch, nu, m->box,
+                     m->nelems);
+}
+
+template <int N>
+int launch_stage3b(gf_mesh *m, double *const *uin, double *const *kp, double *const *acc,
+                   double *const *kout, double bh, double ch, double nu, int acc_mode,
+                   int smooth_v) {
+  using S = gf::Pen3<N>;
+  int nb = xcd_grid((m->nelems + S::EPB - 1) / S::EPB);
+  if (!kp)
+    launch_stage3b_v<N, false, 0, false>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else if (acc_mode == 1 && !smooth_v)
+    launch_stage3b_v<N, true, 1, false>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else if (acc_mode == 2 && !smooth_v)
+    launch_stage3b_v<N, true, 2, false>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else if (acc_mode == 1 && smooth_v)
+    launch_stage3b_v<N, true, 1, true>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else if (acc_mode == 2 && smooth_v)
+    launch_stage3b_v<N, true, 2, true>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else
+    return gf_set_err(m->ctx, GF_ERR_ARG, "stage3b: unsupported stage form");
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int N>
+int launch_rk_out_box(gf_mesh *m, double *const *u, double *const *acc, double *const *kf,
+                      double cl) {
+  constexpr int NN = N * N * N;
+  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;
+  size_t nchunks = (m->nelems + CE - 1) / CE;
+  int nb = xcd_grid(std::min<size_t>(nchunks, 8192));
+  if (m->ndof < ((size_t)1 << 31))
+    hipLaunchKernelGGL((k_rk_out_box<N, int32_t>), dim3(nb), dim3(kThreads), 0, m->ctx->stream,
+                       u[0], u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
+                       m->nelems);
+  else
+    hipLaunchKernelGGL((k_rk_out_box<N, int64_t>), dim3(nb), dim3(kThreads), 0, m->ctx->stream,
+                       u[0], u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
+                       m->nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
 }
 
 template <int N, typename T>
-int launch_helm3p(gf_mesh *m, T p, const T *pf, const T *qf, const T *u, T *uo,
+int launch_helm3p(gf_mesh *m, T p, const T *pf, const T *qf, T qs, const T *u, T *uo,
                   const gf::PenD<T> &D, const T *W, const T *edx, const T *ejac) {
   using S = gf::Pen3<N>;
-  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  int nb = xcd_grid((m->nelems + S::EPB - 1) / S::EPB);
   hipLaunchKernelGGL((k_helm3p<N, T>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream, u, uo, D, W,
-                     edx, ejac, p, pf, qf, m->nelems);
+                     edx, ejac, p, pf, qf, qs, m->nelems);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
 }
@@ -448,7 +626,7 @@ int launch_helm3p(gf_mesh *m, T p, const T *pf, const T *qf, const T *u, T *uo,
 template <int N, typename T>
 int launch_cg_apply3p(gf_mesh *m, const CGApply3Args &a) {
   using S = gf::Pen3<N>;
-  int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  int nb = xcd_grid((m->nelems + S::EPB - 1) / S::EPB);
   gf::PenD<T> Dp;
   for (int c = 0; c < 3; c++) {
     Dp.R[c] = (const T *)a.DR[c];
@@ -456,37 +634,8 @@ int launch_cg_apply3p(gf_mesh *m, const CGApply3Args &a) {
   }
   hipLaunchKernelGGL((k_cg_apply3p<N, T>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream,
                      (const T *)a.r, (const T *)a.dinv, (T *)a.w, (T *)a.q, Dp, (const T *)a.W,
-                     (const T *)a.edx, (const T *)a.ejac, (const T *)a.qf, (const T *)a.mask, a.p,
-                     (const double *)a.scal, a.part, a.nelems);
-  GF_LAUNCH_CHECK(m->ctx);
-  return GF_OK;
-}
-
-template <int N, typename T>
-int launch_helm3c(gf_mesh *m, T p, const T *pf, const T *qf, const T *u, T *uo, const T *D,
-                  const T *W, const T *edx, const T *ejac) {
-  using C = gf::Col3<N>;
-  int nb = (int)((m->nelems + C::EPB - 1) / C::EPB);
-  hipLaunchKernelGGL((k_helm3c<N, T>), dim3(nb), dim3(C::THREADS), 0, m->ctx->stream, u, uo, D, W,
-                     edx, ejac, p, pf, qf, m->nelems);
-  GF_LAUNCH_CHECK(m->ctx);
-  return GF_OK;
-}
-
-template <int N, typename T>
-int launch_cg_apply3c(gf_mesh *m, const CGApply3Args &a) {
-  using C = gf::Col3<N>;
-  int nb = (int)((m->nelems + C::EPB - 1) / C::EPB);
-  hipLaunchKernelGGL((k_cg_apply3c<N, T>), dim3(nb), dim3(C::THREADS), 0, m->ctx->stream, a);
-  GF_LAUNCH_CHECK(m->ctx);
-  return GF_OK;
-}
-
-template <int N>
-int launch_stage3c(gf_mesh *m, const StageArgs &a) {
-  using C = gf::Col3<N>;
-  int nb = (int)((m->nelems + C::EPB - 1) / C::EPB);
-  hipLaunchKernelGGL((k_burgers_stage3c<N>), dim3(nb), dim3(C::THREADS), 0, m->ctx->stream, a);
+                     (const T *)a.edx, (const T *)a.ejac, (const T *)a.qf, (T)a.qs, (const T *)a.mask, a.p,
+                     (const double *)a.scal, a.part, a.nelems, a.topo, a.tmask);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
 }
@@ -494,7 +643,7 @@ int launch_stage3c(gf_mesh *m, const StageArgs &a) {
 // Number of workgroups (= partial sums) the 3-D column apply launches.
 inline int col_blocks(gf_mesh *m) {
   int N = m->Niso, N2 = N * N, epb = N2 >= 64 ? 1 : 64 / N2;
-  return (int)((m->nelems + epb - 1) / epb);
+  return xcd_grid((m->nelems + epb - 1) / epb);
 }
 
 #define GF_DISPATCH_N3(NV, FN, ...)                   \
@@ -514,12 +663,13 @@ inline int col_blocks(gf_mesh *m) {
   }()
 
 template <int N>
-int helm3c_d(gf_mesh *m, double p, const double *pf, const double *qf, const double *u, double *uo) {
-  return launch_helm3p<N, double>(m, p, pf, qf, u, uo, pend_d(m), m->W[0], m->edx, m->ejac);
+int helm3c_d(gf_mesh *m, double p, const double *pf, const double *qf, double qs, const double *u,
+             double *uo) {
+  return launch_helm3p<N, double>(m, p, pf, qf, qs, u, uo, pend_d(m), m->W[0], m->edx, m->ejac);
 }
 template <int N>
-int helm3c_f(gf_mesh *m, float p, const float *qf, const float *u, float *uo) {
-  return launch_helm3p<N, float>(m, p, (const float *)nullptr, qf, u, uo, pend_f(m), m->W32,
+int helm3c_f(gf_mesh *m, float p, const float *qf, float qs, const float *u, float *uo) {
+  return launch_helm3p<N, float>(m, p, (const float *)nullptr, qf, qs, u, uo, pend_f(m), m->W32,
                                  m->edx32, m->ejac32);
 }
 template <int N>
@@ -560,6 +710,7 @@ struct CGApplyArgs {
   void *q;            // T
   const void *D, *W, *edx, *ejac, *qf;  // T
   double p;
+  double qs;  // mass-term scalar when qf is NULL (0: no mass term)
   double *scal;
   double *part;
   size_t nelems;
@@ -622,10 +773,11 @@ __global__ __launch_bounds__(kThreads) void k_cg_apply(CGApplyArgs a) {
 #pragma unroll
     for (int d = 0; d < DIM; d++) acc += gf::contract<DIM, N, true>(&sg[d][el * S::NN], sD, d, i, j, k);
     if (p != (T)1) acc *= p;
-    if (qf) {
+    if (qf || a.qs != 0.0) {
       T M = gf::node_mass<DIM>(sW, i, j, k, ejac[e0 + el]);
       T t = M * su[q];
-      t *= qf[base + q];
+      if (qf) t *= qf[base + q];
+      else if ((T)a.qs != (T)1) t *= (T)a.qs;
       acc += t;
     }
     qv[base + q] = acc;
@@ -688,19 +840,92 @@ __global__ void k_cg_update(T *__restrict__ x, const T *__restrict__ w, T *__res
   T alpha = (T)scal[S_ALPHA];
   double rho = 0.0, nrm = 0.0;
   GRID_LOOP(g, n) {
+    const T dv = dinv ? dinv[g] : T(1);
+    const float im = imult[g];
     T xv = (T)1 * x[g] + alpha * w[g];
-    x[g] = xv;
     T qm = mask ? qv[g] * mask[g] : qv[g];
     T rv = (T)1 * r[g] + (-alpha) * qm;
+    x[g] = xv;
     r[g] = rv;
-    T z = dinv ? dinv[g] * rv : rv;
-    rho += (double)rv * (double)z * (double)imult[g];
+    T z = dinv ? dv * rv : rv;
+    rho += (double)rv * (double)z * (double)im;
     double rd = (double)rv;
     if (NORM == GF_NORM_INF) nrm = fmax(nrm, fabs(rd));
     else if (NORM == GF_NORM_EUC) nrm += rd * rd;
     else if (NORM == GF_NORM_L2) nrm += mass[g] * rd * rd;
     else if (NORM == GF_NORM_L1) nrm += mass[g] * fabs(rd);
-    else nrm += rd * rd * (double)imult[g];
+    else nrm += rd * rd * (double)im;
+  }
+  double a = gf::block_reduce<false>(rho, sred);
+  double b = NORM == GF_NORM_INF ? gf::block_reduce<true>(nrm, sred)
+                                 : gf::block_reduce<false>(nrm, sred);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// k_cg_update on a structured box: q arrives unassembled (straight from the
+// apply) and is assembled here by gather (gf_boxdss.hpp), in the GGFX order,
+// so no DSS pass runs between the apply and the update.  The multiplicity
+// weight is 1/(group size), which equals GGFX::get_imult.  Work is split in
+// chunks of whole elements so that an element's box coordinates are formed
+// once per chunk (and are wave-uniform when a chunk is one element).
+template <typename T, int NORM, int N, typename I>
+__global__ __launch_bounds__(kThreads) void k_cg_update_box(
+    T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r, const T *__restrict__ qv,
+    const T *__restrict__ dinv, const double *__restrict__ mass, gf::BoxTopo topo, double *scal,
+    double *__restrict__ part, size_t nelems) {
+  constexpr int NN = N * N * N;
+  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;  // elements per chunk
+  constexpr int CN = CE * NN;                              // nodes per chunk
+  __shared__ double sred[kThreads / 64];
+  if (scal[S_DONE] != 0.0) return;
+  const T alpha = (T)scal[S_ALPHA];
+  double rho = 0.0, nrm = 0.0;
+  // each (XCD-renumbered) block walks a contiguous range of chunks
+  const size_t nchunks = (nelems + CE - 1) / CE;
+  const size_t cpb = (nchunks + gridDim.x - 1) / gridDim.x;
+  const size_t c0 = xcd_block() * cpb, c1 = c0 + cpb < nchunks ? c0 + cpb : nchunks;
+  for (size_t c = c0; c < c1; c++) {
+    int ec0[3];
+    gf::box_elem_coords(topo, c * CE, ec0);
+    const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);  // uniform when CE == 1
+#pragma unroll
+    for (int q0 = 0; q0 < CN; q0 += kThreads) {
+      const int q = q0 + (int)threadIdx.x;
+      if (q >= CN) break;
+      const int el = CE == 1 ? 0 : q / NN;
+      const int rr = CE == 1 ? q : q - el * NN;
+      const size_t e = c * CE + el;
+      if (e >= nelems) break;
+      gf::BoxElemDeltas de = d0;
+      if (CE > 1 && el) {
+        int ec[3];
+        gf::box_elem_coords(topo, e, ec);
+        de = gf::box_elem_deltas(topo, N, ec);
+      }
+      const int i = rr % N, j = (rr / N) % N, k = rr / (N * N);
+      const I g = (I)(e * NN + rr);
+      // every load of the node is issued before the first store
+      const T dv = dinv ? dinv[g] : T(1);
+      const T xo = x[g], wo = w[g], ro = r[g];
+      int cnt;
+      const T qa = gf::box_gather_sum<N, I, T>(de, qv, g, i, j, k, &cnt);
+      const T xv = (T)1 * xo + alpha * wo;
+      x[g] = xv;
+      const T rv = (T)1 * ro + (-alpha) * qa;
+      r[g] = rv;
+      const T z = dinv ? dv * rv : rv;
+      const double im = gf::box_inv(cnt);
+      rho += (double)rv * (double)z * im;
+      const double rd = (double)rv;
+      if (NORM == GF_NORM_INF) nrm = fmax(nrm, fabs(rd));
+      else if (NORM == GF_NORM_EUC) nrm += rd * rd;
+      else if (NORM == GF_NORM_L2) nrm += mass[g] * rd * rd;
+      else if (NORM == GF_NORM_L1) nrm += mass[g] * fabs(rd);
+      else nrm += rd * rd * im;
+    }
   }
   double a = gf::block_reduce<false>(rho, sred);
   double b = NORM == GF_NORM_INF ? gf::block_reduce<true>(nrm, sred)
@@ -737,6 +962,7 @@ __global__ void k_cg_beta(double *scal, const double *__restrict__ part, int np,
                                       : gf::block_reduce<false>(w, sred);
   if (threadIdx.x == 0) {
     double rhom = scal[S_RHO];
+    scal[S_RHOM] = rhom;
     scal[S_RHO] = rho;
     scal[S_BETA] = rho / rhom;
     double rn = finish_norm(normtype, nr, scal);
@@ -795,6 +1021,9 @@ __global__ void k_mul_t(const T *__restrict__ a, const T *__restrict__ b, T *__r
                         size_t n) {
   GRID_LOOP(g, n) out[g] = a[g] * b[g];
 }
+__global__ void k_fill_f32(float *__restrict__ o, float v, size_t n) {
+  GRID_LOOP(g, n) o[g] = v;
+}
 __global__ void k_to_f32(float *__restrict__ o, const double *__restrict__ i, double s, size_t n) {
   GRID_LOOP(g, n) o[g] = (float)(i[g] * s);
 }
@@ -813,7 +1042,7 @@ template <int DIM, int N>
 __global__ void k_helm_diag(double *__restrict__ d, const double *__restrict__ D,
                             const double *__restrict__ W, const double *__restrict__ edx,
                             const double *__restrict__ ejac, double p,
-                            const double *__restrict__ qf, size_t ndof) {
+                            const double *__restrict__ qf, double qs, size_t ndof) {
   constexpr int NN = Shape<DIM, N>::NN;
   GRID_LOOP(g, ndof) {
     size_t e = g / NN;
@@ -836,6 +1065,7 @@ __global__ void k_helm_diag(double *__restrict__ d, const double *__restrict__ D
     }
     double val = p * lap;
     if (qf) val += qf[g] * gf::node_mass<DIM>(W, idx[0], idx[1], idx[2], jac);
+    else if (qs != 0.0) val += qs * gf::node_mass<DIM>(W, idx[0], idx[1], idx[2], jac);
     d[g] = val;
   }
 }
@@ -847,9 +1077,9 @@ __global__ void k_recip2(double *__restrict__ a, float *__restrict__ af, size_t 
 }
 
 template <int DIM, int N>
-int launch_diag(gf_mesh *m, double p, const double *qf, double *d) {
+int launch_diag(gf_mesh *m, double p, const double *qf, double qs, double *d) {
   hipLaunchKernelGGL((k_helm_diag<DIM, N>), dim3(nblocks(m->ndof)), dim3(kThreads), 0,
-                     m->ctx->stream, d, m->D[0], m->W[0], m->edx, m->ejac, p, qf, m->ndof);
+                     m->ctx->stream, d, m->D[0], m->W[0], m->edx, m->ejac, p, qf, qs, m->ndof);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
 }
@@ -875,7 +1105,7 @@ __global__ __launch_bounds__(kThreads) void k_helm_f32(const float *__restrict__
                                                        const float *__restrict__ W,
                                                        const float *__restrict__ edx,
                                                        const float *__restrict__ ejac, float p,
-                                                       const float *__restrict__ qf,
+                                                       const float *__restrict__ qf, float qs,
                                                        size_t nelems) {
   using S = Shape<DIM, N>;
   __shared__ float su[S::NODES];
@@ -910,30 +1140,31 @@ __global__ __launch_bounds__(kThreads) void k_helm_f32(const float *__restrict__
 #pragma unroll
     for (int d = 0; d < DIM; d++) acc += gf::contract<DIM, N, true>(&sg[d][el * S::NN], sD, d, i, j, k);
     if (p != 1.f) acc *= p;
-    if (qf) {
+    if (qf || qs != 0.f) {
       float M = gf::node_mass<DIM>(sW, i, j, k, ejac[e0 + el]);
       float t = M * su[q];
-      t *= qf[base + q];
+      if (qf) t *= qf[base + q];
+      else if (qs != 1.f) t *= qs;
       acc += t;
     }
     uo[base + q] = acc;
   }
 }
 template <int DIM, int N>
-int launch_helm_f32(gf_mesh *m, float p, const float *qf, const float *u, float *uo) {
+int launch_helm_f32(gf_mesh *m, float p, const float *qf, float qs, const float *u, float *uo) {
   using S = Shape<DIM, N>;
   int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
   hipLaunchKernelGGL((k_helm_f32<DIM, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, uo,
-                     m->D32, m->W32, m->edx32, m->ejac32, p, qf, m->nelems);
+                     m->D32, m->W32, m->edx32, m->ejac32, p, qf, qs, m->nelems);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
 }
 
 }  // namespace
 
-int gf_helm3c(gf_mesh *m, double p, const double *pf, const double *qf, const double *u,
-              double *uo) {
-  return GF_DISPATCH_N3(m->Niso, helm3c_d, m, p, pf, qf, u, uo);
+int gf_helm3c(gf_mesh *m, double p, const double *pf, const double *qf, double qs,
+              const double *u, double *uo) {
+  return GF_DISPATCH_N3(m->Niso, helm3c_d, m, p, pf, qf, qs, u, uo);
 }
 
 int gf_axpy_t(gf_mesh *m, double a, const double *x, double *y) {
@@ -963,13 +1194,13 @@ static int copy(gf_mesh *m, double *y, const double *x) {
 
 extern "C" {
 
-int gf_helmholtz_apply_f32(gf_mesh *m, float p_scalar, const float *q_field, const float *u,
-                           float *uo) {
+int gf_helmholtz_apply_f32(gf_mesh *m, float p_scalar, float q_scalar, const float *q_field,
+                           const float *u, float *uo) {
   if (!m || !u || !uo) return GF_ERR_ARG;
   if (m->gtype != GF_GE_REGULAR || !m->Niso)
     return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "FP32 Helmholtz: regular isotropic meshes");
-  if (m->dim == 3) return GF_DISPATCH_N3(m->Niso, helm3c_f, m, p_scalar, q_field, u, uo);
-  return GF_DISPATCH_N(m->dim, m->Niso, launch_helm_f32, m, p_scalar, q_field, u, uo);
+  if (m->dim == 3) return GF_DISPATCH_N3(m->Niso, helm3c_f, m, p_scalar, q_field, q_scalar, u, uo);
+  return GF_DISPATCH_N(m->dim, m->Niso, launch_helm_f32, m, p_scalar, q_field, q_scalar, u, uo);
 }
 
 // GAdvect::apply (gadvect.ipp:57-99): po = deriv(p,1)*u0 ; po += deriv(p,j+1)*uj ;
@@ -1005,7 +1236,7 @@ int gf_burgers_rhs(gf_mesh *m, int mode, double nu, int nstate, const double *co
   if (!urhs) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
   if (mode == 2) {
     for (int k = 0; k < nstate; k++) {
-      RC(gf_helm_t(m, nu, nullptr, nullptr, u[k], urhs));
+      RC(gf_helm_t(m, nu, nullptr, nullptr, 0.0, u[k], urhs));
       RC(axpby(m, urhs, -1.0, urhs, 0.0, urhs));
       RC(gf_mul_t(m, m->imass, urhs, dudt[k]));
     }
@@ -1013,14 +1244,14 @@ int gf_burgers_rhs(gf_mesh *m, int mode, double nu, int nstate, const double *co
   }
   if (mode == 1) {
     RC(gf_advect(m, u[0], nstate - 1, u + 1, dudt[0]));
-    RC(gf_helm_t(m, nu, nullptr, nullptr, u[0], urhs));
+    RC(gf_helm_t(m, nu, nullptr, nullptr, 0.0, u[0], urhs));
     RC(axpby(m, urhs, -1.0, urhs, -1.0, dudt[0]));
     return gf_mul_t(m, m->imass, urhs, dudt[0]);
   }
   if (nstate < m->dim) return gf_set_err(m->ctx, GF_ERR_ARG, "nonlinear Burgers needs dim components");
   for (int k = 0; k < m->dim; k++) {
     RC(gf_advect(m, u[k], nstate, u, dudt[k]));
-    RC(gf_helm_t(m, nu, nullptr, nullptr, u[k], urhs));
+    RC(gf_helm_t(m, nu, nullptr, nullptr, 0.0, u[k], urhs));
     RC(axpby(m, urhs, -1.0, urhs, -1.0, dudt[k]));
     RC(gf_mul_t(m, m->imass, urhs, dudt[k]));
   }
@@ -1097,7 +1328,7 @@ int rk_step_fused(gf_mesh *m, double nu, int order, double dt, double *const *u)
     }
     a.bh = s > 0 ? beta[s * 4 + (s - 1)] * h : 0.0;
     a.ch = s > 0 ? c[s - 1] * h : 0.0;
-    RC(m->dim == 3 ? GF_DISPATCH_N3(m->Niso, launch_stage3c, m, a)
+    RC(m->dim == 3 ? GF_DISPATCH_N3(m->Niso, launch_stage3p, m, a)
                    : GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, a));
     for (int k = 0; k < nc; k++) RC(gf_dss_t(m, GF_DSS_SMOOTH, K[s % 2][k]));
   }
@@ -1125,7 +1356,7 @@ int rk_step_fused(gf_mesh *m, double nu, int order, double dt, double *const *u)
   }
   a.bh = 0.0;
   a.ch = 0.0;
-  RC(m->dim == 3 ? GF_DISPATCH_N3(m->Niso, launch_stage3c, m, a)
+  RC(m->dim == 3 ? GF_DISPATCH_N3(m->Niso, launch_stage3p, m, a)
                  : GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, a));
   double cl = c[ns - 1] * h;
   for (int k = 0; k < nc; k++) {
@@ -1138,6 +1369,34 @@ int rk_step_fused(gf_mesh *m, double nu, int order, double dt, double *const *u)
     RC(gf_dss_t(m, GF_DSS_SMOOTH, u[k]));
   }
   return GF_OK;
+}
+
+// RK step on a structured box (order >= 2): the same arithmetic as
+// rk_step_fused, with every Smooth done by gather inside the stage kernels
+// and the output kernel.  Per step: `order` stage kernels and one output
+// kernel, no DSS passes.
+int rk_step_box(gf_mesh *m, double nu, int order, double dt, double *const *u) {
+  size_t bytes = m->ndof * sizeof(double);
+  double alpha[4], beta[16], c[4];
+  butcher(order, alpha, beta, c);
+  double *acc[3], *K[2][3];
+  for (int k = 0; k < 3; k++) {
+    acc[k] = (double *)gf_ws(m, WS_RK_BEGIN + k, bytes);
+    for (int s = 0; s < 2; s++) K[s][k] = (double *)gf_ws(m, WS_RK_BEGIN + 6 + 3 * s + k, bytes);
+    if (!acc[k] || !K[0][k] || !K[1][k]) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  }
+  const double h = dt;
+  const int ns = order;
+  // stage 1 from the state itself
+  RC(GF_DISPATCH_N3(m->Niso, launch_stage3b, m, u, (double *const *)nullptr,
+                    (double *const *)nullptr, K[0], 0.0, 0.0, nu, 0, 0));
+  // middle stages and the final stage (whose input is smoothed)
+  for (int s = 1; s < ns; s++) {
+    const bool last = s == ns - 1;
+    RC(GF_DISPATCH_N3(m->Niso, launch_stage3b, m, u, K[(s - 1) % 2], acc, K[s % 2],
+                      beta[s * 4 + (s - 1)] * h, c[s - 1] * h, nu, s == 1 ? 1 : 2, last ? 1 : 0));
+  }
+  return GF_DISPATCH_N3(m->Niso, launch_rk_out_box, m, u, acc, K[(ns - 1) % 2], c[ns - 1] * h);
 }
 
 // General RK step composed from the operators, following step_b literally
@@ -1228,7 +1487,13 @@ int gf_burgers_step(gf_mesh *m, int mode, double nu, int order, int nstate, doub
   for (int k = 0; k < nstate; k++)
     if (!u[k]) return GF_ERR_ARG;
   bool fused = mode == 0 && m->gtype == GF_GE_REGULAR && m->Niso && nstate == m->dim;
-  int rc = fused ? rk_step_fused(m, nu, order, dt, u) : rk_step_general(m, mode, nu, order, nstate, dt, u);
+  int rc;
+  if (fused && m->has_box && order >= 2)
+    rc = rk_step_box(m, nu, order, dt, u);
+  else if (fused)
+    rc = rk_step_fused(m, nu, order, dt, u);
+  else
+    rc = rk_step_general(m, mode, nu, order, nstate, dt, u);
   if (rc) return rc;
   if (check) return check_finite(m, u, mode == 0 ? nstate : 1);
   return GF_OK;
@@ -1243,11 +1508,11 @@ int gf_burgers_steps(gf_mesh *m, int mode, double nu, int order, int nstate, dou
   return GF_OK;
 }
 
-int gf_helmholtz_diag_inv(gf_mesh *m, double p, const double *qf, double *dinv) {
+int gf_helmholtz_diag_inv(gf_mesh *m, double p, double qs, const double *qf, double *dinv) {
   if (!m || !dinv) return GF_ERR_ARG;
   if (m->gtype != GF_GE_REGULAR || !m->Niso)
     return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Jacobi diagonal: regular isotropic meshes");
-  RC(GF_DISPATCH_N(m->dim, m->Niso, launch_diag, m, p, qf, dinv));
+  RC(GF_DISPATCH_N(m->dim, m->Niso, launch_diag, m, p, qf, qs, dinv));
   RC(gf_dss_t(m, GF_DSS_SUM, dinv));
   hipLaunchKernelGGL(k_recip2, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, dinv,
                      (float *)nullptr, m->ndof);
@@ -1272,6 +1537,7 @@ struct CGVecs {
   const float *imult;
   const void *D, *W, *edx, *ejac;
   double p;
+  double qs;  // mass-term scalar when qf is NULL
 };
 
 // Reduce (r.z.imult, norm(r)) at set-up, synchronous; returns both.
@@ -1298,6 +1564,36 @@ double finish_norm_host(int normtype, double v, gf_mesh *m, double volume) {
   }
 }
 
+template <typename T, int NORM, int N>
+int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part) {
+  // 32-bit node indices whenever every index of the mesh fits
+  if (m->ndof < ((size_t)1 << 31))
+    hipLaunchKernelGGL((k_cg_update_box<T, NORM, N, int32_t>), dim3(kUpdBlocks), dim3(kThreads), 0,
+                       m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal, part,
+                       m->nelems);
+  else
+    hipLaunchKernelGGL((k_cg_update_box<T, NORM, N, int64_t>), dim3(kUpdBlocks), dim3(kThreads), 0,
+                       m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal, part,
+                       m->nelems);
+  return GF_OK;
+}
+
+template <typename T, int NORM>
+int launch_update_box(gf_mesh *m, CGVecs<T> &v, double *part) {
+  switch (m->Niso) {
+    case 2: return launch_update_box_n<T, NORM, 2>(m, v, part);
+    case 3: return launch_update_box_n<T, NORM, 3>(m, v, part);
+    case 4: return launch_update_box_n<T, NORM, 4>(m, v, part);
+    case 5: return launch_update_box_n<T, NORM, 5>(m, v, part);
+    case 6: return launch_update_box_n<T, NORM, 6>(m, v, part);
+    case 7: return launch_update_box_n<T, NORM, 7>(m, v, part);
+    case 8: return launch_update_box_n<T, NORM, 8>(m, v, part);
+    case 9: return launch_update_box_n<T, NORM, 9>(m, v, part);
+    case 10: return launch_update_box_n<T, NORM, 10>(m, v, part);
+    default: return GF_ERR_UNSUPPORTED;
+  }
+}
+
 template <typename T, int NORM>
 void launch_update(gf_mesh *m, CGVecs<T> &v, double *part, bool mask_in_q) {
   const T *mk = (m->has_mask && !mask_in_q) ? v.mask : nullptr;
@@ -1312,6 +1608,8 @@ template <typename T>
 int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters) {
   hipStream_t s = m->ctx->stream;
   const bool col = m->dim == 3;
+  // Structured box: the update assembles q by gather (mask already in q).
+  const bool gather = col && m->has_box;
   int nbA = 0;
   if (col) {
     nbA = col_blocks(m);
@@ -1332,6 +1630,7 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a.edx = v.edx;
   a.ejac = v.ejac;
   a.qf = v.qf;
+  a.qs = v.qs;
   a.p = v.p;
   a.scal = m->scal;
   a.part = partA;
@@ -1350,7 +1649,11 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a3.edx = v.edx;
   a3.ejac = v.ejac;
   a3.qf = v.qf;
-  a3.mask = m->has_mask ? v.mask : nullptr;
+  a3.qs = v.qs;
+  // the mask is formed from the box faces when it is known to match them
+  a3.tmask = (m->has_mask && v.mask && m->has_box_mask) ? 1 : 0;
+  a3.topo = m->box;
+  a3.mask = (m->has_mask && !a3.tmask) ? v.mask : nullptr;
   a3.p = v.p;
   a3.scal = m->scal;
   a3.part = partA;
@@ -1365,16 +1668,27 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
         rc = sizeof(T) == 8 ? GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_d, m, a)
                             : GF_DISPATCH_N(m->dim, m->Niso, launch_cg_apply_f, m, a);
       if (rc) return rc;
-      if (m->ngroups)
+      if (m->ngroups && !gather)
         hipLaunchKernelGGL(k_dss_cg<T>, dim3(nblocks((size_t)m->ngroups)), dim3(kThreads), 0, s,
                            v.q, m->goff, m->gmem, m->ngroups, (const double *)m->scal);
       hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kRedThreads), 0, s, m->scal, partA, nbA);
-      switch (normtype) {
-        case GF_NORM_INF: launch_update<T, GF_NORM_INF>(m, v, partU, col); break;
-        case GF_NORM_EUC: launch_update<T, GF_NORM_EUC>(m, v, partU, col); break;
-        case GF_NORM_L2: launch_update<T, GF_NORM_L2>(m, v, partU, col); break;
-        case GF_NORM_L1: launch_update<T, GF_NORM_L1>(m, v, partU, col); break;
-        default: launch_update<T, GF_NORM_ASSEMBLED_L2>(m, v, partU, col); break;
+      if (gather) {
+        switch (normtype) {
+          case GF_NORM_INF: rc = launch_update_box<T, GF_NORM_INF>(m, v, partU); break;
+          case GF_NORM_EUC: rc = launch_update_box<T, GF_NORM_EUC>(m, v, partU); break;
+          case GF_NORM_L2: rc = launch_update_box<T, GF_NORM_L2>(m, v, partU); break;
+          case GF_NORM_L1: rc = launch_update_box<T, GF_NORM_L1>(m, v, partU); break;
+          default: rc = launch_update_box<T, GF_NORM_ASSEMBLED_L2>(m, v, partU); break;
+        }
+        if (rc) return rc;
+      } else {
+        switch (normtype) {
+          case GF_NORM_INF: launch_update<T, GF_NORM_INF>(m, v, partU, col); break;
+          case GF_NORM_EUC: launch_update<T, GF_NORM_EUC>(m, v, partU, col); break;
+          case GF_NORM_L2: launch_update<T, GF_NORM_L2>(m, v, partU, col); break;
+          case GF_NORM_L1: launch_update<T, GF_NORM_L1>(m, v, partU, col); break;
+          default: launch_update<T, GF_NORM_ASSEMBLED_L2>(m, v, partU, col); break;
+        }
       }
       hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kThreads), 0, s, m->scal, partU, kUpdBlocks,
                          normtype, d_resid);
@@ -1427,7 +1741,10 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     // integrate(1) = sum of mass
     RC(gf_dot_t(m, m->mass, nullptr, n, &volume));
   }
-  if (prm->precond) RC(gf_helmholtz_diag_inv(m, prm->p_scalar, prm->q_field, dinv));
+  // mass term: the q field, else the scalar q (GHelmholtz with a one-entry q,
+  // ghelmholtz.ipp:184-186, 370-373)
+  const double qs = prm->q_field ? 0.0 : prm->q_scalar;
+  if (prm->precond) RC(gf_helmholtz_diag_inv(m, prm->p_scalar, qs, prm->q_field, dinv));
   if (bbv) {  // x = x*mask + xb (gcg.ipp:371-375)
     RC(gf_mul_t(m, x, m->mask, x));
     RC(gf_axpy_t(m, 1.0, xb, x));
@@ -1453,7 +1770,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
   double rn_final = 0.0;
   // r = b - A x ; DSS ; mask
   auto residual = [&](double *rr) -> int {
-    RC(gf_helm_t(m, prm->p_scalar, nullptr, prm->q_field, x, q));
+    RC(gf_helm_t(m, prm->p_scalar, nullptr, prm->q_field, qs, x, q));
     hipLaunchKernelGGL(k_residual, dim3(nblocks(n)), dim3(kThreads), 0, s, rr, b, q, n);
     RC(gf_dss_t(m, GF_DSS_SUM, rr));
     if (bbv) RC(gf_mul_t(m, rr, m->mask, rr));
@@ -1485,6 +1802,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     v.mask = mask;
     v.imult = m->imult32;
     v.qf = prm->q_field;
+    v.qs = qs;
     v.D = m->D[0];
     v.W = m->W[0];
     v.edx = m->edx;
@@ -1498,67 +1816,75 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
       GF_HIP(m->ctx, hipMemcpy(prm->resid, d_resid, it * sizeof(double), hipMemcpyDeviceToHost));
     if (it >= prm->maxit && rn_final > prm->tol) status = GF_ERR_NOCONVERGE;
   } else {
-    // FP32 inner CG inside FP64 iterative refinement.
+    // Mixed precision: the CG recurrences (apply, direction, residual and
+    // iterate increments) run in FP32 with FP64 reductions; whenever the
+    // recurrence residual has dropped by the factor `delta` since the last
+    // reliable update, the FP32 increment is added to the FP64 iterate and
+    // the residual is replaced by the true FP64 residual b - A x, keeping the
+    // search direction (reliable updates, van der Vorst & Ye, SIAM J. Sci.
+    // Comput. 22 (2000) 1663-1683).  Convergence is judged on the FP64
+    // residual only.
     float *r32 = (float *)gf_ws(m, WS_CG_BEGIN + 5, n * sizeof(float));
     float *w32 = (float *)gf_ws(m, WS_CG_BEGIN + 6, n * sizeof(float));
     float *q32 = (float *)gf_ws(m, WS_CG_BEGIN + 7, n * sizeof(float));
-    float *d32 = (float *)gf_ws(m, WS_CG_BEGIN + 8, n * sizeof(float));
+    float *x32 = (float *)gf_ws(m, WS_CG_BEGIN + 8, n * sizeof(float));
     float *dinv32 = dinv ? (float *)gf_ws(m, WS_CG_BEGIN + 9, n * sizeof(float)) : nullptr;
     float *qf32 = prm->q_field ? (float *)gf_ws(m, WS_CG_BEGIN + 10, n * sizeof(float)) : nullptr;
-    if (!r32 || !w32 || !q32 || !d32 || (dinv && !dinv32) || (prm->q_field && !qf32))
+    if (!r32 || !w32 || !q32 || !x32 || (dinv && !dinv32) || (prm->q_field && !qf32))
       return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
     if (dinv) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, dinv32, dinv, 1.0, n);
     if (qf32) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, qf32, prm->q_field, 1.0, n);
-    double inner_tol = prm->inner_tol > 0 ? prm->inner_tol : 1e-5;
+    const double delta = prm->inner_tol > 0 ? prm->inner_tol : 0.1;
+    CGVecs<float> v;
+    v.x = x32;
+    v.r = r32;
+    v.w = w32;
+    v.q = q32;
+    v.dinv = dinv32;
+    v.mask = bbv ? m->mask32 : nullptr;
+    v.imult = m->imult32;
+    v.qf = qf32;
+    v.qs = qs;
+    v.D = m->D32;
+    v.W = m->W32;
+    v.edx = m->edx32;
+    v.ejac = m->ejac32;
+    v.p = prm->p_scalar;
     int hist = 0;
-    for (outer = 0; outer < 100; outer++) {
+    bool first = true;
+    for (outer = 0;; outer++) {
+      // reliable update: x += x32 ; r = mask DSS(b - A x) in FP64
+      if (!first)
+        hipLaunchKernelGGL(k_add_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, x, x32, 1.0, n);
       RC(residual(r));
       RC(setup_red<double>(m, r, nullptr, m->imult32, normtype, red2));
-      double rn = finish_norm_host(normtype, red2[1], m, volume);
+      const double rn = finish_norm_host(normtype, red2[1], m, volume);
       rn_final = rn;
       if (prm->resid && hist < prm->maxit) prm->resid[hist++] = rn;
-      if (rn <= rtol || total_iters >= prm->maxit) break;
-      // scale the correction problem to O(1): r32 = r / rscale
-      double rscale = 0.0;
-      RC(setup_red<double>(m, r, nullptr, m->imult32, GF_NORM_INF, red2));
-      rscale = red2[1] > 0 ? red2[1] : 1.0;
-      hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, r32, r, 1.0 / rscale, n);
-      GF_HIP(m->ctx, hipMemsetAsync(d32, 0, n * sizeof(float), s));
+      if (rn <= rtol || total_iters >= prm->maxit || outer >= 10000) break;
+      hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, r32, r, 1.0, n);
+      hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, x32, 0.0f, n);
       float *z32 = q32;
       if (dinv32) hipLaunchKernelGGL(k_mul_t<float>, dim3(nblocks(n)), dim3(kThreads), 0, s, dinv32, r32, z32, n);
       RC(setup_red<float>(m, r32, dinv32 ? z32 : r32, m->imult32, normtype, red2));
-      double rn32 = finish_norm_host(normtype, red2[1], m, volume);
-      double hs[12] = {0};
+      const double rn32 = finish_norm_host(normtype, red2[1], m, volume);
+      double hs[S_COUNT] = {0};
       hs[S_RHO] = red2[0];
-      hs[S_RTOL] = inner_tol * rn32;
-      hs[S_RNORM] = 10.0 * hs[S_RTOL];
+      hs[S_BETA] = first ? 0.0 : red2[0] / m->hscal[S_RHOM];
+      hs[S_RHOM] = first ? 0.0 : m->hscal[S_RHOM];
+      hs[S_RTOL] = std::max(delta * rn32, 0.5 * rtol);
+      hs[S_RNORM] = rn32;
       hs[S_ITER] = 0;
       hs[S_MAXIT] = std::max(1, prm->maxit - total_iters);
       hs[S_DONE] = 0.0;
-      hs[S_FIRST] = 1.0;
+      hs[S_FIRST] = first ? 1.0 : 0.0;
       hs[S_VOL] = volume;
       hs[S_N] = (double)n;
-      GF_HIP(m->ctx, hipMemcpyAsync(m->scal, hs, 12 * sizeof(double), hipMemcpyHostToDevice, s));
-      CGVecs<float> v;
-      v.x = d32;
-      v.r = r32;
-      v.w = w32;
-      v.q = q32;
-      v.dinv = dinv32;
-      v.mask = m->mask32;
-      v.imult = m->imult32;
-      v.qf = qf32;
-      v.D = m->D32;
-      v.W = m->W32;
-      v.edx = m->edx32;
-      v.ejac = m->ejac32;
-      v.p = prm->p_scalar;
-      if (!bbv) v.mask = nullptr;
+      GF_HIP(m->ctx, hipMemcpyAsync(m->scal, hs, sizeof(hs), hipMemcpyHostToDevice, s));
       int it = 0;
       RC(cg_loop<float>(m, v, normtype, (double *)nullptr, &it));
       total_iters += it;
-      hipLaunchKernelGGL(k_add_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, x, d32, rscale, n);
-      GF_LAUNCH_CHECK(m->ctx);
+      first = false;
     }
     if (rn_final > rtol) status = GF_ERR_NOCONVERGE;
   }
@@ -1630,7 +1956,7 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
     sa.nu = 0.02;
     sa.nelems = m->nelems;
     if (m->dim == 3)
-      launch = [&]() { return GF_DISPATCH_N3(m->Niso, launch_stage3c, m, sa); };
+      launch = [&]() { return GF_DISPATCH_N3(m->Niso, launch_stage3p, m, sa); };
     else
       launch = [&]() { return GF_DISPATCH_N(m->dim, m->Niso, launch_stage, m, sa); };
   } else if (w == "cg_apply" || w == "cg_apply_f32") {
@@ -1666,7 +1992,10 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
     ca3.edx = f32 ? (const void *)m->edx32 : (const void *)m->edx;
     ca3.ejac = f32 ? (const void *)m->ejac32 : (const void *)m->ejac;
     ca3.qf = nullptr;
+    ca3.qs = 1.0;
     ca3.mask = nullptr;
+    ca3.topo = m->box;
+    ca3.tmask = (m->has_mask && m->has_box_mask) ? 1 : 0;
     ca3.p = 0.1;
     ca3.scal = m->scal;
     ca3.part = m->red;
@@ -1675,8 +2004,49 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
       launch = [&]() { return GF_DISPATCH_N3(m->Niso, cgapply3c_f, m, ca3); };
     else
       launch = [&]() { return GF_DISPATCH_N3(m->Niso, cgapply3c_d, m, ca3); };
+  } else if (w == "burgers_stage_box") {
+    // structured-box stage, middle-stage form: reads uin, K_prev (gathered),
+    // acc; writes acc, K (5 fields per component)
+    if (m->dim != 3 || !m->has_box)
+      return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "burgers_stage_box timing: structured 3-D boxes");
+    launch = [&]() {
+      return GF_DISPATCH_N3(m->Niso, launch_stage3b, m, buf, buf + 3, buf + 6, buf + 9, 0.5e-5,
+                            1e-5 / 3.0, 0.02, 2, 0);
+    };
+  } else if (w == "cg_update" || w == "cg_update_f32") {
+    // CG update with the Jacobi preconditioner: x, w, r, q, dinv
+    if (m->dim != 3 || !m->has_box)
+      return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "cg_update timing: structured 3-D boxes");
+    bool f32 = w == "cg_update_f32";
+    double hs[S_COUNT] = {0};
+    hs[S_ALPHA] = 1e-3;
+    GF_HIP(m->ctx, hipMemcpyAsync(m->scal, hs, sizeof(hs), hipMemcpyHostToDevice, s));
+    if (f32) {
+      for (int k = 0; k < 5; k++)
+        hipLaunchKernelGGL(k_fill_pattern_f, dim3(nblocks(n)), dim3(kThreads), 0, s,
+                           (float *)buf[k], n, 0.5f + 0.1f * k);
+      launch = [&]() {
+        CGVecs<float> v{};
+        v.x = (float *)buf[0];
+        v.w = (float *)buf[1];
+        v.r = (float *)buf[2];
+        v.q = (float *)buf[3];
+        v.dinv = (const float *)buf[4];
+        return launch_update_box<float, GF_NORM_ASSEMBLED_L2>(m, v, m->red);
+      };
+    } else {
+      launch = [&]() {
+        CGVecs<double> v{};
+        v.x = buf[0];
+        v.w = buf[1];
+        v.r = buf[2];
+        v.q = buf[3];
+        v.dinv = buf[4];
+        return launch_update_box<double, GF_NORM_ASSEMBLED_L2>(m, v, m->red);
+      };
+    }
   } else if (w == "helmholtz") {
-    launch = [&]() { return gf_helm_t(m, 0.1, nullptr, nullptr, buf[0], buf[1]); };
+    launch = [&]() { return gf_helm_t(m, 0.1, nullptr, nullptr, 1.0, buf[0], buf[1]); };
   } else {
     return gf_set_err(m->ctx, GF_ERR_ARG, "unknown kernel name");
   }

@@ -58,6 +58,16 @@ class Context:
             pass
 
 
+def _q_args(q):
+    """Mass-term factor as (field, scalar): a tensor is a per-node field, a
+    number a scalar, None no mass term."""
+    if q is None:
+        return None, 0.0
+    if torch.is_tensor(q):
+        return q, 0.0
+    return None, float(q)
+
+
 class Mesh:
     """Device-resident mesh: GLL basis, metrics, mask and GGFX groups.
 
@@ -66,7 +76,7 @@ class Mesh:
     """
 
     def __init__(self, ctx, grid, gtype=GF_GE_REGULAR, node_dxidx=None, node_jac=None,
-                 xyz=None, use_mask=True):
+                 xyz=None, use_mask=True, structured=True):
         self.ctx = ctx
         self.grid = grid
         self.dim = grid.dim
@@ -107,10 +117,16 @@ class Mesh:
         keep += [offs, mem]
         d.ggfx_offs = offs.ctypes.data_as(_i64p)
         d.ggfx_members = mem.ctypes.data_as(_i64p)
+        if (structured and gtype == GF_GE_REGULAR and grid.dim == 3
+                and getattr(grid, "ne", None) is not None):
+            for k in range(3):
+                d.box_ne[k] = grid.ne[k]
+                d.box_periodic[k] = grid.periodic[k]
         h = _vp()
         check(lib().gf_mesh_create(ctx.h, ctypes.byref(d), ctypes.byref(h)), ctx.h)
         self.h = h
         self.ndof = lib().gf_mesh_ndof(h)
+        self.flags = lib().gf_mesh_flags(h)
 
     def __del__(self):
         try:
@@ -175,15 +191,19 @@ class Mesh:
         return r.value
 
     def helmholtz(self, u, p=1.0, q=None, out=None):
+        """uo = p L u + q M u; p and q may be scalars or per-node fields, q None
+        for the pure Laplacian."""
         out = self.new() if out is None else out
         pf = p if torch.is_tensor(p) else None
         ps = 1.0 if pf is not None else float(p)
-        self._c(lib().gf_helmholtz_apply(self.h, ps, _ptr(pf), _ptr(q), _ptr(u), _ptr(out)))
+        qf, qs = _q_args(q)
+        self._c(lib().gf_helmholtz_apply(self.h, ps, _ptr(pf), qs, _ptr(qf), _ptr(u), _ptr(out)))
         return out
 
     def helmholtz_f32(self, u, p=1.0, q=None, out=None):
         out = self.new(torch.float32) if out is None else out
-        self._c(lib().gf_helmholtz_apply_f32(self.h, float(p), _ptr(q), _ptr(u), _ptr(out)))
+        qf, qs = _q_args(q)
+        self._c(lib().gf_helmholtz_apply_f32(self.h, float(p), qs, _ptr(qf), _ptr(u), _ptr(out)))
         return out
 
     def dss(self, u, op="sum"):
@@ -227,11 +247,12 @@ class Mesh:
 
     def helmholtz_diag_inv(self, p=1.0, q=None):
         out = self.new()
-        self._c(lib().gf_helmholtz_diag_inv(self.h, float(p), _ptr(q), _ptr(out)))
+        qf, qs = _q_args(q)
+        self._c(lib().gf_helmholtz_diag_inv(self.h, float(p), qs, _ptr(qf), _ptr(out)))
         return out
 
     def cg_solve(self, b, x=None, xb=None, p=1.0, q=None, maxit=512, tol=1e-6, normtype=0,
-                 precond=0, precision=0, inner_tol=1e-5, raise_on_error=False):
+                 precond=0, precision=0, inner_tol=0.0, raise_on_error=False):
         x = self.new() if x is None else x
         prm = CGParams()
         prm.maxit = maxit
@@ -242,7 +263,9 @@ class Mesh:
         prm.inner_tol = inner_tol
         prm.p_scalar = float(p)
         prm.p_field = None
-        prm.q_field = _ptr(q)
+        qf, qs = _q_args(q)
+        prm.q_field = _ptr(qf)
+        prm.q_scalar = qs
         hist = np.zeros(max(maxit, 1) + 8)
         prm.resid = hist.ctypes.data_as(_dp)
         res = CGResult()

@@ -81,6 +81,9 @@ class BoxGrid:
                      bdy)
         self.__dict__.update(g)
         self.bdy = list(bdy)
+        self.ne = [int(q) for q in ne][:dim]
+        self.periodic = [int(bdy[2 * d] == GF_BDY_PERIODIC and bdy[2 * d + 1] == GF_BDY_PERIODIC)
+                         for d in range(dim)]
         self.P0 = np.asarray(P0, dtype=np.float64)[:dim]
         self.P1 = np.asarray(P1, dtype=np.float64)[:dim]
         self.N = [q + 1 for q in self.p]

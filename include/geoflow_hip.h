@@ -122,11 +122,23 @@ typedef struct {
   /* GGFX groups from gf_ggfx_build (host) */
   const int64_t *ggfx_offs;
   const int64_t *ggfx_members;
+  /* Optional box structure (3-D GE_REGULAR): elements per direction in
+   * x-fastest order and per-direction periodicity, as given to gf_box_grid.
+   * When set (box_ne[0] > 0) and the GGFX groups above are exactly the
+   * groups this structure implies (checked at creation), fused kernels
+   * assemble by gather instead of a separate DSS pass.  Zero = unstructured. */
+  int box_ne[3];
+  int box_periodic[3];
 } gf_mesh_desc;
 
 int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *desc, gf_mesh **out);
 int gf_mesh_destroy(gf_mesh *mesh);
 size_t gf_mesh_ndof(const gf_mesh *mesh);
+/* Mesh properties (diagnostic; no reference counterpart):
+ * GF_MESH_BOX_GATHER  the structured-box gather assembly is in use;
+ * GF_MESH_BOX_MASK    the boundary mask is formed from the box faces. */
+enum { GF_MESH_BOX_GATHER = 1, GF_MESH_BOX_MASK = 2 };
+int gf_mesh_flags(const gf_mesh *mesh);
 /* Copy a device-resident mesh array to host: "mass", "imass", "imult",
  * "mask" (ndof doubles each).  Synchronous. */
 int gf_mesh_get(gf_mesh *mesh, const char *which, double *host_out);
@@ -164,12 +176,14 @@ int gf_integrate(gf_mesh *mesh, const double *u, double *result);
  * (src/cdg/sem/ghelmholtz.ipp:76-378): uo = p L u + q M u, unassembled.
  * p: p_field (device, ndof) if non-NULL else the scalar p_scalar
  *    (set_Lap_scalar, ghelmholtz.ipp:590-603);
- * q: q_field (device, ndof) or NULL for the pure Laplacian
- *    (set_mass_scalar, ghelmholtz.ipp:613-625). */
-int gf_helmholtz_apply(gf_mesh *mesh, double p_scalar, const double *p_field,
+ * q: q_field (device, ndof) if non-NULL, else the scalar q_scalar; no mass
+ *    term when q_field is NULL and q_scalar is 0 (set_mass_scalar,
+ *    ghelmholtz.ipp:613-625; a one-entry q multiplies M u only when it is
+ *    not 1, ghelmholtz.ipp:184-186, 370-373). */
+int gf_helmholtz_apply(gf_mesh *mesh, double p_scalar, const double *p_field, double q_scalar,
                        const double *q_field, const double *u, double *uo);
 /* Same operator in FP32 (inputs/outputs float) for the mixed-precision solver. */
-int gf_helmholtz_apply_f32(gf_mesh *mesh, float p_scalar, const float *q_field,
+int gf_helmholtz_apply_f32(gf_mesh *mesh, float p_scalar, float q_scalar, const float *q_field,
                            const float *u, float *uo);
 
 /* ---- direct stiffness summation --------------------------------------------
@@ -207,18 +221,19 @@ typedef struct {
   double tol;        /* default 1e-6 */
   int    normtype;   /* GF_NORM_* */
   int    precond;    /* 0: identity (reference);  1: Jacobi on the assembled diagonal */
-  int    precision;  /* 0: FP64 CG;  1: FP32 applies in an FP64 iterative refinement */
-  double inner_tol;  /* mixed: relative reduction per inner FP32 solve (default 1e-5) */
+  int    precision;  /* 0: FP64 CG;  1: FP32 recurrences with FP64 reliable updates */
+  double inner_tol;  /* mixed: residual reduction between FP64 reliable updates (default 0.1) */
   double p_scalar;   /* Laplacian factor when p_field == NULL */
   const double *p_field;
   const double *q_field;
   double *resid;     /* host, optional, length >= maxit: residual history */
+  double q_scalar;   /* mass factor when q_field == NULL (0: no mass term) */
 } gf_cg_params;
 
 typedef struct {
   int    status;     /* GF_OK or GF_ERR_NOCONVERGE ... */
   int    iters;      /* CG iterations (mixed: total inner iterations) */
-  int    outer;      /* mixed: refinement steps */
+  int    outer;      /* mixed: FP64 reliable updates */
   double rnorm;      /* final residual norm in the selected norm */
   double rnorm0;     /* norm of b used for the stopping rule */
 } gf_cg_result;
@@ -228,8 +243,9 @@ typedef struct {
  * Synchronous (reads residual norms on the host each iteration batch). */
 int gf_cg_solve(gf_mesh *mesh, const gf_cg_params *prm, const double *b, const double *xb,
                 double *x, gf_cg_result *res);
-/* Assembled Jacobi diagonal inverse of (p L + q M). */
-int gf_helmholtz_diag_inv(gf_mesh *mesh, double p_scalar, const double *q_field, double *dinv);
+/* Assembled Jacobi diagonal inverse of (p L + q M), q as in gf_helmholtz_apply. */
+int gf_helmholtz_diag_inv(gf_mesh *mesh, double p_scalar, double q_scalar, const double *q_field,
+                          double *dinv);
 
 /* ---- diagnostics ------------------------------------------------------------
  * Average duration in ms of `reps` back-to-back launches of one kernel of the

@@ -261,16 +261,72 @@ def test_cg_poisson_cdg_cg(ctx):
     assert relerr(info2["resid"][:10], res2[:10]) <= 1e-11
 
 
+def test_box_gather_enabled_only_where_groups_match(ctx):
+    """Structured gather assembly is used for fully periodic and fully
+    Dirichlet boxes; a box that mixes periodic and non-periodic directions
+    keeps the GGFX groups (GGridBox::periodize snaps every P1 coordinate of a
+    periodic-face node, which couples the non-periodic faces there) and falls
+    back to the CSR DSS."""
+    for bdy, want in (([6] * 6, 3), ([0] * 6, 3), ([6, 6, 0, 0, 6, 6], 0), ([0, 0, 6, 6, 6, 6], 0)):
+        bg = ga.BoxGrid(3, 3, [2, 3, 2], bdy=bdy)
+        assert Mesh(ctx, bg).flags & 3 == want, bdy
+        assert Mesh(ctx, bg, structured=False).flags & 3 == 0
+    assert Mesh(ctx, ga.BoxGrid(2, 4, [3, 3])).flags & 3 == 0
+
+
+@pytest.mark.parametrize("bdy", [[0] * 6, [6] * 6], ids=["dirichlet", "periodic"])
+def test_cg_gather_path_matches_csr_path(ctx, bdy):
+    """Structured-box CG (gather assembly in the update, mask from the box
+    faces) against the CSR DSS path on the same problem: the assembly is the
+    same sum in the same order, only the partial sums of the dot products are
+    grouped differently, so 20 FP64 iterations agree to rounding and both
+    mixed-precision solves converge to the same solution."""
+    bg = ga.BoxGrid(3, 5, [3, 2, 3], bdy=bdy)
+    f = workloads.unique_normal(bg, seed=11)
+    outs = []
+    for structured in (True, False):
+        m = Mesh(ctx, bg, structured=structured)
+        assert (m.flags & 3) == (3 if structured else 0)
+        b = m.mass_apply(dev(f))
+        xb = torch.zeros_like(b) if bdy[0] == 0 else None
+        xg, info = m.cg_solve(b, xb=xb, p=0.1, q=1.0, maxit=20, tol=1e-300, normtype=0, precond=1)
+        assert info["iters"] == 20
+        outs.append(host(xg))
+        xg, info = m.cg_solve(b, xb=xb, p=0.1, q=1.0, maxit=2000, tol=1e-10, normtype=4, precond=1,
+                              precision=1)
+        assert info["status"] == 0
+        outs.append(host(xg))
+    assert relerr(outs[0], outs[2]) <= 1e-12
+    assert relerr(outs[1], outs[3]) <= 1e-8
+
+
+def test_helmholtz_scalar_q_matches_field_q(ctx):
+    """A one-entry q (set_mass_scalar with a scalar) multiplies M u only when
+    it differs from 1 (ghelmholtz.ipp:184-186); against the oracle with the
+    same q as a full field."""
+    for c in (CASES[0], CASES[2]):
+        og, bg, m = make(ctx, c)
+        u = rand(og.ndof, 3)
+        for qv in (1.0, 2.5):
+            h = host(m.helmholtz(dev(u), 0.3, qv))
+            assert relerr(h, og.helmholtz(u, 0.3, np.full(og.ndof, qv))) <= TOL
+        assert np.array_equal(host(m.helmholtz(dev(u), 0.3, 1.0)),
+                              host(m.helmholtz(dev(u), 0.3, dev(np.ones(og.ndof)))))
+
+
+@pytest.mark.parametrize("qkind", ["field", "scalar"])
 @pytest.mark.parametrize("precond", [0, 1])
-def test_cg_helmholtz_fixed_iterations(ctx, precond):
-    """(M + 0.1 L) x = b on a small 3-D Dirichlet box: the same number of
-    iterations on both sides, x compared to the oracle."""
-    c = dict(dim=3, p=5, ne=[3, 3, 3], bdy=[0] * 6)
+@pytest.mark.parametrize("bdy", [[0] * 6, [6, 6, 0, 0, 6, 6]], ids=["gather", "csr"])
+def test_cg_helmholtz_fixed_iterations(ctx, precond, bdy, qkind):
+    """(M + 0.1 L) x = b on a small 3-D box with Dirichlet faces: the same
+    number of iterations on both sides, x compared to the oracle."""
+    c = dict(dim=3, p=5, ne=[3, 3, 3], bdy=bdy)
     og, bg, m = make(ctx, c)
     f = workloads.unique_normal(bg, seed=67890)
     q = np.ones(og.ndof)
     b = og.mass_apply(f)  # local RHS: GCG assembles b - A x
-    xg, info = m.cg_solve(dev(b), xb=dev(np.zeros(og.ndof)), p=0.1, q=dev(q), maxit=25,
+    xg, info = m.cg_solve(dev(b), xb=dev(np.zeros(og.ndof)), p=0.1,
+                          q=dev(q) if qkind == "field" else 1.0, maxit=25,
                           tol=1e-30, normtype=0, precond=precond)
     xo, ret, it, res = og.cg_solve(b, xb=np.zeros(og.ndof), p=0.1, q=q, maxit=25, tol=1e-30,
                                    normtype=0, precond=precond)
