@@ -1,0 +1,550 @@
+// gf_k3d.hpp -- 3-D regular-element kernels (pencil form, gf_pen3.hpp) and
+// their launchers: Helmholtz apply, CG apply and update, Burgers RK stages
+// and the structured-box RK output.  Included by the gf_k3d_n*.hip
+// translation units, each of which instantiates the launchers for some
+// element orders (GF_K3D_INSTANTIATE).
+#pragma once
+
+#include "gf_solve_common.hpp"
+
+namespace gfs {
+
+// ---------------------------------------------------------------------------
+// Pencil-form 3-D kernels (see gf_pen3.hpp): Helmholtz apply and CG apply.
+// ---------------------------------------------------------------------------
+template <int N, typename T>
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_helm3p(
+    const T *__restrict__ u, T *__restrict__ uo, gf::PenD<T> D, const T *__restrict__ W,
+    const T *__restrict__ edx, const T *__restrict__ ejac, T p, const T *__restrict__ pf,
+    const T *__restrict__ qf, T qs, size_t nelems) {
+  using S = gf::Pen3<N>;
+  __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
+  ColIdx<N> t(nelems);
+  const size_t col = t.e * S::NN + t.r;
+  T ru[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) ru[k] = u[col + k * S::N2];
+  const T jac = ejac[t.e];
+  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
+  auto post = [&](int k, T Lu, T M, T, T) {
+    if (!t.active) return;
+    T acc = Lu;
+    if (!pf && p != T(1)) acc *= p;
+    if (qf) {
+      T tq = M * ru[k];
+      tq *= qf[col + k * S::N2];
+      acc += tq;
+    } else if (qs != T(0)) {
+      T tq = M * ru[k];
+      if (qs != T(1)) tq *= qs;
+      acc += tq;
+    }
+    uo[col + k * S::N2] = acc;
+  };
+  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, D, W, t.i, t.j, jac, f0 * f0,
+                           f1 * f1, f2 * f2, pf ? pf + t.e * S::NN : (const T *)nullptr,
+                           gf::NoPenCB(), gf::NoPenCB(), gf::NoPenCB(), post);
+}
+
+template <int N, typename T>
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
+    const T *__restrict__ r, const T *__restrict__ dinv, T *__restrict__ w, T *__restrict__ qv,
+    gf::PenD<T> Dp, const T *__restrict__ W, const T *__restrict__ edx,
+    const T *__restrict__ ejac, const T *__restrict__ qf, T qs, const T *__restrict__ mask, double pd,
+    const double *__restrict__ scal, double *__restrict__ part, size_t nelems, gf::BoxTopo topo,
+    int tmask) {
+  using S = gf::Pen3<N>;
+  __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
+  __shared__ double sred[4];
+  if (scal[S_DONE] != 0.0) return;
+  const bool first = scal[S_FIRST] != 0.0;
+  const T beta = (T)scal[S_BETA];
+  ColIdx<N> t(nelems);
+  const size_t col = t.e * S::NN + t.r;
+  const T jac = ejac[t.e];
+  const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
+  // all loads of the column first, then the stores of w
+  T ru[N], dv[N], wv[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const size_t g = col + k * S::N2;
+    ru[k] = r[g];
+    dv[k] = dinv ? dinv[g] : T(1);
+    wv[k] = first ? T(0) : w[g];
+  }
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const T z = dinv ? dv[k] * ru[k] : ru[k];
+    ru[k] = first ? z : beta * wv[k] + T(1) * z;
+  }
+  if (t.active) {
+#pragma unroll
+    for (int k = 0; k < N; k++) w[col + k * S::N2] = ru[k];
+  }
+  double dot = 0.0;
+  const T p = (T)pd;
+  int ec[3] = {0, 0, 0};
+  if (tmask) gf::box_elem_coords(topo, t.e, ec);
+  auto post = [&](int k, T Lu, T M, T, T) {
+    if (!t.active) return;
+    const size_t g = col + k * S::N2;
+    T acc = Lu;
+    if (p != T(1)) acc *= p;
+    if (qf) {
+      T tq = M * ru[k];
+      tq *= qf[g];
+      acc += tq;
+    } else if (qs != T(0)) {
+      T tq = M * ru[k];
+      if (qs != T(1)) tq *= qs;
+      acc += tq;
+    }
+    if (mask) acc *= mask[g];
+    else if (tmask && gf::box_masked(topo, N, ec, t.i, t.j, k)) acc *= T(0);
+    qv[g] = acc;
+    dot += (double)ru[k] * (double)acc;
+  };
+  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, Dp, W, t.i, t.j, jac, f0 * f0,
+                           f1 * f1, f2 * f2, (const T *)nullptr, gf::NoPenCB(), gf::NoPenCB(),
+                           gf::NoPenCB(), post);
+  double bs = gf::block_reduce<false>(dot, sred);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// Fused Burgers stage, 3-D pencil form (see k_burgers_stage for the contract).
+// The x- and y-derivative factors of the advection term are carried to the
+// column owner through LDS, so every node's advection is summed in the
+// reference order ((ux rx) u0 + (uy ry) u1) + (uz rz) u2 (gadvect.ipp:58-100).
+template <int N>
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+void k_burgers_stage3p(
+    StageArgs a, gf::PenD<double> Dp) {
+  using S = gf::Pen3<N>;
+  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZ];
+  ColIdx<N> t(a.nelems);
+  const size_t col = t.e * S::NN + t.r;
+  double ru[3][N];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const size_t g = col + k * S::N2;
+      const double ui = a.uin[c][g];
+      double v = ui;
+      if (a.kprev[c]) {
+        const double kp = a.kprev[c][g];
+        v = ui + a.bh * kp;
+        if (a.acc[c] && t.active) a.acc[c][g] = a.accsrc[c][g] + kp * a.ch;
+      }
+      ru[c][k] = v;
+    }
+  }
+  const double jac = a.ejac[t.e];
+  const double rx = a.edx[t.e * 3], ry = a.edx[t.e * 3 + 1], rz = a.edx[t.e * 3 + 2];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    auto zcb = [&](int, double, double) {};
+    auto xcb = [&](int, double ux, double) { return ux * rx; };
+    auto ycb = [&](int, double uy, double) { return uy * ry; };
+    double *kout = a.kout[c];
+    auto post = [&](int k, double L, double M, double hx, double hy) {
+      if (!t.active) return;
+      const size_t g = col + k * S::N2;
+      // z-derivative again (same operands and order as in the z pass)
+      using CP = const __attribute__((address_space(4))) double *;
+      const CP row = gf::opaque_ptr((CP)Dp.R[2] + k * N);
+      double uz = 0.0;
+#pragma unroll
+      for (int m2 = 0; m2 < N; m2++) uz = fma(row[m2], ru[c][m2], uz);
+      double po = hx * ru[0][k];
+      po += hy * ru[1][k];
+      po += (uz * rz) * ru[2][k];
+      const double adv = po * M;
+      if (a.nu != 1.0) L *= a.nu;
+      const double urhs = -1.0 * L + -1.0 * adv;
+      kout[g] = a.imass[g] * urhs;
+    };
+    gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, a.W, t.i, t.j, jac,
+                                 rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
+                                 post);
+    __syncthreads();  // the next component reuses the LDS buffers
+  }
+}
+
+template <int N>
+int launch_stage3p(gf_mesh *m, const StageArgs &a) {
+  using S = gf::Pen3<N>;
+  int nb = xcd_grid((m->nelems + S::EPB - 1) / S::EPB);
+  hipLaunchKernelGGL((k_burgers_stage3p<N>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream, a,
+                     pend_d(m));
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Burgers RK on a structured box (mesh->has_box): the GGFX Smooth of each
+// stage derivative, of the final-stage input and of the step output is formed
+// by gather in the kernels that consume them (gf_boxdss.hpp), in the GGFX
+// summation order, instead of separate DSS passes.  Stage derivatives are
+// stored unsmoothed.
+//   kp       previous stage derivative K_{s-1} (unsmoothed) or NULL (stage 1)
+//   acc_mode 0: no accumulator; 1: acc = uin + Ks*ch; 2: acc += Ks*ch
+//   smooth_v the stage input is the final-stage input of step_b, which is
+//            smoothed: v = Smooth(uin + bh*Ks) (gexrk_stepper.ipp:300-318)
+// ---------------------------------------------------------------------------
+template <int N, bool KP, int ACC, bool SMV>
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict__ ui1,
+                       const double *__restrict__ ui2, const double *__restrict__ kp0,
+                       const double *__restrict__ kp1, const double *__restrict__ kp2,
+                       double *__restrict__ ac0, double *__restrict__ ac1, double *__restrict__ ac2,
+                       double *__restrict__ ko0, double *__restrict__ ko1, double *__restrict__ ko2,
+                       const double *__restrict__ imass, const double *__restrict__ W,
+                       const double *__restrict__ edx, const double *__restrict__ ejac,
+                       gf::PenD<double> Dp, double bh, double ch, double nu, gf::BoxTopo topo,
+                       size_t nelems) {
+  using S = gf::Pen3<N>;
+  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZ];
+  const double *uin[3] = {ui0, ui1, ui2};
+  const double *kp[3] = {kp0, kp1, kp2};
+  double *acc[3] = {ac0, ac1, ac2};
+  double *kout[3] = {ko0, ko1, ko2};
+  ColIdx<N> t(nelems);
+  const size_t col = t.e * S::NN + t.r;
+  gf::BoxElemDeltas de = {{0, 0, 0}, {0, 0, 0}};
+  if (KP) {
+    int ec[3];
+    gf::box_elem_coords(topo, t.e, ec);
+    de = gf::box_elem_deltas(topo, N, ec);
+  }
+  // Stage input (and accumulator) per node; all loads are issued before any
+  // store, the accumulator stores follow the loop.
+  double ru[3][N];
+  double av[ACC ? 3 : 1][ACC ? N : 1];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const size_t g = col + (size_t)k * S::N2;
+    if (!KP) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) ru[c][k] = uin[c][g];
+      continue;
+    }
+    size_t mem[8];
+    const int cnt = gf::box_members<size_t>(de, N, g, t.i, t.j, k, mem);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double kv[8];
+      gf::box_load8<size_t, double>(kp[c], mem, cnt, g, kv);
+      const double ui = uin[c][g];
+      const double ao = ACC == 2 ? acc[c][g] : 0.0;
+      double uv[8];
+      if (SMV) gf::box_load8<size_t, double>(uin[c], mem, cnt, g, uv);
+      // GGFX Smooth: Sum / count (an exact multiply for counts 1, 2, 4, 8)
+      const double ks = cnt == 1 ? kv[0] : gf::box_sum8<double>(kv, cnt) * gf::box_inv(cnt);
+      double v = ui + bh * ks;
+      if (ACC) av[c][k] = (ACC == 1 ? ui : ao) + ks * ch;
+      if (SMV) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) uv[q] = uv[q] + bh * ks;
+        if (cnt > 1) v = gf::box_sum8<double>(uv, cnt) * gf::box_inv(cnt);
+      }
+      ru[c][k] = v;
+    }
+  }
+  if (ACC && t.active) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int k = 0; k < N; k++) acc[c][col + (size_t)k * S::N2] = av[ACC ? c : 0][ACC ? k : 0];
+  }
+  const double jac = ejac[t.e];
+  const double rx = edx[t.e * 3], ry = edx[t.e * 3 + 1], rz = edx[t.e * 3 + 2];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    auto zcb = [&](int, double, double) {};
+    auto xcb = [&](int, double ux, double) { return ux * rx; };
+    auto ycb = [&](int, double uy, double) { return uy * ry; };
+    double *ko = kout[c];
+    auto post = [&](int k, double L, double M, double hx, double hy) {
+      if (!t.active) return;
+      const size_t g = col + k * S::N2;
+      // z-derivative again (same operands and order as in the z pass)
+      using CP = const __attribute__((address_space(4))) double *;
+      const CP row = gf::opaque_ptr((CP)Dp.R[2] + k * N);
+      double uz = 0.0;
+#pragma unroll
+      for (int m2 = 0; m2 < N; m2++) uz = fma(row[m2], ru[c][m2], uz);
+      double po = hx * ru[0][k];
+      po += hy * ru[1][k];
+      po += (uz * rz) * ru[2][k];
+      const double adv = po * M;
+      if (nu != 1.0) L *= nu;
+      const double urhs = -1.0 * L + -1.0 * adv;
+      ko[g] = imass[g] * urhs;
+    };
+    gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, W, t.i, t.j, jac,
+                                 rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
+                                 post);
+    __syncthreads();  // the next component reuses the LDS buffers
+  }
+}
+
+// Step output on a structured box: u = Smooth(1*acc + cl*Smooth(K)) with both
+// smooths by gather (step_b's last update and GGFX Smooth).
+template <int N, typename I>
+__global__ __launch_bounds__(kThreads) void k_rk_out_box(
+    double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2,
+    const double *__restrict__ a0, const double *__restrict__ a1, const double *__restrict__ a2,
+    const double *__restrict__ k0, const double *__restrict__ k1, const double *__restrict__ k2,
+    double cl, gf::BoxTopo topo, size_t nelems) {
+  constexpr int NN = N * N * N;
+  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;
+  constexpr int CN = CE * NN;
+  double *u[3] = {u0, u1, u2};
+  const double *acc[3] = {a0, a1, a2};
+  const double *kf[3] = {k0, k1, k2};
+  const size_t nchunks = (nelems + CE - 1) / CE;
+  const size_t cpb = (nchunks + gridDim.x - 1) / gridDim.x;
+  const size_t c0 = xcd_block() * cpb, c1 = c0 + cpb < nchunks ? c0 + cpb : nchunks;
+  for (size_t ch = c0; ch < c1; ch++) {
+    int ec0[3];
+    gf::box_elem_coords(topo, ch * CE, ec0);
+    const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);
+#pragma unroll
+    for (int q0 = 0; q0 < CN; q0 += kThreads) {
+      const int q = q0 + (int)threadIdx.x;
+      if (q >= CN) break;
+      const int el = CE == 1 ? 0 : q / NN;
+      const int rr = CE == 1 ? q : q - el * NN;
+      const size_t e = ch * CE + el;
+      if (e >= nelems) break;
+      gf::BoxElemDeltas de = d0;
+      if (CE > 1 && el) {
+        int ec[3];
+        gf::box_elem_coords(topo, e, ec);
+        de = gf::box_elem_deltas(topo, N, ec);
+      }
+      const int i = rr % N, j = (rr / N) % N, k = rr / (N * N);
+      const I g = (I)(e * NN + rr);
+      I mem[8];
+      const int cnt = gf::box_members<I>(de, N, g, i, j, k, mem);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double kv[8], av[8];
+        gf::box_load8<I, double>(kf[c], mem, cnt, g, kv);
+        gf::box_load8<I, double>(acc[c], mem, cnt, g, av);
+        const double ks = cnt == 1 ? kv[0] : gf::box_sum8<double>(kv, cnt) * gf::box_inv(cnt);
+#pragma unroll
+        for (int m = 0; m < 8; m++) av[m] = 1.0 * av[m] + cl * ks;
+        u[c][g] = cnt == 1 ? av[0] : gf::box_sum8<double>(av, cnt) * gf::box_inv(cnt);
+      }
+    }
+  }
+}
+
+template <int N, bool KP, int ACC, bool SMV>
+void launch_stage3b_v(gf_mesh *m, int nb, double *const *uin, double *const *kp,
+                      double *const *acc, double *const *kout, double bh, double ch, double nu) {
+  using S = gf::Pen3<N>;
+  hipLaunchKernelGGL((k_burgers_stage3b<N, KP, ACC, SMV>), dim3(nb), dim3(S::THREADS), 0,
+                     m->ctx->stream, uin[0], uin[1], uin[2], kp ? kp[0] : nullptr,
+                     kp ? kp[1] : nullptr, kp ? kp[2] : nullptr, acc ? acc[0] : nullptr,
+                     acc ? acc[1] : nullptr, acc ? acc[2] : nullptr, kout[0], kout[1], kout[2],
+                     m->imass, m->W[0], m->edx, m->ejac, pend_d(m), bh, ch, nu, m->box,
+                     m->nelems);
+}
+
+template <int N>
+int launch_stage3b(gf_mesh *m, double *const *uin, double *const *kp, double *const *acc,
+                   double *const *kout, double bh, double ch, double nu, int acc_mode,
+                   int smooth_v) {
+  using S = gf::Pen3<N>;
+  int nb = xcd_grid((m->nelems + S::EPB - 1) / S::EPB);
+  if (!kp)
+    launch_stage3b_v<N, false, 0, false>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else if (acc_mode == 1 && !smooth_v)
+    launch_stage3b_v<N, true, 1, false>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else if (acc_mode == 2 && !smooth_v)
+    launch_stage3b_v<N, true, 2, false>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else if (acc_mode == 1 && smooth_v)
+    launch_stage3b_v<N, true, 1, true>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else if (acc_mode == 2 && smooth_v)
+    launch_stage3b_v<N, true, 2, true>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else
+    return gf_set_err(m->ctx, GF_ERR_ARG, "stage3b: unsupported stage form");
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int N>
+int launch_rk_out_box(gf_mesh *m, double *const *u, double *const *acc, double *const *kf,
+                      double cl) {
+  constexpr int NN = N * N * N;
+  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;
+  size_t nchunks = (m->nelems + CE - 1) / CE;
+  int nb = xcd_grid(std::min<size_t>(nchunks, 8192));
+  if (m->ndof < ((size_t)1 << 31))
+    hipLaunchKernelGGL((k_rk_out_box<N, int32_t>), dim3(nb), dim3(kThreads), 0, m->ctx->stream,
+                       u[0], u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
+                       m->nelems);
+  else
+    hipLaunchKernelGGL((k_rk_out_box<N, int64_t>), dim3(nb), dim3(kThreads), 0, m->ctx->stream,
+                       u[0], u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
+                       m->nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int N, typename T>
+int launch_helm3p(gf_mesh *m, T p, const T *pf, const T *qf, T qs, const T *u, T *uo,
+                  const gf::PenD<T> &D, const T *W, const T *edx, const T *ejac) {
+  using S = gf::Pen3<N>;
+  int nb = xcd_grid((m->nelems + S::EPB - 1) / S::EPB);
+  hipLaunchKernelGGL((k_helm3p<N, T>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream, u, uo, D, W,
+                     edx, ejac, p, pf, qf, qs, m->nelems);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+template <int N, typename T>
+int launch_cg_apply3p(gf_mesh *m, const CGApply3Args &a) {
+  using S = gf::Pen3<N>;
+  int nb = xcd_grid((m->nelems + S::EPB - 1) / S::EPB);
+  gf::PenD<T> Dp;
+  for (int c = 0; c < 3; c++) {
+    Dp.R[c] = (const T *)a.DR[c];
+    Dp.C[c] = (const T *)a.DC[c];
+  }
+  hipLaunchKernelGGL((k_cg_apply3p<N, T>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream,
+                     (const T *)a.r, (const T *)a.dinv, (T *)a.w, (T *)a.q, Dp, (const T *)a.W,
+                     (const T *)a.edx, (const T *)a.ejac, (const T *)a.qf, (T)a.qs, (const T *)a.mask, a.p,
+                     (const double *)a.scal, a.part, a.nelems, a.topo, a.tmask);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+// Number of workgroups (= partial sums) the 3-D column apply launches.
+template <int N>
+int helm3c_d(gf_mesh *m, double p, const double *pf, const double *qf, double qs, const double *u,
+             double *uo) {
+  return launch_helm3p<N, double>(m, p, pf, qf, qs, u, uo, pend_d(m), m->W[0], m->edx, m->ejac);
+}
+template <int N>
+int helm3c_f(gf_mesh *m, float p, const float *qf, float qs, const float *u, float *uo) {
+  return launch_helm3p<N, float>(m, p, (const float *)nullptr, qf, qs, u, uo, pend_f(m), m->W32,
+                                 m->edx32, m->ejac32);
+}
+template <int N>
+int cgapply3c_d(gf_mesh *m, const CGApply3Args &a) { return launch_cg_apply3p<N, double>(m, a); }
+template <int N>
+int cgapply3c_f(gf_mesh *m, const CGApply3Args &a) { return launch_cg_apply3p<N, float>(m, a); }
+
+// u3 = uin + bh*k ; acc = accsrc + ch*k   (final-stage input of step_b, :315-320)
+
+// k_cg_update on a structured box: q arrives unassembled (straight from the
+// apply) and is assembled here by gather (gf_boxdss.hpp), in the GGFX order,
+// so no DSS pass runs between the apply and the update.  The multiplicity
+// weight is 1/(group size), which equals GGFX::get_imult.  Work is split in
+// chunks of whole elements so that an element's box coordinates are formed
+// once per chunk (and are wave-uniform when a chunk is one element).
+template <typename T, int NORM, int N, typename I>
+__global__ __launch_bounds__(kThreads) void k_cg_update_box(
+    T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r, const T *__restrict__ qv,
+    const T *__restrict__ dinv, const double *__restrict__ mass, gf::BoxTopo topo, double *scal,
+    double *__restrict__ part, size_t nelems) {
+  constexpr int NN = N * N * N;
+  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;  // elements per chunk
+  constexpr int CN = CE * NN;                              // nodes per chunk
+  __shared__ double sred[kThreads / 64];
+  if (scal[S_DONE] != 0.0) return;
+  const T alpha = (T)scal[S_ALPHA];
+  double rho = 0.0, nrm = 0.0;
+  // each (XCD-renumbered) block walks a contiguous range of chunks
+  const size_t nchunks = (nelems + CE - 1) / CE;
+  const size_t cpb = (nchunks + gridDim.x - 1) / gridDim.x;
+  const size_t c0 = xcd_block() * cpb, c1 = c0 + cpb < nchunks ? c0 + cpb : nchunks;
+  for (size_t c = c0; c < c1; c++) {
+    int ec0[3];
+    gf::box_elem_coords(topo, c * CE, ec0);
+    const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);  // uniform when CE == 1
+#pragma unroll
+    for (int q0 = 0; q0 < CN; q0 += kThreads) {
+      const int q = q0 + (int)threadIdx.x;
+      if (q >= CN) break;
+      const int el = CE == 1 ? 0 : q / NN;
+      const int rr = CE == 1 ? q : q - el * NN;
+      const size_t e = c * CE + el;
+      if (e >= nelems) break;
+      gf::BoxElemDeltas de = d0;
+      if (CE > 1 && el) {
+        int ec[3];
+        gf::box_elem_coords(topo, e, ec);
+        de = gf::box_elem_deltas(topo, N, ec);
+      }
+      const int i = rr % N, j = (rr / N) % N, k = rr / (N * N);
+      const I g = (I)(e * NN + rr);
+      // every load of the node is issued before the first store
+      const T dv = dinv ? dinv[g] : T(1);
+      const T xo = x[g], wo = w[g], ro = r[g];
+      int cnt;
+      const T qa = gf::box_gather_sum<N, I, T>(de, qv, g, i, j, k, &cnt);
+      const T xv = (T)1 * xo + alpha * wo;
+      x[g] = xv;
+      const T rv = (T)1 * ro + (-alpha) * qa;
+      r[g] = rv;
+      const T z = dinv ? dv * rv : rv;
+      const double im = gf::box_inv(cnt);
+      rho += (double)rv * (double)z * im;
+      const double rd = (double)rv;
+      if (NORM == GF_NORM_INF) nrm = fmax(nrm, fabs(rd));
+      else if (NORM == GF_NORM_EUC) nrm += rd * rd;
+      else if (NORM == GF_NORM_L2) nrm += mass[g] * rd * rd;
+      else if (NORM == GF_NORM_L1) nrm += mass[g] * fabs(rd);
+      else nrm += rd * rd * im;
+    }
+  }
+  double a = gf::block_reduce<false>(rho, sred);
+  double b = NORM == GF_NORM_INF ? gf::block_reduce<true>(nrm, sred)
+                                 : gf::block_reduce<false>(nrm, sred);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+template <typename T, int NORM, int N>
+int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part) {
+  // 32-bit node indices whenever every index of the mesh fits
+  if (m->ndof < ((size_t)1 << 31))
+    hipLaunchKernelGGL((k_cg_update_box<T, NORM, N, int32_t>), dim3(kUpdBlocks), dim3(kThreads), 0,
+                       m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal, part,
+                       m->nelems);
+  else
+    hipLaunchKernelGGL((k_cg_update_box<T, NORM, N, int64_t>), dim3(kUpdBlocks), dim3(kThreads), 0,
+                       m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal, part,
+                       m->nelems);
+  return GF_OK;
+}
+
+}  // namespace gfs
+
+#define GF_K3D_UPD(T, N)                                                              \
+  template int launch_update_box_n<T, GF_NORM_INF, N>(gf_mesh *, CGVecs<T> &, double *); \
+  template int launch_update_box_n<T, GF_NORM_EUC, N>(gf_mesh *, CGVecs<T> &, double *); \
+  template int launch_update_box_n<T, GF_NORM_L2, N>(gf_mesh *, CGVecs<T> &, double *);  \
+  template int launch_update_box_n<T, GF_NORM_L1, N>(gf_mesh *, CGVecs<T> &, double *);  \
+  template int launch_update_box_n<T, GF_NORM_ASSEMBLED_L2, N>(gf_mesh *, CGVecs<T> &, double *);
+
+#define GF_K3D_INSTANTIATE(N)                                                                \
+  template int launch_stage3p<N>(gf_mesh *, const StageArgs &);                              \
+  template int launch_stage3b<N>(gf_mesh *, double *const *, double *const *, double *const *, \
+                                 double *const *, double, double, double, int, int);         \
+  template int launch_rk_out_box<N>(gf_mesh *, double *const *, double *const *,              \
+                                    double *const *, double);                                \
+  template int helm3c_d<N>(gf_mesh *, double, const double *, const double *, double,         \
+                           const double *, double *);                                        \
+  template int helm3c_f<N>(gf_mesh *, float, const float *, float, const float *, float *);   \
+  template int cgapply3c_d<N>(gf_mesh *, const CGApply3Args &);                              \
+  template int cgapply3c_f<N>(gf_mesh *, const CGApply3Args &);                              \
+  GF_K3D_UPD(float, N)                                                                       \
+  GF_K3D_UPD(double, N)

@@ -1,0 +1,6 @@
+// Instantiation of the 3-D element kernels for element orders N = 10.
+#include "gf_k3d.hpp"
+
+namespace gfs {
+GF_K3D_INSTANTIATE(10)
+}  // namespace gfs
