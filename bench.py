@@ -119,13 +119,19 @@ def main():
     ok = all(bool(torch.isfinite(u).all()) for u in us)
     dofsteps = ndof * args.steps / el
     log("Burgers RK4: %d steps in %.3f s -> %.3e DOF-steps/s (finite=%s)" % (args.steps, el, dofsteps, ok))
-    stage_ms = m.time_kernel("burgers_stage", reps=20)
-    stage_bytes = 5 * 8 * 3 * ndof  # stage 2-3 form: read uin,kprev,acc; write acc,k (per comp)
+    # the stage kernel the step runs: the structured-box form (previous K
+    # assembled by gather inside the stage) when the mesh allows it
+    boxed = bool(m.flags & ga._capi.GF_MESH_BOX_GATHER)
+    stage_ms = m.time_kernel("burgers_stage_box" if boxed else "burgers_stage", reps=20)
+    stage_name = ("k_burgers_stage3b<8> (structured-box RK stage, K gathered in-kernel; "
+                  "middle-stage form)" if boxed else
+                  "k_burgers_stage3p<8> (fused RK stage; middle-stage form)")
+    stage_bytes = 5 * 8 * 3 * ndof  # middle-stage form: read uin,kprev,acc; write acc,k (per comp)
     stage_gbs = stage_bytes / (stage_ms * 1e-3) / 1e9
     step_bytes = 16 * 8 * 3 * ndof
     step_gbs = step_bytes / (el / args.steps) / 1e9
     out.update({"value": dofsteps, "unit": "DOF-steps/s", "ms_per_step": 1e3 * el / args.steps})
-    out["roofline"] = {"kernel": "k_burgers_stage<3,8> (fused RK stage, stages 2-3 form)",
+    out["roofline"] = {"kernel": stage_name,
                        "bound": "hbm", "achieved": stage_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": stage_gbs / HBM_PEAK_GBS, "traffic": None,
                        "avg_launch_ms": stage_ms, "alg_bytes_per_launch": stage_bytes,

@@ -66,6 +66,9 @@ SIGNATURES = {
                                    ctypes.POINTER(ctypes.c_int), _dp, _dp,
                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_dp),
                                    ctypes.POINTER(_dp), _dp, _dp, _dp, _dp]),
+    "gf_icos_grid": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_dp),
+                                    ctypes.POINTER(_dp), _dp, _dp]),
     "gf_ggfx_build": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(_dp),
                                      ctypes.c_double, ctypes.c_size_t, _i64p, _i64p,
                                      ctypes.POINTER(ctypes.c_size_t)]),
@@ -100,6 +103,16 @@ SIGNATURES = {
     "gf_burgers_steps": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, ctypes.POINTER(_vp)]),
+    "gf_burgers_rhs_adjoint": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int,
+                                              ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                              ctypes.POINTER(_vp)]),
+    "gf_burgers_step_adjoint": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_double, ctypes.POINTER(_vp),
+                                               ctypes.POINTER(_vp)]),
+    "gf_burgers_adjoint": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                          ctypes.POINTER(_vp)]),
     "gf_cg_solve": (ctypes.c_int, [_vp, ctypes.POINTER(CGParams), _vp, _vp, _vp,
                                    ctypes.POINTER(CGResult)]),
     "gf_helmholtz_diag_inv": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),

@@ -97,5 +97,7 @@ enum {
   WS_TMP0 = 0, WS_TMP1, WS_TMP2, WS_TMP3, WS_TMP4, WS_TMP5,
   WS_RK_BEGIN = 8,          // RK: acc[3], K[3][3], u3[3]
   WS_CG_BEGIN = 32,         // CG vectors
-  WS_SLOTS = 64
+  WS_ADJ_TMP = 48,          // adjoint RHS scratch (6 slots)
+  WS_SLOTS = 64,            // kernel-timing scratch (16 slots)
+  WS_ADJ_BEGIN = 96         // adjoint stage states and cotangents
 };

@@ -4,6 +4,7 @@
 // Built with -ffp-contract=off so that node coordinates and element metrics
 // round exactly as the reference's expressions do.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -315,6 +316,419 @@ int gf_ggfx_build(size_t ndof, int ncoord, const double *const *xyz, double tol,
     for (int64_t s = goff[gid[n]]; s < goff[gid[n] + 1]; s++) members[c++] = gmem[s];
   }
   offs[ndof] = c;
+  return GF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Icosahedral sphere grid (GGridIcos, 2-D embedded quads on a sphere)
+// ---------------------------------------------------------------------------
+namespace {
+
+const double kPi = 3.14159265358979323846264338328;  // PI (src/cdg/include/gtypes.h:92)
+
+struct V3 {
+  double x, y, z;
+};
+inline V3 vadd(const V3 &a, const V3 &b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 vsub(const V3 &a, const V3 &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 vmul(const V3 &a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+// GTPoint::norm: sqrt(((0 + x^2) + y^2) + z^2)
+inline double vnorm(const V3 &a) {
+  double s = 0.0;
+  s += a.x * a.x;
+  s += a.y * a.y;
+  s += a.z * a.z;
+  return std::sqrt(s);
+}
+
+// Radial projection through latitude/longitude (project2sphere,
+// ggrid_icos.ipp:1457-1556); `r` is the norm as the caller computes it.
+inline V3 to_sphere(double x, double y, double z, double r, double rad) {
+  const double lat = std::asin(z / r);
+  double lon = std::atan2(y, x);
+  lon = lon < 0.0 ? 2.0 * kPi + lon : lon;
+  return {rad * std::cos(lat) * std::cos(lon), rad * std::cos(lat) * std::sin(lon),
+          rad * std::sin(lat)};
+}
+
+// GTVector::sortincreasing (gtvector.ipp:2965-2999, 3013-3243): quicksort with
+// the middle element as pivot that sorts `a` IN PLACE and permutes `is`
+// alongside (is starts as 0..n-1).
+void qsort_s2l(double *a, size_t *is, long lo, long hi) {
+  if (lo >= hi) return;
+  const long pi = lo + (hi - lo) / 2;
+  const double pv = a[pi];
+  std::swap(a[pi], a[lo]);
+  std::swap(is[pi], is[lo]);
+  long i = lo + 1, j = hi;
+  while (i <= j) {
+    while (i <= j && a[i] <= pv) i++;
+    while (i <= j && a[j] > pv) j--;
+    if (i < j) {
+      std::swap(a[i], a[j]);
+      std::swap(is[i], is[j]);
+    }
+  }
+  std::swap(a[i - 1], a[lo]);
+  std::swap(is[i - 1], is[lo]);
+  qsort_s2l(a, is, lo, i - 2);
+  qsort_s2l(a, is, i, hi);
+}
+void sort_increasing(double *a, size_t *is, size_t n) {
+  for (size_t k = 0; k < n; k++) is[k] = k;
+  qsort_s2l(a, is, 0, (long)n - 1);
+}
+
+// xyz2spherical (ggrid_icos.ipp:1698-1720): latitude and longitude in [0,2pi).
+inline void latlon(const V3 &v, double &lat, double &lon) {
+  const double r = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+  lat = std::asin(v.z / r);
+  lon = std::atan2(v.y, v.x);
+  lon = lon < 0.0 ? 2.0 * kPi + lon : lon;
+}
+
+// GLL basis data in long double for basis/derivative evaluation at nodes.
+struct LDBasis {
+  int p;
+  std::vector<LD> x, Pn;
+  std::vector<double> xd;
+};
+
+// Jacobi (here Legendre) recurrence of GLLBasis::computeJacobi
+// (gllbasis.ipp:618-667) with alpha = beta = 0: P_N(x) and P_N'(x).
+void jacobi_ld(int N, LD x, LD &poly, LD &pder) {
+  const LD apb = 0.0L, alpha = 0.0L, beta = 0.0L, rv = 1.0L;
+  poly = 1.0L;
+  pder = 0.0L;
+  if (N == 0) return;
+  LD polylist = poly, pderlist = pder;
+  poly = rv * x;
+  pder = rv;
+  if (N == 1) return;
+  for (int k = 2; k <= N; k++) {
+    const LD fk = (LD)k;
+    const LD a1 = 2.0L * fk * (fk + apb) * (2.0L * fk + apb - 2.0L);
+    const LD a2 = (2.0L * fk + apb - 1.0L) * (alpha * alpha - beta * beta);
+    const LD b3 = 2.0L * fk + apb - 2.0L;
+    const LD a3 = b3 * (b3 + 1.0L) * (b3 + 2.0L);
+    const LD a4 = 2.0L * (fk + alpha - 1.0L) * (fk + beta - 1.0L) * (2.0L * fk + apb);
+    const LD polyn = ((a2 + a3 * x) * poly - a4 * polylist) / a1;
+    const LD pdern = ((a2 + a3 * x) * pder - a4 * pderlist + a3 * poly) / a1;
+    polylist = poly;
+    poly = polyn;
+    pderlist = pder;
+    pder = pdern;
+  }
+}
+
+// h_i(eta) of GLLBasis::evalBasis (2) (gllbasis.ipp:1300-1327).
+double eval_basis(const LDBasis &b, int i, double eta) {
+  const int Np = b.p;
+  const LD fact = -1.0 / (Np * (Np + 1.0));  // formed in double, as the reference does
+  const LD ttiny = 100.0L * std::numeric_limits<LD>::epsilon();
+  const LD tetiny = 100.0L * std::numeric_limits<double>::epsilon();
+  const LD xi = (LD)eta;
+  double f = 1.0;
+  if (xi < -1.0L || xi > 1.0L) f = 0.0;
+  else if (std::fabs(xi - b.x[i]) > tetiny) {
+    LD P, dP;
+    jacobi_ld(Np, xi, P, dP);
+    const LD g = (1.0L - xi * xi) / (xi - b.x[i] + ttiny);
+    f = (double)(fact * g * dP / (b.Pn[i] + ttiny));
+  }
+  return f;
+}
+
+// h_j'(eta) of GLLBasis::evalDBasis (4) (gllbasis.ipp:1628-1665).
+double eval_dbasis(const LDBasis &b, int j, double eta) {
+  const int Np = b.p;
+  const LD fact = -1.0 / (Np * (Np + 1.0));  // formed in double, as the reference does
+  const LD tetiny = 100.0L * std::numeric_limits<double>::epsilon();
+  const LD xi = (LD)eta;
+  const LD g1 = xi - b.x[j];
+  double f = 0.0;
+  if (xi == -1.0L && j == 0) f = (double)(-0.25L * Np * (Np + 1.0L));
+  else if (xi == 1.0L && j == Np) f = (double)(0.25L * Np * (Np + 1.0L));
+  else if (std::fabs(g1) > tetiny) {
+    LD P, dP;
+    jacobi_ld(Np, xi, P, dP);
+    const LD gfact = fact / b.Pn[j];
+    const LD g1i = 1.0L / g1;
+    const LD g2 = (1.0L - xi * xi) * g1i;
+    const LD pdd = 2.0L * xi * dP - Np * (Np + 1.0L) * P;
+    f = (double)(gfact * g1i * (pdd - (2.0L * xi + g2) * dP));
+  }
+  return f;
+}
+
+LDBasis make_ldbasis(int p) {
+  LDBasis b;
+  b.p = p;
+  b.xd.resize(p + 1);
+  std::vector<double> w(p + 1), D((size_t)(p + 1) * (p + 1));
+  gf_gll_basis(p, b.xd.data(), w.data(), D.data());
+  b.x.resize(p + 1);
+  b.Pn.resize(p + 1);
+  for (int i = 0; i <= p; i++) {
+    // nodes in long double: the double nodes are the long-double nodes
+    // rounded, so refine once by Newton on (1-x^2)P_p'(x) from them
+    LD xx = (LD)b.xd[i];
+    if (i > 0 && i < p) {
+      for (int it = 0; it < 4; it++) {
+        LD Pa, dPa, Pb, dPb;
+        legendre(p + 1, xx, Pa, dPa);
+        legendre(p - 1, xx, Pb, dPb);
+        xx -= (Pa - Pb) / (dPa - dPb);
+      }
+    }
+    b.x[i] = xx;
+    LD dP;
+    jacobi_ld(p, xx, b.Pn[i], dP);
+  }
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+// GGridIcos 2-D for one rank (src/cdg/grid/ggrid_icos.ipp): the base
+// icosahedron (init2d, :139-230), 'Lagrangian' refinement of every face into
+// (nrows+1)^2 triangles (lagrefine/lagvert/interleave, :254-328, 2127-2173),
+// projection to the unit sphere, triangle vertex ordering (order_triangles,
+// :2202-2258, including its in-place sort), then three quads per triangle
+// (do_elems2d, :379-513) whose vertices are projected and ordered in gnomonic
+// coordinates (reorderverts2d/cart2gnomonic, :1733-1990), nodes from the
+// bilinear shape functions projected to radius R, and the embedded metrics
+// of GElem_base::dogeom2d/Jac/inv (src/cdg/sem/gelem_base.cpp:877-955,
+// 1066-1170, 1249-1300): rij = dx^k/dxi^m (the third column is the position,
+// from the embedded shape functions, gshapefcn_embed.ipp:356-390),
+// Jac = |det rij| / |x|, dXi/dX = rij^-1.
+//   refine: 0 bisection (nrows = 2^ilevel - 1), 1 Lagrangian (nrows = ilevel)
+// Outputs (caller-allocated, ndof = nelems (p+1)^2; may be NULL to query):
+//   nelems, xyz[3], dxidx[6] (dXi_i/dX_j at [i*3+j], i < 2), jac, minnodedist
+int gf_icos_grid(int p, int ilevel, int refine, double radius, size_t *nelems,
+                 double *const *xyz, double *const *dxidx, double *jac, double *minnodedist) {
+  if (p < 1 || p > 64 || ilevel < 0 || ilevel > 12 || (refine != 0 && refine != 1) ||
+      !(radius > 0.0) || !nelems)
+    return GF_ERR_ARG;
+  const long nrows = refine == 0 ? (1L << ilevel) - 1 : ilevel;
+  const size_t ntri = 20 * (size_t)(nrows * (nrows + 2) + 1);
+  *nelems = 3 * ntri;
+  if (!xyz) return GF_OK;
+  for (int k = 0; k < 3; k++)
+    if (!xyz[k]) return GF_ERR_ARG;
+
+  // base icosahedron, vertex at the top (init2d, ggrid_icos.ipp:163-203)
+  static const double fv0[12][3] = {
+      {0.000000000000000, 0.000000000000000, 1.000000000000000},
+      {0.894427190999916, 0.000000000000000, 0.447213595499958},
+      {-0.894427190999916, 0.000000000000000, -0.447213595499958},
+      {0.000000000000000, 0.000000000000000, -1.000000000000000},
+      {-0.723606797749979, 0.525731112119134, 0.447213595499958},
+      {-0.723606797749979, -0.525731112119134, 0.447213595499958},
+      {0.723606797749979, 0.525731112119134, -0.447213595499958},
+      {0.723606797749979, -0.525731112119134, -0.447213595499958},
+      {0.276393202250021, 0.850650808352040, 0.447213595499958},
+      {0.276393202250021, -0.850650808352040, 0.447213595499958},
+      {-0.276393202250021, 0.850650808352040, -0.447213595499958},
+      {-0.276393202250021, -0.850650808352040, -0.447213595499958}};
+  static const int ifv0[20][3] = {{0, 1, 8},  {0, 8, 4},  {0, 4, 5},  {0, 5, 9},  {0, 9, 1},
+                                  {1, 6, 8},  {8, 6, 10}, {8, 10, 4}, {4, 10, 2}, {4, 2, 5},
+                                  {5, 2, 11}, {5, 11, 9}, {9, 11, 7}, {9, 7, 1},  {1, 7, 6},
+                                  {3, 6, 7},  {3, 7, 11}, {3, 11, 2}, {3, 2, 10}, {3, 10, 6}};
+  const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+  const double vfact = 1.0 / std::sqrt(phi * phi + 1.0);
+  V3 bv[12];
+  for (int i = 0; i < 12; i++) bv[i] = {fv0[i][0] * vfact, fv0[i][1] * vfact, fv0[i][2] * vfact};
+
+  // refined triangles (lagrefine): rows of 'Lagrangian' vertices
+  std::vector<std::array<V3, 3>> tri(ntri);
+  {
+    size_t n = 0;
+    const double rfact = 1.0 / (double)(nrows + 1);
+    auto row = [&](const V3 &a, const V3 &b, const V3 &c, long I, std::vector<V3> &R) {
+      R.resize(I + 1);
+      const double xI = (double)I;
+      const V3 rL = vadd(a, vmul(vsub(b, a), xI * rfact));
+      const V3 rR = vadd(a, vmul(vsub(c, a), xI * rfact));
+      const double f = I > 0 ? 1.0 / (double)I : 1.0;
+      for (long j = 0; j <= I; j++) R[j] = vadd(rL, vmul(vsub(rR, rL), (double)j * f));
+    };
+    std::vector<V3> R0, R1, Rz;
+    for (int t = 0; t < 20; t++) {
+      const V3 a = bv[ifv0[t][0]], b = bv[ifv0[t][1]], c = bv[ifv0[t][2]];
+      for (long l = 0; l < nrows + 1; l++) {
+        row(a, b, c, l, R0);
+        row(a, b, c, l + 1, R1);
+        Rz.resize(2 * l + 3);
+        for (long j = 0; j < 2 * l + 3; j++) Rz[j] = (j % 2 == 0) ? R1[j / 2] : R0[(j - 1) / 2];
+        for (long m = 0; m < 2 * l + 1; m++) tri[n++] = {Rz[m], Rz[m + 1], Rz[m + 2]};
+      }
+    }
+  }
+  // project to the unit sphere, then order the vertices (order_triangles)
+  std::vector<int> iup(ntri, 0);
+  for (size_t t = 0; t < ntri; t++) {
+    for (int j = 0; j < 3; j++) {
+      const V3 v = tri[t][j];
+      tri[t][j] = to_sphere(v.x, v.y, v.z, vnorm(v), 1.0);
+    }
+    const std::array<V3, 3> cv = tri[t];
+    double lat[3], lon[3];
+    for (int j = 0; j < 3; j++) latlon(cv[j], lat[j], lon[j]);
+    size_t is[3];
+    sort_increasing(lon, is, 3);  // lon is now sorted; is refers to the input order
+    if (std::fabs(lon[is[0]] - lon[is[2]]) < kPi)
+      for (int j = 0; j < 3; j++)
+        if (lon[j] > 1.5 * kPi && lon[j] <= 2.0 * kPi) lon[j] -= 2.0 * kPi;
+    sort_increasing(lon, is, 3);
+    if (lat[is[1]] > lat[is[0]] && lat[is[1]] > lat[is[2]]) {
+      tri[t] = {cv[is[0]], cv[is[2]], cv[is[1]]};
+      iup[t] = 1;
+    }
+    if (lat[is[1]] < lat[is[0]] && lat[is[1]] < lat[is[2]]) tri[t] = {cv[is[1]], cv[is[2]], cv[is[0]]};
+  }
+
+  // elements: 3 quads per triangle
+  const int N = p + 1;
+  const size_t nn = (size_t)N * N;
+  const LDBasis lb = make_ldbasis(p);
+  const std::vector<double> &xi = lb.xd;
+  // bilinear shape functions N_m at the nodes (GShapeFcn_linear::Ni_2d)
+  std::vector<double> Nf(4 * nn);
+  for (int m = 0; m < 4; m++)
+    for (int j = 0, q = 0; j < N; j++)
+      for (int i = 0; i < N; i++, q++) {
+        const double rm = 1.0 - xi[i], rp = 1.0 + xi[i], sm = 1.0 - xi[j], sp = 1.0 + xi[j];
+        Nf[m * nn + q] = m == 0 ? rm * sm : (m == 1 ? rp * sm : (m == 2 ? rp * sp : rm * sp));
+      }
+  // embedded shape-function derivatives at the nodes (GShapeFcn_embed::dNdXi_2d)
+  std::vector<double> hb((size_t)N * N), hd((size_t)N * N);  // [basis i][node]
+  for (int i = 0; i < N; i++)
+    for (int q = 0; q < N; q++) {
+      hb[(size_t)i * N + q] = eval_basis(lb, i, xi[q]);
+      hd[(size_t)i * N + q] = eval_dbasis(lb, i, xi[q]);
+    }
+  const double third = 1.0 / 3.0;
+  const double geps = 10.0 * std::numeric_limits<double>::epsilon();
+  size_t e = 0;
+  std::vector<double> X[3];
+  for (int k = 0; k < 3; k++) X[k].resize(nn);
+  for (size_t t = 0; t < ntri; t++) {
+    const V3 v1 = tri[t][0], v2 = tri[t][1], v3 = tri[t][2];
+    const V3 ct = vmul(vadd(v1, vadd(v2, v3)), third);
+    const V3 m12 = vmul(vadd(v1, v2), 0.5), m13 = vmul(vadd(v1, v3), 0.5),
+             m23 = vmul(vadd(v2, v3), 0.5);
+    for (int j = 0; j < 3; j++, e++) {
+      V3 cv[4];
+      if (iup[t] == 1) {
+        if (j == 0) { cv[0] = v1; cv[1] = m12; cv[2] = ct; cv[3] = m13; }
+        else if (j == 1) { cv[0] = m12; cv[1] = v2; cv[2] = m23; cv[3] = ct; }
+        else { cv[0] = ct; cv[1] = m23; cv[2] = v3; cv[3] = m13; }
+      } else {
+        if (j == 0) { cv[0] = v1; cv[1] = m12; cv[2] = ct; cv[3] = m13; }
+        else if (j == 1) { cv[0] = ct; cv[1] = m12; cv[2] = v2; cv[3] = m23; }
+        else { cv[0] = m13; cv[1] = ct; cv[2] = m23; cv[3] = v3; }
+      }
+      for (int m = 0; m < 4; m++) cv[m] = to_sphere(cv[m].x, cv[m].y, cv[m].z, vnorm(cv[m]), 1.0);
+      // reorderverts2d: order the quad's vertices in gnomonic coordinates
+      const V3 c = vmul(vadd(vadd(vadd(cv[0], cv[1]), cv[2]), cv[3]), 0.25);
+      const double latc = std::asin(c.z);
+      double lonc = std::atan2(c.y, c.x);
+      lonc = lonc < 0.0 ? 2 * kPi + lonc : lonc;
+      double gx[4], gy[4];
+      for (int m = 0; m < 4; m++) {
+        const double r = vnorm(cv[m]);
+        const double la = std::asin(cv[m].z / r);
+        double lo = std::atan2(cv[m].y, cv[m].x);
+        lo = lo < 0.0 ? 2.0 * kPi + lo : lo;
+        double den = std::sin(latc) * std::sin(la) + std::cos(latc) * std::cos(la) * std::cos(lo - lonc);
+        den = std::fabs(den) < geps ? 0.0 : 1.0 / den;
+        gx[m] = 1.0 * std::cos(la) * std::sin(lo - lonc) * den;
+        gy[m] = 1.0 * (std::cos(latc) * std::sin(la) - std::sin(latc) * std::cos(la) * std::cos(lo - lonc)) * den;
+      }
+      double sx[4] = {gx[0], gx[1], gx[2], gx[3]};
+      size_t I[4], ord[4];
+      sort_increasing(sx, I, 4);
+      if (gy[I[0]] < gy[I[1]]) { ord[0] = I[0]; ord[3] = I[1]; }
+      else { ord[0] = I[1]; ord[3] = I[0]; }
+      if (gy[I[2]] < gy[I[3]]) { ord[1] = I[2]; ord[2] = I[3]; }
+      else { ord[1] = I[3]; ord[2] = I[2]; }
+      V3 gv[4];
+      for (int m = 0; m < 4; m++) gv[m] = cv[ord[m]];
+      // nodes: sum_m N_m (v_m / 4), projected to radius R
+      const size_t base = e * nn;
+      for (size_t q = 0; q < nn; q++) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int m = 0; m < 4; m++) {
+          acc[0] += Nf[m * nn + q] * (gv[m].x * 0.25);
+          acc[1] += Nf[m * nn + q] * (gv[m].y * 0.25);
+          acc[2] += Nf[m * nn + q] * (gv[m].z * 0.25);
+        }
+        const double r = std::sqrt(acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2]);
+        const V3 s = to_sphere(acc[0], acc[1], acc[2], r, radius);
+        X[0][q] = s.x;
+        X[1][q] = s.y;
+        X[2][q] = s.z;
+        xyz[0][base + q] = s.x;
+        xyz[1][base + q] = s.y;
+        xyz[2][base + q] = s.z;
+      }
+      if (!dxidx && !jac) continue;
+      // rij(k,m) = sum_I (h_m-derivative product) x^k_I, I in (j outer, i inner)
+      for (int qj = 0, q = 0; qj < N; qj++)
+        for (int qi = 0; qi < N; qi++, q++) {
+          double r[3][3];
+          for (int k = 0; k < 3; k++)
+            for (int m = 0; m < 3; m++) {
+              double tmp = 0.0;
+              for (int bj = 0, n = 0; bj < N; bj++)
+                for (int bi = 0; bi < N; bi++, n++) {
+                  const double d0 = m == 0 ? hd[(size_t)bi * N + qi] : hb[(size_t)bi * N + qi];
+                  const double d1 = m == 1 ? hd[(size_t)bj * N + qj] : hb[(size_t)bj * N + qj];
+                  double dn = d0 * d1;
+                  dn *= X[k][n];
+                  tmp += dn;
+                }
+              r[k][m] = tmp;
+            }
+          const double det = r[0][0] * (r[1][1] * r[2][2] - r[1][2] * r[2][1]) -
+                             r[0][1] * (r[1][0] * r[2][2] - r[1][2] * r[2][0]) +
+                             r[0][2] * (r[1][0] * r[2][1] - r[1][1] * r[2][0]);
+          if (jac) {
+            const double x = r[0][2], y = r[1][2], z = r[2][2];
+            jac[base + q] = std::fabs(det) / std::sqrt(x * x + y * y + z * z);
+          }
+          if (dxidx) {
+            const double ij = 1.0 / det;
+            const double i00 = (r[1][1] * r[2][2] - r[1][2] * r[2][1]) * ij;
+            const double i01 = -(r[0][1] * r[2][2] - r[2][1] * r[0][2]) * ij;
+            const double i02 = (r[0][1] * r[1][2] - r[0][2] * r[1][1]) * ij;
+            const double i10 = -(r[1][0] * r[2][2] - r[2][0] * r[1][2]) * ij;
+            const double i11 = (r[0][0] * r[2][2] - r[2][0] * r[0][2]) * ij;
+            const double i12 = -(r[0][0] * r[1][2] - r[0][2] * r[1][0]) * ij;
+            const double iv[6] = {i00, i01, i02, i10, i11, i12};
+            for (int c2 = 0; c2 < 6; c2++)
+              if (dxidx[c2]) dxidx[c2][base + q] = iv[c2];
+          }
+        }
+    }
+  }
+  if (minnodedist) {  // GGrid::find_min_dist (ggrid.ipp:977-1013)
+    const size_t ndof = *nelems * nn;
+    const double tiny = 1000.0 * std::numeric_limits<double>::epsilon();
+    double dmin = std::numeric_limits<double>::max();
+    for (size_t q = 0; q + 1 < ndof; q++) {
+      const double dx = xyz[0][q + 1] - xyz[0][q], dy = xyz[1][q + 1] - xyz[1][q],
+                   dz = xyz[2][q + 1] - xyz[2][q];
+      const double rr = std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (rr > tiny) dmin = std::min(dmin, rr);
+    }
+    *minnodedist = dmin;
+  }
   return GF_OK;
 }
 

@@ -75,11 +75,16 @@ class Mesh:
     edx/ejac (regular) or node metrics (embedded), plus ggfx_offs/members.
     """
 
-    def __init__(self, ctx, grid, gtype=GF_GE_REGULAR, node_dxidx=None, node_jac=None,
+    def __init__(self, ctx, grid, gtype=None, node_dxidx=None, node_jac=None,
                  xyz=None, use_mask=True, structured=True):
         self.ctx = ctx
         self.grid = grid
         self.dim = grid.dim
+        if gtype is None:
+            gtype = getattr(grid, "gtype", GF_GE_REGULAR)
+        if gtype == GF_GE_2DEMBEDDED:
+            node_dxidx = grid.node_dxidx if node_dxidx is None else node_dxidx
+            node_jac = grid.node_jac if node_jac is None else node_jac
         self.gtype = gtype
         d = MeshDesc()
         d.dim = grid.dim
@@ -238,6 +243,26 @@ class Mesh:
         self._c(lib().gf_burgers_steps(self.h, mode, nu, order, len(us), t, dt, nsteps,
                                        _ptrs(us)))
         return us
+
+    def burgers_rhs_adjoint(self, us, ws, lams, nu):
+        """lams[k] += (dF/du)^T w at the state us (nonlinear mode)."""
+        self._c(lib().gf_burgers_rhs_adjoint(self.h, nu, len(us), _ptrs(us), _ptrs(ws),
+                                             _ptrs(lams)))
+        return lams
+
+    def burgers_step_adjoint(self, un, lams, nu, dt, order=4):
+        """lams (dJ/du_{n+1}) becomes dJ/du_n for one step from the state un."""
+        self._c(lib().gf_burgers_step_adjoint(self.h, nu, order, len(un), dt, _ptrs(un),
+                                              _ptrs(lams)))
+        return lams
+
+    def burgers_adjoint(self, u0, lams, nu, dt, nsteps, order=4, ckpt=16, uT=None):
+        """lams (dJ/du(T)) becomes dJ/du(0) for nsteps steps from u0; uT, when
+        given, receives u(T)."""
+        self._c(lib().gf_burgers_adjoint(self.h, nu, order, len(u0), dt, nsteps, ckpt,
+                                         _ptrs(u0), _ptrs(lams),
+                                         _ptrs(uT) if uT is not None else None))
+        return lams
 
     def time_kernel(self, which, reps=20):
         """Average ms per launch of one kernel, timed with HIP events."""

@@ -4,12 +4,15 @@
 of ``init_ggfx`` (src/apps/geoflow_cdg.cpp:731-774): node coordinates,
 regular-element metrics, boundary mask, periodized coordinates and the
 coincidence groups matched with tolerance 0.001 * minnodedist.
+
+``IcosGrid`` mirrors the 2-D GGridIcos (src/cdg/grid/ggrid_icos.ipp): quads on
+the sphere with per-node embedded metrics, and the same GGFX set-up.
 """
 import ctypes
 
 import numpy as np
 
-from ._capi import GF_BDY_PERIODIC, GF_BDY_DIRICHLET, check, lib
+from ._capi import GF_BDY_PERIODIC, GF_BDY_DIRICHLET, GF_GE_REGULAR, GF_GE_2DEMBEDDED, check, lib
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -69,8 +72,50 @@ def ggfx_build(coords, tol, max_dup=0):
     return offs, mem[: nm.value]
 
 
+def icos_grid(p, ilevel, refine=0, radius=1.0):
+    """Run gf_icos_grid; returns dict of host arrays."""
+    ne = ctypes.c_size_t(0)
+    check(lib().gf_icos_grid(p, ilevel, refine, radius, ctypes.byref(ne), None, None, None,
+                             None))
+    nel = ne.value
+    ndof = nel * (p + 1) ** 2
+    xyz = [np.zeros(ndof) for _ in range(3)]
+    dxidx = [[np.zeros(ndof) for _ in range(3)] for _ in range(2)]
+    jac = np.zeros(ndof)
+    md = ctypes.c_double(0)
+    xp = (_dp * 3)(*[_p(a) for a in xyz])
+    dp = (_dp * 6)(*[_p(dxidx[i][j]) for i in range(2) for j in range(3)])
+    check(lib().gf_icos_grid(p, ilevel, refine, radius, ctypes.byref(ne), xp, dp, _p(jac),
+                             ctypes.byref(md)))
+    return dict(xyz=xyz, node_dxidx=dxidx, node_jac=jac, minnodedist=md.value, ndof=ndof,
+                nelems=nel)
+
+
+class IcosGrid:
+    """GGridIcos (2-D) equivalent: quads on the sphere of the given radius,
+    GE_2DEMBEDDED metrics per node; refine 0 is GICOS_BISECTION (ilevel
+    bisections), 1 GICOS_LAGRANGIAN (ilevel rows per base triangle)."""
+
+    gtype = GF_GE_2DEMBEDDED
+
+    def __init__(self, p, ilevel, radius=1.0, refine=0, tol_fac=0.001):
+        self.dim = 2
+        g = icos_grid(p, ilevel, refine, radius)
+        self.__dict__.update(g)
+        self.p = [p, p]
+        self.N = [p + 1, p + 1]
+        self.ilevel, self.refine, self.radius = ilevel, refine, radius
+        self.basis = [gll_basis(p)] * 2
+        self.mask = None
+        self.tol = tol_fac * self.minnodedist
+        # GGridIcos::max_duplicates: 6 quads meet at a base-icosahedron vertex
+        self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyz, self.tol, max_dup=6)
+
+
 class BoxGrid:
     """GGridBox equivalent: host arrays for a box of regular elements."""
+
+    gtype = GF_GE_REGULAR
 
     def __init__(self, dim, p, ne, P0=(0.0, 0.0, 0.0), P1=(1.0, 1.0, 1.0), bdy=None,
                  tol_fac=0.001):

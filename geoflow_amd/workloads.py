@@ -31,9 +31,10 @@ def gaussian_bumps_2d(x, y, seed=12345, nbumps=8, ncomp=2, L=1.0):
     return comps
 
 
-def fourier_field_3d(x, y, z, seed=12345, kmin=1, kmax=6, ncomp=3, L=1.0):
+def fourier_field_3d(x, y, z, seed=12345, kmin=1, kmax=6, ncomp=3, L=1.0, xp=np):
     """Random periodic Fourier field: integer wavevectors with kmin<=|k|<=kmax,
-    amplitude |k|^-2, seeded phases, independent per component."""
+    amplitude |k|^-2, seeded phases, independent per component.  xp is the
+    array module of x, y, z (numpy, or torch for device tensors)."""
     rng = np.random.default_rng(seed)
     ks = []
     for a in range(-kmax, kmax + 1):
@@ -46,9 +47,9 @@ def fourier_field_3d(x, y, z, seed=12345, kmin=1, kmax=6, ncomp=3, L=1.0):
     tp = 2.0 * np.pi / L
     for _ in range(ncomp):
         ph = rng.uniform(0.0, 2.0 * np.pi, len(ks))
-        u = np.zeros_like(x)
+        u = xp.zeros_like(x)
         for (a, b, c, kk), f in zip(ks, ph):
-            u += kk ** -2 * np.cos(tp * (a * x + b * y + c * z) + f)
+            u += kk ** -2 * xp.cos(tp * (a * x + b * y + c * z) + f)
         comps.append(u / np.sqrt(len(ks)))
     return comps
 

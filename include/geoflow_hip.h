@@ -94,6 +94,20 @@ int gf_box_grid(int dim, const int *p, const int *ne, const double *P0, const do
                 const int *bdy, double *const *xyz, double *const *xyzp, double *mask,
                 double *elem_dxidx, double *elem_jac, double *minnodedist);
 
+/* GGridIcos 2-D for one rank (src/cdg/grid/ggrid_icos.ipp:139-513): the
+ * base icosahedron refined into 20 (nrows+1)^2 triangles (refine 0:
+ * GICOS_BISECTION, nrows = 2^ilevel - 1; 1: GICOS_LAGRANGIAN, nrows = ilevel),
+ * three GE_2DEMBEDDED quads per triangle, nodes projected to the sphere of
+ * the given radius, and the embedded metrics of GElem_base::dogeom2d/Jac/inv
+ * (src/cdg/sem/gelem_base.cpp:877-955, 1066-1170, 1249-1300).
+ * Call with xyz == NULL to obtain *nelems; then ndof = nelems (p+1)^2 and
+ *   xyz[3]    node coordinates
+ *   dxidx[6]  dXi_i/dX_j at [i*3 + j], i < 2, j < 3 (entries may be NULL)
+ *   jac       |det(dx/dxi)| / |x| per node (may be NULL)
+ *   minnodedist  GGrid::find_min_dist (may be NULL) */
+int gf_icos_grid(int p, int ilevel, int refine, double radius, size_t *nelems,
+                 double *const *xyz, double *const *dxidx, double *jac, double *minnodedist);
+
 /* GGFX::init local coincidence matching (src/cdg/comm/ggfx.hpp:167-342) on
  * host coordinates xyz[ncoord][ndof] with box tolerance tol.  Produces the
  * group of every node as CSR: members[offs[n] .. offs[n+1]) ascending.
@@ -212,6 +226,26 @@ int gf_burgers_step(gf_mesh *mesh, int mode, double nu, int order, int nstate, d
 /* nsteps consecutive steps (one host call; identical arithmetic). */
 int gf_burgers_steps(gf_mesh *mesh, int mode, double nu, int order, int nstate, double t,
                      double dt, int nsteps, double *const *u);
+
+/* ---- adjoint (no reference counterpart: hand-written adjoint of the two
+ * functions above, for gradients of final-time functionals) ---------------
+ * Nonlinear mode, nstate = velocity components (dim, or 3 on embedded grids).
+ * gf_burgers_rhs_adjoint: lam[k] += (dF/du)^T w at the state u, F the RHS of
+ *   gf_burgers_rhs (GBurgers::dudt_impl, src/cdg/pdes/gburgers.ipp:203-257).
+ * gf_burgers_step_adjoint: lam (in dJ/du_{n+1}) becomes dJ/du_n for one
+ *   step of gf_burgers_step from un (step_b, gexrk_stepper.ipp:252-341);
+ *   the stage states are recomputed.
+ * gf_burgers_adjoint: for u(T) = nsteps steps from u0, lam (in dJ/du(T))
+ *   becomes dJ/du(0); states are checkpointed every ckpt steps on the
+ *   forward sweep and each segment is recomputed on the reverse sweep.  uT
+ *   (optional) receives u(T).  Synchronous; allocates its checkpoints. */
+int gf_burgers_rhs_adjoint(gf_mesh *mesh, double nu, int nstate, const double *const *u,
+                           const double *const *w, double *const *lam);
+int gf_burgers_step_adjoint(gf_mesh *mesh, double nu, int order, int nstate, double dt,
+                            const double *const *un, double *const *lam);
+int gf_burgers_adjoint(gf_mesh *mesh, double nu, int order, int nstate, double dt, int nsteps,
+                       int ckpt, const double *const *u0, double *const *lam,
+                       double *const *uT);
 
 /* ---- conjugate gradient -----------------------------------------------------
  * GCG::solve_impl (src/cdg/solvers/gcg.ipp:239-345, 361-382) applied to the

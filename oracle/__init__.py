@@ -46,6 +46,15 @@ def lib():
         L.or_grid_icos_create.restype = ctypes.c_void_p
         L.or_grid_icos_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_double]
+        L.or_grid_icos_create2.restype = ctypes.c_void_p
+        L.or_grid_icos_create2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_double]
+        L.or_burgers_rhs_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                             ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                             ctypes.POINTER(_dp)]
+        L.or_burgers_step_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_double,
+                                              ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
         L.or_grid_destroy.argtypes = [ctypes.c_void_p]
         for f in ("or_grid_ndof", "or_grid_nelems"):
             getattr(L, f).restype = ctypes.c_size_t
@@ -174,8 +183,9 @@ class Grid:
         return g
 
     @classmethod
-    def icos(cls, p, ilevel, radius=1.0, tol_fac=0.001):
-        g = cls(lib().or_grid_icos_create(p, ilevel, radius, tol_fac))
+    def icos(cls, p, ilevel, radius=1.0, tol_fac=0.001, refine=0):
+        """refine 0: GICOS_BISECTION, 1: GICOS_LAGRANGIAN."""
+        g = cls(lib().or_grid_icos_create2(p, ilevel, refine, radius, tol_fac))
         g.p = [p, p]
         return g
 
@@ -261,6 +271,16 @@ class Grid:
         """Advance the state list in place by one ExRK step."""
         lib().or_burgers_step(self.h, mode, nu, order, len(us), t, dt, _pp(us))
         return us
+
+    def burgers_rhs_adjoint(self, us, ws, lams, nu):
+        """lams[k] += (dF/du)^T w at the state us (nonlinear mode), in place."""
+        lib().or_burgers_rhs_adjoint(self.h, nu, len(us), _pp(us), _pp(ws), _pp(lams))
+        return lams
+
+    def burgers_step_adjoint(self, un, lams, nu, dt, order=4):
+        """lams (dJ/du_{n+1}) becomes dJ/du_n for one step from un, in place."""
+        lib().or_burgers_step_adjoint(self.h, nu, order, len(un), dt, _pp(un), _pp(lams))
+        return lams
 
     def helmholtz_diag_inv(self, p=1.0, q=None):
         pa = np.atleast_1d(np.asarray(p, dtype=np.float64))

@@ -6,6 +6,7 @@
 #include "gf_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cassert>
 #include <cmath>
 #include <cstdint>
@@ -66,6 +67,7 @@ void jacobi(int N, LD alpha, LD beta, LD &poly, LD &pder, LD &polym1, LD &pderm1
 struct Basis {
   int p = 0, N = 0;
   std::vector<double> xi, w, D, DT;  // D(l,j) column-major: D[l + j*N]
+  std::vector<LD> xl, Pnl;           // long-double nodes and P_p(node) (xiNodes_, Pn_)
 };
 
 bool gll(int Np, Basis &b) {
@@ -143,6 +145,8 @@ bool gll(int Np, Basis &b) {
   dphi[Np + (size_t)Np * np1] = -dphi[0];
   b.p = Np;
   b.N = np1;
+  b.xl = x;
+  b.Pnl = Pn;
   b.xi.resize(np1);
   b.w.resize(np1);
   b.D.resize((size_t)np1 * np1);
@@ -907,6 +911,96 @@ void or_burgers_step(OrGrid *g, int mode, double nu, int order, int nstate, doub
   (void)t;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the Burgers RHS and RK step (no reference counterpart; the
+// transposes of the operators above, for the FP64 CPU reference gradient).
+//   F_c(u) = imass * (-nu L u_c - M * sum_j u_j * d_j u_c)
+//   J_F^T w:  z = imass w_c;  lam_c += -nu L z - sum_j d_j^T(u_j M z);
+//             lam_j += -(d_j u_c) M z
+// with d_j^T v = D_j^T(dXi_j/dX_j v) on regular grids.  Regular grids only.
+// ---------------------------------------------------------------------------
+void or_burgers_rhs_adjoint(OrGrid *g, double nu, int nstate, const double *const *u,
+                            const double *const *w, double *const *lam) {
+  size_t n = g->ndof;
+  std::vector<double> z(n), Lz(n), mz(n), t(n), v(n), dt(n);
+  double one = 1.0;
+  for (int c = 0; c < nstate; c++) {
+    for (size_t q = 0; q < n; q++) z[q] = g->imass[q] * w[c][q];
+    or_helmholtz_apply(g, &one, 1, nullptr, z.data(), Lz.data());
+    for (size_t q = 0; q < n; q++) {
+      lam[c][q] -= nu * Lz[q];
+      mz[q] = g->mass[q] * z[q];
+    }
+    for (int j = 0; j < nstate; j++) {
+      or_deriv(g, u[c], j + 1, t.data());
+      for (size_t q = 0; q < n; q++) lam[j][q] -= t[q] * mz[q];
+      for (size_t q = 0; q < n; q++) v[q] = g->dxidx[j][0][q] * (u[j][q] * mz[q]);
+      or_grefderiv(g, v.data(), j + 1, 1, dt.data());
+      for (size_t q = 0; q < n; q++) lam[c][q] -= dt[q];
+    }
+  }
+}
+
+// lam: in dJ/du_{n+1}, out dJ/du_n for one or_burgers_step (mode 0) from un.
+void or_burgers_step_adjoint(OrGrid *g, double nu, int order, int nstate, double dt,
+                             const double *const *un, double *const *lam) {
+  size_t n = g->ndof;
+  std::vector<double> alpha, beta, c;
+  butcher(order, alpha, beta, c);
+  double h = dt;
+  int ns = order;
+  typedef std::vector<std::vector<double>> Fld;
+  // forward: stage inputs us[m] and smoothed stage derivatives K[m]
+  std::vector<Fld> us(ns, Fld(nstate, std::vector<double>(n))), K(ns, Fld(nstate, std::vector<double>(n)));
+  for (int m = 0; m < ns; m++) {
+    for (int k = 0; k < nstate; k++) {
+      for (size_t q = 0; q < n; q++) {
+        double isum = 0.0;
+        for (int j = 0; j < m; j++) isum = 1.0 * isum + beta[(size_t)m * order + j] * h * K[j][k][q];
+        us[m][k][q] = un[k][q] + isum;
+      }
+      if (m == ns - 1) or_dss(g, 1, us[m][k].data());
+    }
+    std::vector<const double *> up(nstate);
+    std::vector<double *> kp(nstate);
+    for (int k = 0; k < nstate; k++) {
+      up[k] = us[m][k].data();
+      kp[k] = K[m][k].data();
+    }
+    or_burgers_rhs(g, 0, nu, nstate, up.data(), kp.data());
+    for (int k = 0; k < nstate; k++) or_dss(g, 1, K[m][k].data());
+  }
+  // reverse
+  Fld lin(nstate, std::vector<double>(n)), ab(nstate, std::vector<double>(n));
+  std::vector<Fld> kb(ns, Fld(nstate, std::vector<double>(n, 0.0)));
+  for (int k = 0; k < nstate; k++) {
+    std::copy(lam[k], lam[k] + n, ab[k].begin());
+    or_dss(g, 1, ab[k].data());
+    lin[k] = ab[k];  // uout starts as uin
+    for (int m = 0; m < ns; m++)
+      for (size_t q = 0; q < n; q++) kb[m][k][q] += c[m] * h * ab[k][q];
+  }
+  for (int m = ns - 1; m >= 0; m--) {
+    Fld y(nstate, std::vector<double>(n, 0.0));
+    std::vector<const double *> up(nstate), wp(nstate);
+    std::vector<double *> yp(nstate);
+    for (int k = 0; k < nstate; k++) {
+      or_dss(g, 1, kb[m][k].data());
+      up[k] = us[m][k].data();
+      wp[k] = kb[m][k].data();
+      yp[k] = y[k].data();
+    }
+    or_burgers_rhs_adjoint(g, nu, nstate, up.data(), wp.data(), yp.data());
+    for (int k = 0; k < nstate; k++) {
+      if (m == ns - 1) or_dss(g, 1, y[k].data());
+      for (size_t q = 0; q < n; q++) lin[k][q] += y[k][q];
+      for (int j = 0; j < m; j++)
+        for (size_t q = 0; q < n; q++) kb[j][k][q] += beta[(size_t)m * order + j] * h * y[k][q];
+    }
+  }
+  for (int k = 0; k < nstate; k++) std::copy(lin[k].begin(), lin[k].end(), lam[k]);
+}
+
 // GCG::solve_impl (gcg.ipp:239-345, 361-382) and compute_norm (395-429)
 int or_cg_solve(OrGrid *g, const double *p, size_t np, const double *q, const double *b,
                 const double *xb, double *x, const OrCGTraits *tr, int *iters, double *resid) {
@@ -1060,7 +1154,364 @@ void or_embed_G(OrGrid *g, int i, int j, std::vector<double> &G) {
           G[q] += g->dxidx[i][k][q] * g->dxidx[j][k][q] * g->basis[0].w[l] * g->basis[1].w[m] * g->jac[q];
 }
 
+// ---------------------------------------------------------------------------
+// Icosahedral sphere grid: GGridIcos 2-D (src/cdg/grid/ggrid_icos.ipp)
+// ---------------------------------------------------------------------------
+namespace icos {
+
+const double PI_ = 3.14159265358979323846264338328;  // gtypes.h:92
+
+struct Pt {  // GTPoint<double>, 3 components
+  double x1 = 0, x2 = 0, x3 = 0;
+  Pt() {}
+  Pt(double a, double b, double c) : x1(a), x2(b), x3(c) {}
+  Pt operator+(const Pt &o) const { return Pt(x1 + o.x1, x2 + o.x2, x3 + o.x3); }
+  Pt operator-(const Pt &o) const { return Pt(x1 - o.x1, x2 - o.x2, x3 - o.x3); }
+  Pt operator*(double s) const { return Pt(x1 * s, x2 * s, x3 * s); }
+  double norm() const {  // gtpoint.hpp:72-74
+    double xn = 0;
+    xn += x1 * x1;
+    xn += x2 * x2;
+    xn += x3 * x3;
+    return std::sqrt(xn);
+  }
+};
+
+// GTVector::sortincreasing (2) via quicksorts2l/partitions2l
+// (gtvector.ipp:2965-2999, 3052-3090, 3229-3243): sorts a in place.
+void partition_sort(std::vector<double> &a, std::vector<size_t> &is, long start, long end) {
+  if (start >= end) return;
+  long P = start + (end - start) / 2;
+  double pivot = a[P];
+  std::swap(a[P], a[start]);
+  std::swap(is[P], is[start]);
+  long i = start + 1, j = end;
+  while (i <= j) {
+    while (i <= j && a[i] <= pivot) i++;
+    while (i <= j && a[j] > pivot) j--;
+    if (i < j) {
+      std::swap(a[i], a[j]);
+      std::swap(is[i], is[j]);
+    }
+  }
+  std::swap(a[i - 1], a[start]);
+  std::swap(is[i - 1], is[start]);
+  partition_sort(a, is, start, i - 2);
+  partition_sort(a, is, i, end);
+}
+void sortincreasing(std::vector<double> &a, std::vector<size_t> &is) {
+  is.resize(a.size());
+  for (size_t j = 0; j < a.size(); j++) is[j] = j;
+  partition_sort(a, is, 0, (long)a.size() - 1);
+}
+
+// lagvert (ggrid_icos.ipp:2140-2173)
+void lagvert(const Pt &a, const Pt &b, const Pt &c, long I, long nrows, std::vector<Pt> &R) {
+  R.resize(I + 1);
+  double fact = 1.0 / static_cast<double>(nrows + 1);
+  double xI = static_cast<double>(I);
+  Pt rL = a + ((b - a) * (xI * fact));
+  Pt rR = a + ((c - a) * (xI * fact));
+  fact = I > 0 ? 1.0 / static_cast<double>(I) : 1.0;
+  for (long j = 0; j < I + 1; j++) {
+    double xJ = static_cast<double>(j);
+    R[j] = rL + (rR - rL) * (xJ * fact);
+  }
+}
+
+// project2sphere (1)/(2) (ggrid_icos.ipp:1459-1515)
+Pt proj(const Pt &v, double rad) {
+  double r = v.norm();
+  double xlat = std::asin(v.x3 / r);
+  double xlong = std::atan2(v.x2, v.x1);
+  xlong = xlong < 0.0 ? 2.0 * PI_ + xlong : xlong;
+  return Pt(rad * std::cos(xlat) * std::cos(xlong), rad * std::cos(xlat) * std::sin(xlong),
+            rad * std::sin(xlat));
+}
+
+// xyz2spherical (3) (ggrid_icos.ipp:1700-1720): (r, lat, long)
+Pt sph(const Pt &p) {
+  double x = p.x1, y = p.x2, z = p.x3;
+  double r = std::sqrt(x * x + y * y + z * z);
+  double lon = std::atan2(y, x);
+  lon = lon < 0.0 ? 2.0 * PI_ + lon : lon;
+  return Pt(r, std::asin(z / r), lon);
+}
+
+// GLLBasis::evalBasis (2), evalDBasis (4) (gllbasis.ipp:1300-1327, 1628-1665)
+double evalBasis(const Basis &b, int i, double eta) {
+  LD ttiny = 100.0L * std::numeric_limits<LD>::epsilon();
+  LD tetiny = 100.0L * std::numeric_limits<double>::epsilon();
+  int Np = b.p;
+  LD fact = -1.0 / (Np * (Np + 1.0)), xi = static_cast<LD>(eta);
+  double fRet = 1.0;
+  if (xi < -1.0L || xi > 1.0L) fRet = 0.0;
+  else if (std::fabs(xi - b.xl[i]) > tetiny) {
+    LD ppn_i = b.Pnl[i], ppn_xi, pder_xi, pm1, pdm1, pm2, pdm2;
+    jacobi(Np, 0.0L, 0.0L, ppn_xi, pder_xi, pm1, pdm1, pm2, pdm2, xi);
+    LD gfact = (1.0L - xi * xi) / (xi - b.xl[i] + ttiny);
+    fRet = static_cast<double>(fact * gfact * pder_xi / (ppn_i + ttiny));
+  }
+  return fRet;
+}
+double evalDBasis(const Basis &b, int j, double eta) {
+  LD tetiny = 100.0L * std::numeric_limits<double>::epsilon();
+  int Np = b.p;
+  LD fact = -1.0 / (Np * (Np + 1.0)), xi = static_cast<LD>(eta);
+  LD g1 = xi - b.xl[j];
+  double fRet = 0.0;
+  if (xi == -1.0L && j == 0) fRet = -0.25 * Np * (Np + 1.0);
+  else if (xi == 1.0L && j == Np) fRet = 0.25 * Np * (Np + 1.0);
+  else if (std::fabs(g1) > tetiny) {
+    LD ppn_j = b.Pnl[j], ppn_xi, pder_xi, pm1, pdm1, pm2, pdm2;
+    jacobi(Np, 0.0L, 0.0L, ppn_xi, pder_xi, pm1, pdm1, pm2, pdm2, xi);
+    LD gfact = fact / ppn_j, g1i = 1.0L / g1, g2 = (1.0L - xi * xi) * g1i;
+    LD pdd = 2.0L * xi * pder_xi - Np * (Np + 1.) * ppn_xi;
+    fRet = static_cast<double>(gfact * g1i * (pdd - (2.0L * xi + g2) * pder_xi));
+  }
+  return fRet;
+}
+
+}  // namespace icos
+
+// GGridIcos::init2d/lagrefine/order_triangles/do_elems2d on one rank, with
+// GElem_base::dogeom2d for GE_2DEMBEDDED (gelem_base.cpp:877-955) using the
+// embedded shape functions (gshapefcn_embed.ipp:185-205, 356-390), Jac
+// (gelem_base.cpp:1066-1170) and inv (:1249-1300); GDD_base::doDD on one
+// rank keeps all triangles in order (gdd_base.ipp:73-102).  refine: 0 for
+// GICOS_BISECTION (nrows = 2^ilevel - 1), 1 for GICOS_LAGRANGIAN.
+extern "C" OrGrid *or_grid_icos_create2(int p, int ilevel, int refine, double radius,
+                                        double tol_fac) {
+  using namespace icos;
+  if (p < 1 || ilevel < 0 || (refine != 0 && refine != 1)) return nullptr;
+  OrGrid *gp = new OrGrid();
+  OrGrid &g = *gp;
+  g.dim = 2;
+  g.gtype = 2;
+  if (!gll(p, g.basis[0]) || !gll(p, g.basis[1])) {
+    delete gp;
+    return nullptr;
+  }
+  g.N[0] = g.N[1] = p + 1;
+  g.N[2] = 1;
+  g.nnodes = (size_t)g.N[0] * g.N[1];
+
+  // init2d (ggrid_icos.ipp:139-230)
+  double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+  double fv0[12][3] = {{0.000000000000000, 0.000000000000000, 1.000000000000000},
+                       {0.894427190999916, 0.000000000000000, 0.447213595499958},
+                       {-0.894427190999916, 0.000000000000000, -0.447213595499958},
+                       {0.000000000000000, 0.000000000000000, -1.000000000000000},
+                       {-0.723606797749979, 0.525731112119134, 0.447213595499958},
+                       {-0.723606797749979, -0.525731112119134, 0.447213595499958},
+                       {0.723606797749979, 0.525731112119134, -0.447213595499958},
+                       {0.723606797749979, -0.525731112119134, -0.447213595499958},
+                       {0.276393202250021, 0.850650808352040, 0.447213595499958},
+                       {0.276393202250021, -0.850650808352040, 0.447213595499958},
+                       {-0.276393202250021, 0.850650808352040, -0.447213595499958},
+                       {-0.276393202250021, -0.850650808352040, -0.447213595499958}};
+  double fact = 1.0 / std::sqrt(phi * phi + 1.0);
+  for (auto &r : fv0)
+    for (double &v : r) v *= fact;
+  int ifv0[20][3] = {{0, 1, 8},  {0, 8, 4},   {0, 4, 5},  {0, 5, 9},  {0, 9, 1},
+                     {1, 6, 8},  {8, 6, 10},  {8, 10, 4}, {4, 10, 2}, {4, 2, 5},
+                     {5, 2, 11}, {5, 11, 9},  {9, 11, 7}, {9, 7, 1},  {1, 7, 6},
+                     {3, 6, 7},  {3, 7, 11},  {3, 11, 2}, {3, 2, 10}, {3, 10, 6}};
+
+  // lagrefine (ggrid_icos.ipp:254-328)
+  long nrows = refine == 0 ? (long)std::pow(2, ilevel) - 1 : ilevel;
+  std::vector<std::array<Pt, 3>> tmesh(20 * (nrows * (nrows + 2) + 1));
+  size_t n = 0;
+  std::vector<Pt> R0, R1, Rz(2 * (nrows + 1) + 1);
+  for (int t = 0; t < 20; t++) {
+    Pt a(fv0[ifv0[t][0]][0], fv0[ifv0[t][0]][1], fv0[ifv0[t][0]][2]);
+    Pt b(fv0[ifv0[t][1]][0], fv0[ifv0[t][1]][1], fv0[ifv0[t][1]][2]);
+    Pt c(fv0[ifv0[t][2]][0], fv0[ifv0[t][2]][1], fv0[ifv0[t][2]][2]);
+    for (long l = 0; l < nrows + 1; l++) {
+      lagvert(a, b, c, l, nrows, R0);
+      lagvert(a, b, c, l + 1, nrows, R1);
+      for (long j = 0; j < 2 * l + 3; j++) Rz[j] = (j % 2 == 0) ? R1[j / 2] : R0[(j - 1) / 2];
+      for (long m = 0; m < 2 * l + 1; m++) tmesh[n++] = {Rz[m], Rz[m + 1], Rz[m + 2]};
+    }
+  }
+  for (auto &tr : tmesh)
+    for (auto &v : tr) v = proj(v, 1.0);
+  // order_triangles (ggrid_icos.ipp:2202-2258)
+  std::vector<int> iup(tmesh.size(), 0);
+  for (size_t i = 0; i < tmesh.size(); i++) {
+    std::array<Pt, 3> cverts = tmesh[i];
+    std::vector<double> lat(3), lon(3);
+    for (int j = 0; j < 3; j++) {
+      Pt s = sph(cverts[j]);
+      lat[j] = s.x2;
+      lon[j] = s.x3;
+    }
+    std::vector<size_t> isl;
+    sortincreasing(lon, isl);
+    if (std::fabs(lon[isl[0]] - lon[isl[2]]) < PI_)
+      for (int j = 0; j < 3; j++)
+        if (lon[j] > 1.5 * PI_ && lon[j] <= 2.0 * PI_) lon[j] -= 2.0 * PI_;
+    sortincreasing(lon, isl);
+    if (lat[isl[1]] > lat[isl[0]] && lat[isl[1]] > lat[isl[2]]) {
+      tmesh[i] = {cverts[isl[0]], cverts[isl[2]], cverts[isl[1]]};
+      iup[i] = 1;
+    }
+    if (lat[isl[1]] < lat[isl[0]] && lat[isl[1]] < lat[isl[2]])
+      tmesh[i] = {cverts[isl[1]], cverts[isl[2]], cverts[isl[0]]};
+  }
+
+  // do_elems2d (ggrid_icos.ipp:379-513)
+  const Basis &B0 = g.basis[0], &B1 = g.basis[1];
+  int N0 = g.N[0], N1 = g.N[1];
+  size_t nn = g.nnodes;
+  g.nelems = 3 * tmesh.size();
+  g.ndof = g.nelems * nn;
+  for (int d = 0; d < 3; d++) {
+    g.xn[d].assign(g.ndof, 0.0);
+    for (int j = 0; j < 3; j++) g.dxidx[d][j].assign(g.ndof, 0.0);
+  }
+  g.jac.assign(g.ndof, 0.0);
+  std::vector<double> Ni(nn), xg[3], dNi(nn), tmp(nn);
+  std::vector<double> rij[3][3];
+  // h_b(xi_q) and h_b'(xi_q) tables, [b*N + q]
+  std::vector<double> h0((size_t)N0 * N0), dh0((size_t)N0 * N0), h1((size_t)N1 * N1),
+      dh1((size_t)N1 * N1);
+  for (int b = 0; b < N0; b++)
+    for (int q = 0; q < N0; q++) {
+      h0[(size_t)b * N0 + q] = evalBasis(B0, b, B0.xi[q]);
+      dh0[(size_t)b * N0 + q] = evalDBasis(B0, b, B0.xi[q]);
+    }
+  for (int b = 0; b < N1; b++)
+    for (int q = 0; q < N1; q++) {
+      h1[(size_t)b * N1 + q] = evalBasis(B1, b, B1.xi[q]);
+      dh1[(size_t)b * N1 + q] = evalDBasis(B1, b, B1.xi[q]);
+    }
+  for (size_t i = 0, icurr = 0; i < tmesh.size(); i++) {
+    Pt v1 = tmesh[i][0], v2 = tmesh[i][1], v3 = tmesh[i][2];
+    double third = 1.0 / 3.0;
+    Pt ct = (v1 + (v2 + v3)) * third;
+    for (int j = 0; j < 3; j++, icurr += nn) {
+      Pt cv[4];
+      if (iup[i] == 1) {
+        if (j == 0) { cv[0] = v1; cv[1] = (v1 + v2) * 0.5; cv[2] = ct; cv[3] = (v1 + v3) * 0.5; }
+        if (j == 1) { cv[0] = (v1 + v2) * 0.5; cv[1] = v2; cv[2] = (v2 + v3) * 0.5; cv[3] = ct; }
+        if (j == 2) { cv[0] = ct; cv[1] = (v2 + v3) * 0.5; cv[2] = v3; cv[3] = (v1 + v3) * 0.5; }
+      } else {
+        if (j == 0) { cv[0] = v1; cv[1] = (v1 + v2) * 0.5; cv[2] = ct; cv[3] = (v1 + v3) * 0.5; }
+        if (j == 1) { cv[0] = ct; cv[1] = (v1 + v2) * 0.5; cv[2] = v2; cv[3] = (v2 + v3) * 0.5; }
+        if (j == 2) { cv[0] = (v1 + v3) * 0.5; cv[1] = ct; cv[2] = (v2 + v3) * 0.5; cv[3] = v3; }
+      }
+      for (auto &v : cv) v = proj(v, 1.0);
+      // reorderverts2d with cart2gnomonic (ggrid_icos.ipp:1733-1775, 1923-1988)
+      Pt c = (cv[0] + cv[1] + cv[2] + cv[3]) * 0.25;
+      double xlatc = std::asin(c.x3), xlongc = std::atan2(c.x2, c.x1);
+      xlongc = xlongc < 0.0 ? 2 * PI_ + xlongc : xlongc;
+      double eps = 10.0 * std::numeric_limits<double>::epsilon();
+      Pt tv[4];
+      for (int m = 0; m < 4; m++) {
+        double r = cv[m].norm();
+        double xlat = std::asin(cv[m].x3 / r), xlong = std::atan2(cv[m].x2, cv[m].x1);
+        xlong = xlong < 0.0 ? 2.0 * PI_ + xlong : xlong;
+        double den = std::sin(xlatc) * std::sin(xlat) +
+                     std::cos(xlatc) * std::cos(xlat) * std::cos(xlong - xlongc);
+        den = std::fabs(den) < eps ? 0.0 : 1.0 / den;
+        tv[m].x1 = 1.0 * std::cos(xlat) * std::sin(xlong - xlongc) * den;
+        tv[m].x2 = 1.0 * (std::cos(xlatc) * std::sin(xlat) -
+                          std::sin(xlatc) * std::cos(xlat) * std::cos(xlong - xlongc)) * den;
+      }
+      std::vector<double> xs = {tv[0].x1, tv[1].x1, tv[2].x1, tv[3].x1};
+      std::vector<size_t> Ixy;
+      sortincreasing(xs, Ixy);
+      size_t is[4];
+      if (tv[Ixy[0]].x2 < tv[Ixy[1]].x2) { is[0] = Ixy[0]; is[3] = Ixy[1]; }
+      else { is[0] = Ixy[1]; is[3] = Ixy[0]; }
+      if (tv[Ixy[2]].x2 < tv[Ixy[3]].x2) { is[1] = Ixy[2]; is[2] = Ixy[3]; }
+      else { is[1] = Ixy[3]; is[2] = Ixy[2]; }
+      Pt gv[4];
+      for (int m = 0; m < 4; m++) gv[m] = cv[is[m]];
+      // node coordinates: sum_m Ni_m * (v_m * 0.25) (GShapeFcn_linear::Ni_2d,
+      // gshapefcn_linear.ipp:171-216), then project2sphere (3) to radius
+      for (int l = 0; l < 3; l++) {
+        xg[l].assign(nn, 0.0);
+        for (int m = 0; m < 4; m++) {
+          size_t q = 0;
+          for (int jj = 0; jj < N1; jj++) {
+            double sm = 1.0 - B1.xi[jj], sp = 1.0 + B1.xi[jj];
+            for (int ii = 0; ii < N0; ii++, q++) {
+              double rm = 1.0 - B0.xi[ii], rp = 1.0 + B0.xi[ii];
+              Ni[q] = m == 0 ? rm * sm : m == 1 ? rp * sm : m == 2 ? rp * sp : rm * sp;
+            }
+          }
+          double vml = l == 0 ? gv[m].x1 : l == 1 ? gv[m].x2 : gv[m].x3;
+          for (size_t q2 = 0; q2 < nn; q2++) xg[l][q2] += Ni[q2] * (vml * 0.25);
+        }
+      }
+      for (size_t q = 0; q < nn; q++) {
+        double x = xg[0][q], y = xg[1][q], z = xg[2][q];
+        double r = std::sqrt(x * x + y * y + z * z);
+        double xlat = std::asin(z / r), xlong = std::atan2(y, x);
+        xlong = xlong < 0.0 ? 2.0 * PI_ + xlong : xlong;
+        g.xn[0][icurr + q] = radius * std::cos(xlat) * std::cos(xlong);
+        g.xn[1][icurr + q] = radius * std::cos(xlat) * std::sin(xlong);
+        g.xn[2][icurr + q] = radius * std::sin(xlat);
+      }
+      // dogeom2d, embedded: rij(k,m) = sum_I dN_I/dxi^m x^k_I
+      for (int m = 0; m < 3; m++)
+        for (int k = 0; k < 3; k++) {
+          tmp.assign(nn, 0.0);
+          for (int jb = 0, nI = 0; jb < N1; jb++)
+            for (int ib = 0; ib < N0; ib++, nI++) {
+              // GShapeFcn_embed::dNdXi_2d (gshapefcn_embed.ipp:356-390)
+              const double *d0 = (m == 0 ? dh0 : h0).data() + (size_t)ib * N0;
+              const double *d1 = (m == 1 ? dh1 : h1).data() + (size_t)jb * N1;
+              for (int qj = 0, q = 0; qj < N1; qj++)
+                for (int qi = 0; qi < N0; qi++, q++) dNi[q] = d0[qi] * d1[qj];
+              double xk = g.xn[k][icurr + nI];
+              for (size_t q = 0; q < nn; q++) {
+                dNi[q] *= xk;
+                tmp[q] += dNi[q];
+              }
+            }
+          rij[k][m] = tmp;
+        }
+      for (size_t q = 0; q < nn; q++) {
+        auto G = [&](int a, int b) { return rij[a][b][q]; };
+        double det = G(0, 0) * (G(1, 1) * G(2, 2) - G(1, 2) * G(2, 1)) -
+                     G(0, 1) * (G(1, 0) * G(2, 2) - G(1, 2) * G(2, 0)) +
+                     G(0, 2) * (G(1, 0) * G(2, 1) - G(1, 1) * G(2, 0));
+        double jv = std::fabs(det);
+        double x = G(0, 2), y = G(1, 2), z = G(2, 2);
+        jv /= std::sqrt(x * x + y * y + z * z);
+        g.jac[icurr + q] = jv;
+        double ijac = 1.0 / det;
+        size_t id = icurr + q;
+        g.dxidx[0][0][id] = (G(1, 1) * G(2, 2) - G(1, 2) * G(2, 1)) * ijac;
+        g.dxidx[0][1][id] = -(G(0, 1) * G(2, 2) - G(2, 1) * G(0, 2)) * ijac;
+        g.dxidx[0][2][id] = (G(0, 1) * G(1, 2) - G(0, 2) * G(1, 1)) * ijac;
+        g.dxidx[1][0][id] = -(G(1, 0) * G(2, 2) - G(2, 0) * G(1, 2)) * ijac;
+        g.dxidx[1][1][id] = (G(0, 0) * G(2, 2) - G(2, 0) * G(0, 2)) * ijac;
+        g.dxidx[1][2][id] = -(G(0, 0) * G(1, 2) - G(0, 2) * G(1, 0)) * ijac;
+        g.dxidx[2][0][id] = (G(1, 0) * G(2, 1) - G(1, 1) * G(2, 0)) * ijac;
+        g.dxidx[2][1][id] = -(G(0, 0) * G(2, 1) - G(0, 1) * G(2, 0)) * ijac;
+        g.dxidx[2][2][id] = (G(0, 0) * G(1, 1) - G(1, 0) * G(0, 1)) * ijac;
+      }
+    }
+  }
+  // find_min_dist (ggrid.ipp:977-1013), no periodicity, no boundaries
+  double tiny = 1000.0 * std::numeric_limits<double>::epsilon(), dmin = std::numeric_limits<double>::max();
+  for (size_t q = 0; q + 1 < g.ndof; q++) {
+    double dx = g.xn[0][q + 1] - g.xn[0][q], dy = g.xn[1][q + 1] - g.xn[1][q],
+           dz = g.xn[2][q + 1] - g.xn[2][q];
+    double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (r > tiny) dmin = std::min(dmin, r);
+  }
+  g.minnodedist = dmin;
+  for (int d = 0; d < 3; d++) g.xp[d] = g.xn[d];
+  g.mask.assign(g.ndof, 1.0);
+  finish_grid(g, tol_fac);
+  return gp;
+}
+
 extern "C" OrGrid *or_grid_icos_create(int p, int ilevel, double radius, double tol_fac) {
-  (void)p; (void)ilevel; (void)radius; (void)tol_fac;
-  return nullptr;  // filled in by the icosahedral restatement
+  return or_grid_icos_create2(p, ilevel, 0, radius, tol_fac);
 }

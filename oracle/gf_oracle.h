@@ -41,6 +41,8 @@ OrGrid *or_grid_box_create(int dim, const int *p, const int *ne,
                            const int *bdy, double tol_fac);
 /* ---- Icosahedral 2-D sphere (src/cdg/grid/ggrid_icos.ipp) --------------- */
 OrGrid *or_grid_icos_create(int p, int ilevel, double radius, double tol_fac);
+/* refine: 0 GICOS_BISECTION, 1 GICOS_LAGRANGIAN */
+OrGrid *or_grid_icos_create2(int p, int ilevel, int refine, double radius, double tol_fac);
 void    or_grid_destroy(OrGrid *g);
 
 size_t or_grid_ndof(const OrGrid *g);
@@ -124,6 +126,13 @@ void or_helmholtz_diag_inv(OrGrid *g, const double *p, size_t np, const double *
 
 /* OpenMP thread count for the element loops (1 = serial reference order). */
 void or_set_threads(int n);
+
+/* Adjoint of the Burgers RHS and RK step (mode 0, regular grids): the
+ * CPU reference for the hand-written GPU adjoint. */
+void or_burgers_rhs_adjoint(OrGrid *g, double nu, int nstate, const double *const *u,
+                            const double *const *w, double *const *lam);
+void or_burgers_step_adjoint(OrGrid *g, double nu, int order, int nstate, double dt,
+                             const double *const *un, double *const *lam);
 
 #ifdef __cplusplus
 }
