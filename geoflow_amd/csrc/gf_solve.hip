@@ -29,6 +29,9 @@ __global__ void k_axpy(double *y, double a, const double *__restrict__ x, size_t
 __global__ void k_copy(double *__restrict__ y, const double *__restrict__ x, size_t n) {
   GRID_LOOP(g, n) y[g] = x[g];
 }
+__global__ void k_zero(double *__restrict__ y, size_t n) {
+  GRID_LOOP(g, n) y[g] = 0.0;
+}
 __global__ void k_nonfinite(const double *__restrict__ x, size_t n, double *flag) {
   bool bad = false;
   GRID_LOOP(g, n) bad |= !isfinite(x[g]);
@@ -638,6 +641,14 @@ int gf_burgers_rhs(gf_mesh *m, int mode, double nu, int nstate, const double *co
     RC(gf_helm_t(m, nu, nullptr, nullptr, 0.0, u[k], urhs));
     RC(axpby(m, urhs, -1.0, urhs, -1.0, dudt[k]));
     RC(gf_mul_t(m, m->imass, urhs, dudt[k]));
+  }
+  // dudt_impl forms the RHS of the first GDIM components only
+  // (gburgers.ipp:247); the third velocity component of a sphere grid gets
+  // a zero RHS and changes only through constrain2sphere.
+  for (int k = m->dim; k < nstate; k++) {
+    hipLaunchKernelGGL(k_zero, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, dudt[k],
+                       m->ndof);
+    GF_LAUNCH_CHECK(m->ctx);
   }
   return GF_OK;
 }

@@ -63,3 +63,62 @@ def unique_normal(grid, seed=67890):
     draw = rng.standard_normal(n)
     first = mem[offs[:-1]]
     return draw[first]
+
+
+def williamson1_velocity(x, y, z, c0, alpha):
+    """Solid-body rotation of Williamson et al. (JCP 102, 1992) test 1:
+    latitude/longitude components
+        v_lat = -c0 sin(lon) sin(alpha)
+        v_lon =  c0 (cos(lat) cos(alpha) + sin(lat) cos(lon) sin(alpha))
+    turned into Cartesian components as GMTK::vsphere2cart does for physical
+    components (gmtk.ipp:3019-3032); this is rotation about the axis
+    (-sin alpha, 0, cos alpha) at angular speed c0 / R.  The reference's
+    set-up (ginitstate_direct_user.ipp:582-593) has the opposite sign on the
+    sin(lat) cos(lon) sin(alpha) term, which is not a rigid rotation for
+    alpha != 0 (the bell would not return after a revolution); the published
+    form is used so that the exact solution is the rotated bell."""
+    r = np.sqrt(x * x + y * y + z * z)
+    lat = np.arcsin(z / r)
+    lon = np.arctan2(y, x)
+    vlat = -c0 * np.sin(lon) * np.sin(alpha)
+    vlon = c0 * (np.cos(lat) * np.cos(alpha) + np.sin(lat) * np.cos(lon) * np.sin(alpha))
+    vx = -vlat * np.sin(lat) * np.cos(lon) - vlon * np.sin(lon)
+    vy = -vlat * np.sin(lat) * np.sin(lon) + vlon * np.cos(lon)
+    vz = vlat * np.cos(lat)
+    return [vx, vy, vz]
+
+
+def cosine_bell(x, y, z, lat0, lon0, radius=1.0, h0=1.0):
+    """Williamson test-1 cosine bell of radius R/3 centred at (lat0, lon0):
+    h = h0/2 (1 + cos(pi r / (R/3))) for great-circle distance r < R/3."""
+    R = radius
+    rr = np.sqrt(x * x + y * y + z * z)
+    lat = np.arcsin(np.clip(z / rr, -1.0, 1.0))
+    lon = np.arctan2(y, x)
+    c = np.sin(lat0) * np.sin(lat) + np.cos(lat0) * np.cos(lat) * np.cos(lon - lon0)
+    r = R * np.arccos(np.clip(c, -1.0, 1.0))
+    rb = R / 3.0
+    return np.where(r < rb, 0.5 * h0 * (1.0 + np.cos(np.pi * r / rb)), 0.0)
+
+
+def rotated_bell(x, y, z, lat0, lon0, alpha, angle, radius=1.0, h0=1.0):
+    """Exact solution of the rotation: the initial bell evaluated at the
+    points rotated back by `angle` about the axis (-sin alpha, 0, cos alpha)
+    (Rodrigues' formula)."""
+    n = np.array([-np.sin(alpha), 0.0, np.cos(alpha)])
+    c, s = np.cos(-angle), np.sin(-angle)
+    dot = n[0] * x + n[1] * y + n[2] * z
+    cx = n[1] * z - n[2] * y
+    cy = n[2] * x - n[0] * z
+    cz = n[0] * y - n[1] * x
+    xr = x * c + cx * s + n[0] * dot * (1 - c)
+    yr = y * c + cy * s + n[1] * dot * (1 - c)
+    zr = z * c + cz * s + n[2] * dot * (1 - c)
+    return cosine_bell(xr, yr, zr, lat0, lon0, radius, h0)
+
+
+def bell_centre(seed=12345):
+    """Seed-chosen bell centre: latitude U(-60, 60) degrees, longitude
+    U[0, 360) degrees, returned in radians."""
+    rng = np.random.default_rng(seed)
+    return np.deg2rad(rng.uniform(-60.0, 60.0)), np.deg2rad(rng.uniform(0.0, 360.0))

@@ -212,8 +212,9 @@ int gf_imult(gf_mesh *mesh, double *imult);
  * GAdvect::apply (src/cdg/sem/gadvect.ipp:57-99): po = M sum_j u_j dp/dx_j. */
 int gf_advect(gf_mesh *mesh, const double *p, int nvel, const double *const *u, double *po);
 /* GBurgers::dudt_impl (src/cdg/pdes/gburgers.ipp:202-257).
- * mode 0: nonlinear, nstate = dim components;  mode 1: pure advection of
- * u[0] by u[1..nstate-1];  mode 2: heat. */
+ * mode 0: nonlinear, nstate = dim components (3 on sphere grids, where, as
+ * in the reference's loop over GDIM, the third component's RHS is zero);
+ * mode 1: pure advection of u[0] by u[1..nstate-1];  mode 2: heat. */
 int gf_burgers_rhs(gf_mesh *mesh, int mode, double nu, int nstate, const double *const *u,
                    double *const *dudt);
 /* GBurgers::step_impl -> GExRKStepper::step_b with Butcher RK of order

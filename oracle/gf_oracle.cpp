@@ -768,6 +768,9 @@ void or_burgers_rhs(OrGrid *g, int mode, double nu, int nstate, const double *co
     for (size_t j = 0; j < n; j++) urhs[j] = -1.0 * urhs[j] + -1.0 * dudt[k][j];
     or_imass_apply(g, urhs.data(), dudt[k]);
   }
+  // the loop runs over GDIM (gburgers.ipp:247): a sphere grid's third
+  // velocity component has no RHS; it is returned as zero
+  for (int k = g->dim; k < nstate; k++) std::fill(dudt[k], dudt[k] + n, 0.0);
 }
 
 }  // extern "C"

@@ -151,10 +151,21 @@ def test_icos_pure_advection_rk4_matches_oracle(ctx):
 @pytest.mark.gpu
 def test_icos_nonlinear_burgers_rk4_matches_oracle(ctx):
     """Three velocity components on the sphere: step_b projects them onto the
-    tangent plane (constrain2sphere, gmtk.ipp:2914-2980)."""
+    tangent plane (constrain2sphere, gmtk.ipp:2914-2980); dudt_impl forms the
+    RHS of the first GDIM = 2 components only (gburgers.ipp:247)."""
     og, ig, m = _make(ctx, p=4)
     st = [0.2 * v for v in _rotation(*ig.xyz)]
     dst = [_dev(a) for a in st]
+    # the RHS, with the third component's output poisoned beforehand
+    outs = [m.new() for _ in range(3)]
+    outs[2].fill_(7.0)
+    from geoflow_amd.device import _ptrs
+    from geoflow_amd._capi import lib
+    m._c(lib().gf_burgers_rhs(m.h, 0, 0.01, 3, _ptrs(dst), _ptrs(outs)))
+    ref = og.burgers_rhs(st, 0.01)
+    for k in range(2):
+        assert relerr(_host(outs[k]), ref[k]) <= TOL, k
+    assert not _host(outs[2]).any() and not ref[2].any()
     for s in range(3):
         m.burgers_step(dst, 0.01, 1e-3)
         og.burgers_step(st, 0.01, 1e-3)
