@@ -72,6 +72,57 @@ def cpu_baseline(seconds=12.0):
                       "3 components) in %.1f s" % (n, og.ndof, dt)}
 
 
+def sphere_advection(ctx, level, p, nsteps, full=False, seed=12345):
+    """Cosine bell (radius R/3, seed-chosen centre) under Williamson test-1
+    solid-body rotation tilted by pi/4 on the icosahedral sphere (GICOS
+    bisection, GE_2DEMBEDDED quads), RK4 pure advection (nu = 0); one
+    revolution takes nsteps steps.  short: times 100 steps; full: runs the
+    revolution and reports L2/Linf errors against the exact (rotated) bell."""
+    import torch
+    import geoflow_amd as ga
+    from geoflow_amd.device import Mesh
+    from geoflow_amd import workloads as W
+    t = time.perf_counter()
+    g = ga.IcosGrid(p, level, radius=1.0)
+    log("icosahedral grid level %d, p=%d: %d quads, %d local DOF, %.1f s"
+        % (level, p, g.nelems, g.ndof, time.perf_counter() - t))
+    m = Mesh(ctx, g)
+    lat0, lon0 = W.bell_centre(seed)
+    alpha, T = np.pi / 4, 1.0
+    c0 = 2 * np.pi / T
+    h0 = W.cosine_bell(*g.xyz, lat0, lon0)
+    st = [torch.from_numpy(a).cuda() for a in [h0] + W.williamson1_velocity(*g.xyz, c0, alpha)]
+    dt = T / nsteps
+    m.burgers_step(st, 0.0, dt, mode=1)  # warm-up
+    st[0].copy_(torch.from_numpy(h0))
+    torch.cuda.synchronize()
+    run = nsteps if full else min(100, nsteps)
+    t0 = time.perf_counter()
+    done = 0
+    while done < run:
+        k = min(1000, run - done)
+        m.burgers_steps(st, 0.0, dt, k, t=done * dt, mode=1)
+        done += k
+        torch.cuda.synchronize()
+        if full:
+            log("sphere: %d / %d steps, %.1f s" % (done, run, time.perf_counter() - t0))
+    el = time.perf_counter() - t0
+    res = {"level": level, "p": p, "quads": g.nelems, "ndof": g.ndof, "steps_per_revolution": nsteps,
+           "dt": dt, "steps_timed": run, "ms_per_step": 1e3 * el / run,
+           "dof_steps_per_s": g.ndof * run / el, "dtype": "f64",
+           "bell_centre_deg": [float(np.rad2deg(lat0)), float(np.rad2deg(lon0))]}
+    if full:
+        h = st[0].cpu().numpy()
+        ex = W.rotated_bell(*g.xyz, lat0, lon0, alpha, c0 * T)
+        err = h - ex
+        mass = m.get("mass")
+        res["l2_error"] = float(np.sqrt(np.dot(mass, err * err) / np.dot(mass, ex * ex)))
+        res["linf_error"] = float(np.abs(err).max() / np.abs(ex).max())
+        log("sphere: L2 %.3e  Linf %.3e" % (res["l2_error"], res["linf_error"]))
+    log("sphere advection: %.3f ms/step, %.3e DOF-steps/s" % (res["ms_per_step"], res["dof_steps_per_s"]))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +133,13 @@ def main():
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--sphere", choices=["off", "short", "full"], default="short",
+                    help="icosahedral sphere advection: short = timed steps only, "
+                         "full = one revolution with L2/Linf errors")
+    ap.add_argument("--sphere-level", type=int, default=6)
+    ap.add_argument("--sphere-steps", type=int, default=10000,
+                    help="RK4 steps per revolution (the stability limit at level 6, p=5 "
+                         "is about 7900)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -205,6 +263,10 @@ def main():
             % (a100, ap_ms, ap32_ms, res["speedup_mixed_vs_fp64"]))
         out["cg"] = res
         del m
+
+    if args.sphere != "off":
+        out["sphere"] = sphere_advection(ctx, args.sphere_level, 5, args.sphere_steps,
+                                         full=args.sphere == "full")
 
     out["config"] = {"workload": "3-D box %d^3 hex elements, p=%d (%d local DOF): Burgers RK4 "
                                  "(3 comps, nu=0.02, dt=1e-5, periodic) and Helmholtz (M+0.1L) "

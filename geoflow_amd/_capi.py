@@ -30,6 +30,7 @@ class GeoflowError(RuntimeError):
 
 GF_MESH_BOX_GATHER = 1
 GF_MESH_BOX_MASK = 2
+GF_MESH_CONST_D = 4
 
 
 class MeshDesc(ctypes.Structure):

@@ -26,6 +26,10 @@ struct gf_mesh_s {
   int dim = 3, gtype = 0;
   int N[3] = {1, 1, 1};
   int Niso = 0;  // isotropic N, 0 if anisotropic
+  // dconst: 3-D isotropic mesh whose D is bit-identical to gf::GllD<Niso>
+  // (gf_dtab.hpp); the pencil kernels, which take D as compile-time
+  // constants, run only then.
+  bool dconst = false;
   size_t nelems = 0, nn = 0, ndof = 0;
   // basis (device)
   double *D[3] = {nullptr, nullptr, nullptr};

@@ -123,6 +123,10 @@ void k_burgers_stage3p(
   __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZ];
   ColIdx<N> t(a.nelems);
   const size_t col = t.e * S::NN + t.r;
+  // the inverse mass of the column, read once for all components
+  double im[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) im[k] = a.imass[col + k * S::N2];
   double ru[3][N];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -151,18 +155,16 @@ void k_burgers_stage3p(
       if (!t.active) return;
       const size_t g = col + k * S::N2;
       // z-derivative again (same operands and order as in the z pass)
-      using CP = const __attribute__((address_space(4))) double *;
-      const CP row = gf::opaque_ptr((CP)Dp.R[2] + k * N);
       double uz = 0.0;
 #pragma unroll
-      for (int m2 = 0; m2 < N; m2++) uz = fma(row[m2], ru[c][m2], uz);
+      for (int m2 = 0; m2 < N; m2++) uz = fma(gf::GllD<N>::at(k, m2), ru[c][m2], uz);
       double po = hx * ru[0][k];
       po += hy * ru[1][k];
       po += (uz * rz) * ru[2][k];
       const double adv = po * M;
       if (a.nu != 1.0) L *= a.nu;
       const double urhs = -1.0 * L + -1.0 * adv;
-      kout[g] = a.imass[g] * urhs;
+      kout[g] = im[k] * urhs;
     };
     gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, a.W, t.i, t.j, jac,
                                  rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
@@ -217,8 +219,13 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
     gf::box_elem_coords(topo, t.e, ec);
     de = gf::box_elem_deltas(topo, N, ec);
   }
+  // The inverse mass of the column, read once for all components.
+  double im[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) im[k] = imass[col + (size_t)k * S::N2];
   // Stage input (and accumulator) per node; all loads are issued before any
-  // store, the accumulator stores follow the loop.
+  // store, the accumulator stores follow the loop.  Absent group members are
+  // not read (box_load8m).
   double ru[3][N];
   double av[ACC ? 3 : 1][ACC ? N : 1];
 #pragma unroll
@@ -234,11 +241,11 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       double kv[8];
-      gf::box_load8<size_t, double>(kp[c], mem, cnt, g, kv);
+      gf::box_load8m<size_t, double>(kp[c], mem, cnt, kv);
       const double ui = uin[c][g];
       const double ao = ACC == 2 ? acc[c][g] : 0.0;
       double uv[8];
-      if (SMV) gf::box_load8<size_t, double>(uin[c], mem, cnt, g, uv);
+      if (SMV) gf::box_load8m<size_t, double>(uin[c], mem, cnt, uv);
       // GGFX Smooth: Sum / count (an exact multiply for counts 1, 2, 4, 8)
       const double ks = cnt == 1 ? kv[0] : gf::box_sum8<double>(kv, cnt) * gf::box_inv(cnt);
       double v = ui + bh * ks;
@@ -269,18 +276,16 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       if (!t.active) return;
       const size_t g = col + k * S::N2;
       // z-derivative again (same operands and order as in the z pass)
-      using CP = const __attribute__((address_space(4))) double *;
-      const CP row = gf::opaque_ptr((CP)Dp.R[2] + k * N);
       double uz = 0.0;
 #pragma unroll
-      for (int m2 = 0; m2 < N; m2++) uz = fma(row[m2], ru[c][m2], uz);
+      for (int m2 = 0; m2 < N; m2++) uz = fma(gf::GllD<N>::at(k, m2), ru[c][m2], uz);
       double po = hx * ru[0][k];
       po += hy * ru[1][k];
       po += (uz * rz) * ru[2][k];
       const double adv = po * M;
       if (nu != 1.0) L *= nu;
       const double urhs = -1.0 * L + -1.0 * adv;
-      ko[g] = imass[g] * urhs;
+      ko[g] = im[k] * urhs;
     };
     gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, W, t.i, t.j, jac,
                                  rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
@@ -290,56 +295,111 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
 }
 
 // Step output on a structured box: u = Smooth(1*acc + cl*Smooth(K)) with both
-// smooths by gather (step_b's last update and GGFX Smooth).
+// smooths by gather (step_b's last update and GGFX Smooth).  One workgroup
+// per element: the element's halo (gf::BoxHalo) of K and acc is staged in
+// LDS, each neighbour value loaded once, and every node sums its group from
+// there in the GGFX order.
+template <int N>
+struct HaloLaunch {
+  static constexpr int NN = N * N * N;
+  static constexpr int THREADS = NN >= 256 ? 256 : ((NN + 63) / 64) * 64;
+  static constexpr int HS = (gf::BoxHalo<N>::SIZE + THREADS - 1) / THREADS;  // halo slots/thread
+  static constexpr int NS = (NN + THREADS - 1) / THREADS;                     // nodes/thread
+};
+
 template <int N, typename I>
-__global__ __launch_bounds__(kThreads) void k_rk_out_box(
+__global__ __launch_bounds__(HaloLaunch<N>::THREADS) void k_rk_out_box(
     double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2,
     const double *__restrict__ a0, const double *__restrict__ a1, const double *__restrict__ a2,
     const double *__restrict__ k0, const double *__restrict__ k1, const double *__restrict__ k2,
     double cl, gf::BoxTopo topo, size_t nelems) {
-  constexpr int NN = N * N * N;
-  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;
-  constexpr int CN = CE * NN;
+  using HB = gf::BoxHalo<N>;
+  using L = HaloLaunch<N>;
+  constexpr int NN = L::NN, T = L::THREADS, H = HB::H;
+  __shared__ double hk[HB::SIZE], ha[HB::SIZE];
   double *u[3] = {u0, u1, u2};
   const double *acc[3] = {a0, a1, a2};
   const double *kf[3] = {k0, k1, k2};
-  const size_t nchunks = (nelems + CE - 1) / CE;
-  const size_t cpb = (nchunks + gridDim.x - 1) / gridDim.x;
-  const size_t c0 = xcd_block() * cpb, c1 = c0 + cpb < nchunks ? c0 + cpb : nchunks;
-  for (size_t ch = c0; ch < c1; ch++) {
-    int ec0[3];
-    gf::box_elem_coords(topo, ch * CE, ec0);
-    const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);
+  const size_t e = xcd_block();
+  if (e >= nelems) return;  // whole workgroup
+  int ec[3];
+  gf::box_elem_coords(topo, e, ec);
+  const gf::BoxElemDeltas de = gf::box_elem_deltas(topo, N, ec);
+  const I gbase = (I)(e * NN);
+  const int tid = (int)threadIdx.x;
+  // halo slots of this thread and their global indices
+  I hg[L::HS];
+  bool hv[L::HS];
 #pragma unroll
-    for (int q0 = 0; q0 < CN; q0 += kThreads) {
-      const int q = q0 + (int)threadIdx.x;
-      if (q >= CN) break;
-      const int el = CE == 1 ? 0 : q / NN;
-      const int rr = CE == 1 ? q : q - el * NN;
-      const size_t e = ch * CE + el;
-      if (e >= nelems) break;
-      gf::BoxElemDeltas de = d0;
-      if (CE > 1 && el) {
-        int ec[3];
-        gf::box_elem_coords(topo, e, ec);
-        de = gf::box_elem_deltas(topo, N, ec);
+  for (int s = 0; s < L::HS; s++) {
+    const int q = s * T + tid;
+    I g = gbase;
+    hv[s] = q < HB::SIZE && gf::box_halo_index<N, I>(de, gbase, q % H, (q / H) % H, q / (H * H), &g);
+    hg[s] = hv[s] ? g : gbase;
+  }
+  // every load (all components) before the first LDS store
+  double kv[3][L::HS], av[3][L::HS];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int s = 0; s < L::HS; s++) {
+      double x = 0.0, y = 0.0;
+      if (hv[s]) {
+        x = kf[c][hg[s]];
+        y = acc[c][hg[s]];
       }
-      const int i = rr % N, j = (rr / N) % N, k = rr / (N * N);
-      const I g = (I)(e * NN + rr);
-      I mem[8];
-      const int cnt = gf::box_members<I>(de, N, g, i, j, k, mem);
+      kv[c][s] = x;
+      av[c][s] = y;
+    }
+  // results are kept until the last barrier: a barrier also waits for the
+  // workgroup's outstanding global stores
+  double uo[3][L::NS];
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        double kv[8], av[8];
-        gf::box_load8<I, double>(kf[c], mem, cnt, g, kv);
-        gf::box_load8<I, double>(acc[c], mem, cnt, g, av);
-        const double ks = cnt == 1 ? kv[0] : gf::box_sum8<double>(kv, cnt) * gf::box_inv(cnt);
+  for (int c = 0; c < 3; c++) {
+    if (c) __syncthreads();  // the previous component's halo reads are done
 #pragma unroll
-        for (int m = 0; m < 8; m++) av[m] = 1.0 * av[m] + cl * ks;
-        u[c][g] = cnt == 1 ? av[0] : gf::box_sum8<double>(av, cnt) * gf::box_inv(cnt);
+    for (int s = 0; s < L::HS; s++) {
+      const int q = s * T + tid;
+      if (q < HB::SIZE) {
+        hk[q] = kv[c][s];
+        ha[q] = av[c][s];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ns = 0; ns < L::NS; ns++) {
+      const int r = ns * T + tid;
+      if (r < NN) {
+        const int i = r % N, j = (r / N) % N, k = r / (N * N);
+        int sl[8];
+        const int cnt = gf::box_member_slots<N>(de, i, j, k, sl);
+        double ks = hk[sl[0]];
+        if (cnt > 1) {
+          double sum = 0.0;
+#pragma unroll
+          for (int m = 0; m < 8; m++)
+            if (m < cnt) sum += hk[sl[m]];
+          ks = sum * gf::box_inv(cnt);
+        }
+        double out = 1.0 * ha[sl[0]] + cl * ks;
+        if (cnt > 1) {
+          double sum = 0.0;
+#pragma unroll
+          for (int m = 0; m < 8; m++)
+            if (m < cnt) sum += 1.0 * ha[sl[m]] + cl * ks;
+          out = sum * gf::box_inv(cnt);
+        }
+        uo[c][ns] = out;
       }
     }
   }
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int ns = 0; ns < L::NS; ns++) {
+      const int r = ns * T + tid;
+      if (r < NN) u[c][gbase + (I)r] = uo[c][ns];
+    }
 }
 
 template <int N, bool KP, int ACC, bool SMV>
@@ -379,17 +439,17 @@ int launch_stage3b(gf_mesh *m, double *const *uin, double *const *kp, double *co
 template <int N>
 int launch_rk_out_box(gf_mesh *m, double *const *u, double *const *acc, double *const *kf,
                       double cl) {
-  constexpr int NN = N * N * N;
-  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;
-  size_t nchunks = (m->nelems + CE - 1) / CE;
-  int nb = xcd_grid(std::min<size_t>(nchunks, 8192));
+  // one workgroup per element (the grid is rounded up to a multiple of the
+  // XCD count; surplus workgroups exit)
+  const int nb = xcd_grid(m->nelems);
+  constexpr int T = HaloLaunch<N>::THREADS;
   if (m->ndof < ((size_t)1 << 31))
-    hipLaunchKernelGGL((k_rk_out_box<N, int32_t>), dim3(nb), dim3(kThreads), 0, m->ctx->stream,
-                       u[0], u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
+    hipLaunchKernelGGL((k_rk_out_box<N, int32_t>), dim3(nb), dim3(T), 0, m->ctx->stream, u[0],
+                       u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
                        m->nelems);
   else
-    hipLaunchKernelGGL((k_rk_out_box<N, int64_t>), dim3(nb), dim3(kThreads), 0, m->ctx->stream,
-                       u[0], u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
+    hipLaunchKernelGGL((k_rk_out_box<N, int64_t>), dim3(nb), dim3(T), 0, m->ctx->stream, u[0],
+                       u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
                        m->nelems);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
@@ -455,6 +515,11 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
   constexpr int NN = N * N * N;
   constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;  // elements per chunk
   constexpr int CN = CE * NN;                              // nodes per chunk
+  constexpr int QPC = (CN + kThreads - 1) / kThreads;      // nodes per thread per chunk
+  // chunks per batch: every load of a batch is issued before its first
+  // store, so a batch should keep enough bytes in flight (FP32 takes two)
+  constexpr int BC = (sizeof(T) == 4 && QPC <= 2) ? 2 : 1;
+  constexpr int NQ = BC * QPC;
   __shared__ double sred[kThreads / 64];
   if (scal[S_DONE] != 0.0) return;
   const T alpha = (T)scal[S_ALPHA];
@@ -463,37 +528,61 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
   const size_t nchunks = (nelems + CE - 1) / CE;
   const size_t cpb = (nchunks + gridDim.x - 1) / gridDim.x;
   const size_t c0 = xcd_block() * cpb, c1 = c0 + cpb < nchunks ? c0 + cpb : nchunks;
-  for (size_t c = c0; c < c1; c++) {
-    int ec0[3];
-    gf::box_elem_coords(topo, c * CE, ec0);
-    const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);  // uniform when CE == 1
+  for (size_t c = c0; c < c1; c += BC) {
+    I gs[NQ];
+    T dv[NQ], xo[NQ], wo[NQ], ro[NQ], qm[NQ][8];
+    int cnt[NQ];
+    bool ok[NQ];
 #pragma unroll
-    for (int q0 = 0; q0 < CN; q0 += kThreads) {
-      const int q = q0 + (int)threadIdx.x;
-      if (q >= CN) break;
-      const int el = CE == 1 ? 0 : q / NN;
-      const int rr = CE == 1 ? q : q - el * NN;
-      const size_t e = c * CE + el;
-      if (e >= nelems) break;
-      gf::BoxElemDeltas de = d0;
-      if (CE > 1 && el) {
-        int ec[3];
-        gf::box_elem_coords(topo, e, ec);
-        de = gf::box_elem_deltas(topo, N, ec);
+    for (int bc = 0; bc < BC; bc++) {
+      const size_t cc = c + bc;
+      int ec0[3];
+      gf::box_elem_coords(topo, cc * CE, ec0);
+      const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);  // uniform when CE == 1
+#pragma unroll
+      for (int qq = 0; qq < QPC; qq++) {
+        const int s = bc * QPC + qq;
+        const int q = qq * kThreads + (int)threadIdx.x;
+        const int el = CE == 1 ? 0 : q / NN;
+        const int rr = CE == 1 ? q : q - el * NN;
+        const size_t e = cc * CE + el;
+        ok[s] = cc < c1 && q < CN && e < nelems;
+        gf::BoxElemDeltas de = d0;
+        if (CE > 1 && el) {
+          int ec[3];
+          gf::box_elem_coords(topo, e, ec);
+          de = gf::box_elem_deltas(topo, N, ec);
+        }
+        const int i = rr % N, j = (rr / N) % N, k = rr / (N * N);
+        // out-of-range slots read node 0 (always valid) and are not stored
+        const I g = ok[s] ? (I)(e * NN + rr) : (I)0;
+        gs[s] = g;
+        I mem[8];
+        if (ok[s]) {
+          cnt[s] = gf::box_members<I>(de, N, g, i, j, k, mem);
+        } else {
+          cnt[s] = 1;
+#pragma unroll
+          for (int m2 = 0; m2 < 8; m2++) mem[m2] = (I)0;
+        }
+        dv[s] = dinv ? dinv[g] : T(1);
+        xo[s] = x[g];
+        wo[s] = w[g];
+        ro[s] = r[g];
+        gf::box_load8m<I, T>(qv, mem, cnt[s], qm[s]);
       }
-      const int i = rr % N, j = (rr / N) % N, k = rr / (N * N);
-      const I g = (I)(e * NN + rr);
-      // every load of the node is issued before the first store
-      const T dv = dinv ? dinv[g] : T(1);
-      const T xo = x[g], wo = w[g], ro = r[g];
-      int cnt;
-      const T qa = gf::box_gather_sum<N, I, T>(de, qv, g, i, j, k, &cnt);
-      const T xv = (T)1 * xo + alpha * wo;
+    }
+#pragma unroll
+    for (int s = 0; s < NQ; s++) {
+      if (!ok[s]) continue;
+      const I g = gs[s];
+      const T qa = gf::box_sum8<T>(qm[s], cnt[s]);
+      const T xv = (T)1 * xo[s] + alpha * wo[s];
       x[g] = xv;
-      const T rv = (T)1 * ro + (-alpha) * qa;
+      const T rv = (T)1 * ro[s] + (-alpha) * qa;
       r[g] = rv;
-      const T z = dinv ? dv * rv : rv;
-      const double im = gf::box_inv(cnt);
+      const T z = dinv ? dv[s] * rv : rv;
+      const double im = gf::box_inv(cnt[s]);
       rho += (double)rv * (double)z * im;
       const double rd = (double)rv;
       if (NORM == GF_NORM_INF) nrm = fmax(nrm, fabs(rd));

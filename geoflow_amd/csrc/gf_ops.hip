@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "gf_dtab.hpp"
 #include "gf_kernels.hpp"
 
 using gf::kThreads;
@@ -109,6 +110,24 @@ int gf_memcpy_d2h(gf_ctx *ctx, void *dst, const void *src, size_t bytes) {
 // ===========================================================================
 // Kernels
 // ===========================================================================
+namespace {
+// Entry (l,m) of the compile-time GLL derivative matrix for N points.
+double gll_const(int N, int l, int m) {
+  switch (N) {
+    case 2: return gf::GllD<2>::at(l, m);
+    case 3: return gf::GllD<3>::at(l, m);
+    case 4: return gf::GllD<4>::at(l, m);
+    case 5: return gf::GllD<5>::at(l, m);
+    case 6: return gf::GllD<6>::at(l, m);
+    case 7: return gf::GllD<7>::at(l, m);
+    case 8: return gf::GllD<8>::at(l, m);
+    case 9: return gf::GllD<9>::at(l, m);
+    case 10: return gf::GllD<10>::at(l, m);
+    default: return 0.0;
+  }
+}
+}  // namespace
+
 namespace {
 
 // ---- derivative along one direction (isotropic N) -------------------------
@@ -565,7 +584,7 @@ int gf_helm_t(gf_mesh *m, double p, const double *pf, const double *qf, double q
               const double *u, double *uo) {
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Helmholtz needs isotropic order");
   if (qf) qs = 0.0;
-  if (m->dim == 3 && m->gtype == GF_GE_REGULAR) return gf_helm3c(m, p, pf, qf, qs, u, uo);
+  if (m->dim == 3 && m->gtype == GF_GE_REGULAR && m->dconst) return gf_helm3c(m, p, pf, qf, qs, u, uo);
   return GF_DISPATCH_N(m->dim, m->Niso, gfl::launch_helm, m, p, pf, qf, qs, u, uo);
 }
 
@@ -645,6 +664,14 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
   for (int k = 1; k < d->dim; k++) iso = iso && m->N[k] == m->N[0];
   int nmax = d->dim == 3 ? 10 : 12;
   m->Niso = (iso && m->N[0] <= nmax) ? m->N[0] : 0;
+  if (m->dim == 3 && m->Niso >= 2 && m->Niso <= 10) {
+    bool same = true;
+    for (int k = 0; k < 3 && same; k++)
+      for (int l = 0; l < m->Niso && same; l++)
+        for (int q = 0; q < m->Niso && same; q++)
+          same = d->D[k][l + q * m->Niso] == gll_const(m->Niso, l, q);
+    m->dconst = same;
+  }
   int rc = GF_OK;
 #define UP(dst, src, n)                      \
   do {                                       \
@@ -848,7 +875,8 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
 
 int gf_mesh_flags(const gf_mesh *m) {
   if (!m) return 0;
-  return (m->has_box ? GF_MESH_BOX_GATHER : 0) | (m->has_box_mask ? GF_MESH_BOX_MASK : 0);
+  return (m->has_box ? GF_MESH_BOX_GATHER : 0) | (m->has_box_mask ? GF_MESH_BOX_MASK : 0) |
+         (m->dconst ? GF_MESH_CONST_D : 0);
 }
 
 int gf_mesh_destroy(gf_mesh *m) {

@@ -9,13 +9,17 @@
 //     the element staged in LDS; gradient, scaling and D^T are all applied
 //     in registers.
 // D(l,m) entries are wave-uniform in every contraction (the loop indices are
-// compile-time), so they come from scalar loads; no thread keeps D in VGPRs.
+// compile-time) and are compile-time constants (gf_dtab.hpp): the kernels
+// run only on meshes whose D is bit-identical to that table, so no thread
+// keeps D in VGPRs and no scalar registers are spent holding it.
 // LDS traffic is ~7 accesses per node.  The LDS element uses row stride N+1
 // and slice stride N(N+1) (= 8 mod 16 doubles for N=8), which makes the
 // column writes and both pencil reads (nearly) bank-conflict free.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include "gf_dtab.hpp"
 
 namespace gf {
 
@@ -57,7 +61,8 @@ struct Pen3 {
 // where each part is D_d^T (M * ((D_d u) * G_d) [* p]) summed in ascending m,
 // i.e. the order of GHelmholtz::reg_prod / GGrid::compute_grefdiv
 // (ghelmholtz.ipp:308-378, ggrid.ipp:1847-1927), and M_k is the lumped mass
-// ((W_k W_j) W_i) Jac (gmass.ipp:200-210).
+// W_i W_j W_k Jac (gmass.ipp:200-210; formed here as W_k ((W_j W_i) Jac), which
+// rounds differently from the stored mass by at most an ulp or two).
 //   ru    the thread's column of u (nodes (a,b,0..N-1))
 //   su,sd two LDS element buffers (Pen3<N>::SZ each) owned by this element
 //   sx    a third LDS element buffer when HOOK is true, else unused
@@ -76,13 +81,14 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
                                         T g2, const T *__restrict__ pe, XCB &&xcb, YCB &&ycb,
                                         ZCB &&zcb, POST &&post) {
   using S = Pen3<N>;
-  // D is read-only for the whole launch: reading it through the constant
-  // address space lets the compiler use scalar loads even when the caller's
-  // output pointers are not marked restrict.
-  using CP = const __attribute__((address_space(4))) T *;
-  const CP Rx = (CP)Dp.R[0], Ry = (CP)Dp.R[1], Rz = (CP)Dp.R[2];
-  const CP Cx = (CP)Dp.C[0], Cy = (CP)Dp.C[1], Cz = (CP)Dp.C[2];
+  (void)Dp;
+  // D(l,m) and D(m,l) as compile-time constants
+  auto dR = [](int l, int m) { return (T)GllD<N>::at(l, m); };
+  auto dC = [](int l, int m) { return (T)GllD<N>::at(m, l); };
   const T Wa = W[a], Wb = W[b];
+  // lumped mass W_i W_j W_k Jac at the thread's nodes: one product per node
+  // (the factor shared by all of them is formed once)
+  const T cab = (Wb * Wa) * jac;
 #pragma unroll
   for (int k = 0; k < N; k++) su[S::at(a, b, k)] = ru[k];
   // z-part on the column, in registers
@@ -92,10 +98,9 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
 #pragma unroll
     for (int k = 0; k < N; k++) {
       T uz = T(0);
-      const CP row = opaque_ptr(Rz + k * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) uz = fma(row[m], ru[m], uz);
-      const T M = W[k] * Wb * Wa * jac;
+      for (int m = 0; m < N; m++) uz = fma(dR(k, m), ru[m], uz);
+      const T M = W[k] * cab;
       zcb(k, uz, M);
       T f = uz * g2;
       if (pe) f *= pe[a + b * N + k * N * N];
@@ -105,9 +110,8 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
 #pragma unroll
     for (int k = 0; k < N; k++) {
       T c = T(0);
-      const CP row = opaque_ptr(Cz + k * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) c = fma(row[m], gz[m], c);
+      for (int m = 0; m < N; m++) c = fma(dC(k, m), gz[m], c);
       dz[k] = c;
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -121,10 +125,9 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
 #pragma unroll
     for (int i = 0; i < N; i++) {
       T ux = T(0);
-      const CP row = opaque_ptr(Rx + i * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) ux = fma(row[m], xp[m], ux);
-      const T M = Wb * Wa * W[i] * jac;
+      for (int m = 0; m < N; m++) ux = fma(dR(i, m), xp[m], ux);
+      const T M = W[i] * cab;
       const T hx = xcb(i, ux, M);
       if (HOOK) sx[S::at(i, a, b)] = hx;
       T g = ux * g0;
@@ -135,9 +138,8 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
 #pragma unroll
     for (int i = 0; i < N; i++) {
       T s = T(0);
-      const CP row = opaque_ptr(Cx + i * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) s = fma(row[m], f[m], s);
+      for (int m = 0; m < N; m++) s = fma(dC(i, m), f[m], s);
       sd[S::at(i, a, b)] = s;
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -151,10 +153,9 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
 #pragma unroll
     for (int j = 0; j < N; j++) {
       T uy = T(0);
-      const CP row = opaque_ptr(Ry + j * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) uy = fma(row[m], yp[m], uy);
-      const T M = Wb * W[j] * Wa * jac;
+      for (int m = 0; m < N; m++) uy = fma(dR(j, m), yp[m], uy);
+      const T M = W[j] * cab;
       const T h = ycb(j, uy, M);
       if (HOOK) hy[j] = h;
       T g = uy * g1;
@@ -165,9 +166,8 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
 #pragma unroll
     for (int j = 0; j < N; j++) {
       T s = T(0);
-      const CP row = opaque_ptr(Cy + j * N);
 #pragma unroll
-      for (int m = 0; m < N; m++) s = fma(row[m], f[m], s);
+      for (int m = 0; m < N; m++) s = fma(dC(j, m), f[m], s);
       dy[j] = s;
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -184,7 +184,7 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const int q = S::at(a, b, k);
-    const T M = W[k] * Wb * Wa * jac;
+    const T M = W[k] * cab;
     post(k, sd[q] + dz[k], M, HOOK ? sx[q] : T(0), HOOK ? su[q] : T(0));
   }
 }

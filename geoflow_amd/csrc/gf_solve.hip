@@ -586,7 +586,8 @@ int gf_helmholtz_apply_f32(gf_mesh *m, float p_scalar, float q_scalar, const flo
   if (!m || !u || !uo) return GF_ERR_ARG;
   if (m->gtype != GF_GE_REGULAR || !m->Niso)
     return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "FP32 Helmholtz: regular isotropic meshes");
-  if (m->dim == 3) return GF_DISPATCH_N3(m->Niso, helm3c_f, m, p_scalar, q_field, q_scalar, u, uo);
+  if (m->dim == 3 && m->dconst)
+    return GF_DISPATCH_N3(m->Niso, helm3c_f, m, p_scalar, q_field, q_scalar, u, uo);
   return GF_DISPATCH_N(m->dim, m->Niso, launch_helm_f32, m, p_scalar, q_field, q_scalar, u, uo);
 }
 
@@ -881,7 +882,8 @@ int gf_burgers_step(gf_mesh *m, int mode, double nu, int order, int nstate, doub
   if (!m || !u || order < 1 || order > 4 || nstate < 1 || nstate > 4) return GF_ERR_ARG;
   for (int k = 0; k < nstate; k++)
     if (!u[k]) return GF_ERR_ARG;
-  bool fused = mode == 0 && m->gtype == GF_GE_REGULAR && m->Niso && nstate == m->dim;
+  bool fused = mode == 0 && m->gtype == GF_GE_REGULAR && m->Niso && nstate == m->dim &&
+               (m->dim == 2 || m->dconst);
   int rc;
   if (fused && m->has_box && order >= 2)
     rc = rk_step_box(m, nu, order, dt, u);
@@ -975,7 +977,7 @@ void launch_update(gf_mesh *m, CGVecs<T> &v, double *part, bool mask_in_q) {
 template <typename T>
 int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters) {
   hipStream_t s = m->ctx->stream;
-  const bool col = m->dim == 3;
+  const bool col = m->dim == 3 && m->dconst;
   // Structured box: the update assembles q by gather (mask already in q).
   const bool gather = col && m->has_box;
   int nbA = 0;
@@ -1305,6 +1307,10 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
   memset(&ca, 0, sizeof(ca));
   CGApply3Args ca3;
   memset(&ca3, 0, sizeof(ca3));
+  const bool pencil = w == "burgers_stage" || w == "cg_apply" || w == "cg_apply_f32" ||
+                      w == "burgers_stage_box";
+  if (pencil && m->dim == 3 && !m->dconst)
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "pencil kernels need the standard GLL D");
   if (w == "burgers_stage") {
     // stages 2-3 form: reads uin, kprev, accsrc; writes acc, kout (5 fields per component)
     for (int c = 0; c < m->dim; c++) {
@@ -1372,6 +1378,13 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
       launch = [&]() { return GF_DISPATCH_N3(m->Niso, cgapply3c_f, m, ca3); };
     else
       launch = [&]() { return GF_DISPATCH_N3(m->Niso, cgapply3c_d, m, ca3); };
+  } else if (w == "rk_out_box") {
+    // structured-box step output: reads acc and K (both gathered), writes u
+    if (m->dim != 3 || !m->has_box)
+      return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "rk_out_box timing: structured 3-D boxes");
+    launch = [&]() {
+      return GF_DISPATCH_N3(m->Niso, launch_rk_out_box, m, buf, buf + 6, buf + 9, 1e-5 / 6.0);
+    };
   } else if (w == "burgers_stage_box") {
     // structured-box stage, middle-stage form: reads uin, K_prev (gathered),
     // acc; writes acc, K (5 fields per component)

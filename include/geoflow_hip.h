@@ -150,8 +150,10 @@ int gf_mesh_destroy(gf_mesh *mesh);
 size_t gf_mesh_ndof(const gf_mesh *mesh);
 /* Mesh properties (diagnostic; no reference counterpart):
  * GF_MESH_BOX_GATHER  the structured-box gather assembly is in use;
- * GF_MESH_BOX_MASK    the boundary mask is formed from the box faces. */
-enum { GF_MESH_BOX_GATHER = 1, GF_MESH_BOX_MASK = 2 };
+ * GF_MESH_BOX_MASK    the boundary mask is formed from the box faces;
+ * GF_MESH_CONST_D     3-D mesh whose D is the standard GLL matrix, so the
+ *                     pencil kernels (D as compile-time constants) are used. */
+enum { GF_MESH_BOX_GATHER = 1, GF_MESH_BOX_MASK = 2, GF_MESH_CONST_D = 4 };
 int gf_mesh_flags(const gf_mesh *mesh);
 /* Copy a device-resident mesh array to host: "mass", "imass", "imult",
  * "mask" (ndof doubles each).  Synchronous. */

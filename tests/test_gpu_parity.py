@@ -10,7 +10,7 @@ import torch
 
 import oracle
 import geoflow_amd as ga
-from geoflow_amd import workloads
+from geoflow_amd import _capi, workloads
 from geoflow_amd.device import Mesh
 
 from conftest import relerr
@@ -349,3 +349,43 @@ def test_cg_helmholtz_converged(ctx, precision):
     assert info["status"] == 0 and ret == 0
     assert info["rnorm"] <= 1e-10 * info["rnorm0"]
     assert relerr(host(xg), xo) <= 1e-8
+
+
+def test_nonstandard_d_falls_back_to_generic_kernels(ctx):
+    """The 3-D pencil kernels take D as compile-time constants and run only
+    when the mesh's D is the standard GLL matrix (GF_MESH_CONST_D).  A mesh
+    whose D differs in one entry by one ulp uses the generic element kernels
+    (Helmholtz, CG, composed RK step); its results must agree with the
+    standard mesh's to rounding."""
+    import copy
+    bg = ga.BoxGrid(3, 5, [3, 2, 2], bdy=[0] * 6)
+    odd = copy.copy(bg)
+    xi, w, D = bg.basis[0]
+    D2 = np.array(D, copy=True)
+    D2[1, 2] = np.nextafter(D2[1, 2], np.inf)
+    odd.basis = [(xi, w, D2)] * 3
+    m1, m2 = Mesh(ctx, bg), Mesh(ctx, odd)
+    assert m1.flags & _capi.GF_MESH_CONST_D and not m2.flags & _capi.GF_MESH_CONST_D
+    rng = np.random.default_rng(4)
+    u = dev(rng.standard_normal(m1.ndof))
+    assert relerr(host(m2.helmholtz(u, 0.3, 1.2)), host(m1.helmholtz(u, 0.3, 1.2))) <= 1e-12
+    f = workloads.unique_normal(bg, seed=12)
+    xs = []
+    for m in (m1, m2):
+        b = m.mass_apply(dev(f))
+        x, info = m.cg_solve(b, xb=torch.zeros_like(b), p=0.1, q=1.0, maxit=15, tol=1e-300,
+                             normtype=0, precond=1)
+        assert info["iters"] == 15
+        xs.append(host(x))
+    assert relerr(xs[1], xs[0]) <= 1e-11
+    pg = ga.BoxGrid(3, 5, [3, 2, 2])
+    podd = copy.copy(pg)
+    podd.basis = [(xi, w, D2)] * 3
+    us = []
+    for g in (pg, podd):
+        m = Mesh(ctx, g)
+        st = [dev(np.sin(2 * np.pi * (g.xyz[0] + c * g.xyz[1])) + 0.2 * g.xyz[2]) for c in range(3)]
+        m.burgers_steps(st, 0.05, 1e-3, 3)
+        us.append([host(a) for a in st])
+    for k in range(3):
+        assert relerr(us[1][k], us[0][k]) <= 1e-12
