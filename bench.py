@@ -27,6 +27,23 @@ METRIC = "Burgers RK4 DOF-steps/s & CG solve s at rel-res 1e-10, mixed vs FP64"
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel, profile="profiles/pmc_32cube_p7_kernels.md"):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (FETCH_SIZE x2 + WRITE_SIZE, calibrated on a copy of known size), or None
+    if the summary has no row for it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), profile)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 4 and cells[1].strip("`").endswith(kernel):
+            try:
+                return (float(cells[2]) + float(cells[3])) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -189,9 +206,11 @@ def main():
     step_bytes = 16 * 8 * 3 * ndof
     step_gbs = step_bytes / (el / args.steps) / 1e9
     out.update({"value": dofsteps, "unit": "DOF-steps/s", "ms_per_step": 1e3 * el / args.steps})
+    traffic = measured_traffic("k_burgers_stage3b<%d, true, 2, false>" % (p + 1)) if boxed else None
     out["roofline"] = {"kernel": stage_name,
                        "bound": "hbm", "achieved": stage_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": stage_gbs / HBM_PEAK_GBS, "traffic": None,
+                       "frac": stage_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                       "traffic_source": "profiles/pmc_32cube_p7_kernels.md" if traffic else None,
                        "avg_launch_ms": stage_ms, "alg_bytes_per_launch": stage_bytes,
                        "step_alg_bytes": step_bytes, "step_achieved_gbs": step_gbs,
                        "step_frac": step_gbs / HBM_PEAK_GBS}

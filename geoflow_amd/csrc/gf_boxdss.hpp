@@ -170,95 +170,6 @@ __host__ __device__ __forceinline__ int box_members(const BoxElemDeltas &d, int 
   return 1 << na;
 }
 
-// Halo of an element: the (N+2)^3 cube of its nodes (coordinates 1..N) and
-// of the coincident nodes of its neighbours, which sit on ghost coordinate 0
-// (across the lower face of a direction) or N+1 (across the upper face).  A
-// node's group members are the node and the ghosts across the faces it lies
-// on, so a group can be summed from a halo staged in LDS, with each
-// neighbour value loaded once per element instead of once per member.
-template <int N>
-struct BoxHalo {
-  static constexpr int H = N + 2;
-  static constexpr int SIZE = H * H * H;
-  __host__ __device__ static constexpr int at(int ih, int jh, int kh) {
-    return (kh * H + jh) * H + ih;
-  }
-};
-
-// Global index of halo slot (ih, jh, kh) of the element whose node (0,0,0)
-// has global index gbase; false when the slot's partner does not exist (a
-// non-periodic domain face).
-template <int N, typename I>
-__host__ __device__ __forceinline__ bool box_halo_index(const BoxElemDeltas &d, I gbase, int ih,
-                                                        int jh, int kh, I *gout) {
-  const int hc[3] = {ih, jh, kh};
-  const I ns[3] = {(I)1, (I)N, (I)(N * N)};
-  I g = gbase;
-  bool ok = true;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    if (hc[c] == 0) {
-      ok = ok && d.lo[c] != 0;
-      g += (I)d.lo[c];
-    } else if (hc[c] == N + 1) {
-      ok = ok && d.hi[c] != 0;
-      g += (I)(N - 1) * ns[c] + (I)d.hi[c];
-    } else {
-      g += (I)(hc[c] - 1) * ns[c];
-    }
-  }
-  *gout = g;
-  return ok;
-}
-
-// Halo slots of the members of node (i,j,k), in ascending global index (the
-// GGFX order, the same members and order as box_members); returns the count.
-template <int N>
-__host__ __device__ __forceinline__ int box_member_slots(const BoxElemDeltas &d, int i, int j,
-                                                         int k, int (&slot)[8]) {
-  const int nc[3] = {i, j, k};
-  int64_t mag[3];
-  int dir[3], neg[3];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const int64_t dl = nc[c] == 0 ? d.lo[c] : (nc[c] == N - 1 ? d.hi[c] : 0);
-    mag[c] = dl < 0 ? -dl : dl;
-    neg[c] = dl < 0;
-    dir[c] = c;
-  }
-  // directions by decreasing partner distance (absent ones, 0, last)
-  auto cs = [&](int x, int y) {
-    if (mag[y] > mag[x]) {
-      const int64_t t = mag[x]; mag[x] = mag[y]; mag[y] = t;
-      const int u = dir[x]; dir[x] = dir[y]; dir[y] = u;
-    }
-  };
-  cs(0, 1);
-  cs(1, 2);
-  cs(0, 1);
-  const int na = (mag[0] != 0) + (mag[1] != 0) + (mag[2] != 0);
-  const int cnt = 1 << na;
-  const int own[3] = {i + 1, j + 1, k + 1};
-  const int ghost[3] = {i == 0 ? 0 : N + 1, j == 0 ? 0 : N + 1, k == 0 ? 0 : N + 1};
-#pragma unroll
-  for (int m = 0; m < 8; m++) {
-    int hc[3] = {own[0], own[1], own[2]};
-#pragma unroll
-    for (int s2 = 0; s2 < 3; s2++) {
-      if (s2 < na) {
-        const int c = dir[s2];
-        const int bit = (m >> (na - 1 - s2)) & 1;
-        // the smallest member takes every partner at a lower index; adding
-        // a magnitude moves that direction to the other element
-        const int b = neg[c] ? 1 - bit : bit;
-        if (b) hc[c] = ghost[c];
-      }
-    }
-    slot[m] = m < cnt ? BoxHalo<N>::at(hc[0], hc[1], hc[2]) : BoxHalo<N>::at(own[0], own[1], own[2]);
-  }
-  return cnt;
-}
-
 // 1/count for a box group (count 1, 2, 4 or 8): exact, so x * box_inv(n)
 // equals x / n bit for bit and no division is needed.
 __host__ __device__ __forceinline__ double box_inv(int n) {

@@ -41,7 +41,8 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_helm3p(
     }
     uo[col + k * S::N2] = acc;
   };
-  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, D, W, t.i, t.j, jac, f0 * f0,
+  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, D, W, t.i, t.j, W[t.i], W[t.j],
+                           jac, f0 * f0,
                            f1 * f1, f2 * f2, pf ? pf + t.e * S::NN : (const T *)nullptr,
                            gf::NoPenCB(), gf::NoPenCB(), gf::NoPenCB(), post);
 }
@@ -104,7 +105,8 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     qv[g] = acc;
     dot += (double)ru[k] * (double)acc;
   };
-  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, Dp, W, t.i, t.j, jac, f0 * f0,
+  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, Dp, W, t.i, t.j, W[t.i], W[t.j],
+                           jac, f0 * f0,
                            f1 * f1, f2 * f2, (const T *)nullptr, gf::NoPenCB(), gf::NoPenCB(),
                            gf::NoPenCB(), post);
   double bs = gf::block_reduce<false>(dot, sred);
@@ -144,6 +146,7 @@ void k_burgers_stage3p(
     }
   }
   const double jac = a.ejac[t.e];
+  const double wa = a.W[t.i], wb = a.W[t.j];
   const double rx = a.edx[t.e * 3], ry = a.edx[t.e * 3 + 1], rz = a.edx[t.e * 3 + 2];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -166,7 +169,7 @@ void k_burgers_stage3p(
       const double urhs = -1.0 * L + -1.0 * adv;
       kout[g] = im[k] * urhs;
     };
-    gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, a.W, t.i, t.j, jac,
+    gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, a.W, t.i, t.j, wa, wb, jac,
                                  rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
                                  post);
     __syncthreads();  // the next component reuses the LDS buffers
@@ -190,7 +193,11 @@ int launch_stage3p(gf_mesh *m, const StageArgs &a) {
 // summation order, instead of separate DSS passes.  Stage derivatives are
 // stored unsmoothed.
 //   kp       previous stage derivative K_{s-1} (unsmoothed) or NULL (stage 1)
-//   acc_mode 0: no accumulator; 1: acc = uin + Ks*ch; 2: acc += Ks*ch
+//   acc_mode 0: no accumulator; 1: acc = Smooth(uin) + Ks*ch; 2: acc += Ks*ch
+//            (step_b accumulates from uin and smooths the output at the end;
+//            starting from Smooth(uin) keeps the accumulator continuous, so
+//            the final Smooth is the identity up to rounding and the output
+//            kernel needs only the last stage derivative's group sums)
 //   smooth_v the stage input is the final-stage input of step_b, which is
 //            smoothed: v = Smooth(uin + bh*Ks) (gexrk_stepper.ipp:300-318)
 // ---------------------------------------------------------------------------
@@ -228,6 +235,7 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
   // not read (box_load8m).
   double ru[3][N];
   double av[ACC ? 3 : 1][ACC ? N : 1];
+  constexpr bool GU = SMV || ACC == 1;  // the stage reads uin's group sums
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const size_t g = col + (size_t)k * S::N2;
@@ -245,11 +253,16 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       const double ui = uin[c][g];
       const double ao = ACC == 2 ? acc[c][g] : 0.0;
       double uv[8];
-      if (SMV) gf::box_load8m<size_t, double>(uin[c], mem, cnt, uv);
+      if (GU) gf::box_load8m<size_t, double>(uin[c], mem, cnt, uv);
       // GGFX Smooth: Sum / count (an exact multiply for counts 1, 2, 4, 8)
       const double ks = cnt == 1 ? kv[0] : gf::box_sum8<double>(kv, cnt) * gf::box_inv(cnt);
       double v = ui + bh * ks;
-      if (ACC) av[c][k] = (ACC == 1 ? ui : ao) + ks * ch;
+      if (ACC == 2) av[c][k] = ao + ks * ch;
+      if (ACC == 1) {
+        // the accumulator starts from Smooth(uin), so it stays continuous
+        const double us = cnt == 1 ? uv[0] : gf::box_sum8<double>(uv, cnt) * gf::box_inv(cnt);
+        av[c][k] = us + ks * ch;
+      }
       if (SMV) {
 #pragma unroll
         for (int q = 0; q < 8; q++) uv[q] = uv[q] + bh * ks;
@@ -265,6 +278,7 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       for (int k = 0; k < N; k++) acc[c][col + (size_t)k * S::N2] = av[ACC ? c : 0][ACC ? k : 0];
   }
   const double jac = ejac[t.e];
+  const double wa = W[t.i], wb = W[t.j];
   const double rx = edx[t.e * 3], ry = edx[t.e * 3 + 1], rz = edx[t.e * 3 + 2];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -287,119 +301,71 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       const double urhs = -1.0 * L + -1.0 * adv;
       ko[g] = im[k] * urhs;
     };
-    gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, W, t.i, t.j, jac,
+    gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, W, t.i, t.j, wa, wb, jac,
                                  rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
                                  post);
     __syncthreads();  // the next component reuses the LDS buffers
   }
 }
 
-// Step output on a structured box: u = Smooth(1*acc + cl*Smooth(K)) with both
-// smooths by gather (step_b's last update and GGFX Smooth).  One workgroup
-// per element: the element's halo (gf::BoxHalo) of K and acc is staged in
-// LDS, each neighbour value loaded once, and every node sums its group from
-// there in the GGFX order.
-template <int N>
-struct HaloLaunch {
-  static constexpr int NN = N * N * N;
-  static constexpr int THREADS = NN >= 256 ? 256 : ((NN + 63) / 64) * 64;
-  static constexpr int HS = (gf::BoxHalo<N>::SIZE + THREADS - 1) / THREADS;  // halo slots/thread
-  static constexpr int NS = (NN + THREADS - 1) / THREADS;                     // nodes/thread
-};
-
+// Step output on a structured box: u = acc + cl*Smooth(K), the smooth by
+// gather.  step_b forms Smooth(acc + cl*Smooth(K)) (its last update and GGFX
+// Smooth); the accumulator here is continuous (it starts from Smooth(uin)
+// and adds smoothed stage derivatives), so that final Smooth is the identity
+// up to rounding and is not formed.
 template <int N, typename I>
-__global__ __launch_bounds__(HaloLaunch<N>::THREADS) void k_rk_out_box(
+__global__ __launch_bounds__(kThreads) void k_rk_out_box(
     double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2,
     const double *__restrict__ a0, const double *__restrict__ a1, const double *__restrict__ a2,
     const double *__restrict__ k0, const double *__restrict__ k1, const double *__restrict__ k2,
     double cl, gf::BoxTopo topo, size_t nelems) {
-  using HB = gf::BoxHalo<N>;
-  using L = HaloLaunch<N>;
-  constexpr int NN = L::NN, T = L::THREADS, H = HB::H;
-  __shared__ double hk[HB::SIZE], ha[HB::SIZE];
+  constexpr int NN = N * N * N;
+  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;
+  constexpr int CN = CE * NN;
   double *u[3] = {u0, u1, u2};
   const double *acc[3] = {a0, a1, a2};
   const double *kf[3] = {k0, k1, k2};
-  const size_t e = xcd_block();
-  if (e >= nelems) return;  // whole workgroup
-  int ec[3];
-  gf::box_elem_coords(topo, e, ec);
-  const gf::BoxElemDeltas de = gf::box_elem_deltas(topo, N, ec);
-  const I gbase = (I)(e * NN);
-  const int tid = (int)threadIdx.x;
-  // halo slots of this thread and their global indices
-  I hg[L::HS];
-  bool hv[L::HS];
+  const size_t nchunks = (nelems + CE - 1) / CE;
+  const size_t cpb = (nchunks + gridDim.x - 1) / gridDim.x;
+  const size_t c0 = xcd_block() * cpb, c1 = c0 + cpb < nchunks ? c0 + cpb : nchunks;
+  for (size_t ch = c0; ch < c1; ch++) {
+    int ec0[3];
+    gf::box_elem_coords(topo, ch * CE, ec0);
+    const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);
 #pragma unroll
-  for (int s = 0; s < L::HS; s++) {
-    const int q = s * T + tid;
-    I g = gbase;
-    hv[s] = q < HB::SIZE && gf::box_halo_index<N, I>(de, gbase, q % H, (q / H) % H, q / (H * H), &g);
-    hg[s] = hv[s] ? g : gbase;
-  }
-  // every load (all components) before the first LDS store
-  double kv[3][L::HS], av[3][L::HS];
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int s = 0; s < L::HS; s++) {
-      double x = 0.0, y = 0.0;
-      if (hv[s]) {
-        x = kf[c][hg[s]];
-        y = acc[c][hg[s]];
+    for (int q0 = 0; q0 < CN; q0 += kThreads) {
+      const int q = q0 + (int)threadIdx.x;
+      if (q >= CN) break;
+      const int el = CE == 1 ? 0 : q / NN;
+      const int rr = CE == 1 ? q : q - el * NN;
+      const size_t e = ch * CE + el;
+      if (e >= nelems) break;
+      gf::BoxElemDeltas de = d0;
+      if (CE > 1 && el) {
+        int ec[3];
+        gf::box_elem_coords(topo, e, ec);
+        de = gf::box_elem_deltas(topo, N, ec);
       }
-      kv[c][s] = x;
-      av[c][s] = y;
-    }
-  // results are kept until the last barrier: a barrier also waits for the
-  // workgroup's outstanding global stores
-  double uo[3][L::NS];
+      const int i = rr % N, j = (rr / N) % N, k = rr / (N * N);
+      const I g = (I)(e * NN + rr);
+      I mem[8];
+      const int cnt = gf::box_members<I>(de, N, g, i, j, k, mem);
+      // every load of the node (all components) before the first store;
+      // absent group members are not read
+      double kv[3][8], av[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    if (c) __syncthreads();  // the previous component's halo reads are done
-#pragma unroll
-    for (int s = 0; s < L::HS; s++) {
-      const int q = s * T + tid;
-      if (q < HB::SIZE) {
-        hk[q] = kv[c][s];
-        ha[q] = av[c][s];
+      for (int c = 0; c < 3; c++) {
+        gf::box_load8m<I, double>(kf[c], mem, cnt, kv[c]);
+        av[c] = acc[c][g];
       }
-    }
-    __syncthreads();
 #pragma unroll
-    for (int ns = 0; ns < L::NS; ns++) {
-      const int r = ns * T + tid;
-      if (r < NN) {
-        const int i = r % N, j = (r / N) % N, k = r / (N * N);
-        int sl[8];
-        const int cnt = gf::box_member_slots<N>(de, i, j, k, sl);
-        double ks = hk[sl[0]];
-        if (cnt > 1) {
-          double sum = 0.0;
-#pragma unroll
-          for (int m = 0; m < 8; m++)
-            if (m < cnt) sum += hk[sl[m]];
-          ks = sum * gf::box_inv(cnt);
-        }
-        double out = 1.0 * ha[sl[0]] + cl * ks;
-        if (cnt > 1) {
-          double sum = 0.0;
-#pragma unroll
-          for (int m = 0; m < 8; m++)
-            if (m < cnt) sum += 1.0 * ha[sl[m]] + cl * ks;
-          out = sum * gf::box_inv(cnt);
-        }
-        uo[c][ns] = out;
+      for (int c = 0; c < 3; c++) {
+        const double ks =
+            cnt == 1 ? kv[c][0] : gf::box_sum8<double>(kv[c], cnt) * gf::box_inv(cnt);
+        u[c][g] = 1.0 * av[c] + cl * ks;
       }
     }
   }
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int ns = 0; ns < L::NS; ns++) {
-      const int r = ns * T + tid;
-      if (r < NN) u[c][gbase + (I)r] = uo[c][ns];
-    }
 }
 
 template <int N, bool KP, int ACC, bool SMV>
@@ -439,17 +405,17 @@ int launch_stage3b(gf_mesh *m, double *const *uin, double *const *kp, double *co
 template <int N>
 int launch_rk_out_box(gf_mesh *m, double *const *u, double *const *acc, double *const *kf,
                       double cl) {
-  // one workgroup per element (the grid is rounded up to a multiple of the
-  // XCD count; surplus workgroups exit)
-  const int nb = xcd_grid(m->nelems);
-  constexpr int T = HaloLaunch<N>::THREADS;
+  constexpr int NN = N * N * N;
+  constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;
+  size_t nchunks = (m->nelems + CE - 1) / CE;
+  int nb = xcd_grid(std::min<size_t>(nchunks, 8192));
   if (m->ndof < ((size_t)1 << 31))
-    hipLaunchKernelGGL((k_rk_out_box<N, int32_t>), dim3(nb), dim3(T), 0, m->ctx->stream, u[0],
-                       u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
+    hipLaunchKernelGGL((k_rk_out_box<N, int32_t>), dim3(nb), dim3(kThreads), 0, m->ctx->stream,
+                       u[0], u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
                        m->nelems);
   else
-    hipLaunchKernelGGL((k_rk_out_box<N, int64_t>), dim3(nb), dim3(T), 0, m->ctx->stream, u[0],
-                       u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
+    hipLaunchKernelGGL((k_rk_out_box<N, int64_t>), dim3(nb), dim3(kThreads), 0, m->ctx->stream,
+                       u[0], u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl, m->box,
                        m->nelems);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;

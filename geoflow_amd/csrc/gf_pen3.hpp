@@ -67,6 +67,7 @@ struct Pen3 {
 //   su,sd two LDS element buffers (Pen3<N>::SZ each) owned by this element
 //   sx    a third LDS element buffer when HOOK is true, else unused
 //   pe    p field restricted to this element, or NULL
+//   Wa,Wb the GLL weights W[a], W[b] (loaded once by the caller)
 // Hooks see the reference derivatives where they are formed:
 //   zcb(k, uz, M)  at column node (a,b,k);
 //   xcb(i, ux, M)  at x-pencil node (i,a,b), returns a value hx;
@@ -77,7 +78,8 @@ struct Pen3 {
 // All threads of the workgroup must call this (it contains barriers).
 template <int N, typename T, bool HOOK, typename XCB, typename YCB, typename ZCB, typename POST>
 __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, const PenD<T> &Dp,
-                                        const T *__restrict__ W, int a, int b, T jac, T g0, T g1,
+                                        const T *__restrict__ W, int a, int b, T Wa, T Wb, T jac,
+                                        T g0, T g1,
                                         T g2, const T *__restrict__ pe, XCB &&xcb, YCB &&ycb,
                                         ZCB &&zcb, POST &&post) {
   using S = Pen3<N>;
@@ -85,7 +87,6 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
   // D(l,m) and D(m,l) as compile-time constants
   auto dR = [](int l, int m) { return (T)GllD<N>::at(l, m); };
   auto dC = [](int l, int m) { return (T)GllD<N>::at(m, l); };
-  const T Wa = W[a], Wb = W[b];
   // lumped mass W_i W_j W_k Jac at the thread's nodes: one product per node
   // (the factor shared by all of them is formed once)
   const T cab = (Wb * Wa) * jac;
