@@ -89,7 +89,7 @@ def cpu_baseline(seconds=12.0):
                       "3 components) in %.1f s" % (n, og.ndof, dt)}
 
 
-def sphere_advection(ctx, level, p, nsteps, full=False, seed=12345):
+def sphere_advection(ctx, level, p, nsteps, full=False, seed=12345, nu=0.0):
     """Cosine bell (radius R/3, seed-chosen centre) under Williamson test-1
     solid-body rotation tilted by pi/4 on the icosahedral sphere (GICOS
     bisection, GE_2DEMBEDDED quads), RK4 pure advection (nu = 0); one
@@ -110,7 +110,7 @@ def sphere_advection(ctx, level, p, nsteps, full=False, seed=12345):
     h0 = W.cosine_bell(*g.xyz, lat0, lon0)
     st = [torch.from_numpy(a).cuda() for a in [h0] + W.williamson1_velocity(*g.xyz, c0, alpha)]
     dt = T / nsteps
-    m.burgers_step(st, 0.0, dt, mode=1)  # warm-up
+    m.burgers_step(st, nu, dt, mode=1)  # warm-up
     st[0].copy_(torch.from_numpy(h0))
     torch.cuda.synchronize()
     run = nsteps if full else min(100, nsteps)
@@ -118,13 +118,14 @@ def sphere_advection(ctx, level, p, nsteps, full=False, seed=12345):
     done = 0
     while done < run:
         k = min(1000, run - done)
-        m.burgers_steps(st, 0.0, dt, k, t=done * dt, mode=1)
+        m.burgers_steps(st, nu, dt, k, t=done * dt, mode=1)
         done += k
         torch.cuda.synchronize()
         if full:
             log("sphere: %d / %d steps, %.1f s" % (done, run, time.perf_counter() - t0))
     el = time.perf_counter() - t0
-    res = {"level": level, "p": p, "quads": g.nelems, "ndof": g.ndof, "steps_per_revolution": nsteps,
+    res = {"level": level, "p": p, "nu": nu, "quads": g.nelems, "ndof": g.ndof,
+           "steps_per_revolution": nsteps,
            "dt": dt, "steps_timed": run, "ms_per_step": 1e3 * el / run,
            "dof_steps_per_s": g.ndof * run / el, "dtype": "f64",
            "bell_centre_deg": [float(np.rad2deg(lat0)), float(np.rad2deg(lon0))]}
@@ -138,6 +139,54 @@ def sphere_advection(ctx, level, p, nsteps, full=False, seed=12345):
         log("sphere: L2 %.3e  Linf %.3e" % (res["l2_error"], res["linf_error"]))
     log("sphere advection: %.3f ms/step, %.3e DOF-steps/s" % (res["ms_per_step"], res["dof_steps_per_s"]))
     return res
+
+
+def ensemble_run(ctx, nmembers, counts, nsteps=50, seed=12345, nu=0.02, dt=5e-5):
+    """Batched ensemble (members drawn by seed from element counts `counts`
+    per direction and orders {4, 6, 8}): nsteps implicit-viscous steps with
+    CG sub-solves to rel-res 1e-10, FP32-inner/FP64-IR versus FP64, and the
+    per-member error of the mixed run against the FP64 run."""
+    import torch
+    from geoflow_amd.ensemble import Ensemble, draw_members
+    from geoflow_amd.workloads import fourier_field_3d
+    members = draw_members(nmembers, seed=seed, counts=counts)
+    t = time.perf_counter()
+    ens = Ensemble(ctx, members)
+    log("ensemble: %d members, %d local DOF, orders %s, set-up %.1f s"
+        % (nmembers, ens.ndof, sorted(ens.mesh), time.perf_counter() - t))
+
+    def fill(i, g):
+        base = fourier_field_3d(*g.xyz, seed=seed, kmax=3)
+        rng = np.random.default_rng(seed + 1 + i)
+        return [b + 1e-3 * rng.standard_normal(g.ndof) for b in base]
+
+    out = {"members": nmembers, "ndof": ens.ndof, "steps": nsteps, "nu": nu, "dt": dt,
+           "orders": {str(p): len(ix) for p, ix in ens.groups.items()},
+           "element_counts": sorted({m.ne for m in members})}
+    finals = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        st = ens.new_state(fill)
+        ens.imex_step(st, nu, dt, precision=prec)  # warm-up step
+        st = ens.new_state(fill)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        iters = 0
+        for _ in range(nsteps):
+            its = ens.imex_step(st, nu, dt, precision=prec)
+            iters += sum(sum(v) for v in its.values())
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[name] = {"seconds": el, "dof_steps_per_s": ens.ndof * nsteps / el, "cg_iters": iters}
+        finals[name] = st
+        log("ensemble %s: %.2f s, %.3e DOF-steps/s, %d CG iterations"
+            % (name, el, ens.ndof * nsteps / el, iters))
+    errs = []
+    for i in range(nmembers):
+        a, b = ens.member(finals["mixed"], i), ens.member(finals["fp64"], i)
+        errs.append(max(float(np.abs(x - y).max() / np.abs(y).max()) for x, y in zip(a, b)))
+    out["mixed_rel_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs))}
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    return out
 
 
 def main():
@@ -154,9 +203,13 @@ def main():
                     help="icosahedral sphere advection: short = timed steps only, "
                          "full = one revolution with L2/Linf errors")
     ap.add_argument("--sphere-level", type=int, default=6)
-    ap.add_argument("--sphere-steps", type=int, default=10000,
-                    help="RK4 steps per revolution (the stability limit at level 6, p=5 "
-                         "is about 7900)")
+    ap.add_argument("--sphere-steps", type=int, default=10000, help="RK4 steps per revolution")
+    ap.add_argument("--sphere-nu", type=float, default=0.0,
+                    help="viscosity of the advected field (0: pure advection)")
+    ap.add_argument("--ensemble", type=int, default=8,
+                    help="members of the batched ensemble (0: off)")
+    ap.add_argument("--ensemble-counts", default="8,12",
+                    help="element counts per direction to draw from (full workload: 8,12,16,24)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -283,9 +336,12 @@ def main():
         out["cg"] = res
         del m
 
+    if args.ensemble > 0:
+        out["ensemble"] = ensemble_run(ctx, args.ensemble,
+                                       tuple(int(c) for c in args.ensemble_counts.split(",")))
     if args.sphere != "off":
         out["sphere"] = sphere_advection(ctx, args.sphere_level, 5, args.sphere_steps,
-                                         full=args.sphere == "full")
+                                         full=args.sphere == "full", nu=args.sphere_nu)
 
     out["config"] = {"workload": "3-D box %d^3 hex elements, p=%d (%d local DOF): Burgers RK4 "
                                  "(3 comps, nu=0.02, dt=1e-5, periodic) and Helmholtz (M+0.1L) "
