@@ -67,26 +67,43 @@ def burgers_ic(torch, xyz, seed=12345, nmodes=12):
 
 
 def cpu_baseline(seconds=12.0):
-    """Oracle (scalar C++ restatement of the reference, 1 thread) on a bounded
-    sample of the same workload: 8^3 elements, p=7, periodic, 3 components."""
+    """Oracle (scalar C++ restatement of the reference) on a bounded sample of
+    the same workload: 8^3 elements, p=7, periodic, 3 components; 1 thread
+    (one MPI rank of the reference), then all of the host's cores (OpenMP
+    over contiguous element ranges, as the reference's element decomposition
+    over ranks would split the work)."""
     import oracle
     import geoflow_amd as ga
-    oracle.set_threads(1)
     og = oracle.Grid.box(3, 7, [8, 8, 8])
     bg = ga.BoxGrid(3, 7, [8, 8, 8])
-    us = [np.cos(2 * np.pi * (bg.xyz[0] + c * bg.xyz[1])) + 0.3 * np.sin(2 * np.pi * bg.xyz[2])
-          for c in range(3)]
-    og.burgers_step(us, 0.02, 1e-5)  # warm-up
-    t0 = time.perf_counter()
-    n = 0
-    while time.perf_counter() - t0 < seconds:
-        og.burgers_step(us, 0.02, 1e-5)
-        n += 1
-    dt = time.perf_counter() - t0
+
+    def run(threads, secs):
+        oracle.set_threads(threads)
+        us = [np.cos(2 * np.pi * (bg.xyz[0] + c * bg.xyz[1])) + 0.3 * np.sin(2 * np.pi * bg.xyz[2])
+              for c in range(3)]
+        og.burgers_step(us, 0.02, 1e-5)  # warm-up
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < secs:
+            og.burgers_step(us, 0.02, 1e-5)
+            n += 1
+        return n, time.perf_counter() - t0
+
+    n, dt = run(1, seconds)
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = max(1, min(ncores, 16))
+    nm, dtm = run(ncores, max(2.0, seconds / 3))
+    oracle.set_threads(1)
     return {"value": og.ndof * n / dt, "unit": "DOF-steps/s", "cores": 1, "kind": "port",
             "sample": "oracle/gf_oracle.cpp (-O3 -march=native -ffp-contract=off), 1 thread: "
                       "%d RK4 Burgers steps of an 8^3-element p=7 periodic box (%d local DOF, "
-                      "3 components) in %.1f s" % (n, og.ndof, dt)}
+                      "3 components) in %.1f s" % (n, og.ndof, dt),
+            "all_cores": {"value": og.ndof * nm / dtm, "cores": ncores,
+                          "sample": "same, OpenMP over %d threads: %d steps in %.1f s"
+                                    % (ncores, nm, dtm)}}
 
 
 def sphere_advection(ctx, level, p, nsteps, full=False, seed=12345, nu=0.0):
