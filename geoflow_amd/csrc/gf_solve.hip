@@ -795,6 +795,182 @@ int rk_step_box(gf_mesh *m, double nu, int order, double dt, double *const *u) {
   return GF_DISPATCH_N3(m->Niso, launch_rk_out_box, m, u, acc, K[(ns - 1) % 2], c[ns - 1] * h);
 }
 
+// ---- pure-advection stage on 2-D embedded (sphere) elements ----------------
+// One kernel per RK stage computes, per element,
+//   us   = uin + bh Kprev            (stage input; Kprev already smoothed)
+//   ust  = ust + ch Kprev            (the step_b accumulation of Kprev)
+//   K    = imass * ( -nu L us - M (c . grad us) )
+// with the advection of GAdvect::apply on an embedded grid (gadvect.ipp:57-99,
+// deriv(p,j) = (D1 p) dXidX(0,j) + (D2 p) dXidX(1,j), ggrid.ipp:1179-1200) and
+// the Laplacian of embed_prod (ghelmholtz.ipp:205-290).  The advection is
+// regrouped as c . grad us = (D1 us) chat_0 + (D2 us) chat_1 with
+// chat_i = sum_j dXidX(i,j) c_j: the first stage of a step forms
+// w_i = imass M chat_i from the velocity and the metric and stores it, later
+// stages read the two w_i instead of three velocity and six metric fields,
+//   K = imass (-nu L us) - ((D1 us) w_0 + (D2 us) w_1).
+// The regrouping changes rounding only (relative 1e-16 per term).
+struct EmbAdvArgs {
+  const double *uin;    // stage input base (state at s = 0, step-start copy after)
+  const double *kprev;  // smoothed K of the previous stage, or null
+  double *ust;          // state, advanced by ch * kprev when kprev is set
+  double *ucopy;        // receives uin when kprev is null (step-start copy)
+  double bh, ch, nu;
+  const double *c[3];   // velocity (read when wform is set)
+  const double *ndx[6]; // dXi_i/dX_j at [i*3+j]
+  const double *G[3];
+  const double *mass, *imass, *D;
+  double *w[2];         // imass M chat_i: written when wform is set, else read
+  int wform;
+  double *kout;
+  size_t nelems;
+};
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_adv_stage_embed(EmbAdvArgs a) {
+  using S = Shape<2, N>;
+  static_assert(S::NODES <= kThreads, "one node per thread");
+  __shared__ double su[S::NODES];
+  __shared__ double sg[2][S::NODES];
+  __shared__ double sD[N * N];
+  const size_t e0 = (size_t)blockIdx.x * S::EPB;
+  const int nel = (int)min((size_t)S::EPB, a.nelems - e0);
+  const size_t base = e0 * S::NN;
+  gf::stage_D(sD, a.D, N);
+  const int q = threadIdx.x;
+  const bool act = q < nel * S::NN;
+  const size_t g = base + q;
+  if (act) {
+    double v = a.uin[g];
+    if (a.kprev) {
+      const double kp = a.kprev[g];
+      a.ust[g] = a.ust[g] + a.ch * kp;
+      v = v + a.bh * kp;
+    } else if (a.ucopy) {
+      a.ucopy[g] = v;
+    }
+    su[q] = v;
+  }
+  __syncthreads();
+  const int el = q / S::NN, r = q - el * S::NN;
+  int i, j, k;
+  gf::node_ijk<2, N>(r, i, j, k);
+  double po = 0.0, im = 0.0;
+  if (act) {
+    const double *se = su + el * S::NN;
+    const double d0 = gf::contract<2, N, false>(se, sD, 0, i, j, k);
+    const double d1 = gf::contract<2, N, false>(se, sD, 1, i, j, k);
+    im = a.imass[g];
+    double w0, w1;
+    if (a.wform) {
+      const double c0 = a.c[0][g], c1 = a.c[1][g], c2 = a.c[2][g];
+      const double mi = im * a.mass[g];
+      w0 = mi * ((a.ndx[0][g] * c0 + a.ndx[1][g] * c1) + a.ndx[2][g] * c2);
+      w1 = mi * ((a.ndx[3][g] * c0 + a.ndx[4][g] * c1) + a.ndx[5][g] * c2);
+      a.w[0][g] = w0;
+      a.w[1][g] = w1;
+    } else {
+      w0 = a.w[0][g];
+      w1 = a.w[1][g];
+    }
+    po = d0 * w0 + d1 * w1;
+    const double G00 = a.G[0][g], G01 = a.G[1][g], G11 = a.G[2][g];
+    double g0 = 0.0 + d0 * G00;
+    g0 += d1 * G01;
+    double g1 = 0.0 + d0 * G01;
+    g1 += d1 * G11;
+    sg[0][q] = g0;
+    sg[1][q] = g1;
+  }
+  __syncthreads();
+  if (act) {
+    double lap = 0.0;
+    lap += gf::contract<2, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
+    lap += gf::contract<2, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
+    if (a.nu != 1.0) lap *= a.nu;
+    a.kout[g] = im * (-lap) - po;
+  }
+}
+
+template <int DIM, int N>
+int launch_adv_stage_embed(gf_mesh *m, const EmbAdvArgs &a) {
+  if (DIM != 2) return GF_ERR_UNSUPPORTED;
+  using S = Shape<2, N>;
+  const size_t nb = (m->nelems + S::EPB - 1) / S::EPB;
+  hipLaunchKernelGGL((k_adv_stage_embed<N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
+// Pure-advection RK step on a 2-D embedded isotropic mesh with three velocity
+// components (mode 1, nstate 4): the arithmetic of rk_step_general with each
+// stage's RHS, stage input and state accumulation fused into one element
+// kernel.  Stage sequence (step_b, gexrk_stepper.ipp:251-341):
+//   s < ns-1:  K_s = F(uin + b_{s,s-1} h K_{s-1});  Smooth K_s;  ust += c_s h K_s
+//   last:      us = uin + b h K_{ns-2};  Smooth us;  K = F(us);  Smooth K;
+//              ust += c h K;  Smooth ust
+// where the ust accumulation of K_{s-1} is done by stage s's kernel.  The
+// Butcher tables of orders 1-4 have only sub-diagonal b, so the reference's
+// isum = sum_j b_{sj} h K_j equals b_{s,s-1} h K_{s-1} exactly.
+int rk_step_embed_adv(gf_mesh *m, double nu, int order, double dt, double *const *ust) {
+  const size_t n = m->ndof, bytes = n * sizeof(double);
+  double alpha[4], beta[16], c[4];
+  butcher(order, alpha, beta, c);
+  const int ns = order;
+  const double h = dt;
+  double *ucopy = (double *)gf_ws(m, WS_RK_BEGIN + 0, bytes);
+  double *us = (double *)gf_ws(m, WS_RK_BEGIN + 3, bytes);
+  double *K[2] = {(double *)gf_ws(m, WS_RK_BEGIN + 6, bytes),
+                  (double *)gf_ws(m, WS_RK_BEGIN + 9, bytes)};
+  double *w0 = (double *)gf_ws(m, WS_RK_BEGIN + 1, bytes);
+  double *w1 = (double *)gf_ws(m, WS_RK_BEGIN + 2, bytes);
+  if (!ucopy || !us || !K[0] || !K[1] || !w0 || !w1)
+    return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  EmbAdvArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nu = nu;
+  for (int k = 0; k < 3; k++) a.c[k] = ust[1 + k];
+  for (int k = 0; k < 6; k++) a.ndx[k] = m->ndx[(k / 3) * 3 + k % 3];
+  for (int k = 0; k < 3; k++) a.G[k] = m->G[k];
+  a.mass = m->mass;
+  a.imass = m->imass;
+  a.D = m->D[0];
+  a.w[0] = w0;
+  a.w[1] = w1;
+  a.nelems = m->nelems;
+  for (int s = 0; s < ns - 1; s++) {
+    a.wform = s == 0;
+    a.uin = s == 0 ? ust[0] : ucopy;
+    a.kprev = s == 0 ? nullptr : K[(s - 1) % 2];
+    a.ust = ust[0];
+    a.ucopy = s == 0 ? ucopy : nullptr;
+    a.bh = s == 0 ? 0.0 : beta[s * 4 + (s - 1)] * h;
+    a.ch = s == 0 ? 0.0 : c[s - 1] * h;
+    a.kout = K[s % 2];
+    RC(GF_DISPATCH_N(2, m->Niso, launch_adv_stage_embed, m, a));
+    RC(gf_dss_t(m, GF_DSS_SMOOTH, K[s % 2]));
+  }
+  if (ns >= 2) {
+    hipLaunchKernelGGL(k_final_input, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, us,
+                       ucopy, K[(ns - 2) % 2], beta[(ns - 1) * 4 + (ns - 2)] * h, ust[0], ust[0],
+                       c[ns - 2] * h, n);
+    GF_LAUNCH_CHECK(m->ctx);
+  } else {
+    RC(copy(m, us, ust[0]));
+  }
+  RC(gf_dss_t(m, GF_DSS_SMOOTH, us));
+  a.uin = us;
+  a.kprev = nullptr;
+  a.ucopy = nullptr;
+  a.wform = ns == 1;
+  a.bh = a.ch = 0.0;
+  a.kout = K[(ns - 1) % 2];
+  RC(GF_DISPATCH_N(2, m->Niso, launch_adv_stage_embed, m, a));
+  double *kl = K[(ns - 1) % 2];
+  RC(gf_dss_t(m, GF_DSS_SMOOTH, kl));
+  RC(axpby(m, ust[0], 1.0, ust[0], c[ns - 1] * h, kl));
+  return gf_dss_t(m, GF_DSS_SMOOTH, ust[0]);
+}
+
 // General RK step composed from the operators, following step_b literally
 // (gexrk_stepper.ipp:251-341).  Used for embedded meshes, pure advection and
 // heat modes, and orders without a fused path.
@@ -885,10 +1061,13 @@ int gf_burgers_step(gf_mesh *m, int mode, double nu, int order, int nstate, doub
   bool fused = mode == 0 && m->gtype == GF_GE_REGULAR && m->Niso && nstate == m->dim &&
                (m->dim == 2 || m->dconst);
   int rc;
+  const bool embed_adv = mode == 1 && m->gtype == GF_GE_2DEMBEDDED && m->Niso && nstate == 4;
   if (fused && m->has_box && order >= 2)
     rc = rk_step_box(m, nu, order, dt, u);
   else if (fused)
     rc = rk_step_fused(m, nu, order, dt, u);
+  else if (embed_adv)
+    rc = rk_step_embed_adv(m, nu, order, dt, u);
   else
     rc = rk_step_general(m, mode, nu, order, nstate, dt, u);
   if (rc) return rc;

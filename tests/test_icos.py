@@ -13,8 +13,9 @@ the oracle pinned by what the geometry fixes:
     triangles;
   * z is a degree-1 spherical harmonic on the unit sphere: the assembled
     Laplacian gives L z = 2 M z up to discretisation error, and L 1 = 0.
-GPU: mass, Helmholtz, derivatives, advection and DSS on the sphere, and RK4
-steps of pure advection and of nonlinear Burgers (with the tangent-plane
+GPU: mass, Helmholtz, derivatives, advection and DSS on the sphere, RK4
+steps of pure advection, advection with viscosity at RK orders 1-4 (the
+fused embedded stage kernel) and nonlinear Burgers (with the tangent-plane
 projection), against the oracle at 1e-12.
 """
 import numpy as np
@@ -146,6 +147,28 @@ def test_icos_pure_advection_rk4_matches_oracle(ctx):
         m.burgers_step(dst, 0.0, 1e-3, mode=1)
         og.burgers_step(st, 0.0, 1e-3, mode=1)
         assert relerr(_host(dst[0]), st[0]) <= TOL, s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_icos_viscous_advection_orders_match_oracle(ctx, order):
+    """The fused embedded advection stage with a viscous term, every RK order
+    (Euler's single stage smooths its input; RK2-4 carry the stage input and
+    the state accumulation inside the stage kernels), and a state whose
+    coincident nodes disagree, which only the reference's smoothing of the
+    last stage's input reconciles."""
+    og, ig, m = _make(ctx, p=4, lev=2)
+    h = _bell(*ig.xyz, lat0=-0.2, lon0=2.0)
+    h = h + 1e-3 * np.random.default_rng(order).standard_normal(h.size)
+    vel = _rotation(*ig.xyz, alpha=0.3)
+    st = [h.copy()] + [v.copy() for v in vel]
+    dst = [_dev(a) for a in st]
+    for s in range(3):
+        m.burgers_step(dst, 2e-3, 5e-4, mode=1, order=order)
+        og.burgers_step(st, 2e-3, 5e-4, mode=1, order=order)
+        assert relerr(_host(dst[0]), st[0]) <= TOL, s
+        for k in range(1, 4):
+            assert np.array_equal(_host(dst[k]), vel[k - 1])
 
 
 @pytest.mark.gpu
