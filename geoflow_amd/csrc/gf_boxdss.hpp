@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace gf {
 
 struct BoxTopo {
@@ -42,13 +44,33 @@ __host__ __device__ __forceinline__ bool box_masked(const BoxTopo &t, int N, con
   return mk;
 }
 
-// Box coordinates (ex, ey, ez) of element e.
+// Box coordinates (ex, ey, ez) of element e (32-bit division whenever the
+// operands fit, which avoids the long 64-bit division sequence).
 __host__ __device__ __forceinline__ void box_elem_coords(const BoxTopo &t, size_t e, int (&ec)[3]) {
   const size_t nxy = (size_t)t.ne[0] * t.ne[1];
+  if (e < ((size_t)1 << 32) && nxy < ((size_t)1 << 32)) {
+    const uint32_t e32 = (uint32_t)e, nx = (uint32_t)t.ne[0], n2 = (uint32_t)nxy;
+    const uint32_t z = e32 / n2, rem = e32 - z * n2, y = rem / nx;
+    ec[2] = (int)z;
+    ec[1] = (int)y;
+    ec[0] = (int)(rem - y * nx);
+    return;
+  }
   ec[2] = (int)(e / nxy);
   const size_t rem = e - (size_t)ec[2] * nxy;
   ec[1] = (int)(rem / (size_t)t.ne[0]);
   ec[0] = (int)(rem - (size_t)ec[1] * t.ne[0]);
+}
+
+// Advances box coordinates to those of the next element in linear order.
+__host__ __device__ __forceinline__ void box_elem_next(const BoxTopo &t, int (&ec)[3]) {
+  if (++ec[0] == t.ne[0]) {
+    ec[0] = 0;
+    if (++ec[1] == t.ne[1]) {
+      ec[1] = 0;
+      ++ec[2];
+    }
+  }
 }
 
 // Members of the group of node (i,j,k) (N nodes per direction) of the element
@@ -136,38 +158,61 @@ __host__ __device__ __forceinline__ BoxElemDeltas box_elem_deltas(const BoxTopo 
 }
 
 // Members of the group of node (i,j,k) with global index gself, from the
-// element's partner offsets; ascending in g[0..count), count returned.  Same
-// construction as box_group (superincreasing magnitudes), in index type I
-// (int32_t when every index of the mesh fits, else int64_t).
+// element's partner offsets; ascending in g[0..count), count returned, and
+// *own the slot of gself itself.  Same construction as box_group
+// (superincreasing magnitudes), in index type I (int32_t when every index of
+// the mesh fits, else int64_t), written without branches or per-slot
+// selects: the non-zero magnitudes in ascending order b0 <= b1 <= b2 (0 where
+// absent) weight bits 0, 1, 2 of the slot number, so slot m is
+// lo + (m&1) b0 + (m&2) b1 + (m&4) b2, and the node's own slot has the bits of
+// the directions whose partner lies below it.  Slots past the count repeat
+// members (their weights are 0) and are never read.
 template <typename I>
-__host__ __device__ __forceinline__ int box_members(const BoxElemDeltas &d, int N, I gself, int i,
-                                                    int j, int k, I (&g)[8]) {
+__host__ __device__ __forceinline__ int box_members_own(const BoxElemDeltas &d, int N, I gself,
+                                                        int i, int j, int k, I (&g)[8], int *own) {
+  static_assert(std::is_signed<I>::value, "partner offsets are signed");
+  using U = typename std::make_unsigned<I>::type;
   const int nc[3] = {i, j, k};
-  I lo = gself, a[3];
+  I lo = gself;
+  U key[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const I dl = nc[c] == 0 ? (I)d.lo[c] : (nc[c] == N - 1 ? (I)d.hi[c] : (I)0);
     lo += dl < 0 ? dl : (I)0;
-    a[c] = dl < 0 ? -dl : dl;
+    // 2 magnitude + (partner below) - 2 as unsigned: an absent direction
+    // (magnitude 0) wraps to the largest key and sorts last; magnitudes are
+    // distinct, so the sign bit never decides the order
+    key[c] = ((((U)(dl < 0 ? -dl : dl)) << 1) | (U)(dl < 0)) - (U)2;
   }
-  auto cs = [](I &x, I &y) {
-    const I h = x > y ? x : y, l = x > y ? y : x;
-    x = h;
-    y = l;
+  auto cs = [](U &x, U &y) {
+    const U l = x < y ? x : y, h = x < y ? y : x;
+    x = l;
+    y = h;
   };
-  cs(a[0], a[1]);
-  cs(a[1], a[2]);
-  cs(a[0], a[1]);
-  const int na = (a[0] != 0) + (a[1] != 0) + (a[2] != 0);
-#pragma unroll
-  for (int m = 0; m < 8; m++) {
-    I v = lo;
-    if (na >= 1) v += ((m >> (na - 1)) & 1) ? a[0] : (I)0;
-    if (na >= 2) v += ((m >> (na - 2)) & 1) ? a[1] : (I)0;
-    if (na >= 3) v += (m & 1) ? a[2] : (I)0;
-    g[m] = v;
-  }
+  cs(key[0], key[1]);
+  cs(key[1], key[2]);
+  cs(key[0], key[1]);
+  const I b0 = (I)((key[0] + (U)2) >> 1), b1 = (I)((key[1] + (U)2) >> 1),
+          b2 = (I)((key[2] + (U)2) >> 1);
+  *own = (int)(key[0] & 1) | ((int)(key[1] & 1) << 1) | ((int)(key[2] & 1) << 2);
+  const int na = (b0 != 0) + (b1 != 0) + (b2 != 0);
+  const I l1 = lo + b1;
+  g[0] = lo;
+  g[1] = lo + b0;
+  g[2] = l1;
+  g[3] = l1 + b0;
+  g[4] = lo + b2;
+  g[5] = g[4] + b0;
+  g[6] = l1 + b2;
+  g[7] = g[6] + b0;
   return 1 << na;
+}
+
+template <typename I>
+__host__ __device__ __forceinline__ int box_members(const BoxElemDeltas &d, int N, I gself, int i,
+                                                    int j, int k, I (&g)[8]) {
+  int own;
+  return box_members_own<I>(d, N, gself, i, j, k, g, &own);
 }
 
 // 1/count for a box group (count 1, 2, 4 or 8): exact, so x * box_inv(n)
@@ -215,6 +260,18 @@ __device__ __forceinline__ T box_sum8(const T (&v)[8], int n) {
   T s = T(0);
 #pragma unroll
   for (int m = 0; m < 8; m++) s = m < n ? s + v[m] : s;
+  return n == 1 ? v[0] : s;
+}
+
+// box_sum8 for slots filled by box_load8m (absent slots hold +0): adding
+// them changes nothing, because a sum that starts from +0 is never -0 (in
+// round-to-nearest x + y is -0 only when both are -0), and s + (+0) == s for
+// every other s.  Same result bit for bit, without a select per slot.
+template <typename T>
+__device__ __forceinline__ T box_sum8z(const T (&v)[8], int n) {
+  T s = T(0);
+#pragma unroll
+  for (int m = 0; m < 8; m++) s = s + v[m];
   return n == 1 ? v[0] : s;
 }
 

@@ -244,23 +244,33 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       for (int c = 0; c < 3; c++) ru[c][k] = uin[c][g];
       continue;
     }
-    size_t mem[8];
-    const int cnt = gf::box_members<size_t>(de, N, g, t.i, t.j, k, mem);
+    int64_t mem[8];
+    int own;
+    const int cnt = gf::box_members_own<int64_t>(de, N, (int64_t)g, t.i, t.j, k, mem, &own);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       double kv[8];
-      gf::box_load8m<size_t, double>(kp[c], mem, cnt, kv);
-      const double ui = uin[c][g];
+      gf::box_load8m<int64_t, double>(kp[c], mem, cnt, kv);
       const double ao = ACC == 2 ? acc[c][g] : 0.0;
       double uv[8];
-      if (GU) gf::box_load8m<size_t, double>(uin[c], mem, cnt, uv);
+      double ui;
+      if (GU) gf::box_load8m<int64_t, double>(uin[c], mem, cnt, uv);
+      if (SMV) {
+        // the node's own value is one of the gathered slots (picking it
+        // there measured faster than a separate load in this form only)
+        ui = uv[0];
+#pragma unroll
+        for (int m = 1; m < 8; m++) ui = own == m ? uv[m] : ui;
+      } else {
+        ui = uin[c][g];
+      }
       // GGFX Smooth: Sum / count (an exact multiply for counts 1, 2, 4, 8)
-      const double ks = cnt == 1 ? kv[0] : gf::box_sum8<double>(kv, cnt) * gf::box_inv(cnt);
+      const double ks = cnt == 1 ? kv[0] : gf::box_sum8z<double>(kv, cnt) * gf::box_inv(cnt);
       double v = ui + bh * ks;
       if (ACC == 2) av[c][k] = ao + ks * ch;
       if (ACC == 1) {
         // the accumulator starts from Smooth(uin), so it stays continuous
-        const double us = cnt == 1 ? uv[0] : gf::box_sum8<double>(uv, cnt) * gf::box_inv(cnt);
+        const double us = cnt == 1 ? uv[0] : gf::box_sum8z<double>(uv, cnt) * gf::box_inv(cnt);
         av[c][k] = us + ks * ch;
       }
       if (SMV) {
@@ -328,9 +338,18 @@ __global__ __launch_bounds__(kThreads) void k_rk_out_box(
   const size_t nchunks = (nelems + CE - 1) / CE;
   const size_t cpb = (nchunks + gridDim.x - 1) / gridDim.x;
   const size_t c0 = xcd_block() * cpb, c1 = c0 + cpb < nchunks ? c0 + cpb : nchunks;
+  int ecw[3] = {0, 0, 0};
+  if (CE == 1) gf::box_elem_coords(topo, c0, ecw);
   for (size_t ch = c0; ch < c1; ch++) {
     int ec0[3];
-    gf::box_elem_coords(topo, ch * CE, ec0);
+    if (CE == 1) {
+      ec0[0] = ecw[0];
+      ec0[1] = ecw[1];
+      ec0[2] = ecw[2];
+      gf::box_elem_next(topo, ecw);
+    } else {
+      gf::box_elem_coords(topo, ch * CE, ec0);
+    }
     const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);
 #pragma unroll
     for (int q0 = 0; q0 < CN; q0 += kThreads) {
@@ -361,7 +380,7 @@ __global__ __launch_bounds__(kThreads) void k_rk_out_box(
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const double ks =
-            cnt == 1 ? kv[c][0] : gf::box_sum8<double>(kv[c], cnt) * gf::box_inv(cnt);
+            cnt == 1 ? kv[c][0] : gf::box_sum8z<double>(kv[c], cnt) * gf::box_inv(cnt);
         u[c][g] = 1.0 * av[c] + cl * ks;
       }
     }
@@ -494,6 +513,9 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
   const size_t nchunks = (nelems + CE - 1) / CE;
   const size_t cpb = (nchunks + gridDim.x - 1) / gridDim.x;
   const size_t c0 = xcd_block() * cpb, c1 = c0 + cpb < nchunks ? c0 + cpb : nchunks;
+  // box coordinates of the walk's next element (chunks of one element)
+  int ecw[3] = {0, 0, 0};
+  if (CE == 1) gf::box_elem_coords(topo, c0, ecw);
   for (size_t c = c0; c < c1; c += BC) {
     I gs[NQ];
     T dv[NQ], xo[NQ], wo[NQ], ro[NQ], qm[NQ][8];
@@ -503,7 +525,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
     for (int bc = 0; bc < BC; bc++) {
       const size_t cc = c + bc;
       int ec0[3];
-      gf::box_elem_coords(topo, cc * CE, ec0);
+      if (CE == 1) {
+        ec0[0] = ecw[0];
+        ec0[1] = ecw[1];
+        ec0[2] = ecw[2];
+        gf::box_elem_next(topo, ecw);
+      } else {
+        gf::box_elem_coords(topo, cc * CE, ec0);
+      }
       const gf::BoxElemDeltas d0 = gf::box_elem_deltas(topo, N, ec0);  // uniform when CE == 1
 #pragma unroll
       for (int qq = 0; qq < QPC; qq++) {
@@ -542,7 +571,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
     for (int s = 0; s < NQ; s++) {
       if (!ok[s]) continue;
       const I g = gs[s];
-      const T qa = gf::box_sum8<T>(qm[s], cnt[s]);
+      const T qa = gf::box_sum8z<T>(qm[s], cnt[s]);
       const T xv = (T)1 * xo[s] + alpha * wo[s];
       x[g] = xv;
       const T rv = (T)1 * ro[s] + (-alpha) * qa;
