@@ -206,6 +206,32 @@ def ensemble_run(ctx, nmembers, counts, nsteps=50, seed=12345, nu=0.02, dt=5e-5)
     return out
 
 
+def kernel_set(ctx, ne, p, reps):
+    """Average time (ms, HIP events) of each kernel of the path on the
+    periodic (Burgers) and Dirichlet (CG) boxes, after a plain device copy of
+    known size that profiler counter passes use for calibration."""
+    import torch
+    import geoflow_amd as ga
+    from geoflow_amd.device import Mesh
+    a = torch.ones(ne ** 3 * (p + 1) ** 3, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    del a, b
+    res = {}
+    for kind in ("periodic", "dirichlet"):
+        bdy = [ga.GF_BDY_DIRICHLET] * 6 if kind == "dirichlet" else None
+        m = Mesh(ctx, ga.BoxGrid(3, p, [ne] * 3, bdy=bdy))
+        names = ("burgers_stage_box", "rk_out_box", "helmholtz") if kind == "periodic" else \
+                ("cg_apply", "cg_apply_f32", "cg_update", "cg_update_f32")
+        for k in names:
+            res[k] = m.time_kernel(k, reps)
+            log("%s %s %.1f us" % (kind, k, res[k] * 1e3))
+        del m
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +253,9 @@ def main():
                     help="members of the batched ensemble (0: off)")
     ap.add_argument("--ensemble-counts", default="8,12",
                     help="element counts per direction to draw from (full workload: 8,12,16,24)")
+    ap.add_argument("--kernels", type=int, default=0, metavar="REPS",
+                    help="only time the path's kernels REPS times each on the 32^3 p=7 boxes "
+                         "(for profiler counter passes) and print their average times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -240,6 +269,9 @@ def main():
 
     ne, p = args.ne, args.p
     ctx = Context(0)
+    if args.kernels:
+        print(json.dumps(kernel_set(ctx, ne, p, args.kernels)), flush=True)
+        return
     out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic"}

@@ -274,6 +274,30 @@ def test_box_gather_enabled_only_where_groups_match(ctx):
     assert Mesh(ctx, ga.BoxGrid(2, 4, [3, 3])).flags & 3 == 0
 
 
+def test_box_rk_steps_stay_continuous(ctx):
+    """Every copy of a shared node sums its group in the same (ascending
+    global) order inside the gather kernels, so the box-path RK4 state stays
+    continuous bit for bit: GGFX Max and Min of each component equal the
+    component.  The same steps on the CSR path (DSS kernels) agree to
+    rounding."""
+    print("box rk continuity", flush=True)
+    bg = ga.BoxGrid(3, 4, [3, 2, 4])
+    comps = [0.5 * workloads.unique_normal(bg, seed=40 + c) for c in range(3)]
+    res = []
+    for structured in (True, False):
+        m = Mesh(ctx, bg, structured=structured)
+        assert (m.flags & 1) == (1 if structured else 0)
+        us = [dev(a) for a in comps]
+        m.burgers_steps(us, 0.02, 1e-3, 4)
+        res.append([host(u) for u in us])
+        if structured:
+            for h in res[-1]:
+                assert np.array_equal(host(m.dss(dev(h), "max")), h)
+                assert np.array_equal(host(m.dss(dev(h), "min")), h)
+    for c in range(3):
+        assert relerr(res[0][c], res[1][c]) <= 1e-13
+
+
 @pytest.mark.parametrize("bdy", [[0] * 6, [6] * 6], ids=["dirichlet", "periodic"])
 def test_cg_gather_path_matches_csr_path(ctx, bdy):
     """Structured-box CG (gather assembly in the update, mask from the box
