@@ -200,6 +200,11 @@ int launch_stage3p(gf_mesh *m, const StageArgs &a) {
 //            kernel needs only the last stage derivative's group sums)
 //   smooth_v the stage input is the final-stage input of step_b, which is
 //            smoothed: v = Smooth(uin + bh*Ks) (gexrk_stepper.ipp:300-318)
+//   acc_mode 3: the previous step's output fused with this step's first
+//            stage: uin is the previous accumulator, bh the last weight c h,
+//            and v = acc + c h Smooth(K_last) -- the output kernel's value,
+//            in the same operations -- is stored through `acc` (the state)
+//            and is this stage's input
 // ---------------------------------------------------------------------------
 template <int N, bool KP, int ACC, bool SMV>
 __global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4)))
@@ -268,6 +273,7 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       const double ks = cnt == 1 ? kv[0] : gf::box_sum8z<double>(kv, cnt) * gf::box_inv(cnt);
       double v = ui + bh * ks;
       if (ACC == 2) av[c][k] = ao + ks * ch;
+      if (ACC == 3) av[c][k] = v;
       if (ACC == 1) {
         // the accumulator starts from Smooth(uin), so it stays continuous
         const double us = cnt == 1 ? uv[0] : gf::box_sum8z<double>(uv, cnt) * gf::box_inv(cnt);
@@ -415,6 +421,8 @@ int launch_stage3b(gf_mesh *m, double *const *uin, double *const *kp, double *co
     launch_stage3b_v<N, true, 1, true>(m, nb, uin, kp, acc, kout, bh, ch, nu);
   else if (acc_mode == 2 && smooth_v)
     launch_stage3b_v<N, true, 2, true>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+  else if (acc_mode == 3 && !smooth_v)
+    launch_stage3b_v<N, true, 3, false>(m, nb, uin, kp, acc, kout, bh, ch, nu);
   else
     return gf_set_err(m->ctx, GF_ERR_ARG, "stage3b: unsupported stage form");
   GF_LAUNCH_CHECK(m->ctx);

@@ -767,11 +767,15 @@ int rk_step_fused(gf_mesh *m, double nu, int order, double dt, double *const *u)
   return GF_OK;
 }
 
-// RK step on a structured box (order >= 2): the same arithmetic as
+// RK steps on a structured box (order >= 2): the same arithmetic as
 // rk_step_fused, with every Smooth done by gather inside the stage kernels
 // and the output kernel.  Per step: `order` stage kernels and one output
-// kernel, no DSS passes.
-int rk_step_box(gf_mesh *m, double nu, int order, double dt, double *const *u) {
+// kernel, no DSS passes.  Between consecutive steps the output kernel and the
+// next step's first stage are one kernel (acc_mode 3), which computes the
+// output value with the output kernel's operations, stores it and continues
+// with it as the stage input, so n steps give the same bits as n single
+// steps.
+int rk_steps_box(gf_mesh *m, double nu, int order, double dt, int nsteps, double *const *u) {
   size_t bytes = m->ndof * sizeof(double);
   double alpha[4], beta[16], c[4];
   butcher(order, alpha, beta, c);
@@ -783,16 +787,31 @@ int rk_step_box(gf_mesh *m, double nu, int order, double dt, double *const *u) {
   }
   const double h = dt;
   const int ns = order;
-  // stage 1 from the state itself
-  RC(GF_DISPATCH_N3(m->Niso, launch_stage3b, m, u, (double *const *)nullptr,
-                    (double *const *)nullptr, K[0], 0.0, 0.0, nu, 0, 0));
-  // middle stages and the final stage (whose input is smoothed)
-  for (int s = 1; s < ns; s++) {
-    const bool last = s == ns - 1;
-    RC(GF_DISPATCH_N3(m->Niso, launch_stage3b, m, u, K[(s - 1) % 2], acc, K[s % 2],
-                      beta[s * 4 + (s - 1)] * h, c[s - 1] * h, nu, s == 1 ? 1 : 2, last ? 1 : 0));
+  int kb = 0;  // buffer of the current step's first stage derivative
+  bool have_k0 = false;
+  for (int n = 0; n < nsteps; n++) {
+    // stage 1 from the state itself (unless the previous step's output
+    // kernel formed it)
+    if (!have_k0)
+      RC(GF_DISPATCH_N3(m->Niso, launch_stage3b, m, u, (double *const *)nullptr,
+                        (double *const *)nullptr, K[kb], 0.0, 0.0, nu, 0, 0));
+    // middle stages and the final stage (whose input is smoothed)
+    for (int s = 1; s < ns; s++) {
+      const bool last = s == ns - 1;
+      RC(GF_DISPATCH_N3(m->Niso, launch_stage3b, m, u, K[(kb + s - 1) % 2], acc, K[(kb + s) % 2],
+                        beta[s * 4 + (s - 1)] * h, c[s - 1] * h, nu, s == 1 ? 1 : 2, last ? 1 : 0));
+    }
+    const int kl = (kb + ns - 1) % 2;
+    if (n + 1 < nsteps) {
+      kb = (kl + 1) % 2;
+      RC(GF_DISPATCH_N3(m->Niso, launch_stage3b, m, acc, K[kl], u, K[kb], c[ns - 1] * h, 0.0, nu,
+                        3, 0));
+      have_k0 = true;
+    } else {
+      RC(GF_DISPATCH_N3(m->Niso, launch_rk_out_box, m, u, acc, K[kl], c[ns - 1] * h));
+    }
   }
-  return GF_DISPATCH_N3(m->Niso, launch_rk_out_box, m, u, acc, K[(ns - 1) % 2], c[ns - 1] * h);
+  return GF_OK;
 }
 
 // ---- pure-advection stage on 2-D embedded (sphere) elements ----------------
@@ -1063,7 +1082,7 @@ int gf_burgers_step(gf_mesh *m, int mode, double nu, int order, int nstate, doub
   int rc;
   const bool embed_adv = mode == 1 && m->gtype == GF_GE_2DEMBEDDED && m->Niso && nstate == 4;
   if (fused && m->has_box && order >= 2)
-    rc = rk_step_box(m, nu, order, dt, u);
+    rc = rk_steps_box(m, nu, order, dt, 1, u);
   else if (fused)
     rc = rk_step_fused(m, nu, order, dt, u);
   else if (embed_adv)
@@ -1077,6 +1096,13 @@ int gf_burgers_step(gf_mesh *m, int mode, double nu, int order, int nstate, doub
 
 int gf_burgers_steps(gf_mesh *m, int mode, double nu, int order, int nstate, double t, double dt,
                      int nsteps, double *const *u) {
+  if (!m || !u || nsteps < 0) return GF_ERR_ARG;
+  if (nsteps > 1 && mode == 0 && m->gtype == GF_GE_REGULAR && m->Niso && m->dim == 3 &&
+      m->dconst && m->has_box && nstate == 3 && order >= 2 && order <= 4) {
+    for (int k = 0; k < nstate; k++)
+      if (!u[k]) return GF_ERR_ARG;
+    return rk_steps_box(m, nu, order, dt, nsteps, u);
+  }
   for (int s = 0; s < nsteps; s++) {
     int rc = gf_burgers_step(m, mode, nu, order, nstate, t + s * dt, dt, u, 0);
     if (rc) return rc;
