@@ -13,6 +13,7 @@ GF_BDY_DIRICHLET = 0
 GF_BDY_PERIODIC = 6
 GF_BDY_NONE = 7
 GF_GE_REGULAR = 0
+GF_GE_DEFORMED = 1
 GF_GE_2DEMBEDDED = 2
 GF_DSS_SUM, GF_DSS_SMOOTH, GF_DSS_MAX, GF_DSS_MIN = 0, 1, 2, 3
 GF_NORM_INF, GF_NORM_EUC, GF_NORM_L2, GF_NORM_L1, GF_NORM_ASSEMBLED_L2 = 0, 1, 2, 3, 4
@@ -70,6 +71,10 @@ SIGNATURES = {
     "gf_icos_grid": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_dp),
                                     ctypes.POINTER(_dp), _dp, _dp]),
+    "gf_terrain_box_grid": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp,
+                                           ctypes.POINTER(ctypes.c_int), ctypes.c_double,
+                                           ctypes.c_int, ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                           _dp, ctypes.POINTER(_dp), _dp, _dp]),
     "gf_ggfx_build": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(_dp),
                                      ctypes.c_double, ctypes.c_size_t, _i64p, _i64p,
                                      ctypes.POINTER(ctypes.c_size_t)]),

@@ -639,10 +639,12 @@ extern "C" {
 int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
   if (!ctx || !d || !out) return GF_ERR_ARG;
   if (d->dim != 2 && d->dim != 3) return gf_set_err(ctx, GF_ERR_ARG, "dim must be 2 or 3");
-  if (d->gtype != GF_GE_REGULAR && d->gtype != GF_GE_2DEMBEDDED)
-    return gf_set_err(ctx, GF_ERR_UNSUPPORTED, "only GE_REGULAR and GE_2DEMBEDDED meshes");
+  if (d->gtype != GF_GE_REGULAR && d->gtype != GF_GE_2DEMBEDDED && d->gtype != GF_GE_DEFORMED)
+    return gf_set_err(ctx, GF_ERR_ARG, "unknown element type");
   if (d->gtype == GF_GE_2DEMBEDDED && d->dim != 2)
     return gf_set_err(ctx, GF_ERR_ARG, "embedded meshes are 2-D");
+  if (d->gtype == GF_GE_DEFORMED && d->dim != 2)
+    return gf_set_err(ctx, GF_ERR_UNSUPPORTED, "deformed elements: 2-D meshes only");
   if (!d->ggfx_offs || !d->ggfx_members) return gf_set_err(ctx, GF_ERR_ARG, "missing GGFX maps");
   GF_HIP(ctx, hipSetDevice(ctx->device));
   gf_mesh *m = new gf_mesh_s();
@@ -719,21 +721,23 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     UP(m->edx32, a.data(), a.size());
     UP(m->ejac32, b.data(), b.size());
   } else {
+    // Cartesian coordinates per node: 3 on an embedded surface, else dim
+    const int nxy = d->gtype == GF_GE_2DEMBEDDED ? 3 : d->dim;
     if (!d->node_jac) {
       gf_mesh_destroy(m);
-      return gf_set_err(ctx, GF_ERR_ARG, "embedded mesh needs node metrics");
+      return gf_set_err(ctx, GF_ERR_ARG, "embedded or deformed mesh needs node metrics");
     }
     for (int i = 0; i < 2; i++)
-      for (int j = 0; j < 3; j++) {
+      for (int j = 0; j < nxy; j++) {
         if (!d->node_dxidx[i * 3 + j]) {
           gf_mesh_destroy(m);
-          return gf_set_err(ctx, GF_ERR_ARG, "embedded mesh needs dXi/dX for i<2, j<3");
+          return gf_set_err(ctx, GF_ERR_ARG, "node metrics need dXi_i/dX_j for i<2, j<nxy");
         }
         UP(m->ndx[i * 3 + j], d->node_dxidx[i * 3 + j], n);
       }
     UP(m->njac, d->node_jac, n);
-    // Metric G(i,j) = sum_k dXidX(i,k) dXidX(j,k) W0 W1 Jac (def_init 2-D,
-    // src/cdg/sem/ghelmholtz.ipp:476-491), built on the host in that order.
+    // Metric G(i,j) = sum_{k<nxy} dXidX(i,k) dXidX(j,k) W0 W1 Jac (def_init
+    // 2-D, src/cdg/sem/ghelmholtz.ipp:476-491), built on the host in that order.
     std::vector<double> G[3];
     int Ni = m->N[0], Nj = m->N[1];
     for (int c = 0; c < 3; c++) G[c].assign(n, 0.0);
@@ -744,7 +748,7 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
       for (size_t e = 0; e < m->nelems; e++)
         for (int mm = 0; mm < Nj; mm++)
           for (int l = 0; l < Ni; l++, q++)
-            for (int k = 0; k < 3; k++)
+            for (int k = 0; k < nxy; k++)
               G[c][q] += d->node_dxidx[ii[c] * 3 + k][q] * d->node_dxidx[jj[c] * 3 + k][q] *
                          d->W[0][l] * d->W[1][mm] * d->node_jac[q];
       UP(m->G[c], G[c].data(), n);
@@ -965,8 +969,9 @@ int gf_deriv(gf_mesh *m, int idir, const double *u, double *du) {
     if (idir < 1 || idir > m->dim) return gf_set_err(m->ctx, GF_ERR_ARG, "idir out of range");
     return gf_grefderiv_t(m, idir, 0, u, du, m->edx);
   }
-  // embedded: du = (D1 u) dXidX(0,idir-1) + (D2 u) dXidX(1,idir-1)
-  if (idir < 1 || idir > 3) return gf_set_err(m->ctx, GF_ERR_ARG, "idir out of range");
+  // embedded or deformed: du = (D1 u) dXidX(0,idir-1) + (D2 u) dXidX(1,idir-1)
+  if (idir < 1 || idir > (m->gtype == GF_GE_2DEMBEDDED ? 3 : m->dim))
+    return gf_set_err(m->ctx, GF_ERR_ARG, "idir out of range");
   double *t = (double *)gf_ws(m, WS_TMP0, m->ndof * sizeof(double));
   double *t1 = (double *)gf_ws(m, WS_TMP1, m->ndof * sizeof(double));
   if (!t || !t1) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");

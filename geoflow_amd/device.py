@@ -10,7 +10,8 @@ import numpy as np
 import torch
 
 from . import _capi
-from ._capi import (CGParams, CGResult, MeshDesc, GF_GE_REGULAR, GF_GE_2DEMBEDDED, check, lib)
+from ._capi import (CGParams, CGResult, MeshDesc, GF_GE_REGULAR, GF_GE_2DEMBEDDED, GF_GE_DEFORMED,
+                    check, lib)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -82,7 +83,7 @@ class Mesh:
         self.dim = grid.dim
         if gtype is None:
             gtype = getattr(grid, "gtype", GF_GE_REGULAR)
-        if gtype == GF_GE_2DEMBEDDED:
+        if gtype in (GF_GE_2DEMBEDDED, GF_GE_DEFORMED):
             node_dxidx = grid.node_dxidx if node_dxidx is None else node_dxidx
             node_jac = grid.node_jac if node_jac is None else node_jac
         self.gtype = gtype
@@ -108,7 +109,7 @@ class Mesh:
             d.elem_jac = host(grid.ejac)
         else:
             for i in range(2):
-                for j in range(3):
+                for j in range(len(node_dxidx[i])):
                     d.node_dxidx[i * 3 + j] = host(node_dxidx[i][j])
             d.node_jac = host(node_jac)
         coords = xyz if xyz is not None else getattr(grid, "xyz", None)

@@ -7,12 +7,17 @@ coincidence groups matched with tolerance 0.001 * minnodedist.
 
 ``IcosGrid`` mirrors the 2-D GGridIcos (src/cdg/grid/ggrid_icos.ipp): quads on
 the sphere with per-node embedded metrics, and the same GGFX set-up.
+
+``TerrainBoxGrid`` is a 2-D box of GE_DEFORMED quads whose bottom follows a
+terrain profile (the result of GGrid::add_terrain, src/cdg/grid/ggrid.ipp:1486-1548,
+with an analytic terrain-following map), with per-node deformed metrics.
 """
 import ctypes
 
 import numpy as np
 
-from ._capi import GF_BDY_PERIODIC, GF_BDY_DIRICHLET, GF_GE_REGULAR, GF_GE_2DEMBEDDED, check, lib
+from ._capi import (GF_BDY_PERIODIC, GF_BDY_DIRICHLET, GF_GE_REGULAR, GF_GE_2DEMBEDDED,
+                    GF_GE_DEFORMED, check, lib)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -110,6 +115,62 @@ class IcosGrid:
         self.tol = tol_fac * self.minnodedist
         # GGridIcos::max_duplicates: 6 quads meet at a base-icosahedron vertex
         self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyz, self.tol, max_dup=6)
+
+
+def terrain_box_grid(p, ne, P0, P1, bdy, amp, reference_inverse=True):
+    """Run gf_terrain_box_grid; returns dict of host arrays (node_dxidx[i][j],
+    i, j < 2)."""
+    ne = [int(q) for q in ne]
+    nel = ne[0] * ne[1]
+    ndof = nel * (p + 1) ** 2
+    xyz = [np.zeros(ndof) for _ in range(2)]
+    xyzp = [np.zeros(ndof) for _ in range(2)]
+    mask = np.zeros(ndof)
+    dxidx = [[np.zeros(ndof) for _ in range(2)] for _ in range(2)]
+    jac = np.zeros(ndof)
+    md = ctypes.c_double(0)
+    ia = lambda v: (ctypes.c_int * len(v))(*v)
+    P0a = np.ascontiguousarray(P0[:2], dtype=np.float64)
+    P1a = np.ascontiguousarray(P1[:2], dtype=np.float64)
+    xp = (_dp * 2)(*[_p(a) for a in xyz])
+    xpp = (_dp * 2)(*[_p(a) for a in xyzp])
+    dp = (_dp * 4)(*[_p(dxidx[i][j]) for i in range(2) for j in range(2)])
+    check(lib().gf_terrain_box_grid(p, ia(ne), _p(P0a), _p(P1a), ia(list(bdy)), float(amp),
+                                    int(bool(reference_inverse)), xp, xpp, _p(mask), dp, _p(jac),
+                                    ctypes.byref(md)))
+    return dict(xyz=xyz, xyzp=xyzp, mask=mask, node_dxidx=dxidx, node_jac=jac,
+                minnodedist=md.value, ndof=ndof, nelems=nel)
+
+
+class TerrainBoxGrid:
+    """2-D box [P0, P1] of GE_DEFORMED quads with the bottom raised to
+    h(x) = amp (1 - cos(2 pi (x - P0x)/Lx)) / 2 and every node moved by the
+    terrain-following map y' = y + h(x) (P1y - y)/(P1y - P0y).  With
+    reference_inverse (default) the metrics are the reference's, whose 2x2
+    inverse is transposed (GElem_base::inv, gelem_base.cpp:1249-1266);
+    otherwise dXi/dX is the true inverse of dX/dXi."""
+
+    gtype = GF_GE_DEFORMED
+
+    def __init__(self, p, ne, amp=0.1, P0=(0.0, 0.0), P1=(1.0, 1.0), bdy=None,
+                 reference_inverse=True, tol_fac=0.001):
+        if bdy is None:
+            bdy = [GF_BDY_PERIODIC, GF_BDY_PERIODIC, GF_BDY_DIRICHLET, GF_BDY_DIRICHLET]
+        self.dim = 2
+        g = terrain_box_grid(p, ne, np.asarray(P0, dtype=np.float64),
+                             np.asarray(P1, dtype=np.float64), bdy, amp, reference_inverse)
+        self.__dict__.update(g)
+        self.p = [p, p]
+        self.N = [p + 1, p + 1]
+        self.ne = [int(q) for q in ne]
+        self.bdy = list(bdy)
+        self.amp = amp
+        self.reference_inverse = reference_inverse
+        self.P0 = np.asarray(P0, dtype=np.float64)[:2]
+        self.P1 = np.asarray(P1, dtype=np.float64)[:2]
+        self.basis = [gll_basis(p)] * 2
+        self.tol = tol_fac * self.minnodedist
+        self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyzp, self.tol, max_dup=4)
 
 
 class BoxGrid:

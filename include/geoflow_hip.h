@@ -108,6 +108,20 @@ int gf_box_grid(int dim, const int *p, const int *ne, const double *P0, const do
 int gf_icos_grid(int p, int ilevel, int refine, double radius, size_t *nelems,
                  double *const *xyz, double *const *dxidx, double *jac, double *minnodedist);
 
+/* GGrid::add_terrain's result for a 2-D box (src/cdg/grid/ggrid.ipp:1486-1548)
+ * with an analytic terrain-following map in place of its CG solve, and the
+ * GE_DEFORMED geometry of GElem_base::dogeom2d / Jac / inv
+ * (src/cdg/sem/gelem_base.cpp:877-955, 1121-1136, 1249-1266): the regular box
+ * of gf_box_grid (y not periodic) with every node moved to
+ *   y' = y + h(x) (P1y - y)/(P1y - P0y),  h(x) = amp (1 - cos(2 pi (x-P0x)/Lx))/2.
+ * dxidx[4] at [i*2 + j] = dXi_i/dX_j per node.  reference_inverse != 0 keeps
+ * GElem_base::inv's 2x2 result, the transpose of dx/dxi^-1 (so the metrics
+ * equal the reference's); 0 gives the true inverse.  jac = det(dx/dxi). */
+int gf_terrain_box_grid(int p, const int *ne, const double *P0, const double *P1, const int *bdy,
+                        double amp, int reference_inverse, double *const *xyz,
+                        double *const *xyzp, double *mask, double *const *dxidx, double *jac,
+                        double *minnodedist);
+
 /* GGFX::init local coincidence matching (src/cdg/comm/ggfx.hpp:167-342) on
  * host coordinates xyz[ncoord][ndof] with box tolerance tol.  Produces the
  * group of every node as CSR: members[offs[n] .. offs[n+1]) ascending.
@@ -119,7 +133,7 @@ int gf_ggfx_build(size_t ndof, int ncoord, const double *const *xyz, double tol,
 /* ---- mesh (device-resident operator data) ------------------------------ */
 typedef struct {
   int dim;               /* 2 or 3 (GDIM is a runtime value here)                  */
-  int gtype;             /* GF_GE_REGULAR or GF_GE_2DEMBEDDED                       */
+  int gtype;             /* GF_GE_REGULAR, GF_GE_2DEMBEDDED or (2-D) GF_GE_DEFORMED */
   int N[3];              /* nodes per direction, p+1 (N[2] ignored when dim == 2)   */
   size_t nelems;
   const double *D[3];    /* host: column-major N x N derivative matrices            */
@@ -127,8 +141,9 @@ typedef struct {
   /* GE_REGULAR: per-element constants (host) */
   const double *elem_dxidx;  /* nelems*dim */
   const double *elem_jac;    /* nelems     */
-  /* GE_2DEMBEDDED: per-node metrics (host), dXi_i/dX_j, i < dim, j < 3 at
-   * node_dxidx[i*3+j], and node Jacobian. */
+  /* GE_2DEMBEDDED and GE_DEFORMED: per-node metrics (host), dXi_i/dX_j,
+   * i < dim, j < nxy (3 embedded, dim deformed) at node_dxidx[i*3+j], and
+   * node Jacobian. */
   const double *node_dxidx[9];
   const double *node_jac;
   const double *xyz[3];  /* host node coordinates (optional; needed for sphere ops) */

@@ -49,6 +49,10 @@ def lib():
         L.or_grid_icos_create2.restype = ctypes.c_void_p
         L.or_grid_icos_create2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_double, ctypes.c_double]
+        L.or_grid_terrain_create.restype = ctypes.c_void_p
+        L.or_grid_terrain_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp,
+                                             ctypes.POINTER(ctypes.c_int), ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_double]
         L.or_burgers_rhs_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
                                              ctypes.POINTER(_dp), ctypes.POINTER(_dp),
                                              ctypes.POINTER(_dp)]
@@ -187,6 +191,24 @@ class Grid:
         """refine 0: GICOS_BISECTION, 1: GICOS_LAGRANGIAN."""
         g = cls(lib().or_grid_icos_create2(p, ilevel, refine, radius, tol_fac))
         g.p = [p, p]
+        return g
+
+    @classmethod
+    def terrain(cls, p, ne, amp, P0=(0.0, 0.0), P1=(1.0, 1.0), bdy=None, ref_inverse=True,
+                tol_fac=0.001):
+        """2-D box of GE_DEFORMED quads under the terrain-following map
+        y' = y + h(x)(P1y - y)/H, h = amp (1 - cos(2 pi (x-P0x)/Lx))/2;
+        ref_inverse keeps the reference's transposed 2x2 dXi/dX."""
+        if bdy is None:
+            bdy = [BDY_PERIODIC, BDY_PERIODIC, 0, 0]
+        P0 = np.ascontiguousarray(P0, dtype=np.float64)
+        P1 = np.ascontiguousarray(P1, dtype=np.float64)
+        na = (ctypes.c_int * 2)(*ne)
+        ba = (ctypes.c_int * 4)(*bdy)
+        g = cls(lib().or_grid_terrain_create(p, na, _p(P0), _p(P1), ba, amp, int(bool(ref_inverse)),
+                                             tol_fac))
+        g.p = [p, p]
+        g.ne = list(ne)
         return g
 
     def __del__(self):

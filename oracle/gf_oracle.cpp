@@ -190,7 +190,7 @@ void or_embed_G(OrGrid *g, int i, int j, std::vector<double> &G);
 // ---------------------------------------------------------------------------
 struct OrGrid {
   int dim = 3;
-  int gtype = 0;  // 0 GE_REGULAR, 2 GE_2DEMBEDDED (src/cdg/include/gtypes.h:171)
+  int gtype = 0;  // 0 GE_REGULAR, 1 GE_DEFORMED, 2 GE_2DEMBEDDED (src/cdg/include/gtypes.h:171)
   int N[3] = {1, 1, 1};
   Basis basis[3];
   size_t nelems = 0, ndof = 0, nnodes = 0;  // nnodes per element
@@ -1517,4 +1517,77 @@ extern "C" OrGrid *or_grid_icos_create2(int p, int ilevel, int refine, double ra
 
 extern "C" OrGrid *or_grid_icos_create(int p, int ilevel, double radius, double tol_fac) {
   return or_grid_icos_create2(p, ilevel, 0, radius, tol_fac);
+}
+
+// 2-D box of GE_DEFORMED quads: the regular box (or_grid_box_create) with
+// every node moved by y' = y + h(x) (P1y - y)/H, h(x) = amp (1 - cos(2 pi
+// (x - P0x)/Lx))/2 (the coordinates GGrid::add_terrain would install,
+// ggrid.ipp:1486-1548, for this terrain), then GElem_base::dogeom2d for
+// GE_DEFORMED (gelem_base.cpp:877-955): rij(k,m) = sum_I dN_I/dxi^m x^k_I
+// with the tensor-product GLL shape functions, Jac = rij00 rij11 - rij01 rij10
+// (GElem_base::Jac, 2x2, gelem_base.cpp:1121-1136) and the 2x2 branch of
+// GElem_base::inv (gelem_base.cpp:1255-1266), whose off-diagonal entries are
+// -rij10/jac at (0,1) and -rij01/jac at (1,0).  ref_inverse = 0 swaps those
+// two (the true inverse).  Mask and boundary classes stay those of the base
+// box (add_terrain recomputes only the geometry).
+extern "C" OrGrid *or_grid_terrain_create(int p, const int *ne, const double *P0, const double *P1,
+                                          const int *bdy, double amp, int ref_inverse,
+                                          double tol_fac) {
+  using namespace icos;
+  const int pp[2] = {p, p};
+  OrGrid *gp = or_grid_box_create(2, pp, ne, P0, P1, bdy, tol_fac);
+  if (!gp) return nullptr;
+  OrGrid &g = *gp;
+  g.gtype = 1;
+  const double Lx = P1[0] - P0[0], H = P1[1] - P0[1];
+  for (std::vector<double> *c : {g.xn, g.xp})
+    for (size_t q = 0; q < g.ndof; q++) {
+      double h = amp * 0.5 * (1.0 - std::cos(2.0 * PI_ * (c[0][q] - P0[0]) / Lx));
+      c[1][q] = c[1][q] + h * (P1[1] - c[1][q]) / H;
+    }
+  const int N = p + 1;
+  const size_t nn = (size_t)N * N;
+  const Basis &B = g.basis[0];
+  std::vector<double> h0((size_t)N * N), dh0((size_t)N * N);  // [basis][node]
+  for (int i = 0; i < N; i++)
+    for (int q = 0; q < N; q++) {
+      h0[(size_t)i * N + q] = evalBasis(B, i, B.xi[q]);
+      dh0[(size_t)i * N + q] = evalDBasis(B, i, B.xi[q]);
+    }
+  std::vector<double> dNi(nn), tmp(nn), rij[2][2];
+  for (size_t e = 0; e < g.nelems; e++) {
+    const size_t b0 = e * nn;
+    for (int m = 0; m < 2; m++)
+      for (int k = 0; k < 2; k++) {
+        tmp.assign(nn, 0.0);
+        for (int jb = 0, nI = 0; jb < N; jb++)
+          for (int ib = 0; ib < N; ib++, nI++) {
+            // dN_I/dxi^m at every node: h'_ib(xi) h_jb(eta) or h_ib(xi) h'_jb(eta)
+            const double *d0 = (m == 0 ? dh0 : h0).data() + (size_t)ib * N;
+            const double *d1 = (m == 1 ? dh0 : h0).data() + (size_t)jb * N;
+            for (int qj = 0, q = 0; qj < N; qj++)
+              for (int qi = 0; qi < N; qi++, q++) dNi[q] = d0[qi] * d1[qj];
+            double xk = g.xn[k][b0 + nI];
+            for (size_t q = 0; q < nn; q++) {
+              dNi[q] *= xk;
+              tmp[q] += dNi[q];
+            }
+          }
+        rij[k][m] = tmp;
+      }
+    for (size_t q = 0; q < nn; q++) {
+      auto G = [&](int a, int b) { return rij[a][b][q]; };
+      double jv = G(0, 0) * G(1, 1) - G(0, 1) * G(1, 0);
+      double ijac = 1.0 / (G(0, 0) * G(1, 1) - G(1, 0) * G(0, 1));
+      size_t id = b0 + q;
+      g.jac[id] = jv;
+      g.dxidx[0][0][id] = G(1, 1) * ijac;
+      g.dxidx[0][1][id] = -(ref_inverse ? G(1, 0) : G(0, 1)) * ijac;
+      g.dxidx[1][0][id] = -(ref_inverse ? G(0, 1) : G(1, 0)) * ijac;
+      g.dxidx[1][1][id] = G(0, 0) * ijac;
+    }
+  }
+  g.minnodedist = find_min_dist(g);
+  finish_grid(g, tol_fac);
+  return gp;
 }

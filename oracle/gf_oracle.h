@@ -43,6 +43,12 @@ OrGrid *or_grid_box_create(int dim, const int *p, const int *ne,
 OrGrid *or_grid_icos_create(int p, int ilevel, double radius, double tol_fac);
 /* refine: 0 GICOS_BISECTION, 1 GICOS_LAGRANGIAN */
 OrGrid *or_grid_icos_create2(int p, int ilevel, int refine, double radius, double tol_fac);
+/* ---- 2-D terrain-following GE_DEFORMED box (GGrid::add_terrain result,
+ * GElem_base::dogeom2d/Jac/inv for GE_DEFORMED, gelem_base.cpp:877-955,
+ * 1121-1136, 1249-1266); ref_inverse keeps the reference's transposed 2x2
+ * inverse ------------------------------------------------------------------ */
+OrGrid *or_grid_terrain_create(int p, const int *ne, const double *P0, const double *P1,
+                               const int *bdy, double amp, int ref_inverse, double tol_fac);
 void    or_grid_destroy(OrGrid *g);
 
 size_t or_grid_ndof(const OrGrid *g);
