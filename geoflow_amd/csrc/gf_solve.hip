@@ -1409,7 +1409,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
       return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
     if (dinv) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, dinv32, dinv, 1.0, n);
     if (qf32) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, qf32, prm->q_field, 1.0, n);
-    const double delta = prm->inner_tol > 0 ? prm->inner_tol : 0.1;
+    const double delta = prm->inner_tol > 0 ? prm->inner_tol : 0.01;
     CGVecs<float> v;
     v.x = x32;
     v.r = r32;

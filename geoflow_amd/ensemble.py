@@ -108,9 +108,12 @@ class Ensemble:
         p, off, n, _ = self.slices[i]
         return [st[p][c][off:off + n].cpu().numpy() for c in range(3)]
 
-    def imex_step(self, st, nu, dt, precision=0, tol=1e-10, maxit=2000):
+    def imex_step(self, st, nu, dt, precision=0, tol=1e-10, maxit=2000, inner_tol=1e-5):
         """One implicit-viscous step of every member; returns the CG
-        iteration counts, per group and component."""
+        iteration counts, per group and component.  The sub-solves take ~5
+        iterations, so the FP32-inner solver refines only when its residual
+        has fallen by inner_tol (1e-5: one or two FP64 reliable updates per
+        solve; the solver's default 0.01 would need five)."""
         import torch
         its = {}
         for p, m in self.mesh.items():
@@ -121,7 +124,7 @@ class Ensemble:
                 b -= dt * m.advect(us[c], us)
                 x = us[c].clone()
                 x, info = m.cg_solve(b, x=x, p=nu * dt, q=1.0, maxit=maxit, tol=tol, normtype=4,
-                                     precond=1, precision=precision)
+                                     precond=1, precision=precision, inner_tol=inner_tol)
                 if info["status"] != 0:
                     raise RuntimeError("ensemble CG did not converge (order %d, comp %d)" % (p, c))
                 its.setdefault(p, []).append(info["iters"])

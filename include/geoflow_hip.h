@@ -274,7 +274,7 @@ typedef struct {
   int    normtype;   /* GF_NORM_* */
   int    precond;    /* 0: identity (reference);  1: Jacobi on the assembled diagonal */
   int    precision;  /* 0: FP64 CG;  1: FP32 recurrences with FP64 reliable updates */
-  double inner_tol;  /* mixed: residual reduction between FP64 reliable updates (default 0.1) */
+  double inner_tol;  /* mixed: residual reduction between FP64 reliable updates (default 0.01) */
   double p_scalar;   /* Laplacian factor when p_field == NULL */
   const double *p_field;
   const double *q_field;
