@@ -167,6 +167,8 @@ struct CGApplyArgs {
   void *w;            // T
   void *q;            // T
   const void *D, *W, *edx, *ejac, *qf;  // T
+  const double *G[3];  // node metrics G00, G01, G11 (embedded or deformed 2-D meshes)
+  const double *mass;  // lumped mass per node (embedded or deformed 2-D meshes)
   double p;
   double qs;  // mass-term scalar when qf is NULL (0: no mass term)
   double *scal;
@@ -245,6 +247,72 @@ __global__ __launch_bounds__(kThreads) void k_cg_apply(CGApplyArgs a) {
   if (threadIdx.x == 0) a.part[blockIdx.x] = bsum;
 }
 
+// The same for 2-D meshes with per-node metrics (GE_2DEMBEDDED and
+// GE_DEFORMED): q = p sum_j D_j^T (sum_i G(i,j) D_i w) + q M w in the
+// operation order of embed_prod / def_prod (ghelmholtz.ipp:108-191, 205-290),
+// as k_helm_embed forms it.  FP64.
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_cg_apply_g(CGApplyArgs a) {
+  using S = Shape<2, N>;
+  __shared__ double su[S::NODES];
+  __shared__ double sg[2][S::NODES];
+  __shared__ double sD[N * N];
+  __shared__ double sred[kThreads / 64];
+  const double *scal = a.scal;
+  if (scal[S_DONE] != 0.0) return;
+  const double *r = (const double *)a.r, *dinv = (const double *)a.dinv;
+  double *w = (double *)a.w, *qv = (double *)a.q;
+  const double *qf = (const double *)a.qf;
+  const bool first = scal[S_FIRST] != 0.0;
+  const double beta = scal[S_BETA];
+  const size_t e0 = (size_t)blockIdx.x * S::EPB;
+  const int nel = (int)min((size_t)S::EPB, a.nelems - e0);
+  const size_t base = e0 * S::NN;
+  const int cnt = nel * S::NN;
+  gf::stage_D(sD, (const double *)a.D, N);
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    const size_t g = base + q;
+    const double z = dinv ? dinv[g] * r[g] : r[g];
+    const double wn = first ? z : beta * w[g] + 1.0 * z;
+    w[g] = wn;
+    su[q] = wn;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
+    gf::node_ijk<2, N>(rr, i, j, k);
+    const double d0 = gf::contract<2, N, false>(su + el * S::NN, sD, 0, i, j, k);
+    const double d1 = gf::contract<2, N, false>(su + el * S::NN, sD, 1, i, j, k);
+    const size_t g = base + q;
+    double g0 = 0.0 + d0 * a.G[0][g];
+    g0 += d1 * a.G[1][g];
+    double g1 = 0.0 + d0 * a.G[1][g];
+    g1 += d1 * a.G[2][g];
+    sg[0][q] = g0;
+    sg[1][q] = g1;
+  }
+  __syncthreads();
+  double dot = 0.0;
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
+    gf::node_ijk<2, N>(rr, i, j, k);
+    double acc = 0.0;
+    acc += gf::contract<2, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
+    acc += gf::contract<2, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
+    if (a.p != 1.0) acc *= a.p;
+    if (qf || a.qs != 0.0) {
+      double t = a.mass[base + q] * su[q];
+      if (qf) t *= qf[base + q];
+      else if (a.qs != 1.0) t *= a.qs;
+      acc += t;
+    }
+    qv[base + q] = acc;
+    dot += su[q] * acc;
+  }
+  const double bsum = gf::block_reduce<false>(dot, sred);
+  if (threadIdx.x == 0) a.part[blockIdx.x] = bsum;
+}
+
 template <typename T>
 __global__ void k_dss_cg(T *__restrict__ u, const int *__restrict__ goff,
                          const int *__restrict__ gmem, int ngroups, const double *scal) {
@@ -290,7 +358,7 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_alpha(double *scal,
 template <typename T, int NORM>
 __global__ void k_cg_update(T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r,
                             const T *__restrict__ qv, const T *__restrict__ mask,
-                            const T *__restrict__ dinv, const float *__restrict__ imult,
+                            const T *__restrict__ dinv, const T *__restrict__ imult,
                             const double *__restrict__ mass, double *scal,
                             double *__restrict__ part, size_t n) {
   __shared__ double sred[kThreads / 64];
@@ -299,7 +367,7 @@ __global__ void k_cg_update(T *__restrict__ x, const T *__restrict__ w, T *__res
   double rho = 0.0, nrm = 0.0;
   GRID_LOOP(g, n) {
     const T dv = dinv ? dinv[g] : T(1);
-    const float im = imult[g];
+    const double im = (double)imult[g];
     T xv = (T)1 * x[g] + alpha * w[g];
     T qm = mask ? qv[g] * mask[g] : qv[g];
     T rv = (T)1 * r[g] + (-alpha) * qm;
@@ -366,7 +434,7 @@ __global__ void k_cg_beta(double *scal, const double *__restrict__ part, int np,
 // Generic reductions used at solver set-up: partial (sum_i a*b*c) and norm.
 template <typename T>
 __global__ void k_setup_red(const T *__restrict__ r, const T *__restrict__ z,
-                            const float *__restrict__ imult, const double *__restrict__ mass,
+                            const T *__restrict__ imult, const double *__restrict__ mass,
                             int normtype, double *__restrict__ part, size_t n) {
   __shared__ double sred[kThreads / 64];
   double rho = 0.0, nrm = 0.0;
@@ -475,6 +543,12 @@ template <int DIM, int N, typename T>
 int launch_cg_apply(gf_mesh *m, const CGApplyArgs &a) {
   using S = Shape<DIM, N>;
   int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
+  if (m->gtype != GF_GE_REGULAR) {
+    if (DIM != 2 || sizeof(T) != 8) return GF_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL((k_cg_apply_g<N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
+    GF_LAUNCH_CHECK(m->ctx);
+    return GF_OK;
+  }
   hipLaunchKernelGGL((k_cg_apply<DIM, N, T>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
@@ -1131,7 +1205,7 @@ namespace {
 
 // Reduce (r.z.imult, norm(r)) at set-up, synchronous; returns both.
 template <typename T>
-int setup_red(gf_mesh *m, const T *r, const T *z, const float *imult, int normtype, double out[2]) {
+int setup_red(gf_mesh *m, const T *r, const T *z, const T *imult, int normtype, double out[2]) {
   hipLaunchKernelGGL(k_setup_red<T>, dim3(kUpdBlocks), dim3(kThreads), 0, m->ctx->stream, r, z,
                      imult, m->mass, normtype, m->red, m->ndof);
   hipLaunchKernelGGL(k_setup_fin, dim3(1), dim3(kThreads), 0, m->ctx->stream, m->red, kUpdBlocks,
@@ -1206,6 +1280,8 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a.ejac = v.ejac;
   a.qf = v.qf;
   a.qs = v.qs;
+  for (int c = 0; c < 3; c++) a.G[c] = m->G[c];
+  a.mass = m->mass;
   a.p = v.p;
   a.scal = m->scal;
   a.part = partA;
@@ -1283,8 +1359,10 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
                            double *x, gf_cg_result *res) {
   if (!m || !prm || !b || !x || !res) return GF_ERR_ARG;
   memset(res, 0, sizeof(*res));
-  if (m->gtype != GF_GE_REGULAR || !m->Niso)
-    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: regular isotropic meshes");
+  if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: isotropic meshes");
+  if (m->gtype != GF_GE_REGULAR && (prm->precision != 0 || prm->precond))
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED,
+                      "CG on embedded or deformed meshes: FP64 without preconditioner");
   if (prm->maxit < 1) return gf_set_err(m->ctx, GF_ERR_BADITERNUM, "maxit < 1");
   if (prm->normtype < GF_NORM_INF || prm->normtype > GF_NORM_ASSEMBLED_L2) return GF_ERR_ARG;
   if (prm->p_field) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: p field not supported");
@@ -1332,11 +1410,11 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     RC(copy(m, q, b));
     RC(gf_dss_t(m, GF_DSS_SUM, q));
     if (bbv) RC(gf_mul_t(m, q, m->mask, q));
-    RC(setup_red<double>(m, q, nullptr, m->imult32, normtype, red2));
+    RC(setup_red<double>(m, q, nullptr, m->imult, normtype, red2));
     rnorm0 = finish_norm_host(normtype, red2[1], m, volume);
     rtol = prm->tol * rnorm0;
   } else {
-    RC(setup_red<double>(m, b, nullptr, m->imult32, normtype, red2));
+    RC(setup_red<double>(m, b, nullptr, m->imult, normtype, red2));
     rnorm0 = finish_norm_host(normtype, red2[1], m, volume);
     rtol = rnorm0 < 1.0 ? prm->tol : prm->tol * rnorm0;
   }
@@ -1356,7 +1434,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     // z = P^-1 r ; w = z (set inside the first apply) ; rho = r.z.imult
     double *z = q;
     if (dinv) RC(gf_mul_t(m, dinv, r, z));
-    RC(setup_red<double>(m, r, dinv ? z : r, m->imult32, normtype, red2));
+    RC(setup_red<double>(m, r, dinv ? z : r, m->imult, normtype, red2));
     double hs[12] = {0};
     hs[S_RHO] = red2[0];
     hs[S_RTOL] = rtol;
@@ -1375,7 +1453,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     v.q = q;
     v.dinv = dinv;
     v.mask = mask;
-    v.imult = m->imult32;
+    v.imult = m->imult;
     v.qf = prm->q_field;
     v.qs = qs;
     v.D = m->D[0];
@@ -1432,7 +1510,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
       if (!first)
         hipLaunchKernelGGL(k_add_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, x, x32, 1.0, n);
       RC(residual(r));
-      RC(setup_red<double>(m, r, nullptr, m->imult32, normtype, red2));
+      RC(setup_red<double>(m, r, nullptr, m->imult, normtype, red2));
       const double rn = finish_norm_host(normtype, red2[1], m, volume);
       rn_final = rn;
       if (prm->resid && hist < prm->maxit) prm->resid[hist++] = rn;

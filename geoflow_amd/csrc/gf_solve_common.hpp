@@ -140,7 +140,7 @@ template <typename T>
 struct CGVecs {
   T *x, *r, *w, *q;
   const T *dinv, *mask, *qf;
-  const float *imult;
+  const T *imult;  // 1/multiplicity in the recurrence precision (exact in FP64)
   const void *D, *W, *edx, *ejac;
   double p;
   double qs;  // mass-term scalar when qf is NULL
