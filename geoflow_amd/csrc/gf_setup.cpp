@@ -732,55 +732,33 @@ int gf_icos_grid(int p, int ilevel, int refine, double radius, size_t *nelems,
   return GF_OK;
 }
 
-// 2-D box of GE_DEFORMED quads following a terrain profile: the nodes of the
-// regular box (gf_box_grid) are moved by the terrain-following map
-//   y' = y + h(x) (P1y - y) / (P1y - P0y),  h(x) = amp (1 - cos(2 pi (x - P0x) / Lx)) / 2,
-// which raises the bottom boundary to h(x) and keeps the top.  The geometry
-// is GElem_base::dogeom2d for GE_DEFORMED elements
-// (src/cdg/sem/gelem_base.cpp:877-955): R(k,m) = dx_k/dxi_m from the nodal
-// coordinates with the GLL derivative matrix (the shape-function derivatives
-// at the nodes), jac = R00 R11 - R01 R10 (GElem_base::Jac, 2x2 branch, no
-// absolute value, gelem_base.cpp:1121-1136), and dXi/dX from GElem_base::inv
-// (gelem_base.cpp:1249-1266).  That 2x2 inverse returns the transpose of
-// R^-1 (its (0,1) entry is -R10/jac); reference_inverse != 0 keeps that, so
-// the metrics equal the reference's, and 0 gives the true inverse.
-//   p, ne[2], P0[2], P1[2], bdy[4] as gf_box_grid (y may not be periodic);
-//   xyz[2], xyzp[2] (periodized), mask: as gf_box_grid, for the moved nodes;
-//   dxidx[4] at [i*2+j] = dXi_i/dX_j per node; jac per node; minnodedist of
-//   the moved nodes (GGrid::find_min_dist, src/cdg/grid/ggrid.ipp:977-1013).
-int gf_terrain_box_grid(int p, const int *ne, const double *P0, const double *P1, const int *bdy,
-                        double amp, int reference_inverse, double *const *xyz,
-                        double *const *xyzp, double *mask, double *const *dxidx, double *jac,
-                        double *minnodedist) {
-  if (p < 1 || !ne || !P0 || !P1 || !bdy || !xyz || !xyz[0] || !xyz[1] || !dxidx || !jac)
-    return GF_ERR_ARG;
+// GE_DEFORMED geometry of 2-D elements from their node coordinates:
+// GElem_base::dogeom2d (src/cdg/sem/gelem_base.cpp:877-955): R(k,m) = dx_k/dxi_m
+// from the tensor-product GLL shape-function derivatives at the nodes
+// (GLLBasis::evalBasis / evalDBasis), jac = R00 R11 - R01 R10 (GElem_base::Jac,
+// 2x2 branch without absolute value, gelem_base.cpp:1121-1136) and dXi/dX
+// from GElem_base::inv (gelem_base.cpp:1249-1266).  That 2x2 inverse returns
+// the transpose of R^-1 (its (0,1) entry is -R10/jac); reference_inverse != 0
+// keeps it, so the metrics equal the reference's, 0 gives the true inverse.
+//   xyz[2]: nelems * (p+1)^2 node coordinates; dxidx[4] at [i*2+j] =
+//   dXi_i/dX_j per node; jac per node.
+int gf_deformed_geometry2d(int p, size_t nelems, const double *const *xyz, int reference_inverse,
+                           double *const *dxidx, double *jac) {
+  if (p < 1 || !xyz || !xyz[0] || !xyz[1] || !dxidx || !jac) return GF_ERR_ARG;
   for (int k = 0; k < 4; k++)
     if (!dxidx[k]) return GF_ERR_ARG;
-  if (bdy[2] == GF_BDY_PERIODIC || bdy[3] == GF_BDY_PERIODIC) return GF_ERR_ARG;
-  const int pp[2] = {p, p};
-  int rc = gf_box_grid(2, pp, ne, P0, P1, bdy, xyz, xyzp, mask, nullptr, nullptr, nullptr);
-  if (rc) return rc;
   const int N = p + 1;
   const LDBasis lb = make_ldbasis(p);
-  // tensor-product shape-function values and derivatives at the nodes
-  // (GLLBasis::evalBasis / evalDBasis), [basis i][node q]
+  // tensor-product shape-function values and derivatives at the nodes,
+  // [basis i][node q]
   std::vector<double> hb((size_t)N * N), hd((size_t)N * N);
   for (int i = 0; i < N; i++)
     for (int q = 0; q < N; q++) {
       hb[(size_t)i * N + q] = eval_basis(lb, i, lb.xd[q]);
       hd[(size_t)i * N + q] = eval_dbasis(lb, i, lb.xd[q]);
     }
-  const double Lx = P1[0] - P0[0], H = P1[1] - P0[1];
-  const size_t nn = (size_t)N * N, ndof = (size_t)ne[0] * ne[1] * nn;
-  auto move = [&](double *const *c) {
-    for (size_t q = 0; q < ndof; q++) {
-      const double h = amp * 0.5 * (1.0 - std::cos(2.0 * kPi * (c[0][q] - P0[0]) / Lx));
-      c[1][q] = c[1][q] + h * (P1[1] - c[1][q]) / H;
-    }
-  };
-  move(xyz);
-  if (xyzp && xyzp[0] && xyzp[1]) move(xyzp);
-  for (size_t e = 0; e < (size_t)ne[0] * ne[1]; e++) {
+  const size_t nn = (size_t)N * N;
+  for (size_t e = 0; e < nelems; e++) {
     const size_t base = e * nn;
     for (int qj = 0, q = 0; qj < N; qj++)
       for (int qi = 0; qi < N; qi++, q++) {
@@ -809,16 +787,54 @@ int gf_terrain_box_grid(int p, const int *ne, const double *P0, const double *P1
         dxidx[2][id] = (reference_inverse ? -r[0][1] : -r[1][0]) * ij;
       }
   }
-  if (minnodedist) {  // GGrid::find_min_dist (ggrid.ipp:977-1013)
-    const double tiny = 1000.0 * std::numeric_limits<double>::epsilon();
-    double dmin = std::numeric_limits<double>::max();
-    for (size_t q = 0; q + 1 < ndof; q++) {
-      const double dx = xyz[0][q + 1] - xyz[0][q], dy = xyz[1][q + 1] - xyz[1][q];
-      const double rr = std::sqrt(dx * dx + dy * dy);
-      if (rr > tiny) dmin = std::min(dmin, rr);
-    }
-    *minnodedist = dmin;
+  return GF_OK;
+}
+
+// GGrid::find_min_dist over consecutive local nodes (ggrid.ipp:977-1013), 2-D.
+static double min_dist2d(const double *const *xyz, size_t ndof) {
+  const double tiny = 1000.0 * std::numeric_limits<double>::epsilon();
+  double dmin = std::numeric_limits<double>::max();
+  for (size_t q = 0; q + 1 < ndof; q++) {
+    const double dx = xyz[0][q + 1] - xyz[0][q], dy = xyz[1][q + 1] - xyz[1][q];
+    const double rr = std::sqrt(dx * dx + dy * dy);
+    if (rr > tiny) dmin = std::min(dmin, rr);
   }
+  return dmin;
+}
+
+// 2-D box of GE_DEFORMED quads following a terrain profile: the nodes of the
+// regular box (gf_box_grid) are moved by the terrain-following map
+//   y' = y + h(x) (P1y - y) / (P1y - P0y),  h(x) = amp (1 - cos(2 pi (x - P0x) / Lx)) / 2,
+// which raises the bottom boundary to h(x) and keeps the top; geometry as
+// gf_deformed_geometry2d.
+//   p, ne[2], P0[2], P1[2], bdy[4] as gf_box_grid (y may not be periodic);
+//   xyz[2], xyzp[2] (periodized), mask: as gf_box_grid, for the moved nodes;
+//   dxidx[4] at [i*2+j] = dXi_i/dX_j per node; jac per node; minnodedist of
+//   the moved nodes (GGrid::find_min_dist, src/cdg/grid/ggrid.ipp:977-1013).
+int gf_terrain_box_grid(int p, const int *ne, const double *P0, const double *P1, const int *bdy,
+                        double amp, int reference_inverse, double *const *xyz,
+                        double *const *xyzp, double *mask, double *const *dxidx, double *jac,
+                        double *minnodedist) {
+  if (p < 1 || !ne || !P0 || !P1 || !bdy || !xyz || !xyz[0] || !xyz[1] || !dxidx || !jac)
+    return GF_ERR_ARG;
+  if (bdy[2] == GF_BDY_PERIODIC || bdy[3] == GF_BDY_PERIODIC) return GF_ERR_ARG;
+  const int pp[2] = {p, p};
+  int rc = gf_box_grid(2, pp, ne, P0, P1, bdy, xyz, xyzp, mask, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  const int N = p + 1;
+  const double Lx = P1[0] - P0[0], H = P1[1] - P0[1];
+  const size_t nel = (size_t)ne[0] * ne[1], ndof = nel * N * N;
+  auto move = [&](double *const *c) {
+    for (size_t q = 0; q < ndof; q++) {
+      const double h = amp * 0.5 * (1.0 - std::cos(2.0 * kPi * (c[0][q] - P0[0]) / Lx));
+      c[1][q] = c[1][q] + h * (P1[1] - c[1][q]) / H;
+    }
+  };
+  move(xyz);
+  if (xyzp && xyzp[0] && xyzp[1]) move(xyzp);
+  rc = gf_deformed_geometry2d(p, nel, xyz, reference_inverse, dxidx, jac);
+  if (rc) return rc;
+  if (minnodedist) *minnodedist = min_dist2d(xyz, ndof);
   return GF_OK;
 }
 

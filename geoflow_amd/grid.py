@@ -142,6 +142,20 @@ def terrain_box_grid(p, ne, P0, P1, bdy, amp, reference_inverse=True):
                 minnodedist=md.value, ndof=ndof, nelems=nel)
 
 
+def deformed_geometry2d(p, xyz, reference_inverse=True):
+    """Run gf_deformed_geometry2d: per-node dXi/dX (node_dxidx[i][j]) and
+    Jacobian of 2-D elements from their node coordinates."""
+    xyz = [np.ascontiguousarray(a, dtype=np.float64) for a in xyz[:2]]
+    ndof = xyz[0].size
+    nel = ndof // (p + 1) ** 2
+    dxidx = [[np.zeros(ndof) for _ in range(2)] for _ in range(2)]
+    jac = np.zeros(ndof)
+    xp = (_dp * 2)(*[_p(a) for a in xyz])
+    dp = (_dp * 4)(*[_p(dxidx[i][j]) for i in range(2) for j in range(2)])
+    check(lib().gf_deformed_geometry2d(p, nel, xp, int(bool(reference_inverse)), dp, _p(jac)))
+    return dxidx, jac
+
+
 class TerrainBoxGrid:
     """2-D box [P0, P1] of GE_DEFORMED quads with the bottom raised to
     h(x) = amp (1 - cos(2 pi (x - P0x)/Lx)) / 2 and every node moved by the
@@ -169,6 +183,32 @@ class TerrainBoxGrid:
         self.P0 = np.asarray(P0, dtype=np.float64)[:2]
         self.P1 = np.asarray(P1, dtype=np.float64)[:2]
         self.basis = [gll_basis(p)] * 2
+        self.tol = tol_fac * self.minnodedist
+        self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyzp, self.tol, max_dup=4)
+
+
+class DeformedGrid2D:
+    """2-D GE_DEFORMED grid from explicit node coordinates on the elements of
+    a box grid (what GGrid::add_terrain leaves, src/cdg/grid/ggrid.ipp:1522-1547):
+    geometry recomputed from the coordinates, mask kept from the base grid,
+    GGFX matched on the new coordinates (no periodic directions)."""
+
+    gtype = GF_GE_DEFORMED
+
+    def __init__(self, base, xyz, reference_inverse=True, tol_fac=0.001):
+        self.dim = 2
+        self.p, self.N, self.ne = list(base.p), list(base.N), list(base.ne)
+        self.basis = base.basis
+        self.nelems, self.ndof = base.nelems, base.ndof
+        self.mask = base.mask
+        self.reference_inverse = reference_inverse
+        self.xyz = [np.ascontiguousarray(a, dtype=np.float64) for a in xyz[:2]]
+        self.xyzp = self.xyz
+        self.node_dxidx, self.node_jac = deformed_geometry2d(self.p[0], self.xyz,
+                                                             reference_inverse)
+        d = [np.hypot(np.diff(self.xyz[0]), np.diff(self.xyz[1]))]
+        tiny = 1000.0 * np.finfo(np.float64).eps
+        self.minnodedist = float(d[0][d[0] > tiny].min())  # GGrid::find_min_dist
         self.tol = tol_fac * self.minnodedist
         self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyzp, self.tol, max_dup=4)
 

@@ -122,6 +122,15 @@ int gf_terrain_box_grid(int p, const int *ne, const double *P0, const double *P1
                         double *const *xyzp, double *mask, double *const *dxidx, double *jac,
                         double *minnodedist);
 
+/* GE_DEFORMED geometry of 2-D elements from their node coordinates
+ * (GElem_base::dogeom2d / Jac / inv, src/cdg/sem/gelem_base.cpp:877-955,
+ * 1121-1136, 1249-1266), e.g. after GGrid::add_terrain has moved the nodes
+ * (src/cdg/grid/ggrid.ipp:1528-1540 recomputes the geometry the same way).
+ *   xyz[2]: nelems*(p+1)^2 coordinates; dxidx[4] at [i*2 + j] = dXi_i/dX_j;
+ *   jac = det(dx/dxi); reference_inverse as gf_terrain_box_grid. */
+int gf_deformed_geometry2d(int p, size_t nelems, const double *const *xyz, int reference_inverse,
+                           double *const *dxidx, double *jac);
+
 /* GGFX::init local coincidence matching (src/cdg/comm/ggfx.hpp:167-342) on
  * host coordinates xyz[ncoord][ndof] with box tolerance tol.  Produces the
  * group of every node as CSR: members[offs[n] .. offs[n+1]) ascending.

@@ -53,6 +53,7 @@ def lib():
         L.or_grid_terrain_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp,
                                              ctypes.POINTER(ctypes.c_int), ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double]
+        L.or_grid_set_coords2d.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double]
         L.or_burgers_rhs_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
                                              ctypes.POINTER(_dp), ctypes.POINTER(_dp),
                                              ctypes.POINTER(_dp)]
@@ -210,6 +211,18 @@ class Grid:
         g.p = [p, p]
         g.ne = list(ne)
         return g
+
+    def set_coords(self, x, y, ref_inverse=True, tol_fac=0.001):
+        """New node coordinates for a 2-D deformed grid without periodic
+        directions (the end of GGrid::add_terrain): geometry, GGFX, mass and
+        volume recomputed."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if lib().or_grid_set_coords2d(self.h, _p(x), _p(y), int(bool(ref_inverse)), tol_fac) != 0:
+            raise ValueError("set_coords: 2-D deformed grids only")
+        self.minnodedist = lib().or_grid_minnodedist(self.h)
+        self.volume = lib().or_grid_volume(self.h)
+        return self
 
     def __del__(self):
         try:

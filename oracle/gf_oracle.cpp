@@ -1519,33 +1519,17 @@ extern "C" OrGrid *or_grid_icos_create(int p, int ilevel, double radius, double 
   return or_grid_icos_create2(p, ilevel, 0, radius, tol_fac);
 }
 
-// 2-D box of GE_DEFORMED quads: the regular box (or_grid_box_create) with
-// every node moved by y' = y + h(x) (P1y - y)/H, h(x) = amp (1 - cos(2 pi
-// (x - P0x)/Lx))/2 (the coordinates GGrid::add_terrain would install,
-// ggrid.ipp:1486-1548, for this terrain), then GElem_base::dogeom2d for
-// GE_DEFORMED (gelem_base.cpp:877-955): rij(k,m) = sum_I dN_I/dxi^m x^k_I
-// with the tensor-product GLL shape functions, Jac = rij00 rij11 - rij01 rij10
-// (GElem_base::Jac, 2x2, gelem_base.cpp:1121-1136) and the 2x2 branch of
-// GElem_base::inv (gelem_base.cpp:1255-1266), whose off-diagonal entries are
-// -rij10/jac at (0,1) and -rij01/jac at (1,0).  ref_inverse = 0 swaps those
-// two (the true inverse).  Mask and boundary classes stay those of the base
-// box (add_terrain recomputes only the geometry).
-extern "C" OrGrid *or_grid_terrain_create(int p, const int *ne, const double *P0, const double *P1,
-                                          const int *bdy, double amp, int ref_inverse,
-                                          double tol_fac) {
+namespace {
+// GElem_base::dogeom2d for GE_DEFORMED (gelem_base.cpp:877-955) from g.xn:
+// rij(k,m) = sum_I dN_I/dxi^m x^k_I with the tensor-product GLL shape
+// functions, Jac = rij00 rij11 - rij01 rij10 (GElem_base::Jac, 2x2,
+// gelem_base.cpp:1121-1136) and the 2x2 branch of GElem_base::inv
+// (gelem_base.cpp:1255-1266), whose off-diagonal entries are -rij10/jac at
+// (0,1) and -rij01/jac at (1,0); ref_inverse = 0 swaps those two (the true
+// inverse).
+void deformed_geom2d(OrGrid &g, int ref_inverse) {
   using namespace icos;
-  const int pp[2] = {p, p};
-  OrGrid *gp = or_grid_box_create(2, pp, ne, P0, P1, bdy, tol_fac);
-  if (!gp) return nullptr;
-  OrGrid &g = *gp;
-  g.gtype = 1;
-  const double Lx = P1[0] - P0[0], H = P1[1] - P0[1];
-  for (std::vector<double> *c : {g.xn, g.xp})
-    for (size_t q = 0; q < g.ndof; q++) {
-      double h = amp * 0.5 * (1.0 - std::cos(2.0 * PI_ * (c[0][q] - P0[0]) / Lx));
-      c[1][q] = c[1][q] + h * (P1[1] - c[1][q]) / H;
-    }
-  const int N = p + 1;
+  const int N = g.N[0];
   const size_t nn = (size_t)N * N;
   const Basis &B = g.basis[0];
   std::vector<double> h0((size_t)N * N), dh0((size_t)N * N);  // [basis][node]
@@ -1587,7 +1571,48 @@ extern "C" OrGrid *or_grid_terrain_create(int p, const int *ne, const double *P0
       g.dxidx[1][1][id] = G(0, 0) * ijac;
     }
   }
+}
+}  // namespace
+
+// 2-D box of GE_DEFORMED quads: the regular box (or_grid_box_create) with
+// every node moved by y' = y + h(x) (P1y - y)/H, h(x) = amp (1 - cos(2 pi
+// (x - P0x)/Lx))/2, and the deformed geometry of those nodes.  Mask and
+// boundary classes stay those of the base box (add_terrain recomputes only
+// the geometry, ggrid.ipp:1522-1540).
+extern "C" OrGrid *or_grid_terrain_create(int p, const int *ne, const double *P0, const double *P1,
+                                          const int *bdy, double amp, int ref_inverse,
+                                          double tol_fac) {
+  using namespace icos;
+  const int pp[2] = {p, p};
+  OrGrid *gp = or_grid_box_create(2, pp, ne, P0, P1, bdy, tol_fac);
+  if (!gp) return nullptr;
+  OrGrid &g = *gp;
+  g.gtype = 1;
+  const double Lx = P1[0] - P0[0], H = P1[1] - P0[1];
+  for (std::vector<double> *c : {g.xn, g.xp})
+    for (size_t q = 0; q < g.ndof; q++) {
+      double h = amp * 0.5 * (1.0 - std::cos(2.0 * PI_ * (c[0][q] - P0[0]) / Lx));
+      c[1][q] = c[1][q] + h * (P1[1] - c[1][q]) / H;
+    }
+  deformed_geom2d(g, ref_inverse);
   g.minnodedist = find_min_dist(g);
   finish_grid(g, tol_fac);
   return gp;
+}
+
+// The second half of GGrid::add_terrain (ggrid.ipp:1522-1547) on a 2-D
+// deformed grid without periodic directions: install new node coordinates,
+// recompute the deformed geometry, the minimum node distance and (here) the
+// GGFX maps, mass and volume from them.
+extern "C" int or_grid_set_coords2d(OrGrid *g, const double *x, const double *y, int ref_inverse,
+                                    double tol_fac) {
+  if (!g || g->dim != 2 || g->gtype != 1 || !x || !y) return 1;
+  std::copy(x, x + g->ndof, g->xn[0].begin());
+  std::copy(y, y + g->ndof, g->xn[1].begin());
+  g->xp[0] = g->xn[0];
+  g->xp[1] = g->xn[1];
+  deformed_geom2d(*g, ref_inverse);
+  g->minnodedist = find_min_dist(*g);
+  finish_grid(*g, tol_fac);
+  return 0;
 }

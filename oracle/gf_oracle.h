@@ -49,6 +49,11 @@ OrGrid *or_grid_icos_create2(int p, int ilevel, int refine, double radius, doubl
  * inverse ------------------------------------------------------------------ */
 OrGrid *or_grid_terrain_create(int p, const int *ne, const double *P0, const double *P1,
                                const int *bdy, double amp, int ref_inverse, double tol_fac);
+/* new node coordinates for a 2-D deformed grid (no periodic directions):
+ * geometry, minnodedist, GGFX, mass, volume recomputed (add_terrain,
+ * ggrid.ipp:1522-1547) */
+int     or_grid_set_coords2d(OrGrid *g, const double *x, const double *y, int ref_inverse,
+                             double tol_fac);
 void    or_grid_destroy(OrGrid *g);
 
 size_t or_grid_ndof(const OrGrid *g);
