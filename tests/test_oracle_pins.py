@@ -363,3 +363,61 @@ def test_heat_mode_evolves_first_component_only():
     c1 = st[1].copy()
     g.burgers_step(st, 0.1, 1e-3, mode=2)
     assert np.array_equal(st[1], c1)
+
+
+def test_burgers_rhs_closed_form():
+    """The nonlinear, pure-advection and heat right-hand sides of
+    or_burgers_rhs, GGFX-smoothed as the stepper does, against the strong
+    form at the nodes for a smooth periodic field (spectral accuracy at p=10):
+    nonlinear dudt_c = -(u.grad) u_c + nu Lap u_c; advection dudt = -c.grad s
+    + nu Lap s; heat dudt_c = nu Lap u_c.  A dropped term, a wrong sign or a
+    transposed operand fails by O(1)."""
+    og = oracle.Grid.box(2, 10, [4, 4])
+    x, y = og.get("x"), og.get("y")
+    k, A, B, nu = 2 * np.pi, 0.7, 0.4, 0.03
+    sx, cx, sy, cy = np.sin(k * x), np.cos(k * x), np.sin(k * y), np.cos(k * y)
+    u0, u1 = A * sx * cy, B * cx * sy
+    u0x, u0y = A * k * cx * cy, -A * k * sx * sy
+    u1x, u1y = -B * k * sx * sy, B * k * cx * cy
+    want = [-(u0 * u0x + u1 * u0y) - nu * 2 * k * k * u0,
+            -(u0 * u1x + u1 * u1y) - nu * 2 * k * k * u1]
+    got = og.burgers_rhs([u0, u1], nu, mode=0)
+    scale = max(np.abs(w).max() for w in want)
+    for c in range(2):
+        err = np.abs(og.dss(got[c], "smooth") - want[c]).max() / scale
+        assert err <= 1e-7, (c, err)
+    # pure advection of s by the velocity (u0, u1)
+    s = 0.5 * cx * cy
+    sxd, syd = -0.5 * k * sx * cy, -0.5 * k * cx * sy
+    got = og.burgers_rhs([s, u0, u1], nu, mode=1)
+    want_s = -(u0 * sxd + u1 * syd) - nu * 2 * k * k * s
+    assert np.abs(og.dss(got[0], "smooth") - want_s).max() <= 1e-7 * np.abs(want_s).max()
+    # heat
+    got = og.burgers_rhs([u0, u1], nu, mode=2)
+    assert np.abs(og.dss(got[0], "smooth") + nu * 2 * k * k * u0).max() <= \
+        1e-7 * nu * 2 * k * k * A
+
+
+@pytest.mark.parametrize("dim,p,ne,bdy", [(2, 3, [2, 3], [6] * 4), (2, 4, [2, 2], [0] * 4),
+                                          (3, 2, [2, 2, 2], [0] * 6)])
+def test_jacobi_diagonal_by_unit_vectors(dim, p, ne, bdy):
+    """or_helmholtz_diag_inv (the build's Jacobi preconditioner) against the
+    assembled diagonal found by brute force: apply (p L + q M) to the
+    continuous unit field of each node group, assemble (GGFX Sum) and read
+    the group's own entry."""
+    og = oracle.Grid.box(dim, p, ne, bdy=bdy)
+    q = 1.0 + 0.5 * np.random.default_rng(3).random(og.ndof)
+    q = og.dss(q, "smooth")
+    dinv = og.helmholtz_diag_inv(0.3, q)
+    offs, mem = og.ggfx()
+    done = np.zeros(og.ndof, dtype=bool)
+    for l in range(og.ndof):
+        if done[l]:
+            continue
+        group = mem[offs[l]:offs[l + 1]]
+        e = np.zeros(og.ndof)
+        e[group] = 1.0
+        a = og.dss(og.helmholtz(e, 0.3, q), "sum")
+        for g in group:
+            assert abs(dinv[g] * a[g] - 1.0) <= 1e-12, (l, g)
+        done[group] = True
