@@ -17,11 +17,12 @@ from ._capi import (GF_OK, GF_BDY_DIRICHLET, GF_BDY_PERIODIC, GF_BDY_NONE, GF_DS
                     GF_DSS_SMOOTH, GF_DSS_MAX, GF_DSS_MIN, GF_NORM_INF, GF_NORM_EUC, GF_NORM_L2,
                     GF_NORM_L1, GF_NORM_ASSEMBLED_L2, GeoflowError, lib, lib_path)
 from .grid import (gll_basis, box_grid, icos_grid, terrain_box_grid, deformed_geometry2d,
-                   ggfx_build, BoxGrid, IcosGrid, TerrainBoxGrid, DeformedGrid2D)
+                   ggfx_build, BoxGrid, IcosGrid, TerrainBoxGrid, TerrainBoxGrid3D,
+                   DeformedGrid2D)
 
 __all__ = ["lib", "lib_path", "gll_basis", "box_grid", "icos_grid", "terrain_box_grid",
            "deformed_geometry2d", "ggfx_build", "BoxGrid", "IcosGrid", "TerrainBoxGrid",
-           "DeformedGrid2D", "GeoflowError", "Context", "Mesh"]
+           "TerrainBoxGrid3D", "DeformedGrid2D", "GeoflowError", "Context", "Mesh"]
 
 
 def __getattr__(name):

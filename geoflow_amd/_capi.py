@@ -73,6 +73,12 @@ SIGNATURES = {
                                     ctypes.POINTER(_dp), _dp, _dp]),
     "gf_deformed_geometry2d": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_dp),
                                               ctypes.c_int, ctypes.POINTER(_dp), _dp]),
+    "gf_deformed_geometry3d": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_dp),
+                                              ctypes.POINTER(_dp), _dp]),
+    "gf_terrain_box_grid3d": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp,
+                                             ctypes.POINTER(ctypes.c_int), ctypes.c_double,
+                                             ctypes.POINTER(_dp), ctypes.POINTER(_dp), _dp,
+                                             ctypes.POINTER(_dp), _dp, _dp]),
     "gf_terrain_box_grid": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp,
                                            ctypes.POINTER(ctypes.c_int), ctypes.c_double,
                                            ctypes.c_int, ctypes.POINTER(_dp), ctypes.POINTER(_dp),

@@ -44,7 +44,8 @@ struct gf_mesh_s {
   float *edx32 = nullptr, *ejac32 = nullptr;
   double *ndx[9] = {nullptr};  // embedded: dXi_i/dX_j at [i*3+j]
   double *njac = nullptr;
-  double *G[3] = {nullptr, nullptr, nullptr};  // embedded metric G00, G01, G11
+  // node metric G: 2-D G00, G01, G11; 3-D deformed G00, G01, G11, G02, G12, G22
+  double *G[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double *xyz[3] = {nullptr, nullptr, nullptr};
   double *mask = nullptr, *mass = nullptr, *imass = nullptr, *imult = nullptr;
   float *mask32 = nullptr, *imult32 = nullptr;

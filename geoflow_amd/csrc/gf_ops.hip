@@ -334,6 +334,74 @@ __global__ __launch_bounds__(kThreads) void k_helm_embed(
   }
 }
 
+// ---- 3-D deformed Helmholtz (def_prod, ghelmholtz.ipp:108-191) ------------
+// gdu_j = ((0 + G(0,j) D1 u) + G(1,j) D2 u) + G(2,j) D3 u ;
+// uo = p (((0 + D1^T gdu_0) + D2^T gdu_1) + D3^T gdu_2) + q M u,
+// with the six stored metric vectors (G00, G01, G11, G02, G12, G22).
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_helm_def3(
+    const double *__restrict__ u, double *__restrict__ uo, const double *__restrict__ D,
+    const double *__restrict__ G00, const double *__restrict__ G01,
+    const double *__restrict__ G11, const double *__restrict__ G02,
+    const double *__restrict__ G12, const double *__restrict__ G22,
+    const double *__restrict__ mass, double pscal, const double *__restrict__ pfield,
+    const double *__restrict__ qfield, double qscal, size_t nelems) {
+  using S = Shape<3, N>;
+  __shared__ double su[S::NODES];
+  __shared__ double sg[3][S::NODES];
+  __shared__ double sD[N * N];
+  size_t e0 = (size_t)blockIdx.x * S::EPB;
+  int nel = (int)min((size_t)S::EPB, nelems - e0);
+  size_t base = e0 * S::NN;
+  gf::stage_D(sD, D, N);
+  gf::stage(su, u, base, nel * S::NN);
+  __syncthreads();
+  for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x) {
+    int el = q / S::NN, r = q - el * S::NN, i, j, k;
+    gf::node_ijk<3, N>(r, i, j, k);
+    const double d0 = gf::contract<3, N, false>(su + el * S::NN, sD, 0, i, j, k);
+    const double d1 = gf::contract<3, N, false>(su + el * S::NN, sD, 1, i, j, k);
+    const double d2 = gf::contract<3, N, false>(su + el * S::NN, sD, 2, i, j, k);
+    const size_t g = base + q;
+    const double a00 = G00[g], a01 = G01[g], a11 = G11[g], a02 = G02[g], a12 = G12[g],
+                 a22 = G22[g];
+    double g0 = 0.0 + d0 * a00;
+    g0 += d1 * a01;
+    g0 += d2 * a02;
+    double g1 = 0.0 + d0 * a01;
+    g1 += d1 * a11;
+    g1 += d2 * a12;
+    double g2 = 0.0 + d0 * a02;
+    g2 += d1 * a12;
+    g2 += d2 * a22;
+    if (pfield) {
+      g0 *= pfield[g];
+      g1 *= pfield[g];
+      g2 *= pfield[g];
+    }
+    sg[0][q] = g0;
+    sg[1][q] = g1;
+    sg[2][q] = g2;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nel * S::NN; q += blockDim.x) {
+    int el = q / S::NN, r = q - el * S::NN, i, j, k;
+    gf::node_ijk<3, N>(r, i, j, k);
+    double acc = 0.0;
+    acc += gf::contract<3, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
+    acc += gf::contract<3, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
+    acc += gf::contract<3, N, true>(&sg[2][el * S::NN], sD, 2, i, j, k);
+    if (!pfield && pscal != 1.0) acc *= pscal;
+    if (qfield || qscal != 0.0) {
+      double t = mass[base + q] * su[q];
+      if (qfield) t *= qfield[base + q];
+      else if (qscal != 1.0) t *= qscal;
+      acc += t;
+    }
+    uo[base + q] = acc;
+  }
+}
+
 // ---- generic (anisotropic) derivative: one thread per node, operands from
 // global memory through the cache.
 __global__ void k_deriv_generic(const double *__restrict__ u, double *__restrict__ du,
@@ -531,10 +599,13 @@ int launch_helm(gf_mesh *m, double p, const double *pf, const double *qf, double
   if (m->gtype == GF_GE_REGULAR) {
     hipLaunchKernelGGL((k_helm<DIM, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, uo,
                        m->D[0], m->W[0], m->edx, m->ejac, p, pf, qf, qs, m->nelems);
-  } else {
-    if (DIM != 2) return GF_ERR_UNSUPPORTED;
+  } else if (DIM == 2) {
     hipLaunchKernelGGL((k_helm_embed<2, N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, uo,
                        m->D[0], m->G[0], m->G[1], m->G[2], m->mass, p, pf, qf, qs, m->nelems);
+  } else {
+    hipLaunchKernelGGL((k_helm_def3<N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, u, uo,
+                       m->D[0], m->G[0], m->G[1], m->G[2], m->G[3], m->G[4], m->G[5], m->mass, p,
+                       pf, qf, qs, m->nelems);
   }
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
@@ -643,8 +714,6 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     return gf_set_err(ctx, GF_ERR_ARG, "unknown element type");
   if (d->gtype == GF_GE_2DEMBEDDED && d->dim != 2)
     return gf_set_err(ctx, GF_ERR_ARG, "embedded meshes are 2-D");
-  if (d->gtype == GF_GE_DEFORMED && d->dim != 2)
-    return gf_set_err(ctx, GF_ERR_UNSUPPORTED, "deformed elements: 2-D meshes only");
   if (!d->ggfx_offs || !d->ggfx_members) return gf_set_err(ctx, GF_ERR_ARG, "missing GGFX maps");
   GF_HIP(ctx, hipSetDevice(ctx->device));
   gf_mesh *m = new gf_mesh_s();
@@ -727,22 +796,22 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
       gf_mesh_destroy(m);
       return gf_set_err(ctx, GF_ERR_ARG, "embedded or deformed mesh needs node metrics");
     }
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < d->dim; i++)
       for (int j = 0; j < nxy; j++) {
         if (!d->node_dxidx[i * 3 + j]) {
           gf_mesh_destroy(m);
-          return gf_set_err(ctx, GF_ERR_ARG, "node metrics need dXi_i/dX_j for i<2, j<nxy");
+          return gf_set_err(ctx, GF_ERR_ARG, "node metrics need dXi_i/dX_j for i<dim, j<nxy");
         }
         UP(m->ndx[i * 3 + j], d->node_dxidx[i * 3 + j], n);
       }
     UP(m->njac, d->node_jac, n);
     // Metric G(i,j) = sum_{k<nxy} dXidX(i,k) dXidX(j,k) W0 W1 Jac (def_init
     // 2-D, src/cdg/sem/ghelmholtz.ipp:476-491), built on the host in that order.
-    std::vector<double> G[3];
+    std::vector<double> G[6];
     int Ni = m->N[0], Nj = m->N[1];
     for (int c = 0; c < 3; c++) G[c].assign(n, 0.0);
     const int ii[3] = {0, 0, 1}, jj[3] = {0, 1, 1};
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < (d->dim == 2 ? 3 : 0); c++) {
       std::fill(G[c].begin(), G[c].end(), 0.0);
       size_t q = 0;
       for (size_t e = 0; e < m->nelems; e++)
@@ -752,6 +821,31 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
               G[c][q] += d->node_dxidx[ii[c] * 3 + k][q] * d->node_dxidx[jj[c] * 3 + k][q] *
                          d->W[0][l] * d->W[1][mm] * d->node_jac[q];
       UP(m->G[c], G[c].data(), n);
+    }
+    if (d->dim == 3) {
+      // def_init 3-D (ghelmholtz.ipp:494-514): both loops run over all (i, j)
+      // while G(i,j) and G(j,i) share one vector, so each off-diagonal vector
+      // receives its terms twice, in this order of passes.  Slots:
+      // 0 G00, 1 G01, 2 G11, 3 G02, 4 G12, 5 G22.
+      auto slot = [](int i, int j) {
+        const int a = i < j ? i : j, b = i < j ? j : i;
+        return a == b ? (a == 0 ? 0 : a == 1 ? 2 : 5) : (b == 1 ? 1 : a == 0 ? 3 : 4);
+      };
+      const int Nk = m->N[2];
+      for (int c = 0; c < 6; c++) G[c].assign(n, 0.0);
+      for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+          std::vector<double> &Gv = G[slot(i, j)];
+          size_t q = 0;
+          for (size_t e = 0; e < m->nelems; e++)
+            for (int pp = 0; pp < Nk; pp++)
+              for (int mm = 0; mm < Nj; mm++)
+                for (int l = 0; l < Ni; l++, q++)
+                  for (int k = 0; k < 3; k++)
+                    Gv[q] += d->node_dxidx[i * 3 + k][q] * d->node_dxidx[j * 3 + k][q] *
+                             d->W[0][l] * d->W[1][mm] * d->W[2][pp] * d->node_jac[q];
+        }
+      for (int c = 0; c < 6; c++) UP(m->G[c], G[c].data(), n);
     }
   }
   for (int k = 0; k < 3; k++)
@@ -891,9 +985,9 @@ int gf_mesh_destroy(gf_mesh *m) {
   for (int k = 0; k < 3; k++) {
     F(m->D[k]);
     F(m->W[k]);
-    F(m->G[k]);
     F(m->xyz[k]);
   }
+  for (int k = 0; k < 6; k++) F(m->G[k]);
   for (int k = 0; k < 9; k++) F(m->ndx[k]);
   for (int k = 0; k < 3; k++) {
     F(m->pDR[k]);
@@ -969,7 +1063,8 @@ int gf_deriv(gf_mesh *m, int idir, const double *u, double *du) {
     if (idir < 1 || idir > m->dim) return gf_set_err(m->ctx, GF_ERR_ARG, "idir out of range");
     return gf_grefderiv_t(m, idir, 0, u, du, m->edx);
   }
-  // embedded or deformed: du = (D1 u) dXidX(0,idir-1) + (D2 u) dXidX(1,idir-1)
+  // embedded or deformed (GGrid::deriv, ggrid.ipp:1179-1189):
+  // du = (D1 u) dXidX(0,idir-1); du += (D_j u) dXidX(j-1,idir-1), j = 2..dim
   if (idir < 1 || idir > (m->gtype == GF_GE_2DEMBEDDED ? 3 : m->dim))
     return gf_set_err(m->ctx, GF_ERR_ARG, "idir out of range");
   double *t = (double *)gf_ws(m, WS_TMP0, m->ndof * sizeof(double));
@@ -979,11 +1074,15 @@ int gf_deriv(gf_mesh *m, int idir, const double *u, double *du) {
   if (rc) return rc;
   rc = gf_mul_t(m, du, m->ndx[0 * 3 + idir - 1], du);
   if (rc) return rc;
-  rc = gf_grefderiv_t(m, 2, 0, u, t, nullptr);
-  if (rc) return rc;
-  rc = gf_mul_t(m, t, m->ndx[1 * 3 + idir - 1], t);
-  if (rc) return rc;
-  return gf_axpy_t(m, 1.0, t, du);
+  for (int j = 1; j < m->dim; j++) {
+    rc = gf_grefderiv_t(m, j + 1, 0, u, t, nullptr);
+    if (rc) return rc;
+    rc = gf_mul_t(m, t, m->ndx[j * 3 + idir - 1], t);
+    if (rc) return rc;
+    rc = gf_axpy_t(m, 1.0, t, du);
+    if (rc) return rc;
+  }
+  return GF_OK;
 }
 
 int gf_grad(gf_mesh *m, const double *u, double *const *du) {

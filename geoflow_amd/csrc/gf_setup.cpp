@@ -802,6 +802,113 @@ static double min_dist2d(const double *const *xyz, size_t ndof) {
   return dmin;
 }
 
+// GE_DEFORMED geometry of 3-D elements from their node coordinates:
+// GElem_base::dogeom3d (src/cdg/sem/gelem_base.cpp:972-1048): R(l,m) =
+// dx_l/dxi_m from the tensor-product GLL shape-function derivatives at the
+// nodes, jac = |det R| (GElem_base::Jac, 3x3 branch, gelem_base.cpp:1148-1168)
+// and dXi/dX = R^-1 by cofactors over det R (GElem_base::inv,
+// gelem_base.cpp:1268-1283; the 3x3 branch is the true inverse).
+//   xyz[3]: nelems * (p+1)^3 coordinates; dxidx[9] at [i*3+j] = dXi_i/dX_j.
+int gf_deformed_geometry3d(int p, size_t nelems, const double *const *xyz,
+                           double *const *dxidx, double *jac) {
+  if (p < 1 || !xyz || !xyz[0] || !xyz[1] || !xyz[2] || !dxidx || !jac) return GF_ERR_ARG;
+  for (int k = 0; k < 9; k++)
+    if (!dxidx[k]) return GF_ERR_ARG;
+  const int N = p + 1;
+  const LDBasis lb = make_ldbasis(p);
+  std::vector<double> hb((size_t)N * N), hd((size_t)N * N);  // [basis i][node q]
+  for (int i = 0; i < N; i++)
+    for (int q = 0; q < N; q++) {
+      hb[(size_t)i * N + q] = eval_basis(lb, i, lb.xd[q]);
+      hd[(size_t)i * N + q] = eval_dbasis(lb, i, lb.xd[q]);
+    }
+  const size_t nn = (size_t)N * N * N;
+  for (size_t e = 0; e < nelems; e++) {
+    const size_t base = e * nn;
+    for (int qk = 0, q = 0; qk < N; qk++)
+      for (int qj = 0; qj < N; qj++)
+        for (int qi = 0; qi < N; qi++, q++) {
+          double r[3][3];
+          for (int m = 0; m < 3; m++)
+            for (int l = 0; l < 3; l++) {
+              // R(l,m) = sum_I dN_I/dxi_m x_l[I], I in (k, j, i) order
+              double tmp = 0.0;
+              for (int bk = 0, n = 0; bk < N; bk++)
+                for (int bj = 0; bj < N; bj++)
+                  for (int bi = 0; bi < N; bi++, n++) {
+                    const double d0 = m == 0 ? hd[(size_t)bi * N + qi] : hb[(size_t)bi * N + qi];
+                    const double d1 = m == 1 ? hd[(size_t)bj * N + qj] : hb[(size_t)bj * N + qj];
+                    const double d2 = m == 2 ? hd[(size_t)bk * N + qk] : hb[(size_t)bk * N + qk];
+                    double dn = d0 * d1 * d2;
+                    dn *= xyz[l][base + n];
+                    tmp += dn;
+                  }
+              r[l][m] = tmp;
+            }
+          auto G = [&](int a, int b) { return r[a][b]; };
+          const double det = G(0, 0) * (G(1, 1) * G(2, 2) - G(1, 2) * G(2, 1)) -
+                             G(0, 1) * (G(1, 0) * G(2, 2) - G(1, 2) * G(2, 0)) +
+                             G(0, 2) * (G(1, 0) * G(2, 1) - G(1, 1) * G(2, 0));
+          const size_t id = base + q;
+          jac[id] = std::fabs(det);
+          const double ij = 1.0 / det;
+          dxidx[0][id] = (G(1, 1) * G(2, 2) - G(1, 2) * G(2, 1)) * ij;
+          dxidx[1][id] = -(G(0, 1) * G(2, 2) - G(2, 1) * G(0, 2)) * ij;
+          dxidx[2][id] = (G(0, 1) * G(1, 2) - G(0, 2) * G(1, 1)) * ij;
+          dxidx[3][id] = -(G(1, 0) * G(2, 2) - G(2, 0) * G(1, 2)) * ij;
+          dxidx[4][id] = (G(0, 0) * G(2, 2) - G(2, 0) * G(0, 2)) * ij;
+          dxidx[5][id] = -(G(0, 0) * G(1, 2) - G(0, 2) * G(1, 0)) * ij;
+          dxidx[6][id] = (G(1, 0) * G(2, 1) - G(1, 1) * G(2, 0)) * ij;
+          dxidx[7][id] = -(G(0, 0) * G(2, 1) - G(0, 1) * G(2, 0)) * ij;
+          dxidx[8][id] = (G(0, 0) * G(1, 1) - G(1, 0) * G(0, 1)) * ij;
+        }
+  }
+  return GF_OK;
+}
+
+// 3-D box of GE_DEFORMED hexahedra whose bottom (z = P0z) follows
+//   h(x,y) = amp (1 - cos(2 pi (x-P0x)/Lx)) (1 - cos(2 pi (y-P0y)/Ly)) / 4,
+// every node moved by z' = z + h(x,y) (P1z - z)/(P1z - P0z); arguments as
+// gf_terrain_box_grid with 3 directions (z may not be periodic) and
+// dxidx[9] at [i*3+j]; geometry as gf_deformed_geometry3d.
+int gf_terrain_box_grid3d(int p, const int *ne, const double *P0, const double *P1,
+                          const int *bdy, double amp, double *const *xyz, double *const *xyzp,
+                          double *mask, double *const *dxidx, double *jac, double *minnodedist) {
+  if (p < 1 || !ne || !P0 || !P1 || !bdy || !xyz || !xyz[0] || !xyz[1] || !xyz[2] || !dxidx ||
+      !jac)
+    return GF_ERR_ARG;
+  if (bdy[4] == GF_BDY_PERIODIC || bdy[5] == GF_BDY_PERIODIC) return GF_ERR_ARG;
+  const int pp[3] = {p, p, p};
+  int rc = gf_box_grid(3, pp, ne, P0, P1, bdy, xyz, xyzp, mask, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  const int N = p + 1;
+  const double Lx = P1[0] - P0[0], Ly = P1[1] - P0[1], H = P1[2] - P0[2];
+  const size_t nel = (size_t)ne[0] * ne[1] * ne[2], ndof = nel * N * N * N;
+  auto move = [&](double *const *c) {
+    for (size_t q = 0; q < ndof; q++) {
+      const double h = amp * 0.25 * (1.0 - std::cos(2.0 * kPi * (c[0][q] - P0[0]) / Lx)) *
+                       (1.0 - std::cos(2.0 * kPi * (c[1][q] - P0[1]) / Ly));
+      c[2][q] = c[2][q] + h * (P1[2] - c[2][q]) / H;
+    }
+  };
+  move(xyz);
+  if (xyzp && xyzp[0] && xyzp[1] && xyzp[2]) move(xyzp);
+  rc = gf_deformed_geometry3d(p, nel, xyz, dxidx, jac);
+  if (rc) return rc;
+  if (minnodedist) {  // GGrid::find_min_dist (ggrid.ipp:977-1013)
+    const double tiny = 1000.0 * std::numeric_limits<double>::epsilon();
+    double dmin = std::numeric_limits<double>::max();
+    for (size_t q = 0; q + 1 < ndof; q++) {
+      const double dx = xyz[0][q + 1] - xyz[0][q], dy = xyz[1][q + 1] - xyz[1][q],
+                   dz = xyz[2][q + 1] - xyz[2][q];
+      const double rr = std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (rr > tiny) dmin = std::min(dmin, rr);
+    }
+    *minnodedist = dmin;
+  }
+  return GF_OK;
+}
+
 // 2-D box of GE_DEFORMED quads following a terrain profile: the nodes of the
 // regular box (gf_box_grid) are moved by the terrain-following map
 //   y' = y + h(x) (P1y - y) / (P1y - P0y),  h(x) = amp (1 - cos(2 pi (x - P0x) / Lx)) / 2,

@@ -1360,9 +1360,9 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
   if (!m || !prm || !b || !x || !res) return GF_ERR_ARG;
   memset(res, 0, sizeof(*res));
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: isotropic meshes");
-  if (m->gtype != GF_GE_REGULAR && (prm->precision != 0 || prm->precond))
+  if (m->gtype != GF_GE_REGULAR && (prm->precision != 0 || prm->precond || m->dim != 2))
     return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED,
-                      "CG on embedded or deformed meshes: FP64 without preconditioner");
+                      "CG on embedded or deformed meshes: 2-D, FP64 without preconditioner");
   if (prm->maxit < 1) return gf_set_err(m->ctx, GF_ERR_BADITERNUM, "maxit < 1");
   if (prm->normtype < GF_NORM_INF || prm->normtype > GF_NORM_ASSEMBLED_L2) return GF_ERR_ARG;
   if (prm->p_field) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: p field not supported");

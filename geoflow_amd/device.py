@@ -108,7 +108,7 @@ class Mesh:
             d.elem_dxidx = host(grid.edx)
             d.elem_jac = host(grid.ejac)
         else:
-            for i in range(2):
+            for i in range(len(node_dxidx)):
                 for j in range(len(node_dxidx[i])):
                     d.node_dxidx[i * 3 + j] = host(node_dxidx[i][j])
             d.node_jac = host(node_jac)

@@ -187,6 +187,48 @@ class TerrainBoxGrid:
         self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyzp, self.tol, max_dup=4)
 
 
+class TerrainBoxGrid3D:
+    """3-D box [P0, P1] of GE_DEFORMED hexahedra with the bottom raised to
+    h(x,y) = amp (1 - cos(2 pi x/Lx)) (1 - cos(2 pi y/Ly)) / 4 and every node
+    moved by z' = z + h (P1z - z)/(P1z - P0z); per-node metrics of
+    GElem_base::dogeom3d (3x3 inverse, the true one)."""
+
+    gtype = GF_GE_DEFORMED
+
+    def __init__(self, p, ne, amp=0.1, P0=(0.0, 0.0, 0.0), P1=(1.0, 1.0, 1.0), bdy=None,
+                 tol_fac=0.001):
+        if bdy is None:
+            bdy = [GF_BDY_PERIODIC] * 4 + [GF_BDY_DIRICHLET] * 2
+        ne = [int(q) for q in ne]
+        nel = ne[0] * ne[1] * ne[2]
+        ndof = nel * (p + 1) ** 3
+        xyz = [np.zeros(ndof) for _ in range(3)]
+        xyzp = [np.zeros(ndof) for _ in range(3)]
+        mask = np.zeros(ndof)
+        dxidx = [[np.zeros(ndof) for _ in range(3)] for _ in range(3)]
+        jac = np.zeros(ndof)
+        md = ctypes.c_double(0)
+        ia = lambda v: (ctypes.c_int * len(v))(*v)
+        P0a = np.ascontiguousarray(P0[:3], dtype=np.float64)
+        P1a = np.ascontiguousarray(P1[:3], dtype=np.float64)
+        check(lib().gf_terrain_box_grid3d(
+            p, ia(ne), _p(P0a), _p(P1a), ia(list(bdy)), float(amp),
+            (_dp * 3)(*[_p(a) for a in xyz]), (_dp * 3)(*[_p(a) for a in xyzp]), _p(mask),
+            (_dp * 9)(*[_p(dxidx[i][j]) for i in range(3) for j in range(3)]), _p(jac),
+            ctypes.byref(md)))
+        self.dim = 3
+        self.xyz, self.xyzp, self.mask = xyz, xyzp, mask
+        self.node_dxidx, self.node_jac = dxidx, jac
+        self.minnodedist = md.value
+        self.ndof, self.nelems = ndof, nel
+        self.p, self.N, self.ne = [p] * 3, [p + 1] * 3, ne
+        self.bdy, self.amp = list(bdy), amp
+        self.P0, self.P1 = P0a, P1a
+        self.basis = [gll_basis(p)] * 3
+        self.tol = tol_fac * self.minnodedist
+        self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyzp, self.tol, max_dup=8)
+
+
 class DeformedGrid2D:
     """2-D GE_DEFORMED grid from explicit node coordinates on the elements of
     a box grid (what GGrid::add_terrain leaves, src/cdg/grid/ggrid.ipp:1522-1547):

@@ -131,6 +131,20 @@ int gf_terrain_box_grid(int p, const int *ne, const double *P0, const double *P1
 int gf_deformed_geometry2d(int p, size_t nelems, const double *const *xyz, int reference_inverse,
                            double *const *dxidx, double *jac);
 
+/* GE_DEFORMED geometry of 3-D elements from their node coordinates
+ * (GElem_base::dogeom3d / Jac / inv, src/cdg/sem/gelem_base.cpp:972-1048,
+ * 1148-1168, 1268-1283): dxidx[9] at [i*3 + j] = dXi_i/dX_j (the 3x3 inverse
+ * is the true one), jac = |det(dx/dxi)|. */
+int gf_deformed_geometry3d(int p, size_t nelems, const double *const *xyz,
+                           double *const *dxidx, double *jac);
+/* 3-D box of GE_DEFORMED hexahedra, bottom raised to
+ * h(x,y) = amp (1 - cos(2 pi (x-P0x)/Lx)) (1 - cos(2 pi (y-P0y)/Ly)) / 4 by
+ * z' = z + h (P1z - z)/(P1z - P0z); arguments as gf_terrain_box_grid in 3-D
+ * (z not periodic), dxidx[9] at [i*3 + j]. */
+int gf_terrain_box_grid3d(int p, const int *ne, const double *P0, const double *P1,
+                          const int *bdy, double amp, double *const *xyz, double *const *xyzp,
+                          double *mask, double *const *dxidx, double *jac, double *minnodedist);
+
 /* GGFX::init local coincidence matching (src/cdg/comm/ggfx.hpp:167-342) on
  * host coordinates xyz[ncoord][ndof] with box tolerance tol.  Produces the
  * group of every node as CSR: members[offs[n] .. offs[n+1]) ascending.

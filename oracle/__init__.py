@@ -53,6 +53,10 @@ def lib():
         L.or_grid_terrain_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp,
                                              ctypes.POINTER(ctypes.c_int), ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double]
+        L.or_grid_terrain3_create.restype = ctypes.c_void_p
+        L.or_grid_terrain3_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp,
+                                              _dp, ctypes.POINTER(ctypes.c_int), ctypes.c_double,
+                                              ctypes.c_double]
         L.or_grid_set_coords2d.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double]
         L.or_burgers_rhs_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
                                              ctypes.POINTER(_dp), ctypes.POINTER(_dp),
@@ -209,6 +213,21 @@ class Grid:
         g = cls(lib().or_grid_terrain_create(p, na, _p(P0), _p(P1), ba, amp, int(bool(ref_inverse)),
                                              tol_fac))
         g.p = [p, p]
+        g.ne = list(ne)
+        return g
+
+    @classmethod
+    def terrain3(cls, p, ne, amp, P0=(0.0, 0.0, 0.0), P1=(1.0, 1.0, 1.0), bdy=None,
+                 tol_fac=0.001):
+        """3-D box of GE_DEFORMED hexahedra, bottom raised to
+        h = amp (1 - cos(2 pi x/Lx))(1 - cos(2 pi y/Ly))/4."""
+        if bdy is None:
+            bdy = [BDY_PERIODIC] * 4 + [0, 0]
+        P0 = np.ascontiguousarray(P0, dtype=np.float64)
+        P1 = np.ascontiguousarray(P1, dtype=np.float64)
+        g = cls(lib().or_grid_terrain3_create(p, (ctypes.c_int * 3)(*ne), _p(P0), _p(P1),
+                                              (ctypes.c_int * 6)(*bdy), amp, tol_fac))
+        g.p = [p] * 3
         g.ne = list(ne)
         return g
 

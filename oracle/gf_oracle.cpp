@@ -1148,6 +1148,26 @@ void finish_grid(OrGrid &g, double tol_fac) {
 void or_embed_G(OrGrid *g, int i, int j, std::vector<double> &G) {
   size_t n = g->ndof;
   G.assign(n, 0.0);
+  if (g->dim == 3) {
+    // def_init 3-D (ghelmholtz.ipp:494-514): passes over all (ii, jj) in
+    // (jj outer, ii inner) order, with G(ii,jj) and G(jj,ii) one vector, so
+    // the off-diagonal vector of (i, j) takes its terms in the (min, max) pass
+    // order of both members.
+    auto same = [&](int a, int b) { return (a == i && b == j) || (a == j && b == i); };
+    for (int jj = 0; jj < 3; jj++)
+      for (int ii = 0; ii < 3; ii++) {
+        if (!same(ii, jj)) continue;
+        size_t q = 0;
+        for (size_t e = 0; e < g->nelems; e++)
+          for (int pp = 0; pp < g->N[2]; pp++)
+            for (int m = 0; m < g->N[1]; m++)
+              for (int l = 0; l < g->N[0]; l++, q++)
+                for (int k = 0; k < 3; k++)
+                  G[q] += g->dxidx[ii][k][q] * g->dxidx[jj][k][q] * g->basis[0].w[l] *
+                          g->basis[1].w[m] * g->basis[2].w[pp] * g->jac[q];
+      }
+    return;
+  }
   int nxy = g->nxy();
   size_t q = 0;
   for (size_t e = 0; e < g->nelems; e++)
@@ -1595,6 +1615,92 @@ extern "C" OrGrid *or_grid_terrain_create(int p, const int *ne, const double *P0
       c[1][q] = c[1][q] + h * (P1[1] - c[1][q]) / H;
     }
   deformed_geom2d(g, ref_inverse);
+  g.minnodedist = find_min_dist(g);
+  finish_grid(g, tol_fac);
+  return gp;
+}
+
+namespace {
+// GElem_base::dogeom3d for GE_DEFORMED (gelem_base.cpp:972-1048) from g.xn:
+// rij(l,m) = sum_I dN_I/dxi^m x^l_I with the tensor-product shape functions
+// (factors multiplied x, y, z), Jac = |det rij| (gelem_base.cpp:1148-1168) and
+// the 3x3 branch of GElem_base::inv (cofactors over det, gelem_base.cpp:1268-1283).
+void deformed_geom3d(OrGrid &g) {
+  using namespace icos;
+  const int N = g.N[0];
+  const size_t nn = (size_t)N * N * N;
+  const Basis &B = g.basis[0];
+  std::vector<double> h0((size_t)N * N), dh0((size_t)N * N);
+  for (int i = 0; i < N; i++)
+    for (int q = 0; q < N; q++) {
+      h0[(size_t)i * N + q] = evalBasis(B, i, B.xi[q]);
+      dh0[(size_t)i * N + q] = evalDBasis(B, i, B.xi[q]);
+    }
+  std::vector<double> dNi(nn), tmp(nn), rij[3][3];
+  for (size_t e = 0; e < g.nelems; e++) {
+    const size_t b0 = e * nn;
+    for (int m = 0; m < 3; m++)
+      for (int l = 0; l < 3; l++) {
+        tmp.assign(nn, 0.0);
+        for (int kb = 0, nI = 0; kb < N; kb++)
+          for (int jb = 0; jb < N; jb++)
+            for (int ib = 0; ib < N; ib++, nI++) {
+              const double *d0 = (m == 0 ? dh0 : h0).data() + (size_t)ib * N;
+              const double *d1 = (m == 1 ? dh0 : h0).data() + (size_t)jb * N;
+              const double *d2 = (m == 2 ? dh0 : h0).data() + (size_t)kb * N;
+              for (int qk = 0, q = 0; qk < N; qk++)
+                for (int qj = 0; qj < N; qj++)
+                  for (int qi = 0; qi < N; qi++, q++) dNi[q] = d0[qi] * d1[qj] * d2[qk];
+              double xl = g.xn[l][b0 + nI];
+              for (size_t q = 0; q < nn; q++) {
+                dNi[q] *= xl;
+                tmp[q] += dNi[q];
+              }
+            }
+        rij[l][m] = tmp;
+      }
+    for (size_t q = 0; q < nn; q++) {
+      auto G = [&](int a, int b) { return rij[a][b][q]; };
+      double det = G(0, 0) * (G(1, 1) * G(2, 2) - G(1, 2) * G(2, 1)) -
+                   G(0, 1) * (G(1, 0) * G(2, 2) - G(1, 2) * G(2, 0)) +
+                   G(0, 2) * (G(1, 0) * G(2, 1) - G(1, 1) * G(2, 0));
+      size_t id = b0 + q;
+      g.jac[id] = std::fabs(det);
+      double ijac = 1.0 / det;
+      g.dxidx[0][0][id] = (G(1, 1) * G(2, 2) - G(1, 2) * G(2, 1)) * ijac;
+      g.dxidx[0][1][id] = -(G(0, 1) * G(2, 2) - G(2, 1) * G(0, 2)) * ijac;
+      g.dxidx[0][2][id] = (G(0, 1) * G(1, 2) - G(0, 2) * G(1, 1)) * ijac;
+      g.dxidx[1][0][id] = -(G(1, 0) * G(2, 2) - G(2, 0) * G(1, 2)) * ijac;
+      g.dxidx[1][1][id] = (G(0, 0) * G(2, 2) - G(2, 0) * G(0, 2)) * ijac;
+      g.dxidx[1][2][id] = -(G(0, 0) * G(1, 2) - G(0, 2) * G(1, 0)) * ijac;
+      g.dxidx[2][0][id] = (G(1, 0) * G(2, 1) - G(1, 1) * G(2, 0)) * ijac;
+      g.dxidx[2][1][id] = -(G(0, 0) * G(2, 1) - G(0, 1) * G(2, 0)) * ijac;
+      g.dxidx[2][2][id] = (G(0, 0) * G(1, 1) - G(1, 0) * G(0, 1)) * ijac;
+    }
+  }
+}
+}  // namespace
+
+// 3-D box of GE_DEFORMED hexahedra: the regular box with every node moved by
+// z' = z + h(x,y)(P1z - z)/H, h = amp (1 - cos(2 pi (x-P0x)/Lx))(1 - cos(2 pi
+// (y-P0y)/Ly))/4, and the deformed geometry of those nodes.
+extern "C" OrGrid *or_grid_terrain3_create(int p, const int *ne, const double *P0,
+                                           const double *P1, const int *bdy, double amp,
+                                           double tol_fac) {
+  using namespace icos;
+  const int pp[3] = {p, p, p};
+  OrGrid *gp = or_grid_box_create(3, pp, ne, P0, P1, bdy, tol_fac);
+  if (!gp) return nullptr;
+  OrGrid &g = *gp;
+  g.gtype = 1;
+  const double Lx = P1[0] - P0[0], Ly = P1[1] - P0[1], H = P1[2] - P0[2];
+  for (std::vector<double> *c : {g.xn, g.xp})
+    for (size_t q = 0; q < g.ndof; q++) {
+      double h = amp * 0.25 * (1.0 - std::cos(2.0 * PI_ * (c[0][q] - P0[0]) / Lx)) *
+                 (1.0 - std::cos(2.0 * PI_ * (c[1][q] - P0[1]) / Ly));
+      c[2][q] = c[2][q] + h * (P1[2] - c[2][q]) / H;
+    }
+  deformed_geom3d(g);
   g.minnodedist = find_min_dist(g);
   finish_grid(g, tol_fac);
   return gp;

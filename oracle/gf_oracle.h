@@ -52,6 +52,10 @@ OrGrid *or_grid_terrain_create(int p, const int *ne, const double *P0, const dou
 /* new node coordinates for a 2-D deformed grid (no periodic directions):
  * geometry, minnodedist, GGFX, mass, volume recomputed (add_terrain,
  * ggrid.ipp:1522-1547) */
+/* 3-D terrain-following GE_DEFORMED box (dogeom3d/Jac/inv,
+ * gelem_base.cpp:972-1048, 1148-1168, 1268-1283) */
+OrGrid *or_grid_terrain3_create(int p, const int *ne, const double *P0, const double *P1,
+                                const int *bdy, double amp, double tol_fac);
 int     or_grid_set_coords2d(OrGrid *g, const double *x, const double *y, int ref_inverse,
                              double tol_fac);
 void    or_grid_destroy(OrGrid *g);
