@@ -43,6 +43,12 @@ class MeshDesc(ctypes.Structure):
                 ("box_ne", ctypes.c_int * 3), ("box_periodic", ctypes.c_int * 3)]
 
 
+class GioInfo(ctypes.Structure):
+    _fields_ = [("ivers", ctypes.c_int), ("dim", ctypes.c_int), ("nelems", ctypes.c_uint64),
+                ("gtype", ctypes.c_int), ("cycle", ctypes.c_uint64), ("time", ctypes.c_double),
+                ("multivar", ctypes.c_int), ("nporder", ctypes.c_uint64)]
+
+
 class CGParams(ctypes.Structure):
     _fields_ = [("maxit", ctypes.c_int), ("tol", ctypes.c_double), ("normtype", ctypes.c_int),
                 ("precond", ctypes.c_int), ("precision", ctypes.c_int),
@@ -73,6 +79,13 @@ SIGNATURES = {
                                     ctypes.POINTER(_dp), _dp, _dp]),
     "gf_deformed_geometry2d": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_dp),
                                               ctypes.c_int, ctypes.POINTER(_dp), _dp]),
+    "gf_gio_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(GioInfo),
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64),
+                                    ctypes.POINTER(_dp), ctypes.c_int, ctypes.c_size_t]),
+    "gf_gio_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(GioInfo),
+                                   ctypes.POINTER(ctypes.c_int), ctypes.c_size_t,
+                                   ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t,
+                                   ctypes.POINTER(_dp), ctypes.c_int, ctypes.c_size_t]),
     "gf_deformed_geometry3d": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_dp),
                                               ctypes.POINTER(_dp), _dp]),
     "gf_terrain_box_grid3d": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp,

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -800,6 +801,80 @@ static double min_dist2d(const double *const *xyz, size_t ndof) {
     if (rr > tiny) dmin = std::min(dmin, rr);
   }
   return dmin;
+}
+
+// GIO state files (src/cdg/io/gio.ipp): the header of GIO::read_header
+// (gio.ipp:588-640) -- int ivers, int dim, size_t nelems, int porder[rows*dim]
+// (rows = 1 for ivers 0, else nelems), int gtype, size_t cycle, double time,
+// int multivar, size_t keys[nelems] -- of GIO::sz_header bytes
+// (gio.ipp:678-690), followed by the fields' doubles (one field, or all of
+// them in order when multivar is set).  Native byte order, as fwrite.
+int gf_gio_write(const char *path, const gf_gio_info *info, const int *porder,
+                 const uint64_t *keys, const double *const *fields, int nfields, size_t npts) {
+  if (!path || !info || !porder || !fields || nfields < 1 || info->dim < 1 || info->dim > 3)
+    return GF_ERR_ARG;
+  if (!info->multivar && nfields != 1) return GF_ERR_ARG;
+  for (int f = 0; f < nfields; f++)
+    if (!fields[f]) return GF_ERR_ARG;
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) return GF_ERR_ARG;
+  const int ivers = info->ivers, dim = info->dim, gtype = info->gtype, multi = info->multivar;
+  const size_t nelems = (size_t)info->nelems, cycle = (size_t)info->cycle;
+  const double time = info->time;
+  const size_t nrow = ivers == 0 ? 1 : nelems;
+  bool ok = std::fwrite(&ivers, sizeof(int), 1, fp) == 1 && std::fwrite(&dim, sizeof(int), 1, fp) == 1 &&
+            std::fwrite(&nelems, sizeof(size_t), 1, fp) == 1 &&
+            std::fwrite(porder, sizeof(int), nrow * dim, fp) == nrow * dim &&
+            std::fwrite(&gtype, sizeof(int), 1, fp) == 1 &&
+            std::fwrite(&cycle, sizeof(size_t), 1, fp) == 1 &&
+            std::fwrite(&time, sizeof(double), 1, fp) == 1 &&
+            std::fwrite(&multi, sizeof(int), 1, fp) == 1;
+  for (size_t e = 0; ok && e < nelems; e++) {
+    const size_t k = keys ? (size_t)keys[e] : e;
+    ok = std::fwrite(&k, sizeof(size_t), 1, fp) == 1;
+  }
+  for (int f = 0; ok && f < nfields; f++) ok = std::fwrite(fields[f], sizeof(double), npts, fp) == npts;
+  ok = (std::fclose(fp) == 0) && ok;
+  return ok ? GF_OK : GF_ERR_ARG;
+}
+
+// Reads a GIO file written as above.  Call with fields == NULL to read the
+// header only (porder_cap / keys_cap bound the arrays, which may be NULL);
+// then with nfields field buffers of npts doubles.
+int gf_gio_read(const char *path, gf_gio_info *info, int *porder, size_t porder_cap,
+                uint64_t *keys, size_t keys_cap, double *const *fields, int nfields, size_t npts) {
+  if (!path || !info) return GF_ERR_ARG;
+  FILE *fp = std::fopen(path, "rb");
+  if (!fp) return GF_ERR_ARG;
+  int ivers = 0, dim = 0, gtype = 0, multi = 0;
+  size_t nelems = 0, cycle = 0;
+  double time = 0.0;
+  bool ok = std::fread(&ivers, sizeof(int), 1, fp) == 1 && std::fread(&dim, sizeof(int), 1, fp) == 1 &&
+            std::fread(&nelems, sizeof(size_t), 1, fp) == 1 && dim >= 1 && dim <= 3;
+  const size_t nrow = ok ? (ivers == 0 ? 1 : nelems) : 0;
+  std::vector<int> po(ok ? nrow * dim : 0);
+  ok = ok && std::fread(po.data(), sizeof(int), po.size(), fp) == po.size() &&
+       std::fread(&gtype, sizeof(int), 1, fp) == 1 && std::fread(&cycle, sizeof(size_t), 1, fp) == 1 &&
+       std::fread(&time, sizeof(double), 1, fp) == 1 && std::fread(&multi, sizeof(int), 1, fp) == 1;
+  std::vector<size_t> ks(ok ? nelems : 0);
+  ok = ok && std::fread(ks.data(), sizeof(size_t), ks.size(), fp) == ks.size();
+  if (ok) {
+    info->ivers = ivers;
+    info->dim = dim;
+    info->nelems = nelems;
+    info->gtype = gtype;
+    info->cycle = cycle;
+    info->time = time;
+    info->multivar = multi;
+    info->nporder = po.size();
+    if (porder) std::copy(po.begin(), po.begin() + std::min(porder_cap, po.size()), porder);
+    if (keys)
+      for (size_t e = 0; e < std::min(keys_cap, ks.size()); e++) keys[e] = (uint64_t)ks[e];
+  }
+  for (int f = 0; ok && fields && f < nfields; f++)
+    ok = fields[f] && std::fread(fields[f], sizeof(double), npts, fp) == npts;
+  std::fclose(fp);
+  return ok ? GF_OK : GF_ERR_ARG;
 }
 
 // GE_DEFORMED geometry of 3-D elements from their node coordinates:

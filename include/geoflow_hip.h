@@ -131,6 +131,31 @@ int gf_terrain_box_grid(int p, const int *ne, const double *P0, const double *P1
 int gf_deformed_geometry2d(int p, size_t nelems, const double *const *xyz, int reference_inverse,
                            double *const *dxidx, double *jac);
 
+/* ---- GIO state files (src/cdg/io/gio.ipp) ---------------------------------
+ * The layout GIO::read_header reads (gio.ipp:588-640) and GIO::sz_header
+ * sizes (gio.ipp:678-690): int ivers, int dim, size_t nelems,
+ * int porder[(ivers == 0 ? 1 : nelems) * dim], int gtype, size_t cycle,
+ * double time, int multivar, size_t keys[nelems], then the field doubles
+ * (one field, or every field in order when multivar != 0). */
+typedef struct {
+  int      ivers;     /* GIO version: 0 = one porder row, else one per element */
+  int      dim;
+  uint64_t nelems;
+  int      gtype;     /* GF_GE_* */
+  uint64_t cycle;
+  double   time;
+  int      multivar;  /* 0: one field per file */
+  uint64_t nporder;   /* filled by gf_gio_read: ints in porder */
+} gf_gio_info;
+/* porder: (ivers == 0 ? 1 : nelems) * dim ints; keys: nelems ids or NULL
+ * (0 .. nelems-1); fields: nfields host arrays of npts doubles. */
+int gf_gio_write(const char *path, const gf_gio_info *info, const int *porder,
+                 const uint64_t *keys, const double *const *fields, int nfields, size_t npts);
+/* fields == NULL reads the header only; porder/keys (may be NULL) receive at
+ * most porder_cap / keys_cap entries. */
+int gf_gio_read(const char *path, gf_gio_info *info, int *porder, size_t porder_cap,
+                uint64_t *keys, size_t keys_cap, double *const *fields, int nfields, size_t npts);
+
 /* GE_DEFORMED geometry of 3-D elements from their node coordinates
  * (GElem_base::dogeom3d / Jac / inv, src/cdg/sem/gelem_base.cpp:972-1048,
  * 1148-1168, 1268-1283): dxidx[9] at [i*3 + j] = dXi_i/dX_j (the 3x3 inverse
