@@ -735,7 +735,8 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
   for (int k = 1; k < d->dim; k++) iso = iso && m->N[k] == m->N[0];
   int nmax = d->dim == 3 ? 10 : 12;
   m->Niso = (iso && m->N[0] <= nmax) ? m->N[0] : 0;
-  if (m->dim == 3 && m->Niso >= 2 && m->Niso <= 10) {
+  // the constant-D pencil kernels serve regular elements only
+  if (m->dim == 3 && m->Niso >= 2 && m->Niso <= 10 && d->gtype == GF_GE_REGULAR) {
     bool same = true;
     for (int k = 0; k < 3 && same; k++)
       for (int l = 0; l < m->Niso && same; l++)

@@ -167,7 +167,7 @@ struct CGApplyArgs {
   void *w;            // T
   void *q;            // T
   const void *D, *W, *edx, *ejac, *qf;  // T
-  const double *G[3];  // node metrics G00, G01, G11 (embedded or deformed 2-D meshes)
+  const double *G[6];  // node metrics: 2-D G00, G01, G11; 3-D deformed also G02, G12, G22
   const double *mass;  // lumped mass per node (embedded or deformed 2-D meshes)
   double p;
   double qs;  // mass-term scalar when qf is NULL (0: no mass term)
@@ -299,6 +299,80 @@ __global__ __launch_bounds__(kThreads) void k_cg_apply_g(CGApplyArgs a) {
     double acc = 0.0;
     acc += gf::contract<2, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
     acc += gf::contract<2, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
+    if (a.p != 1.0) acc *= a.p;
+    if (qf || a.qs != 0.0) {
+      double t = a.mass[base + q] * su[q];
+      if (qf) t *= qf[base + q];
+      else if (a.qs != 1.0) t *= a.qs;
+      acc += t;
+    }
+    qv[base + q] = acc;
+    dot += su[q] * acc;
+  }
+  const double bsum = gf::block_reduce<false>(dot, sred);
+  if (threadIdx.x == 0) a.part[blockIdx.x] = bsum;
+}
+
+// The same for 3-D deformed meshes: def_prod with the six metric vectors
+// (G00, G01, G11, G02, G12, G22), as k_helm_def3 forms it.  FP64.
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_cg_apply_g3(CGApplyArgs a) {
+  using S = Shape<3, N>;
+  __shared__ double su[S::NODES];
+  __shared__ double sg[3][S::NODES];
+  __shared__ double sD[N * N];
+  __shared__ double sred[kThreads / 64];
+  const double *scal = a.scal;
+  if (scal[S_DONE] != 0.0) return;
+  const double *r = (const double *)a.r, *dinv = (const double *)a.dinv;
+  double *w = (double *)a.w, *qv = (double *)a.q;
+  const double *qf = (const double *)a.qf;
+  const bool first = scal[S_FIRST] != 0.0;
+  const double beta = scal[S_BETA];
+  const size_t e0 = (size_t)blockIdx.x * S::EPB;
+  const int nel = (int)min((size_t)S::EPB, a.nelems - e0);
+  const size_t base = e0 * S::NN;
+  const int cnt = nel * S::NN;
+  gf::stage_D(sD, (const double *)a.D, N);
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    const size_t g = base + q;
+    const double z = dinv ? dinv[g] * r[g] : r[g];
+    const double wn = first ? z : beta * w[g] + 1.0 * z;
+    w[g] = wn;
+    su[q] = wn;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
+    gf::node_ijk<3, N>(rr, i, j, k);
+    const double d0 = gf::contract<3, N, false>(su + el * S::NN, sD, 0, i, j, k);
+    const double d1 = gf::contract<3, N, false>(su + el * S::NN, sD, 1, i, j, k);
+    const double d2 = gf::contract<3, N, false>(su + el * S::NN, sD, 2, i, j, k);
+    const size_t g = base + q;
+    const double a00 = a.G[0][g], a01 = a.G[1][g], a11 = a.G[2][g], a02 = a.G[3][g],
+                 a12 = a.G[4][g], a22 = a.G[5][g];
+    double g0 = 0.0 + d0 * a00;
+    g0 += d1 * a01;
+    g0 += d2 * a02;
+    double g1 = 0.0 + d0 * a01;
+    g1 += d1 * a11;
+    g1 += d2 * a12;
+    double g2 = 0.0 + d0 * a02;
+    g2 += d1 * a12;
+    g2 += d2 * a22;
+    sg[0][q] = g0;
+    sg[1][q] = g1;
+    sg[2][q] = g2;
+  }
+  __syncthreads();
+  double dot = 0.0;
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
+    gf::node_ijk<3, N>(rr, i, j, k);
+    double acc = 0.0;
+    acc += gf::contract<3, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
+    acc += gf::contract<3, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
+    acc += gf::contract<3, N, true>(&sg[2][el * S::NN], sD, 2, i, j, k);
     if (a.p != 1.0) acc *= a.p;
     if (qf || a.qs != 0.0) {
       double t = a.mass[base + q] * su[q];
@@ -544,8 +618,11 @@ int launch_cg_apply(gf_mesh *m, const CGApplyArgs &a) {
   using S = Shape<DIM, N>;
   int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
   if (m->gtype != GF_GE_REGULAR) {
-    if (DIM != 2 || sizeof(T) != 8) return GF_ERR_UNSUPPORTED;
-    hipLaunchKernelGGL((k_cg_apply_g<N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
+    if (sizeof(T) != 8) return GF_ERR_UNSUPPORTED;
+    if (DIM == 2)
+      hipLaunchKernelGGL((k_cg_apply_g<N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
+    else
+      hipLaunchKernelGGL((k_cg_apply_g3<N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
     GF_LAUNCH_CHECK(m->ctx);
     return GF_OK;
   }
@@ -1280,7 +1357,7 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a.ejac = v.ejac;
   a.qf = v.qf;
   a.qs = v.qs;
-  for (int c = 0; c < 3; c++) a.G[c] = m->G[c];
+  for (int c = 0; c < 6; c++) a.G[c] = m->G[c];
   a.mass = m->mass;
   a.p = v.p;
   a.scal = m->scal;
@@ -1360,9 +1437,9 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
   if (!m || !prm || !b || !x || !res) return GF_ERR_ARG;
   memset(res, 0, sizeof(*res));
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: isotropic meshes");
-  if (m->gtype != GF_GE_REGULAR && (prm->precision != 0 || prm->precond || m->dim != 2))
+  if (m->gtype != GF_GE_REGULAR && (prm->precision != 0 || prm->precond))
     return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED,
-                      "CG on embedded or deformed meshes: 2-D, FP64 without preconditioner");
+                      "CG on embedded or deformed meshes: FP64 without preconditioner");
   if (prm->maxit < 1) return gf_set_err(m->ctx, GF_ERR_BADITERNUM, "maxit < 1");
   if (prm->normtype < GF_NORM_INF || prm->normtype > GF_NORM_ASSEMBLED_L2) return GF_ERR_ARG;
   if (prm->p_field) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: p field not supported");

@@ -214,7 +214,7 @@ size_t gf_mesh_ndof(const gf_mesh *mesh);
 /* Mesh properties (diagnostic; no reference counterpart):
  * GF_MESH_BOX_GATHER  the structured-box gather assembly is in use;
  * GF_MESH_BOX_MASK    the boundary mask is formed from the box faces;
- * GF_MESH_CONST_D     3-D mesh whose D is the standard GLL matrix, so the
+ * GF_MESH_CONST_D     3-D regular mesh whose D is the standard GLL matrix, so the
  *                     pencil kernels (D as compile-time constants) are used. */
 enum { GF_MESH_BOX_GATHER = 1, GF_MESH_BOX_MASK = 2, GF_MESH_CONST_D = 4 };
 int gf_mesh_flags(const gf_mesh *mesh);
@@ -341,7 +341,7 @@ typedef struct {
 /* b, xb, x: device fields.  xb != NULL selects solve_impl(A,b,xb,x): the
  * boundary mask is applied and xb is added back (gcg.ipp:361-382).
  * Synchronous (reads residual norms on the host each iteration batch).
- * Regular meshes: every option.  2-D embedded and deformed meshes: FP64
+ * Regular meshes: every option.  Embedded and deformed meshes: FP64
  * without preconditioner (other options give GF_ERR_UNSUPPORTED). */
 int gf_cg_solve(gf_mesh *mesh, const gf_cg_params *prm, const double *b, const double *xb,
                 double *x, gf_cg_result *res);
