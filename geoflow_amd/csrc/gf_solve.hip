@@ -598,6 +598,43 @@ __global__ void k_helm_diag(double *__restrict__ d, const double *__restrict__ D
     d[g] = val;
   }
 }
+// Element diagonal of (p L + q M) on meshes with node metrics (embedded and
+// deformed): the operator sum_{a,b} D_a^T diag(G_ab) D_b has diagonal
+//   sum_a sum_l D(l, n_a)^2 G_aa(line_a(n, l)) + 2 sum_{a<b} D(n_a,n_a) D(n_b,n_b) G_ab(n)
+// with G_ab in slot b(b+1)/2 + a (G00, G01, G11, G02, G12, G22).
+struct DiagG {
+  const double *G[6];
+};
+template <int DIM, int N>
+__global__ void k_helm_diag_g(double *__restrict__ d, const double *__restrict__ D, DiagG gm,
+                              const double *__restrict__ mass, double p,
+                              const double *__restrict__ qf, double qs, size_t ndof) {
+  constexpr int NN = Shape<DIM, N>::NN;
+  GRID_LOOP(g, ndof) {
+    size_t e = g / NN;
+    int r = (int)(g - e * NN), idx[3];
+    gf::node_ijk<DIM, N>(r, idx[0], idx[1], idx[2]);
+    const size_t stride[3] = {1, (size_t)N, (size_t)N * N};
+    double lap = 0.0;
+    for (int a = 0; a < DIM; a++) {
+      const double *Ga = gm.G[a * (a + 1) / 2 + a];
+      const size_t line = g - (size_t)idx[a] * stride[a];
+      double s = 0.0;
+      for (int l = 0; l < N; l++) {
+        double Dl = D[l + idx[a] * N];
+        s += Dl * Dl * Ga[line + (size_t)l * stride[a]];
+      }
+      lap += s;
+    }
+    for (int a = 0; a < DIM; a++)
+      for (int b = a + 1; b < DIM; b++)
+        lap += 2.0 * D[idx[a] * (N + 1)] * D[idx[b] * (N + 1)] * gm.G[b * (b + 1) / 2 + a][g];
+    double val = p * lap;
+    if (qf) val += qf[g] * mass[g];
+    else if (qs != 0.0) val += qs * mass[g];
+    d[g] = val;
+  }
+}
 __global__ void k_recip2(double *__restrict__ a, float *__restrict__ af, size_t n) {
   GRID_LOOP(g, n) {
     a[g] = 1.0 / a[g];
@@ -607,6 +644,14 @@ __global__ void k_recip2(double *__restrict__ a, float *__restrict__ af, size_t 
 
 template <int DIM, int N>
 int launch_diag(gf_mesh *m, double p, const double *qf, double qs, double *d) {
+  if (m->gtype != GF_GE_REGULAR) {
+    DiagG gm;
+    for (int c = 0; c < 6; c++) gm.G[c] = m->G[c];
+    hipLaunchKernelGGL((k_helm_diag_g<DIM, N>), dim3(nblocks(m->ndof)), dim3(kThreads), 0,
+                       m->ctx->stream, d, m->D[0], gm, m->mass, p, qf, qs, m->ndof);
+    GF_LAUNCH_CHECK(m->ctx);
+    return GF_OK;
+  }
   hipLaunchKernelGGL((k_helm_diag<DIM, N>), dim3(nblocks(m->ndof)), dim3(kThreads), 0,
                      m->ctx->stream, d, m->D[0], m->W[0], m->edx, m->ejac, p, qf, qs, m->ndof);
   GF_LAUNCH_CHECK(m->ctx);
@@ -1263,8 +1308,7 @@ int gf_burgers_steps(gf_mesh *m, int mode, double nu, int order, int nstate, dou
 
 int gf_helmholtz_diag_inv(gf_mesh *m, double p, double qs, const double *qf, double *dinv) {
   if (!m || !dinv) return GF_ERR_ARG;
-  if (m->gtype != GF_GE_REGULAR || !m->Niso)
-    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Jacobi diagonal: regular isotropic meshes");
+  if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Jacobi diagonal: isotropic meshes");
   RC(GF_DISPATCH_N(m->dim, m->Niso, launch_diag, m, p, qf, qs, dinv));
   RC(gf_dss_t(m, GF_DSS_SUM, dinv));
   hipLaunchKernelGGL(k_recip2, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, dinv,
@@ -1437,9 +1481,8 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
   if (!m || !prm || !b || !x || !res) return GF_ERR_ARG;
   memset(res, 0, sizeof(*res));
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: isotropic meshes");
-  if (m->gtype != GF_GE_REGULAR && (prm->precision != 0 || prm->precond))
-    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED,
-                      "CG on embedded or deformed meshes: FP64 without preconditioner");
+  if (m->gtype != GF_GE_REGULAR && prm->precision != 0)
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG on embedded or deformed meshes: FP64");
   if (prm->maxit < 1) return gf_set_err(m->ctx, GF_ERR_BADITERNUM, "maxit < 1");
   if (prm->normtype < GF_NORM_INF || prm->normtype > GF_NORM_ASSEMBLED_L2) return GF_ERR_ARG;
   if (prm->p_field) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: p field not supported");

@@ -341,11 +341,15 @@ typedef struct {
 /* b, xb, x: device fields.  xb != NULL selects solve_impl(A,b,xb,x): the
  * boundary mask is applied and xb is added back (gcg.ipp:361-382).
  * Synchronous (reads residual norms on the host each iteration batch).
- * Regular meshes: every option.  Embedded and deformed meshes: FP64
- * without preconditioner (other options give GF_ERR_UNSUPPORTED). */
+ * Regular meshes: every option.  Embedded and deformed meshes: FP64, with
+ * or without the Jacobi preconditioner (precision = 1 gives
+ * GF_ERR_UNSUPPORTED). */
 int gf_cg_solve(gf_mesh *mesh, const gf_cg_params *prm, const double *b, const double *xb,
                 double *x, gf_cg_result *res);
-/* Assembled Jacobi diagonal inverse of (p L + q M), q as in gf_helmholtz_apply. */
+/* Assembled Jacobi diagonal inverse of (p L + q M), q as in gf_helmholtz_apply;
+ * on embedded and deformed meshes the diagonal includes the off-diagonal
+ * metric terms.  No reference counterpart (GCG's default preconditioner is
+ * the identity, gcg.ipp:38). */
 int gf_helmholtz_diag_inv(gf_mesh *mesh, double p_scalar, double q_scalar, const double *q_field,
                           double *dinv);
 

@@ -677,43 +677,75 @@ void or_helmholtz_apply(OrGrid *g, const double *p, size_t np, const double *q, 
 }
 
 // Assembled diagonal of (p L + q M) for the Jacobi preconditioner (a
-// build-side addition; the reference has no Jacobi preconditioner).  The
-// element diagonal is evaluated by applying the operator to unit vectors
-// one element-local node at a time would be O(n^2); instead use the closed
-// form of the regular-element operator:
-//   diag_e(i,j,k) = p * M * sum_d G_dd * sum_m (D(m,i_d) ^2 W(m)/W(i_d))
-//                   + q * M
-// where M is the node's lumped mass.  For embedded grids only the mass
-// part and the metric diagonal are used.
+// build-side addition; the reference has no Jacobi preconditioner), from the
+// closed form of the element operator sum_{a,b} D_a^T diag(G_ab) D_b + q M,
+// D_a being the derivative along direction a (D(l, n_a) on the line of
+// nodes through n).  Its diagonal entry at node n is
+//   sum_a sum_l D(l, n_a)^2 G_aa(line_a(n, l))
+//     + sum_{a != b} D(n_a, n_a) D(n_b, n_b) G_ab(n)      (times p)  + q M(n).
+// On regular elements G_ab = 0 for a != b and G_aa = M (dXi_a/dx_a)^2, so the
+// line sum becomes G_aa Jac prod_{o != a} W(n_o) sum_l D(l, n_a)^2 W(l).  On
+// meshes with node metrics G_ab is the vector of or_embed_G (3-D: with the
+// doubled off-diagonals the operator applies).  The element diagonals are
+// assembled with a GGFX Sum.
 void or_helmholtz_diag_inv(OrGrid *g, const double *p, size_t np, const double *q, double *dinv) {
   size_t n = g->ndof;
   std::vector<double> d(n, 0.0);
   int N0 = g->N[0], N1 = g->N[1], N2 = g->N[2];
   size_t nn = g->nnodes;
+  const int dim = g->dim;
+  const bool pfull = p && np >= n;
+  std::vector<std::vector<double>> G(9);
+  if (g->gtype != 0)
+    for (int b = 0; b < dim; b++)
+      for (int a = 0; a <= b; a++) or_embed_G(g, a, b, G[a * 3 + b]);
+  const size_t stride[3] = {1, (size_t)N0, (size_t)N0 * N1};
   for (size_t e = 0; e < g->nelems; e++)
     for (int k = 0; k < N2; k++)
       for (int j = 0; j < N1; j++)
         for (int i = 0; i < N0; i++) {
           size_t id = e * nn + (size_t)k * N0 * N1 + (size_t)j * N0 + i;
           int idx[3] = {i, j, k};
-          double Wn[3] = {g->basis[0].w[i], g->basis[1].w[j], g->dim == 3 ? g->basis[2].w[k] : 1.0};
           double lap = 0.0;
-          for (int dd = 0; dd < g->dim; dd++) {
-            const Basis &b = g->basis[dd];
-            int Nd = b.N;
-            double s = 0.0;
-            for (int m = 0; m < Nd; m++) {
-              double Dm = b.D[m + (size_t)idx[dd] * Nd];
-              // mass at the node with index m along dd
-              s += Dm * Dm * b.w[m];
+          if (g->gtype == 0) {
+            double Wn[3] = {g->basis[0].w[i], g->basis[1].w[j], dim == 3 ? g->basis[2].w[k] : 1.0};
+            for (int dd = 0; dd < dim; dd++) {
+              const Basis &b = g->basis[dd];
+              int Nd = b.N;
+              double s = 0.0;
+              for (int m = 0; m < Nd; m++) {
+                double Dm = b.D[m + (size_t)idx[dd] * Nd];
+                s += Dm * Dm * b.w[m];
+              }
+              double other = 1.0;
+              for (int o = 0; o < dim; o++)
+                if (o != dd) other *= Wn[o];
+              double Gd = g->dxidx[dd][0][id] * g->dxidx[dd][0][id];
+              lap += Gd * s * other * g->jac[id];
             }
-            double other = 1.0;
-            for (int o = 0; o < g->dim; o++)
-              if (o != dd) other *= Wn[o];
-            double G = g->gtype == 0 ? g->dxidx[dd][0][id] * g->dxidx[dd][0][id] : 0.0;
-            lap += G * s * other * g->jac[id];
+          } else {
+            for (int a = 0; a < dim; a++) {
+              const Basis &b = g->basis[a];
+              const std::vector<double> &Ga = G[a * 3 + a];
+              size_t line = id - (size_t)idx[a] * stride[a];
+              double s = 0.0;
+              for (int l = 0; l < b.N; l++) {
+                double Dl = b.D[l + (size_t)idx[a] * b.N];
+                size_t ql = line + (size_t)l * stride[a];
+                double gv = pfull ? p[ql] * Ga[ql] : Ga[ql];
+                s += Dl * Dl * gv;
+              }
+              lap += s;
+            }
+            for (int a = 0; a < dim; a++)
+              for (int b = a + 1; b < dim; b++) {
+                double Daa = g->basis[a].D[idx[a] * (g->basis[a].N + 1)];
+                double Dbb = g->basis[b].D[idx[b] * (g->basis[b].N + 1)];
+                double gv = pfull ? p[id] * G[a * 3 + b][id] : G[a * 3 + b][id];
+                lap += 2.0 * Daa * Dbb * gv;
+              }
           }
-          double pv = p ? (np >= n ? p[id] : p[0]) : 1.0;
+          double pv = p ? (pfull ? (g->gtype == 0 ? p[id] : 1.0) : p[0]) : 1.0;
           double val = pv * lap;
           if (q) val += q[id] * g->mass[id];
           d[id] = val;

@@ -192,13 +192,15 @@ def test_terrain_burgers_rk4_matches_oracle(ctx):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precond", [0, 1])
 @pytest.mark.parametrize("mesh", ["terrain", "sphere"])
-def test_cg_on_node_metric_meshes(ctx, mesh):
+def test_cg_on_node_metric_meshes(ctx, mesh, precond):
     """GCG on 2-D meshes with per-node metrics (deformed and embedded): the
     Helmholtz (M + 0.1 L) on the terrain box with Dirichlet top and bottom
-    (xb = 0) and on the icosahedral sphere (no boundary): 25 fixed iterations
-    against the oracle, then a solve to rel-res 1e-10 whose assembled
-    residual is checked directly."""
+    (xb = 0) and on the icosahedral sphere (no boundary), without and with the
+    Jacobi preconditioner (whose diagonal is first checked against the
+    oracle's): 25 fixed iterations against the oracle, then a solve to
+    rel-res 1e-10 whose assembled residual is checked directly."""
     from geoflow_amd.device import Mesh
     import torch
     print("cg on %s" % mesh, flush=True)
@@ -213,15 +215,18 @@ def test_cg_on_node_metric_meshes(ctx, mesh):
     f = og.dss(f, "smooth")
     b = og.mass_apply(f)
     xb = np.zeros(og.ndof) if mesh == "terrain" else None
+    if precond:
+        assert relerr(_host(m.helmholtz_diag_inv(0.1, 1.0)),
+                      og.helmholtz_diag_inv(0.1, np.ones(og.ndof))) <= 1e-13
     xg, info = m.cg_solve(_dev(b), xb=_dev(xb) if xb is not None else None, p=0.1, q=1.0,
-                          maxit=25, tol=1e-30, normtype=0)
+                          maxit=25, tol=1e-30, normtype=0, precond=precond)
     xo, ret, it, res = og.cg_solve(b, xb=xb, p=0.1, q=np.ones(og.ndof), maxit=25, tol=1e-30,
-                                   normtype=0)
+                                   normtype=0, precond=precond)
     assert info["iters"] == it == 25
     assert relerr(_host(xg), xo) <= 1e-11
     assert relerr(info["resid"][:25], res[:25]) <= 1e-10
     xg, info = m.cg_solve(_dev(b), xb=_dev(xb) if xb is not None else None, p=0.1, q=1.0,
-                          maxit=3000, tol=1e-10, normtype=4)
+                          maxit=3000, tol=1e-10, normtype=4, precond=precond)
     assert info["status"] == 0
     r = og.dss(b - og.helmholtz(_host(xg), 0.1, np.ones(og.ndof)), "sum")
     if xb is not None:

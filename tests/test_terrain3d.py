@@ -140,10 +140,12 @@ def test_terrain3_operators_and_steps_match_oracle(ctx):
 
 
 @pytest.mark.gpu
-def test_cg_on_3d_deformed_mesh(ctx):
-    """GCG (FP64, no preconditioner) on the 3-D deformed box with Dirichlet
-    top and bottom: 25 fixed iterations against the oracle, then a solve to
-    rel-res 1e-10 whose assembled residual is checked directly."""
+@pytest.mark.parametrize("precond", [0, 1])
+def test_cg_on_3d_deformed_mesh(ctx, precond):
+    """GCG (FP64, without and with the Jacobi preconditioner, whose diagonal
+    is first checked against the oracle's) on the 3-D deformed box with
+    Dirichlet top and bottom: 25 fixed iterations against the oracle, then a
+    solve to rel-res 1e-10 whose assembled residual is checked directly."""
     from geoflow_amd.device import Mesh
     p, ne, amp = CASE
     print("cg on terrain3", flush=True)
@@ -153,13 +155,18 @@ def test_cg_on_3d_deformed_mesh(ctx):
     f = og.dss(np.random.default_rng(22).standard_normal(og.ndof), "smooth")
     b = og.mass_apply(f)
     xb = np.zeros(og.ndof)
-    xg, info = m.cg_solve(_dev(b), xb=_dev(xb), p=0.1, q=1.0, maxit=25, tol=1e-30, normtype=0)
+    if precond:
+        assert relerr(_host(m.helmholtz_diag_inv(0.1, 1.0)),
+                      og.helmholtz_diag_inv(0.1, np.ones(og.ndof))) <= 1e-13
+    xg, info = m.cg_solve(_dev(b), xb=_dev(xb), p=0.1, q=1.0, maxit=25, tol=1e-30, normtype=0,
+                          precond=precond)
     xo, ret, it, res = og.cg_solve(b, xb=xb, p=0.1, q=np.ones(og.ndof), maxit=25, tol=1e-30,
-                                   normtype=0)
+                                   normtype=0, precond=precond)
     assert info["iters"] == it == 25
     assert relerr(_host(xg), xo) <= 1e-11
     assert relerr(info["resid"][:25], res[:25]) <= 1e-10
-    xg, info = m.cg_solve(_dev(b), xb=_dev(xb), p=0.1, q=1.0, maxit=3000, tol=1e-10, normtype=4)
+    xg, info = m.cg_solve(_dev(b), xb=_dev(xb), p=0.1, q=1.0, maxit=3000, tol=1e-10, normtype=4,
+                          precond=precond)
     assert info["status"] == 0
     mask = og.get("mask")
     r = og.dss(b - og.helmholtz(_host(xg), 0.1, np.ones(og.ndof)), "sum") * mask
