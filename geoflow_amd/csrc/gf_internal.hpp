@@ -103,6 +103,7 @@ enum {
   WS_RK_BEGIN = 8,          // RK: acc[3], K[3][3], u3[3]
   WS_CG_BEGIN = 32,         // CG vectors
   WS_ADJ_TMP = 48,          // adjoint RHS scratch (6 slots)
+  WS_G32 = 54,              // mixed CG: FP32 node metrics G[6] and mass (7 slots)
   WS_SLOTS = 64,            // kernel-timing scratch (16 slots)
   WS_ADJ_BEGIN = 96         // adjoint stage states and cotangents
 };

@@ -167,8 +167,8 @@ struct CGApplyArgs {
   void *w;            // T
   void *q;            // T
   const void *D, *W, *edx, *ejac, *qf;  // T
-  const double *G[6];  // node metrics: 2-D G00, G01, G11; 3-D deformed also G02, G12, G22
-  const double *mass;  // lumped mass per node (embedded or deformed 2-D meshes)
+  const void *G[6];  // T node metrics: 2-D G00, G01, G11; 3-D deformed also G02, G12, G22
+  const void *mass;  // T lumped mass per node (meshes with node metrics)
   double p;
   double qs;  // mass-term scalar when qf is NULL (0: no mass term)
   double *scal;
@@ -247,141 +247,88 @@ __global__ __launch_bounds__(kThreads) void k_cg_apply(CGApplyArgs a) {
   if (threadIdx.x == 0) a.part[blockIdx.x] = bsum;
 }
 
-// The same for 2-D meshes with per-node metrics (GE_2DEMBEDDED and
-// GE_DEFORMED): q = p sum_j D_j^T (sum_i G(i,j) D_i w) + q M w in the
-// operation order of embed_prod / def_prod (ghelmholtz.ipp:108-191, 205-290),
-// as k_helm_embed forms it.  FP64.
-template <int N>
-__global__ __launch_bounds__(kThreads) void k_cg_apply_g(CGApplyArgs a) {
-  using S = Shape<2, N>;
-  __shared__ double su[S::NODES];
-  __shared__ double sg[2][S::NODES];
-  __shared__ double sD[N * N];
+// The same for meshes with per-node metrics (GE_2DEMBEDDED and GE_DEFORMED):
+// q = p sum_j D_j^T (sum_i G(i,j) D_i w) + q M w in the operation order of
+// embed_prod / def_prod (ghelmholtz.ipp:108-191, 205-290, 494-514), as
+// k_helm_embed and k_helm_def3 form it; 2-D with G00, G01, G11, 3-D with the
+// six vectors G00, G01, G11, G02, G12, G22.  T is the recurrence precision:
+// FP64, or FP32 with FP32 copies of G, the mass and D.
+template <int DIM, int N, typename T>
+__global__ __launch_bounds__(kThreads) void k_cg_apply_gm(CGApplyArgs a) {
+  using S = Shape<DIM, N>;
+  __shared__ T su[S::NODES];
+  __shared__ T sg[DIM][S::NODES];
+  __shared__ T sD[N * N];
   __shared__ double sred[kThreads / 64];
   const double *scal = a.scal;
   if (scal[S_DONE] != 0.0) return;
-  const double *r = (const double *)a.r, *dinv = (const double *)a.dinv;
-  double *w = (double *)a.w, *qv = (double *)a.q;
-  const double *qf = (const double *)a.qf;
+  const T *r = (const T *)a.r, *dinv = (const T *)a.dinv;
+  T *w = (T *)a.w, *qv = (T *)a.q;
+  const T *qf = (const T *)a.qf, *mass = (const T *)a.mass;
+  const T *G0 = (const T *)a.G[0], *G1 = (const T *)a.G[1], *G2 = (const T *)a.G[2];
   const bool first = scal[S_FIRST] != 0.0;
-  const double beta = scal[S_BETA];
+  const T beta = (T)scal[S_BETA];
   const size_t e0 = (size_t)blockIdx.x * S::EPB;
   const int nel = (int)min((size_t)S::EPB, a.nelems - e0);
   const size_t base = e0 * S::NN;
   const int cnt = nel * S::NN;
-  gf::stage_D(sD, (const double *)a.D, N);
+  gf::stage_D(sD, (const T *)a.D, N);
   for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
     const size_t g = base + q;
-    const double z = dinv ? dinv[g] * r[g] : r[g];
-    const double wn = first ? z : beta * w[g] + 1.0 * z;
+    const T z = dinv ? dinv[g] * r[g] : r[g];
+    const T wn = first ? z : beta * w[g] + (T)1 * z;
     w[g] = wn;
     su[q] = wn;
   }
   __syncthreads();
   for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
     int el = q / S::NN, rr = q - el * S::NN, i, j, k;
-    gf::node_ijk<2, N>(rr, i, j, k);
-    const double d0 = gf::contract<2, N, false>(su + el * S::NN, sD, 0, i, j, k);
-    const double d1 = gf::contract<2, N, false>(su + el * S::NN, sD, 1, i, j, k);
+    gf::node_ijk<DIM, N>(rr, i, j, k);
     const size_t g = base + q;
-    double g0 = 0.0 + d0 * a.G[0][g];
-    g0 += d1 * a.G[1][g];
-    double g1 = 0.0 + d0 * a.G[1][g];
-    g1 += d1 * a.G[2][g];
-    sg[0][q] = g0;
-    sg[1][q] = g1;
+    const T d0 = gf::contract<DIM, N, false>(su + el * S::NN, sD, 0, i, j, k);
+    const T d1 = gf::contract<DIM, N, false>(su + el * S::NN, sD, 1, i, j, k);
+    if constexpr (DIM == 2) {
+      T g0 = (T)0 + d0 * G0[g];
+      g0 += d1 * G1[g];
+      T g1 = (T)0 + d0 * G1[g];
+      g1 += d1 * G2[g];
+      sg[0][q] = g0;
+      sg[1][q] = g1;
+    } else {
+      const T d2 = gf::contract<DIM, N, false>(su + el * S::NN, sD, 2, i, j, k);
+      const T a00 = G0[g], a01 = G1[g], a11 = G2[g], a02 = ((const T *)a.G[3])[g],
+              a12 = ((const T *)a.G[4])[g], a22 = ((const T *)a.G[5])[g];
+      T g0 = (T)0 + d0 * a00;
+      g0 += d1 * a01;
+      g0 += d2 * a02;
+      T g1 = (T)0 + d0 * a01;
+      g1 += d1 * a11;
+      g1 += d2 * a12;
+      T g2 = (T)0 + d0 * a02;
+      g2 += d1 * a12;
+      g2 += d2 * a22;
+      sg[0][q] = g0;
+      sg[1][q] = g1;
+      sg[DIM - 1][q] = g2;
+    }
   }
   __syncthreads();
   double dot = 0.0;
   for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
     int el = q / S::NN, rr = q - el * S::NN, i, j, k;
-    gf::node_ijk<2, N>(rr, i, j, k);
-    double acc = 0.0;
-    acc += gf::contract<2, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
-    acc += gf::contract<2, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
-    if (a.p != 1.0) acc *= a.p;
+    gf::node_ijk<DIM, N>(rr, i, j, k);
+    T acc = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) acc += gf::contract<DIM, N, true>(&sg[d][el * S::NN], sD, d, i, j, k);
+    if (a.p != 1.0) acc *= (T)a.p;
     if (qf || a.qs != 0.0) {
-      double t = a.mass[base + q] * su[q];
+      T t = mass[base + q] * su[q];
       if (qf) t *= qf[base + q];
-      else if (a.qs != 1.0) t *= a.qs;
+      else if (a.qs != 1.0) t *= (T)a.qs;
       acc += t;
     }
     qv[base + q] = acc;
-    dot += su[q] * acc;
-  }
-  const double bsum = gf::block_reduce<false>(dot, sred);
-  if (threadIdx.x == 0) a.part[blockIdx.x] = bsum;
-}
-
-// The same for 3-D deformed meshes: def_prod with the six metric vectors
-// (G00, G01, G11, G02, G12, G22), as k_helm_def3 forms it.  FP64.
-template <int N>
-__global__ __launch_bounds__(kThreads) void k_cg_apply_g3(CGApplyArgs a) {
-  using S = Shape<3, N>;
-  __shared__ double su[S::NODES];
-  __shared__ double sg[3][S::NODES];
-  __shared__ double sD[N * N];
-  __shared__ double sred[kThreads / 64];
-  const double *scal = a.scal;
-  if (scal[S_DONE] != 0.0) return;
-  const double *r = (const double *)a.r, *dinv = (const double *)a.dinv;
-  double *w = (double *)a.w, *qv = (double *)a.q;
-  const double *qf = (const double *)a.qf;
-  const bool first = scal[S_FIRST] != 0.0;
-  const double beta = scal[S_BETA];
-  const size_t e0 = (size_t)blockIdx.x * S::EPB;
-  const int nel = (int)min((size_t)S::EPB, a.nelems - e0);
-  const size_t base = e0 * S::NN;
-  const int cnt = nel * S::NN;
-  gf::stage_D(sD, (const double *)a.D, N);
-  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-    const size_t g = base + q;
-    const double z = dinv ? dinv[g] * r[g] : r[g];
-    const double wn = first ? z : beta * w[g] + 1.0 * z;
-    w[g] = wn;
-    su[q] = wn;
-  }
-  __syncthreads();
-  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
-    gf::node_ijk<3, N>(rr, i, j, k);
-    const double d0 = gf::contract<3, N, false>(su + el * S::NN, sD, 0, i, j, k);
-    const double d1 = gf::contract<3, N, false>(su + el * S::NN, sD, 1, i, j, k);
-    const double d2 = gf::contract<3, N, false>(su + el * S::NN, sD, 2, i, j, k);
-    const size_t g = base + q;
-    const double a00 = a.G[0][g], a01 = a.G[1][g], a11 = a.G[2][g], a02 = a.G[3][g],
-                 a12 = a.G[4][g], a22 = a.G[5][g];
-    double g0 = 0.0 + d0 * a00;
-    g0 += d1 * a01;
-    g0 += d2 * a02;
-    double g1 = 0.0 + d0 * a01;
-    g1 += d1 * a11;
-    g1 += d2 * a12;
-    double g2 = 0.0 + d0 * a02;
-    g2 += d1 * a12;
-    g2 += d2 * a22;
-    sg[0][q] = g0;
-    sg[1][q] = g1;
-    sg[2][q] = g2;
-  }
-  __syncthreads();
-  double dot = 0.0;
-  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-    int el = q / S::NN, rr = q - el * S::NN, i, j, k;
-    gf::node_ijk<3, N>(rr, i, j, k);
-    double acc = 0.0;
-    acc += gf::contract<3, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
-    acc += gf::contract<3, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
-    acc += gf::contract<3, N, true>(&sg[2][el * S::NN], sD, 2, i, j, k);
-    if (a.p != 1.0) acc *= a.p;
-    if (qf || a.qs != 0.0) {
-      double t = a.mass[base + q] * su[q];
-      if (qf) t *= qf[base + q];
-      else if (a.qs != 1.0) t *= a.qs;
-      acc += t;
-    }
-    qv[base + q] = acc;
-    dot += su[q] * acc;
+    dot += (double)su[q] * (double)acc;
   }
   const double bsum = gf::block_reduce<false>(dot, sred);
   if (threadIdx.x == 0) a.part[blockIdx.x] = bsum;
@@ -663,11 +610,7 @@ int launch_cg_apply(gf_mesh *m, const CGApplyArgs &a) {
   using S = Shape<DIM, N>;
   int nb = (int)((m->nelems + S::EPB - 1) / S::EPB);
   if (m->gtype != GF_GE_REGULAR) {
-    if (sizeof(T) != 8) return GF_ERR_UNSUPPORTED;
-    if (DIM == 2)
-      hipLaunchKernelGGL((k_cg_apply_g<N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
-    else
-      hipLaunchKernelGGL((k_cg_apply_g3<N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
+    hipLaunchKernelGGL((k_cg_apply_gm<DIM, N, T>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
     GF_LAUNCH_CHECK(m->ctx);
     return GF_OK;
   }
@@ -1401,8 +1344,8 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a.ejac = v.ejac;
   a.qf = v.qf;
   a.qs = v.qs;
-  for (int c = 0; c < 6; c++) a.G[c] = m->G[c];
-  a.mass = m->mass;
+  for (int c = 0; c < 6; c++) a.G[c] = v.G[c];
+  a.mass = v.mass;
   a.p = v.p;
   a.scal = m->scal;
   a.part = partA;
@@ -1481,8 +1424,6 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
   if (!m || !prm || !b || !x || !res) return GF_ERR_ARG;
   memset(res, 0, sizeof(*res));
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: isotropic meshes");
-  if (m->gtype != GF_GE_REGULAR && prm->precision != 0)
-    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG on embedded or deformed meshes: FP64");
   if (prm->maxit < 1) return gf_set_err(m->ctx, GF_ERR_BADITERNUM, "maxit < 1");
   if (prm->normtype < GF_NORM_INF || prm->normtype > GF_NORM_ASSEMBLED_L2) return GF_ERR_ARG;
   if (prm->p_field) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: p field not supported");
@@ -1580,6 +1521,8 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     v.W = m->W[0];
     v.edx = m->edx;
     v.ejac = m->ejac;
+    for (int c = 0; c < 6; c++) v.G[c] = m->G[c];
+    v.mass = m->mass;
     v.p = prm->p_scalar;
     int it = 0;
     RC(cg_loop<double>(m, v, normtype, d_resid, &it));
@@ -1622,6 +1565,21 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     v.W = m->W32;
     v.edx = m->edx32;
     v.ejac = m->ejac32;
+    for (int c = 0; c < 6; c++) v.G[c] = nullptr;
+    v.mass = nullptr;
+    if (m->gtype != GF_GE_REGULAR) {
+      // FP32 copies of the node metrics and the lumped mass
+      for (int c = 0; c < 7; c++) {
+        const double *src = c < 6 ? m->G[c] : m->mass;
+        if (!src) continue;
+        float *dst = (float *)gf_ws(m, WS_G32 + c, n * sizeof(float));
+        if (!dst) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+        hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, dst, src, 1.0, n);
+        if (c < 6) v.G[c] = dst;
+        else v.mass = dst;
+      }
+      GF_LAUNCH_CHECK(m->ctx);
+    }
     v.p = prm->p_scalar;
     int hist = 0;
     bool first = true;

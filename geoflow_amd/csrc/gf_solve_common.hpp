@@ -142,6 +142,8 @@ struct CGVecs {
   const T *dinv, *mask, *qf;
   const T *imult;  // 1/multiplicity in the recurrence precision (exact in FP64)
   const void *D, *W, *edx, *ejac;
+  const void *G[6];  // T node metrics and lumped mass (meshes with node metrics)
+  const void *mass;
   double p;
   double qs;  // mass-term scalar when qf is NULL
 };

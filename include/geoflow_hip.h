@@ -341,9 +341,8 @@ typedef struct {
 /* b, xb, x: device fields.  xb != NULL selects solve_impl(A,b,xb,x): the
  * boundary mask is applied and xb is added back (gcg.ipp:361-382).
  * Synchronous (reads residual norms on the host each iteration batch).
- * Regular meshes: every option.  Embedded and deformed meshes: FP64, with
- * or without the Jacobi preconditioner (precision = 1 gives
- * GF_ERR_UNSUPPORTED). */
+ * Every option on regular, embedded and deformed meshes (the mixed
+ * precision keeps FP32 copies of the node metrics in the mesh workspace). */
 int gf_cg_solve(gf_mesh *mesh, const gf_cg_params *prm, const double *b, const double *xb,
                 double *x, gf_cg_result *res);
 /* Assembled Jacobi diagonal inverse of (p L + q M), q as in gf_helmholtz_apply;

@@ -145,7 +145,8 @@ def test_cg_on_3d_deformed_mesh(ctx, precond):
     """GCG (FP64, without and with the Jacobi preconditioner, whose diagonal
     is first checked against the oracle's) on the 3-D deformed box with
     Dirichlet top and bottom: 25 fixed iterations against the oracle, then a
-    solve to rel-res 1e-10 whose assembled residual is checked directly."""
+    FP64 and mixed-precision solves to rel-res 1e-10 whose assembled residuals
+    are checked directly."""
     from geoflow_amd.device import Mesh
     p, ne, amp = CASE
     print("cg on terrain3", flush=True)
@@ -173,3 +174,10 @@ def test_cg_on_3d_deformed_mesh(ctx, precond):
     bA = og.dss(b.copy(), "sum") * mask
     im = og.imult()
     assert np.sqrt(np.sum(r * r * im)) <= 1.01e-10 * np.sqrt(np.sum(bA * bA * im))
+    x64 = _host(xg)
+    xm, info = m.cg_solve(_dev(b), xb=_dev(xb), p=0.1, q=1.0, maxit=3000, tol=1e-10, normtype=4,
+                          precond=precond, precision=1)
+    assert info["status"] == 0
+    r = og.dss(b - og.helmholtz(_host(xm), 0.1, np.ones(og.ndof)), "sum") * mask
+    assert np.sqrt(np.sum(r * r * im)) <= 1.01e-10 * np.sqrt(np.sum(bA * bA * im))
+    assert relerr(_host(xm), x64) <= 1e-8
