@@ -93,6 +93,7 @@ def lib():
         L.or_dss.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.or_imult.argtypes = [ctypes.c_void_p, _dp]
         L.or_advect.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.POINTER(_dp), _dp]
+        L.or_constrain2sphere.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.or_burgers_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
         L.or_burgers_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
@@ -314,6 +315,11 @@ class Grid:
         o = np.zeros(self.ndof)
         lib().or_advect(self.h, _p(p), len(us), _pp(us), _p(o))
         return o
+
+    def constrain2sphere(self, vs):
+        """In-place GMTK::constrain2sphere of the three components vs."""
+        lib().or_constrain2sphere(self.h, _p(vs[0]), _p(vs[1]), _p(vs[2]))
+        return vs
 
     def burgers_rhs(self, us, nu, mode=0):
         # heat (2) and nonlinear (0) modes produce one output per state component

@@ -712,10 +712,14 @@ void or_helmholtz_diag_inv(OrGrid *g, const double *p, size_t np, const double *
             for (int dd = 0; dd < dim; dd++) {
               const Basis &b = g->basis[dd];
               int Nd = b.N;
+              // p (a full field) multiplies pointwise before D^T, so it
+              // enters the line sum at each line node m
+              size_t line = id - (size_t)idx[dd] * stride[dd];
               double s = 0.0;
               for (int m = 0; m < Nd; m++) {
                 double Dm = b.D[m + (size_t)idx[dd] * Nd];
-                s += Dm * Dm * b.w[m];
+                double wm = pfull ? b.w[m] * p[line + (size_t)m * stride[dd]] : b.w[m];
+                s += Dm * Dm * wm;
               }
               double other = 1.0;
               for (int o = 0; o < dim; o++)
@@ -745,7 +749,7 @@ void or_helmholtz_diag_inv(OrGrid *g, const double *p, size_t np, const double *
                 lap += 2.0 * Daa * Dbb * gv;
               }
           }
-          double pv = p ? (pfull ? (g->gtype == 0 ? p[id] : 1.0) : p[0]) : 1.0;
+          double pv = p ? (pfull ? 1.0 : p[0]) : 1.0;
           double val = pv * lap;
           if (q) val += q[id] * g->mass[id];
           d[id] = val;
@@ -824,6 +828,15 @@ void constrain2sphere(OrGrid *g, std::vector<double *> &v) {
     v[2][j] = (-v[0][j] * z * x - v[1][j] * z * y + v[2][j] * (r2 - z * z)) * ri;
   }
 }
+
+}  // namespace
+
+extern "C" void or_constrain2sphere(OrGrid *g, double *v0, double *v1, double *v2) {
+  std::vector<double *> v = {v0, v1, v2};
+  constrain2sphere(g, v);
+}
+
+namespace {
 
 void butcher(int order, std::vector<double> &alpha, std::vector<double> &beta,
              std::vector<double> &c) {

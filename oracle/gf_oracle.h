@@ -109,6 +109,10 @@ void or_dss(OrGrid *g, int op, double *u);
 void or_imult(OrGrid *g, double *imult);
 /* GAdvect::apply (src/cdg/sem/gadvect.ipp:57-99): nu velocity components */
 void or_advect(OrGrid *g, const double *p, int nu, const double *const *u, double *po);
+/* GMTK::constrain2sphere (2) (gmtk.ipp:2956-2980): in-place tangent projection of
+ * (v0, v1, v2) with the reference's aliasing (the updated v0 feeds v1, both feed v2);
+ * a no-op unless the grid is GE_2DEMBEDDED. */
+void or_constrain2sphere(OrGrid *g, double *v0, double *v1, double *v2);
 /* GBurgers::dudt_impl (src/cdg/pdes/gburgers.ipp:202-257).
  * mode 0 nonlinear (n_c = dim comps), 1 pure advection (u[0] advected by
  * u[1..nvel]), 2 heat. */

@@ -412,9 +412,10 @@ def _diag_grid(kind):
     return oracle.Grid.icos(3, 1, radius=1.3)
 
 
+@pytest.mark.parametrize("pfield", [False, True], ids=["p_scalar", "p_field"])
 @pytest.mark.parametrize("kind", ["box2_dirichlet", "box2_periodic", "box3_periodic", "terrain2",
                                   "terrain3", "sphere"])
-def test_jacobi_diagonal_by_unit_vectors(kind):
+def test_jacobi_diagonal_by_unit_vectors(kind, pfield):
     """or_helmholtz_diag_inv (the build's Jacobi preconditioner) against the
     assembled diagonal found by brute force: apply (p L + q M) to the
     continuous unit field of each node group, assemble (GGFX Sum) and read
@@ -424,7 +425,11 @@ def test_jacobi_diagonal_by_unit_vectors(kind):
     og = _diag_grid(kind)
     q = 1.0 + 0.5 * np.random.default_rng(3).random(og.ndof)
     q = og.dss(q, "smooth")
-    dinv = og.helmholtz_diag_inv(0.3, q)
+    # p as a full field varies inside the elements, so it must enter the
+    # diagonal at every node of the line sum (the reference multiplies it in
+    # pointwise before D^T); a continuous field, as a Laplacian factor is
+    p = og.dss(0.2 + 0.2 * np.random.default_rng(5).random(og.ndof), "smooth") if pfield else 0.3
+    dinv = og.helmholtz_diag_inv(p, q)
     offs, mem = og.ggfx()
     done = np.zeros(og.ndof, dtype=bool)
     for l in range(og.ndof):
@@ -433,7 +438,142 @@ def test_jacobi_diagonal_by_unit_vectors(kind):
         group = mem[offs[l]:offs[l + 1]]
         e = np.zeros(og.ndof)
         e[group] = 1.0
-        a = og.dss(og.helmholtz(e, 0.3, q), "sum")
+        a = og.dss(og.helmholtz(e, p, q), "sum")
         for g in group:
             assert abs(dinv[g] * a[g] - 1.0) <= 1e-12, (l, g)
         done[group] = True
+
+
+# ---------------------------------------------------------------------------
+# Sphere path (GE_2DEMBEDDED): constrain2sphere, the embedded gradient and
+# GAdvect against closed forms of the sphere's geometry
+# ---------------------------------------------------------------------------
+def _sphere_xyz(og):
+    return [og.get(c) for c in "xyz"]
+
+
+def test_constrain2sphere_tangent_fields_unchanged():
+    """A tangent field (x . v = 0) is a fixed point of the projection, also
+    with the reference's in-place aliasing (gmtk.ipp:2972-2974): v0' = v0
+    exactly up to rounding, so v1 and v2 see the original v0."""
+    og = oracle.Grid.icos(4, 1, radius=1.7)
+    X = _sphere_xyz(og)
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal(3)
+    # v = a x X is tangent everywhere
+    v = [a[1] * X[2] - a[2] * X[1], a[2] * X[0] - a[0] * X[2], a[0] * X[1] - a[1] * X[0]]
+    got = og.constrain2sphere([c.copy() for c in v])
+    scale = max(np.abs(c).max() for c in v)
+    for c in range(3):
+        assert np.abs(got[c] - v[c]).max() <= 1e-14 * scale, c
+
+
+def test_constrain2sphere_radial_field_aliased_closed_form():
+    """A radial field v = X: the true projection is 0, but the reference
+    updates in place, so v1 and v2 are formed from the already-projected v0.
+    With r^2 = |X|^2 (every node on the sphere): v0' = X(r^2 - |X|^2)/r^2 = 0,
+    v1' = y x^2 / r^2 (the -v0' y x term vanishes, the rest is y(r^2 - y^2 - z^2)),
+    v2' = z (x^2 (r^2 - y^2) / r^2 + y^2) / r^2.  A projection without the
+    aliasing (all three from the original v) would give 0 in every component."""
+    R = 1.3
+    og = oracle.Grid.icos(3, 1, radius=R)
+    X = _sphere_xyz(og)
+    x, y, z = X
+    got = og.constrain2sphere([c.copy() for c in X])
+    r2 = R * R
+    want = [np.zeros_like(x), y * x * x / r2, z * (x * x * (r2 - y * y) / r2 + y * y) / r2]
+    for c in range(3):
+        assert np.abs(got[c] - want[c]).max() <= 1e-13 * R, c
+    assert np.abs(got[1]).max() > 0.1  # the aliasing is visible, not round-off
+
+
+@pytest.mark.parametrize("p", [6, 8, 10])
+def test_embedded_gradient_of_coordinates_is_tangent_projection(p):
+    """GGrid::deriv on a GE_2DEMBEDDED grid (ggrid.ipp:1163-1191): the surface
+    gradient of the coordinate function x_k is the tangent projection of e_k,
+    d x_k / d x_j = delta_kj - x_k x_j / R^2.  The metric comes from the
+    embedded shape functions, the derivative from D on nodal values, so the
+    two meet to spectral accuracy: the error falls ~100x per +2 in p (1e-9 at
+    p=8, 1e-11 at p=10 on the level-1 sphere).  A transposed metric, a
+    dropped xi-direction or a wrong radial column fails at O(1)."""
+    og = oracle.Grid.icos(p, 1, radius=1.0)
+    X = _sphere_xyz(og)
+    R2 = X[0] ** 2 + X[1] ** 2 + X[2] ** 2
+    err = 0.0
+    for k in range(3):
+        for j in range(3):
+            d = og.deriv(X[k].copy(), j + 1)
+            want = (1.0 if k == j else 0.0) - X[k] * X[j] / R2
+            err = max(err, np.abs(d - want).max())
+    bound = {6: 1e-6, 8: 1e-8, 10: 1e-10}[p]
+    assert err <= bound, err
+
+
+@pytest.mark.parametrize("p", [6, 8, 10])
+def test_embedded_advection_solid_body_rotation_closed_forms(p):
+    """GAdvect::apply (gadvect.ipp:57-99) on the sphere: po = M (c . grad z).
+    Rotation about z (c = (-y, x, 0)) moves z along its level sets, so
+    c . grad z = 0; rotation about x (c = (0, -z, y)) gives c . grad z = y."""
+    og = oracle.Grid.icos(p, 1, radius=1.0)
+    x, y, z = _sphere_xyz(og)
+    mass = og.get("mass")
+    zero = np.zeros_like(x)
+    po = og.advect(z.copy(), [-y, x, zero])
+    bound = {6: 1e-6, 8: 1e-8, 10: 1e-10}[p]
+    assert np.abs(po / mass).max() <= bound
+    po = og.advect(z.copy(), [zero, -z, y])
+    assert np.abs(po / mass - y).max() <= bound
+
+
+def _assembled_advection_matrix(og, c):
+    """The pure-advection RHS K = Smooth(M^-1 (-GAdvect(u, c))) of GBurgers
+    (ν = 0) as a matrix on the unique nodes (one column per node group)."""
+    offs, mem = og.ggfx()
+    n = og.ndof
+    uid = -np.ones(n, dtype=np.int64)
+    nu = 0
+    for g in range(len(offs) - 1):
+        uid[mem[offs[g]:offs[g + 1]]] = nu
+        nu += 1
+    for i in range(n):
+        if uid[i] < 0:
+            uid[i] = nu
+            nu += 1
+    rep = np.zeros(nu, dtype=np.int64)
+    rep[uid] = np.arange(n)
+    A = np.zeros((nu, nu))
+    for j in range(nu):
+        e = (uid == j).astype(float)
+        k = og.dss(og.imass_apply(-og.advect(e, c)), "smooth")
+        A[:, j] = k[rep]
+    return A
+
+
+def test_advection_spectrum_flat_box_neutral_sphere_growing():
+    """Why pure advection (ν = 0) on the icosahedral sphere grows without
+    bound: the collocation form of GAdvect on curved (embedded) elements has
+    eigenvalues with positive real part, and they grow with resolution.  On
+    a flat periodic box the same operator with a variable divergence-free
+    swirl is neutral to round-off (max Re λ ~ 1e-13 against |Im λ| ~ 250);
+    on the sphere under solid-body rotation (|c| ≤ 2π) max Re λ is 0.5 at
+    level 0, p=3, and rises with p.  The operator itself is pinned above
+    (gradient and advection closed forms), so the growth is the scheme's:
+    grid-scale modes of the under-integrated, non-skew advection on curved
+    elements, which the reference damps with filters (not on this path)."""
+    og = oracle.Grid.box(2, 5, [4, 4])
+    x, y = og.get("x"), og.get("y")
+    tp = 2 * np.pi
+    c = [tp * np.sin(tp * x) * np.cos(tp * y), -tp * np.cos(tp * x) * np.sin(tp * y)]
+    lam = np.linalg.eigvals(_assembled_advection_matrix(og, c))
+    assert np.abs(lam.imag).max() > 100
+    assert lam.real.max() < 1e-10 * np.abs(lam.imag).max()
+    res = []
+    for p in (3, 5):
+        og = oracle.Grid.icos(p, 0, radius=1.0)
+        X = _sphere_xyz(og)
+        ax = np.array([-np.sin(np.pi / 4), 0.0, np.cos(np.pi / 4)]) * tp
+        c = [ax[1] * X[2] - ax[2] * X[1], ax[2] * X[0] - ax[0] * X[2],
+             ax[0] * X[1] - ax[1] * X[0]]
+        lam = np.linalg.eigvals(_assembled_advection_matrix(og, c))
+        res.append(lam.real.max())
+    assert res[0] > 0.3 and res[1] > 1.3 * res[0], res
