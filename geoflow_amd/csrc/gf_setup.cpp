@@ -19,71 +19,115 @@ namespace {
 
 typedef long double LD;
 
-// Legendre P_n and P_n' at x by the three-term recurrence.
-void legendre(int n, LD x, LD &P, LD &dP) {
-  LD p0 = 1.0L, p1 = x, d0 = 0.0L, d1 = 1.0L;
-  if (n == 0) {
-    P = 1.0L;
-    dP = 0.0L;
-    return;
-  }
+// Legendre P_n(x), P_n'(x), P_{n-1}(x) and P_{n-1}'(x) by the Jacobi
+// three-term recurrence of GLLBasis::computeJacobi (gllbasis.ipp:618-667)
+// with alpha = beta = 0.  The coefficients are exact integers in long double,
+// and every product and quotient is the recurrence's own (a2 = 0 drops out
+// exactly), so the values equal that routine's bit for bit.
+struct Leg {
+  LD p, dp, p1, dp1;  // P_n, P_n', P_{n-1}, P_{n-1}'
+};
+
+Leg legendre(int n, LD x) {
+  Leg r{1.0L, 0.0L, 0.0L, 0.0L};
+  if (n == 0) return r;
+  LD pm = r.p, dm = r.dp;
+  r.p = x;
+  r.dp = 1.0L;
   for (int k = 2; k <= n; k++) {
-    LD p2 = ((2.0L * k - 1.0L) * x * p1 - (k - 1.0L) * p0) / (LD)k;
-    LD d2 = d0 + (2.0L * k - 1.0L) * p1;
-    p0 = p1;
-    p1 = p2;
-    d0 = d1;
-    d1 = d2;
+    const LD fk = (LD)k;
+    const LD a1 = 2.0L * fk * fk * (2.0L * fk - 2.0L);
+    const LD b3 = 2.0L * fk - 2.0L;
+    const LD a3 = b3 * (b3 + 1.0L) * (b3 + 2.0L);
+    const LD a4 = 2.0L * (fk - 1.0L) * (fk - 1.0L) * (2.0L * fk);
+    const LD ax = a3 * x;
+    const LD pn = (ax * r.p - a4 * pm) / a1;
+    const LD dn = (ax * r.dp - a4 * dm + a3 * r.p) / a1;
+    pm = r.p;
+    dm = r.dp;
+    r.p = pn;
+    r.dp = dn;
   }
-  P = p1;
-  dP = d1;
+  r.p1 = pm;
+  r.dp1 = dm;
+  return r;
+}
+
+// GLL nodes in long double, ascending, by GLLBasis::computeNodes'
+// algorithm (gllbasis.ipp:410-501); see gf_gll_basis.
+std::vector<LD> gll_nodes_ld(int p) {
+  const int N = p + 1;
+  std::vector<LD> x(N, 0.0L);
+  // q = P_{p+1} + a P_p + b P_{p-1} vanishes at +1 and -1: a 2x2 solve with
+  // the values the recurrence gives at the end points
+  const Leg hi = legendre(N, 1.0L), lo = legendre(N, -1.0L);         // P_{p+1}, P_p
+  const LD pm_hi = legendre(p, 1.0L).p1, pm_lo = legendre(p, -1.0L).p1;  // P_{p-1}
+  const LD dt = hi.p1 * pm_lo - lo.p1 * pm_hi;
+  const LD rp = -hi.p, rm = -lo.p;
+  const LD ca = (rp * pm_lo - rm * pm_hi) / dt;
+  const LD cb = (rm * hi.p1 - rp * lo.p1) / dt;
+  // nodes from largest to smallest while they are found
+  x[0] = 1.0L;
+  x[p] = -1.0L;
+  const int nh = (p + 1) / 2;
+  // the seed angle is formed in double, as the reference's literal and
+  // operands are (gllbasis.ipp:452), then used in long double
+  const LD th = (LD)(3.1415926535898 / (2.0 * p + 1.0));
+  const LD c2 = std::cos(2.0L * th), s2 = std::sin(2.0L * th);
+  LD c1 = std::cos(th), s1 = std::sin(th);
+  for (int j = 1; j <= nh; j++) {
+    LD xx = c1;
+    for (int it = 0; it < 128; it++) {
+      const Leg a = legendre(N, xx);      // P_{p+1}, P_p and derivatives
+      const Leg b = legendre(p, xx);      // P_{p-1} and its derivative in .p1/.dp1
+      const LD q = a.p + ca * a.p1 + cb * b.p1;
+      const LD dq = a.dp + ca * a.dp1 + cb * b.dp1;
+      LD defl = 0.0L;
+      for (int i = 0; i < j; i++) defl += (1.0L / (xx - x[i]));
+      const LD step = -q / (dq - defl * q);
+      xx += step;
+      if (std::fabs(step) / std::fabs(xx) < (LD)1.0e-8) break;
+    }
+    x[j] = xx;
+    const LD cn = c1 * c2 - s1 * s2;
+    s1 = c1 * s2 + s1 * c2;
+    c1 = cn;
+  }
+  for (int i = 0; i < nh; i++) x[N - 1 - i] = -x[i];
+  if (p % 2 == 0) x[nh] = 0.0L;
+  std::reverse(x.begin(), x.end());
+  return x;
 }
 
 }  // namespace
 
 extern "C" {
 
-// GLL nodes are the roots of (1-x^2) P_p'(x).  Interior roots are found by
-// Newton on q(x) = P_{p+1}(x) - P_{p-1}(x) (proportional to (1-x^2)P_p'),
-// seeded with Chebyshev-Gauss-Lobatto points and iterated to long-double
-// convergence, then symmetrised.  Weights 2/(p(p+1) P_p(x)^2); derivative
-// matrix D(l,j) = P_p(x_l)/(P_p(x_j)(x_l-x_j) + 100 eps_ld) off the diagonal
-// and -/+ p(p+1)/4 at the two corners, zero elsewhere on the diagonal -- the
-// formulas of GLLBasis::computeWeights/computeDerivMatrix
-// (src/cdg/sem/gllbasis.ipp:514-531, 579-607), including the reference's
-// 100*eps(long double) shift in the denominator (gllbasis.ipp:592,596).
+// GLL nodes, weights and derivative matrix with the reference's algorithm,
+// so that GeoFLOW's own D and this library's agree bit for bit:
+//  * nodes (GLLBasis::computeNodes, gllbasis.ipp:410-501): the interior roots
+//    of q = P_{p+1} + a P_p + b P_{p-1}, with a, b chosen so q(+-1) = 0, by
+//    Newton iteration with deflation of the roots already found, seeded at
+//    cos((2j-1) pi/(2p+1)) (advanced by the angle-addition recurrence) and
+//    stopped when the relative step falls below 1e-8 (at most 128 steps);
+//    the upper half is found, mirrored, and the centre set to 0 for even p;
+//  * weights 2/(p(p+1) P_p(x)^2) (computeWeights, :514-531);
+//  * D(l,j) = P_p(x_l)/(P_p(x_j)(x_l-x_j) + 100 eps_ld) off the diagonal and
+//    -/+ p(p+1)/4 at the two corners (computeDerivMatrix, :579-607).
+// All of it in long double, cast to double at the end, except where the
+// reference's own expressions are double: the seed angle (its literal
+// 3.1415926535898 divided by 2p+1), the stopping threshold and the weight
+// factor 2/(p(p+1)).
 int gf_gll_basis(int p, double *xi, double *w, double *D) {
   if (p < 1 || p > 64 || !xi || !w || !D) return GF_ERR_ARG;
-  int N = p + 1;
-  std::vector<LD> x(N), P(N);
-  x[0] = -1.0L;
-  x[p] = 1.0L;
-  const LD pi = 3.14159265358979323846264338327950288L;
-  for (int j = 1; j < p; j++) {
-    LD xx = -std::cos(pi * (LD)j / (LD)p);
-    for (int it = 0; it < 100; it++) {
-      LD Pa, dPa, Pb, dPb;
-      legendre(p + 1, xx, Pa, dPa);
-      legendre(p - 1, xx, Pb, dPb);
-      LD f = Pa - Pb, df = dPa - dPb;
-      LD dx = f / df;
-      xx -= dx;
-      if (std::fabs(dx) <= 4.0L * std::numeric_limits<LD>::epsilon() * (1.0L + std::fabs(xx))) break;
-    }
-    x[j] = xx;
-  }
-  for (int j = 0; j < N / 2; j++) {  // exact symmetry about 0
-    LD s = 0.5L * (x[N - 1 - j] - x[j]);
-    x[j] = -s;
-    x[N - 1 - j] = s;
-  }
-  if (N % 2 == 1) x[N / 2] = 0.0L;
-  const LD fact = 2.0L / ((LD)p * ((LD)p + 1.0L));
+  const int N = p + 1;
+  const std::vector<LD> x = gll_nodes_ld(p);
+  std::vector<LD> P(N);
+  const LD fact = (LD)(2.0 / (p * (p + 1.0)));  // formed in double (gllbasis.ipp:520)
   for (int i = 0; i < N; i++) {
-    LD dP;
-    legendre(p, x[i], P[i], dP);
+    P[i] = legendre(p, x[i]).p;
     xi[i] = (double)x[i];
-    w[i] = (double)(fact / (P[i] * P[i]));
+    w[i] = (double)(P[i] == 0.0L ? 0.0L : fact / (P[i] * P[i]));
   }
   const LD ttiny = 100.0L * std::numeric_limits<LD>::epsilon();
   for (int j = 0; j < N; j++)
@@ -92,8 +136,9 @@ int gf_gll_basis(int p, double *xi, double *w, double *D) {
       if (l != j) v = P[l] / (P[j] * (x[l] - x[j]) + ttiny);
       D[l + (size_t)j * N] = (double)v;
     }
-  D[0] = (double)(-0.25L * p * (p + 1.0L));
-  D[(size_t)p + (size_t)p * N] = (double)(0.25L * p * (p + 1.0L));
+  const LD d00 = -0.25L * p * (p + 1.0L);
+  D[0] = (double)d00;
+  D[(size_t)p + (size_t)p * N] = (double)(-d00);
   return GF_OK;
 }
 
@@ -397,31 +442,11 @@ struct LDBasis {
   std::vector<double> xd;
 };
 
-// Jacobi (here Legendre) recurrence of GLLBasis::computeJacobi
-// (gllbasis.ipp:618-667) with alpha = beta = 0: P_N(x) and P_N'(x).
+// P_N(x) and P_N'(x) by GLLBasis::computeJacobi's recurrence (see legendre).
 void jacobi_ld(int N, LD x, LD &poly, LD &pder) {
-  const LD apb = 0.0L, alpha = 0.0L, beta = 0.0L, rv = 1.0L;
-  poly = 1.0L;
-  pder = 0.0L;
-  if (N == 0) return;
-  LD polylist = poly, pderlist = pder;
-  poly = rv * x;
-  pder = rv;
-  if (N == 1) return;
-  for (int k = 2; k <= N; k++) {
-    const LD fk = (LD)k;
-    const LD a1 = 2.0L * fk * (fk + apb) * (2.0L * fk + apb - 2.0L);
-    const LD a2 = (2.0L * fk + apb - 1.0L) * (alpha * alpha - beta * beta);
-    const LD b3 = 2.0L * fk + apb - 2.0L;
-    const LD a3 = b3 * (b3 + 1.0L) * (b3 + 2.0L);
-    const LD a4 = 2.0L * (fk + alpha - 1.0L) * (fk + beta - 1.0L) * (2.0L * fk + apb);
-    const LD polyn = ((a2 + a3 * x) * poly - a4 * polylist) / a1;
-    const LD pdern = ((a2 + a3 * x) * pder - a4 * pderlist + a3 * poly) / a1;
-    polylist = poly;
-    poly = polyn;
-    pderlist = pder;
-    pder = pdern;
-  }
+  const Leg r = legendre(N, x);
+  poly = r.p;
+  pder = r.dp;
 }
 
 // h_i(eta) of GLLBasis::evalBasis (2) (gllbasis.ipp:1300-1327).
@@ -472,21 +497,13 @@ LDBasis make_ldbasis(int p) {
   gf_gll_basis(p, b.xd.data(), w.data(), D.data());
   b.x.resize(p + 1);
   b.Pn.resize(p + 1);
+  // the long-double nodes themselves (GLLBasis::xiNodes_), as evalBasis and
+  // evalDBasis use them
+  b.x = gll_nodes_ld(p);
+  b.Pn.resize(p + 1);
   for (int i = 0; i <= p; i++) {
-    // nodes in long double: the double nodes are the long-double nodes
-    // rounded, so refine once by Newton on (1-x^2)P_p'(x) from them
-    LD xx = (LD)b.xd[i];
-    if (i > 0 && i < p) {
-      for (int it = 0; it < 4; it++) {
-        LD Pa, dPa, Pb, dPb;
-        legendre(p + 1, xx, Pa, dPa);
-        legendre(p - 1, xx, Pb, dPb);
-        xx -= (Pa - Pb) / (dPa - dPb);
-      }
-    }
-    b.x[i] = xx;
     LD dP;
-    jacobi_ld(p, xx, b.Pn[i], dP);
+    jacobi_ld(p, b.x[i], b.Pn[i], dP);
   }
   return b;
 }

@@ -73,7 +73,7 @@ struct Basis {
 bool gll(int Np, Basis &b) {
   if (Np < 1) return false;
   const int kstop = 128;
-  const LD eps = 1.0e-8L;
+  const LD eps = 1.0e-8;  // eps_(1.0e-8), a double literal (gllbasis.hpp:43)
   const LD ttiny = 100.0L * std::numeric_limits<LD>::epsilon();
   const LD ximin = -1.0L, ximax = 1.0L;
   const LD alpha = 0, beta = 0;
@@ -90,8 +90,9 @@ bool gll(int Np, Basis &b) {
   x[Np] = ximin;
   int nh = (Np + 1) / 2;
   // gllbasis.ipp:452-456: the reference's literal pi and libm trig seed
-  // the Newton iteration.
-  LD dth = 3.1415926535898L / (2.0L * Np + 1.0L);
+  // the Newton iteration; the literal and 2.0*Np_+1.0 are double, so the
+  // quotient is formed in double and then held in long double.
+  LD dth = 3.1415926535898 / (2.0 * Np + 1.0);
   LD cd = std::cos(2.0L * dth), sd = std::sin(2.0L * dth);
   LD cs = std::cos(dth), ss = std::sin(dth);
   for (int j = 1; j < nh + 1; j++) {
@@ -121,7 +122,7 @@ bool gll(int Np, Basis &b) {
     x[i] = t;
   }
   // computeWeights (gllbasis.ipp:514-531)
-  LD fact = 2.0L / (Np * (Np + 1.0L));
+  LD fact = 2.0 / (Np * (Np + 1.0));  // double operands (gllbasis.ipp:520)
   for (int i = 0; i < Np + 1; i++) {
     LD ppn, pder, pm1, pdm1, pm2, pdm2;
     jacobi(Np, alpha, beta, ppn, pder, pm1, pdm1, pm2, pdm2, x[i]);

@@ -318,6 +318,27 @@ def test_box_multistep_equals_single_steps(ctx, order):
         assert np.array_equal(host(a[c]), host(b[c])), c
 
 
+def test_box_multistep_p7_against_oracle(ctx):
+    """The benchmark's own kernels at N=8 (p=7) against the oracle: one
+    gf_burgers_steps call of n steps (first stage, middle stages, final-stage
+    form, and for n >= 2 the output fused with the next step's first stage)
+    on a periodic 4x3x3 box, for n = 1..6, each compared with n RK4 steps of
+    the oracle (GExRKStepper::step_b, gexrk_stepper.ipp:251-341) at 1e-12."""
+    print("box multistep p7 vs oracle", flush=True)
+    c = dict(dim=3, p=7, ne=[4, 3, 3], bdy=[6] * 6)
+    og, bg, m = make(ctx, c)
+    assert m.flags & 1
+    us = workloads.fourier_field_3d(*bg.xyz, seed=21, kmax=3)
+    ref = [u.copy() for u in us]
+    nu, dt = 0.02, 2e-4
+    for n in range(1, 7):
+        og.burgers_step(ref, nu, dt, t=(n - 1) * dt)
+        dus = [dev(u) for u in us]
+        m.burgers_steps(dus, nu, dt, n)
+        for k in range(3):
+            assert relerr(host(dus[k]), ref[k]) <= TOL, (n, k)
+
+
 @pytest.mark.parametrize("bdy", [[0] * 6, [6] * 6], ids=["dirichlet", "periodic"])
 def test_cg_gather_path_matches_csr_path(ctx, bdy):
     """Structured-box CG (gather assembly in the update, mask from the box
