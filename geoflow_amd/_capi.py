@@ -32,6 +32,16 @@ class GeoflowError(RuntimeError):
 GF_MESH_BOX_GATHER = 1
 GF_MESH_BOX_MASK = 2
 GF_MESH_CONST_D = 4
+GF_ERR_PRECOND = 3
+GF_ERR_ALLOC = 12
+GF_RESERVE_RK, GF_RESERVE_CG, GF_RESERVE_CG_MIXED, GF_RESERVE_ADJOINT = 1, 2, 4, 8
+GF_RESERVE_FREEZE = 256
+
+
+class ReserveParams(ctypes.Structure):
+    _fields_ = [("nstate", ctypes.c_int), ("cg_maxit", ctypes.c_int),
+                ("adj_nsteps", ctypes.c_int), ("adj_ckpt", ctypes.c_int),
+                ("adj_order", ctypes.c_int)]
 
 
 class MeshDesc(ctypes.Structure):
@@ -104,6 +114,7 @@ SIGNATURES = {
     "gf_mesh_ndof": (ctypes.c_size_t, [_vp]),
     "gf_mesh_flags": (ctypes.c_int, [_vp]),
     "gf_mesh_get": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
+    "gf_mesh_reserve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ReserveParams)]),
     "gf_malloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "gf_free": (ctypes.c_int, [_vp, _vp]),
     "gf_memcpy_h2d": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),

@@ -252,18 +252,13 @@ int gf_burgers_adjoint(gf_mesh *m, double nu, int order, int nstate, double dt, 
   if (nstate != nvel_of(m)) return gf_set_err(m->ctx, GF_ERR_ARG, "adjoint: nonlinear mode needs nstate == velocity components");
   const size_t bytes = m->ndof * sizeof(double);
   const int ncp = (nsteps + ckpt - 1) / ckpt;  // checkpoints at steps 0, ckpt, 2 ckpt, ...
-  // checkpoint and segment storage: (ncp + ckpt) states of nstate fields
+  // checkpoint and segment storage: (ncp + ckpt + 1) states of nstate
+  // fields, in the mesh workspace (gf_mesh_reserve can allocate it ahead)
   std::vector<double *> store((size_t)(ncp + ckpt + 1) * nstate, nullptr);
-  auto release = [&]() {
-    for (double *p : store)
-      if (p) (void)hipFree(p);
-  };
-  for (auto &p : store)
-    if (hipMalloc((void **)&p, bytes) != hipSuccess) {
-      p = nullptr;
-      release();
-      return gf_set_err(m->ctx, GF_ERR_ALLOC, "adjoint checkpoints");
-    }
+  for (size_t i = 0; i < store.size(); i++) {
+    store[i] = (double *)gf_ws(m, WS_ADJ_CKPT + (int)i, bytes);
+    if (!store[i]) return gf_set_err(m->ctx, GF_ERR_ALLOC, "adjoint checkpoint workspace");
+  }
   auto cp = [&](int i) { return &store[(size_t)i * nstate]; };
   auto seg = [&](int i) { return &store[(size_t)(ncp + i) * nstate]; };
   auto run = [&]() -> int {
@@ -300,9 +295,7 @@ int gf_burgers_adjoint(gf_mesh *m, double nu, int order, int nstate, double dt, 
     GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
     return GF_OK;
   };
-  int rc = run();
-  release();
-  return rc;
+  return run();
 }
 
 }  // extern "C"

@@ -62,10 +62,17 @@ struct gf_mesh_s {
   // has_box_mask: the boundary mask equals gf::box_masked for `box`, so
   // kernels may form it from the node position instead of reading it.
   bool has_box_mask = false;
-  // reusable device workspace
+  // reusable device workspace; ws_frozen: gf_mesh_reserve(GF_RESERVE_FREEZE)
+  // was called, so gf_ws hands out only what is already allocated
   std::vector<void *> ws;
   std::vector<size_t> ws_cap;
-  double *red = nullptr;    // reduction partials, GF_RED_MAX_BLOCKS*8 doubles
+  bool ws_frozen = false;
+  // FP32 copies of the node metrics G[6] and the lumped mass for the
+  // mixed-precision CG on embedded and deformed meshes (made at creation)
+  float *G32[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  float *mass32 = nullptr;
+  double *red = nullptr;    // reduction partials, red_cap doubles
+  size_t red_cap = 0;
   double *scal = nullptr;   // solver scalars
   double *hscal = nullptr;  // pinned host mirror of scal
 };
@@ -103,7 +110,11 @@ enum {
   WS_RK_BEGIN = 8,          // RK: acc[3], K[3][3], u3[3]
   WS_CG_BEGIN = 32,         // CG vectors
   WS_ADJ_TMP = 48,          // adjoint RHS scratch (6 slots)
-  WS_G32 = 54,              // mixed CG: FP32 node metrics G[6] and mass (7 slots)
   WS_SLOTS = 64,            // kernel-timing scratch (16 slots)
-  WS_ADJ_BEGIN = 96         // adjoint stage states and cotangents
+  WS_ADJ_BEGIN = 96,        // adjoint stage states and cotangents (up to 36 slots)
+  WS_ADJ_CKPT = 160         // adjoint checkpoints and segment states
 };
+
+// Reduction partials a mesh needs: one per apply block (at most one block
+// per element) plus the update kernels' pairs.
+size_t gf_red_need(size_t nelems);

@@ -31,12 +31,22 @@ int gf_check_hip(gf_ctx *ctx, hipError_t e, const char *what) {
   return e == hipErrorOutOfMemory ? GF_ERR_ALLOC : GF_ERR_HIP;
 }
 
+size_t gf_red_need(size_t nelems) {
+  // apply partials (one per block, at most one block per element) at the
+  // front, then the update kernels' (rho, norm) pairs and the set-up slots
+  const size_t need = nelems + 64 + 2 * 1024 + 128;
+  const size_t base = (size_t)GF_RED_MAX_BLOCKS * 8;
+  return need > base ? need : base;
+}
+
 void *gf_ws(gf_mesh *m, int slot, size_t bytes) {
   if ((int)m->ws.size() <= slot) {
+    if (m->ws_frozen) return nullptr;
     m->ws.resize(slot + 1, nullptr);
     m->ws_cap.resize(slot + 1, 0);
   }
   if (m->ws[slot] && m->ws_cap[slot] >= bytes) return m->ws[slot];
+  if (m->ws_frozen) return nullptr;  // reserved workspace only
   if (m->ws[slot]) (void)hipFree(m->ws[slot]);
   m->ws[slot] = nullptr;
   m->ws_cap[slot] = 0;
@@ -476,6 +486,12 @@ __global__ void k_recip(double *__restrict__ a, size_t n) {
   for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < n;
        g += (size_t)gridDim.x * blockDim.x)
     a[g] = 1.0 / a[g];
+}
+
+__global__ void k_narrow_f32(float *__restrict__ o, const double *__restrict__ a, size_t n) {
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < n;
+       g += (size_t)gridDim.x * blockDim.x)
+    o[g] = (float)a[g];
 }
 
 // ---- DSS over coincidence groups --------------------------------------------
@@ -933,7 +949,8 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     m->has_box = ok;
     m->has_box_mask = ok && mask_ok;
   }
-  GF_HIP(ctx, hipMalloc((void **)&m->red, GF_RED_MAX_BLOCKS * 8 * sizeof(double)));
+  m->red_cap = gf_red_need(m->nelems);
+  GF_HIP(ctx, hipMalloc((void **)&m->red, m->red_cap * sizeof(double)));
   GF_HIP(ctx, hipMalloc((void **)&m->scal, 64 * sizeof(double)));
   GF_HIP(ctx, hipHostMalloc((void **)&m->hscal, 64 * sizeof(double), hipHostMallocDefault));
   GF_HIP(ctx, hipMalloc((void **)&m->mass, n * sizeof(double)));
@@ -955,6 +972,18 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
   rc = gf_dss_t(m, GF_DSS_SMOOTH, m->imass);
   if (rc) { gf_mesh_destroy(m); return rc; }
   hipLaunchKernelGGL(k_recip, dim3(nblocks(n)), dim3(kThreads), 0, s, m->imass, n);
+  if (d->gtype != GF_GE_REGULAR) {
+    // FP32 copies of the node metrics and the mass for the mixed-precision CG
+    for (int c = 0; c < 7; c++) {
+      const double *src = c < 6 ? m->G[c] : m->mass;
+      if (!src) continue;
+      float *dst = nullptr;
+      GF_HIP(ctx, hipMalloc((void **)&dst, n * sizeof(float)));
+      if (c < 6) m->G32[c] = dst;
+      else m->mass32 = dst;
+      hipLaunchKernelGGL(k_narrow_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, dst, src, n);
+    }
+  }
   hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kThreads), 0, s, m->imult, 1.0, n);
   if (m->ngroups)
     hipLaunchKernelGGL(k_imult_groups, dim3(nblocks((size_t)m->ngroups)), dim3(kThreads), 0, s,
@@ -999,6 +1028,8 @@ int gf_mesh_destroy(gf_mesh *m) {
   F(m->D32); F(m->W32); F(m->edx); F(m->ejac); F(m->edx32); F(m->ejac32); F(m->njac);
   F(m->mask); F(m->mass); F(m->imass); F(m->imult); F(m->mask32); F(m->imult32);
   F(m->goff); F(m->gmem); F(m->red); F(m->scal);
+  for (int k = 0; k < 6; k++) F(m->G32[k]);
+  F(m->mass32);
   if (m->hscal) (void)hipHostFree(m->hscal);
   for (size_t k = 0; k < m->ws.size(); k++) F(m->ws[k]);
   delete m;

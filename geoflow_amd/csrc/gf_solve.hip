@@ -582,11 +582,15 @@ __global__ void k_helm_diag_g(double *__restrict__ d, const double *__restrict__
     d[g] = val;
   }
 }
-__global__ void k_recip2(double *__restrict__ a, float *__restrict__ af, size_t n) {
+// dinv = 1/d; *bad = 1 when some entry of d is not positive (or NaN)
+__global__ void k_recip_pos(double *__restrict__ a, double *__restrict__ bad, size_t n) {
+  bool any = false;
   GRID_LOOP(g, n) {
-    a[g] = 1.0 / a[g];
-    if (af) af[g] = (float)a[g];
+    const double d = a[g];
+    any = any || !(d > 0.0);
+    a[g] = 1.0 / d;
   }
+  if (any) *bad = 1.0;
 }
 
 template <int DIM, int N>
@@ -1254,9 +1258,19 @@ int gf_helmholtz_diag_inv(gf_mesh *m, double p, double qs, const double *qf, dou
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Jacobi diagonal: isotropic meshes");
   RC(GF_DISPATCH_N(m->dim, m->Niso, launch_diag, m, p, qf, qs, dinv));
   RC(gf_dss_t(m, GF_DSS_SUM, dinv));
-  hipLaunchKernelGGL(k_recip2, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, dinv,
-                     (float *)nullptr, m->ndof);
+  // a Jacobi preconditioner needs a positive assembled diagonal; a strongly
+  // sheared deformed element (whose doubled off-diagonal metric terms enter
+  // it) or p = q = 0 can break that
+  double *bad = m->scal + 63;
+  GF_HIP(m->ctx, hipMemsetAsync(bad, 0, sizeof(double), m->ctx->stream));
+  hipLaunchKernelGGL(k_recip_pos, dim3(nblocks(m->ndof)), dim3(kThreads), 0, m->ctx->stream, dinv,
+                     bad, m->ndof);
   GF_LAUNCH_CHECK(m->ctx);
+  double hb = 0.0;
+  GF_HIP(m->ctx, hipMemcpyAsync(&hb, bad, sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+  GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+  if (hb != 0.0)
+    return gf_set_err(m->ctx, GF_ERR_PRECOND, "Jacobi: assembled diagonal not positive");
   return GF_OK;
 }
 
@@ -1438,16 +1452,6 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
   double *d_resid = (double *)gf_ws(m, WS_CG_BEGIN + 4, (size_t)(prm->maxit + 8) * sizeof(double));
   if (!r || !w || !q || (prm->precond && !dinv) || !d_resid)
     return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
-  // partial-sum buffer large enough for one partial per apply block
-  {
-    size_t need = (m->nelems + 64) + 2 * kUpdBlocks + 64;  // >= apply blocks + update partials
-    if (need > (size_t)GF_RED_MAX_BLOCKS * 8) {
-      double *nr = nullptr;
-      GF_HIP(m->ctx, hipMalloc((void **)&nr, need * sizeof(double)));
-      (void)hipFree(m->red);
-      m->red = nr;
-    }
-  }
   double volume = 0.0;
   {
     double *one = w;
@@ -1565,21 +1569,10 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     v.W = m->W32;
     v.edx = m->edx32;
     v.ejac = m->ejac32;
-    for (int c = 0; c < 6; c++) v.G[c] = nullptr;
-    v.mass = nullptr;
-    if (m->gtype != GF_GE_REGULAR) {
-      // FP32 copies of the node metrics and the lumped mass
-      for (int c = 0; c < 7; c++) {
-        const double *src = c < 6 ? m->G[c] : m->mass;
-        if (!src) continue;
-        float *dst = (float *)gf_ws(m, WS_G32 + c, n * sizeof(float));
-        if (!dst) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
-        hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, dst, src, 1.0, n);
-        if (c < 6) v.G[c] = dst;
-        else v.mass = dst;
-      }
-      GF_LAUNCH_CHECK(m->ctx);
-    }
+    // FP32 copies of the node metrics and the lumped mass (made at mesh
+    // creation for embedded and deformed meshes)
+    for (int c = 0; c < 6; c++) v.G[c] = m->G32[c];
+    v.mass = m->gtype != GF_GE_REGULAR ? m->mass32 : nullptr;
     v.p = prm->p_scalar;
     int hist = 0;
     bool first = true;
@@ -1629,6 +1622,48 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
   res->outer = outer;
   res->rnorm = rn_final;
   return status;
+}
+
+// ===========================================================================
+// Workspace reservation: the slots each family of entry points takes from
+// the mesh workspace (gf_ws), allocated ahead of the calls.
+// ===========================================================================
+extern "C" int gf_mesh_reserve(gf_mesh *m, int what, const gf_reserve_params *prm) {
+  if (!m) return GF_ERR_ARG;
+  const size_t n = m->ndof, bytes = n * sizeof(double);
+  const gf_reserve_params z = {0, 0, 0, 0, 0};
+  const gf_reserve_params &p = prm ? *prm : z;
+  auto want = [&](int slot, size_t b) -> int {
+    if (!gf_ws(m, slot, b)) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace reservation");
+    return GF_OK;
+  };
+  if (what & (GF_RESERVE_RK | GF_RESERVE_ADJOINT)) {
+    // every RK form (fused, structured box, embedded advection, composed)
+    // and the composed operators (gf_burgers_rhs, gf_advect, gf_deriv)
+    for (int k = 0; k < 16; k++) RC(want(WS_RK_BEGIN + k, bytes));
+    for (int k = WS_TMP0; k <= WS_TMP3; k++) RC(want(k, bytes));
+  }
+  if (what & (GF_RESERVE_CG | GF_RESERVE_CG_MIXED)) {
+    for (int k = 0; k < 4; k++) RC(want(WS_CG_BEGIN + k, bytes));
+    const int maxit = p.cg_maxit > 0 ? p.cg_maxit : 512;
+    RC(want(WS_CG_BEGIN + 4, (size_t)(maxit + 8) * sizeof(double)));
+  }
+  if (what & GF_RESERVE_CG_MIXED)
+    for (int k = 5; k <= 10; k++) RC(want(WS_CG_BEGIN + k, n * sizeof(float)));
+  if (what & GF_RESERVE_ADJOINT) {
+    const int nv = m->gtype == GF_GE_2DEMBEDDED ? 3 : m->dim;
+    const int order = p.adj_order >= 1 && p.adj_order <= 4 ? p.adj_order : 4;
+    for (int k = 0; k < 6; k++) RC(want(WS_ADJ_TMP + k, bytes));
+    for (int k = 0; k < 2 * order * nv + 4 * nv; k++) RC(want(WS_ADJ_BEGIN + k, bytes));
+    if (p.adj_nsteps > 0 && p.adj_ckpt > 0) {
+      const int ns = p.nstate > 0 ? p.nstate : nv;
+      const int ncp = (p.adj_nsteps + p.adj_ckpt - 1) / p.adj_ckpt;
+      for (int k = 0; k < (ncp + p.adj_ckpt + 1) * ns; k++) RC(want(WS_ADJ_CKPT + k, bytes));
+    }
+  }
+  if (what & GF_RESERVE_FREEZE) m->ws_frozen = true;
+  GF_HIP(m->ctx, hipDeviceSynchronize());
+  return GF_OK;
 }
 
 // ===========================================================================
@@ -1702,13 +1737,6 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
     hs[S_FIRST] = 0.0;
     hs[S_DONE] = 0.0;
     GF_HIP(m->ctx, hipMemcpyAsync(m->scal, hs, sizeof(hs), hipMemcpyHostToDevice, s));
-    size_t need = m->nelems + 64;
-    if (need > (size_t)GF_RED_MAX_BLOCKS * 8) {
-      double *nr = nullptr;
-      GF_HIP(m->ctx, hipMalloc((void **)&nr, (need + 2 * kUpdBlocks + 64) * sizeof(double)));
-      (void)hipFree(m->red);
-      m->red = nr;
-    }
     if (f32) {
       for (int k = 0; k < 4; k++)
         hipLaunchKernelGGL(k_fill_pattern_f, dim3(nblocks(n)), dim3(kThreads), 0, s,

@@ -271,6 +271,13 @@ class Mesh:
         self._c(lib().gf_time_kernel(self.h, which.encode(), reps, ctypes.byref(ms)))
         return ms.value
 
+    def reserve(self, what, nstate=0, cg_maxit=0, adj_nsteps=0, adj_ckpt=0, adj_order=4):
+        """gf_mesh_reserve: allocate the workspace of the selected entry
+        points now (what: GF_RESERVE_* flags, GF_RESERVE_FREEZE to forbid
+        later allocation)."""
+        prm = _capi.ReserveParams(nstate, cg_maxit, adj_nsteps, adj_ckpt, adj_order)
+        self._c(lib().gf_mesh_reserve(self.h, int(what), ctypes.byref(prm)))
+
     def helmholtz_diag_inv(self, p=1.0, q=None):
         out = self.new()
         qf, qs = _q_args(q)

@@ -13,8 +13,10 @@
  *
  * Threading: one gf_ctx per host thread.  Operator calls are enqueued on the
  * context's HIP stream; they return once enqueued unless documented as
- * synchronous (those that return host scalars).  Hot-path entry points do
- * not allocate device memory after a mesh's first use of that entry point.
+ * synchronous (those that return host scalars).  Hot-path entry points take
+ * device scratch from the mesh's workspace, allocated on a mesh's first use
+ * of that entry point or up front by gf_mesh_reserve; after that they do not
+ * allocate.
  */
 #ifndef GEOFLOW_HIP_H
 #define GEOFLOW_HIP_H
@@ -181,7 +183,7 @@ int gf_ggfx_build(size_t ndof, int ncoord, const double *const *xyz, double tol,
 /* ---- mesh (device-resident operator data) ------------------------------ */
 typedef struct {
   int dim;               /* 2 or 3 (GDIM is a runtime value here)                  */
-  int gtype;             /* GF_GE_REGULAR, GF_GE_2DEMBEDDED or (2-D) GF_GE_DEFORMED */
+  int gtype;             /* GF_GE_REGULAR, GF_GE_2DEMBEDDED or GF_GE_DEFORMED (2-D, 3-D) */
   int N[3];              /* nodes per direction, p+1 (N[2] ignored when dim == 2)   */
   size_t nelems;
   const double *D[3];    /* host: column-major N x N derivative matrices            */
@@ -191,7 +193,7 @@ typedef struct {
   const double *elem_jac;    /* nelems     */
   /* GE_2DEMBEDDED and GE_DEFORMED: per-node metrics (host), dXi_i/dX_j,
    * i < dim, j < nxy (3 embedded, dim deformed) at node_dxidx[i*3+j], and
-   * node Jacobian. */
+   * node Jacobian (deformed: node_dxidx[i*3+j] for i, j < dim). */
   const double *node_dxidx[9];
   const double *node_jac;
   const double *xyz[3];  /* host node coordinates (optional; needed for sphere ops) */
@@ -221,6 +223,36 @@ int gf_mesh_flags(const gf_mesh *mesh);
 /* Copy a device-resident mesh array to host: "mass", "imass", "imult",
  * "mask" (ndof doubles each).  Synchronous. */
 int gf_mesh_get(gf_mesh *mesh, const char *which, double *host_out);
+
+/* Workspace reservation (no reference counterpart: GeoFLOW's callers pass
+ * scratch pools, State& utmp, to every operator; this library keeps them per
+ * mesh).  Operators take device scratch from the mesh workspace, allocated
+ * on first use.  gf_mesh_reserve allocates up front what the selected entry
+ * points need, so later calls on this mesh allocate nothing:
+ *   GF_RESERVE_RK       gf_burgers_step(s), gf_burgers_rhs, gf_advect, gf_deriv
+ *   GF_RESERVE_CG       gf_cg_solve, FP64 (residual history of cg_maxit entries)
+ *   GF_RESERVE_CG_MIXED gf_cg_solve, mixed precision (FP32 recurrence vectors)
+ *   GF_RESERVE_ADJOINT  gf_burgers_step_adjoint, gf_burgers_adjoint with
+ *                       adj_nsteps steps, checkpoints every adj_ckpt, nstate
+ *                       components and RK order adj_order
+ *   GF_RESERVE_FREEZE   from now on a call that would need more workspace
+ *                       returns GF_ERR_ALLOC instead of allocating.
+ * Synchronous. */
+enum {
+  GF_RESERVE_RK = 1,
+  GF_RESERVE_CG = 2,
+  GF_RESERVE_CG_MIXED = 4,
+  GF_RESERVE_ADJOINT = 8,
+  GF_RESERVE_FREEZE = 256
+};
+typedef struct {
+  int nstate;      /* state components (RK, adjoint) */
+  int cg_maxit;    /* CG iteration limit (residual history) */
+  int adj_nsteps;  /* adjoint window length in steps */
+  int adj_ckpt;    /* adjoint checkpoint interval */
+  int adj_order;   /* adjoint RK order (1..4) */
+} gf_reserve_params;
+int gf_mesh_reserve(gf_mesh *mesh, int what, const gf_reserve_params *prm);
 
 /* ---- device memory helpers (optional; any device allocation works) ----- */
 int gf_malloc(gf_ctx *ctx, size_t bytes, void **dptr);
@@ -304,7 +336,9 @@ int gf_burgers_steps(gf_mesh *mesh, int mode, double nu, int order, int nstate, 
  * gf_burgers_adjoint: for u(T) = nsteps steps from u0, lam (in dJ/du(T))
  *   becomes dJ/du(0); states are checkpointed every ckpt steps on the
  *   forward sweep and each segment is recomputed on the reverse sweep.  uT
- *   (optional) receives u(T).  Synchronous; allocates its checkpoints. */
+ *   (optional) receives u(T).  Synchronous.  The checkpoints live in the
+ *   mesh workspace ((ceil(nsteps/ckpt) + ckpt + 1) nstate fields), kept for
+ *   later calls; gf_mesh_reserve(GF_RESERVE_ADJOINT) allocates them ahead. */
 int gf_burgers_rhs_adjoint(gf_mesh *mesh, double nu, int nstate, const double *const *u,
                            const double *const *w, double *const *lam);
 int gf_burgers_step_adjoint(gf_mesh *mesh, double nu, int order, int nstate, double dt,
@@ -345,10 +379,13 @@ typedef struct {
  * precision keeps FP32 copies of the node metrics in the mesh workspace). */
 int gf_cg_solve(gf_mesh *mesh, const gf_cg_params *prm, const double *b, const double *xb,
                 double *x, gf_cg_result *res);
+
 /* Assembled Jacobi diagonal inverse of (p L + q M), q as in gf_helmholtz_apply;
  * on embedded and deformed meshes the diagonal includes the off-diagonal
  * metric terms.  No reference counterpart (GCG's default preconditioner is
- * the identity, gcg.ipp:38). */
+ * the identity, gcg.ipp:38).  Returns GF_ERR_PRECOND (and leaves dinv
+ * holding 1/diagonal) when an assembled diagonal entry is not positive, as
+ * a sheared deformed element or p = q = 0 can make it. */
 int gf_helmholtz_diag_inv(gf_mesh *mesh, double p_scalar, double q_scalar, const double *q_field,
                           double *dinv);
 
