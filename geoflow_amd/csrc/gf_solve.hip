@@ -1433,25 +1433,35 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
 
 }  // namespace
 
-extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b, const double *xb,
-                           double *x, gf_cg_result *res) {
+namespace {
+
+// Device vectors of one CG solve: FP64 residual, direction, operator output,
+// Jacobi diagonal inverse and residual history; the FP32 recurrence vectors
+// of the mixed solver.
+struct CGWork {
+  double *r = nullptr, *w = nullptr, *q = nullptr, *dinv = nullptr, *resid = nullptr;
+  float *r32 = nullptr, *w32 = nullptr, *q32 = nullptr, *x32 = nullptr, *dinv32 = nullptr,
+        *qf32 = nullptr;
+};
+
+int cg_check_args(gf_mesh *m, const gf_cg_params *prm, const double *b, double *x,
+                  gf_cg_result *res) {
   if (!m || !prm || !b || !x || !res) return GF_ERR_ARG;
   memset(res, 0, sizeof(*res));
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: isotropic meshes");
   if (prm->maxit < 1) return gf_set_err(m->ctx, GF_ERR_BADITERNUM, "maxit < 1");
   if (prm->normtype < GF_NORM_INF || prm->normtype > GF_NORM_ASSEMBLED_L2) return GF_ERR_ARG;
   if (prm->p_field) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: p field not supported");
+  return GF_OK;
+}
+
+int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const double *xb,
+                  double *x, gf_cg_result *res, const CGWork &wk) {
   hipStream_t s = m->ctx->stream;
-  size_t n = m->ndof, bytes = n * sizeof(double);
+  size_t n = m->ndof;
   bool bbv = xb != nullptr;
   const double *mask = bbv ? m->mask : nullptr;
-  double *r = (double *)gf_ws(m, WS_CG_BEGIN + 0, bytes);
-  double *w = (double *)gf_ws(m, WS_CG_BEGIN + 1, bytes);
-  double *q = (double *)gf_ws(m, WS_CG_BEGIN + 2, bytes);
-  double *dinv = prm->precond ? (double *)gf_ws(m, WS_CG_BEGIN + 3, bytes) : nullptr;
-  double *d_resid = (double *)gf_ws(m, WS_CG_BEGIN + 4, (size_t)(prm->maxit + 8) * sizeof(double));
-  if (!r || !w || !q || (prm->precond && !dinv) || !d_resid)
-    return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  double *r = wk.r, *w = wk.w, *q = wk.q, *dinv = wk.dinv, *d_resid = wk.resid;
   double volume = 0.0;
   {
     double *one = w;
@@ -1544,14 +1554,9 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
     // search direction (reliable updates, van der Vorst & Ye, SIAM J. Sci.
     // Comput. 22 (2000) 1663-1683).  Convergence is judged on the FP64
     // residual only.
-    float *r32 = (float *)gf_ws(m, WS_CG_BEGIN + 5, n * sizeof(float));
-    float *w32 = (float *)gf_ws(m, WS_CG_BEGIN + 6, n * sizeof(float));
-    float *q32 = (float *)gf_ws(m, WS_CG_BEGIN + 7, n * sizeof(float));
-    float *x32 = (float *)gf_ws(m, WS_CG_BEGIN + 8, n * sizeof(float));
-    float *dinv32 = dinv ? (float *)gf_ws(m, WS_CG_BEGIN + 9, n * sizeof(float)) : nullptr;
-    float *qf32 = prm->q_field ? (float *)gf_ws(m, WS_CG_BEGIN + 10, n * sizeof(float)) : nullptr;
-    if (!r32 || !w32 || !q32 || !x32 || (dinv && !dinv32) || (prm->q_field && !qf32))
-      return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+    float *r32 = wk.r32, *w32 = wk.w32, *q32 = wk.q32, *x32 = wk.x32;
+    float *dinv32 = dinv ? wk.dinv32 : nullptr;
+    float *qf32 = prm->q_field ? wk.qf32 : nullptr;
     if (dinv) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, dinv32, dinv, 1.0, n);
     if (qf32) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, qf32, prm->q_field, 1.0, n);
     const double delta = prm->inner_tol > 0 ? prm->inner_tol : 0.01;
@@ -1622,6 +1627,35 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
   res->outer = outer;
   res->rnorm = rn_final;
   return status;
+}
+
+}  // namespace
+
+extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b, const double *xb,
+                           double *x, gf_cg_result *res) {
+  RC(cg_check_args(m, prm, b, x, res));
+  const size_t n = m->ndof, bytes = n * sizeof(double);
+  CGWork wk;
+  wk.r = (double *)gf_ws(m, WS_CG_BEGIN + 0, bytes);
+  wk.w = (double *)gf_ws(m, WS_CG_BEGIN + 1, bytes);
+  wk.q = (double *)gf_ws(m, WS_CG_BEGIN + 2, bytes);
+  wk.dinv = prm->precond ? (double *)gf_ws(m, WS_CG_BEGIN + 3, bytes) : nullptr;
+  wk.resid = (double *)gf_ws(m, WS_CG_BEGIN + 4, (size_t)(prm->maxit + 8) * sizeof(double));
+  if (!wk.r || !wk.w || !wk.q || (prm->precond && !wk.dinv) || !wk.resid)
+    return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  if (prm->precision != 0) {
+    const size_t b32 = n * sizeof(float);
+    wk.r32 = (float *)gf_ws(m, WS_CG_BEGIN + 5, b32);
+    wk.w32 = (float *)gf_ws(m, WS_CG_BEGIN + 6, b32);
+    wk.q32 = (float *)gf_ws(m, WS_CG_BEGIN + 7, b32);
+    wk.x32 = (float *)gf_ws(m, WS_CG_BEGIN + 8, b32);
+    wk.dinv32 = prm->precond ? (float *)gf_ws(m, WS_CG_BEGIN + 9, b32) : nullptr;
+    wk.qf32 = prm->q_field ? (float *)gf_ws(m, WS_CG_BEGIN + 10, b32) : nullptr;
+    if (!wk.r32 || !wk.w32 || !wk.q32 || !wk.x32 || (prm->precond && !wk.dinv32) ||
+        (prm->q_field && !wk.qf32))
+      return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  }
+  return cg_solve_impl(m, prm, b, xb, x, res, wk);
 }
 
 // ===========================================================================
