@@ -127,6 +127,7 @@ SIGNATURES = {
     "gf_mass_apply": (ctypes.c_int, [_vp, _vp, _vp]),
     "gf_imass_apply": (ctypes.c_int, [_vp, _vp, _vp]),
     "gf_integrate": (ctypes.c_int, [_vp, _vp, _dp]),
+    "gf_integrate_noalloc": (ctypes.c_int, [_vp, _vp, _vp]),
     "gf_helmholtz_apply": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp,
                                           _vp]),
     "gf_helmholtz_apply_f32": (ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]),

@@ -16,6 +16,9 @@ struct gf_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // device address of the file-scope scratch array of gf_ops.hip on this
+  // context's device (hipGetSymbolAddress at gf_ctx_create)
+  double *scratch = nullptr;
 };
 
 // Reduction slot layout on the device (see gf_reduce.hpp).

@@ -16,6 +16,14 @@
 using gf::kThreads;
 using gf::Shape;
 
+// Scratch of gf_integrate_noalloc: the per-block partial sums of the
+// integral, one slot per block of its partial-sum kernel.  Each device the
+// library runs on holds its own copy until the process ends; host code uses
+// the per-device address that gf_ctx_create takes with hipGetSymbolAddress
+// (gf_ctx::scratch), never this name.
+constexpr int kNoAllocParts = 1024;
+static __device__ __align__(256) double scratch[kNoAllocParts];
+
 // ===========================================================================
 // Context, errors, workspace
 // ===========================================================================
@@ -69,6 +77,14 @@ int gf_ctx_create(int device, void *stream, gf_ctx **out) {
   gf_ctx *c = new gf_ctx_s();
   c->device = device;
   c->stream = (hipStream_t)stream;
+  // this device's copy of the scratch array (with the device current)
+  void *sp = nullptr;
+  hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !sp) {
+    delete c;
+    return GF_ERR_HIP;
+  }
+  c->scratch = (double *)sp;
   *out = c;
   return GF_OK;
 }
@@ -1146,6 +1162,22 @@ int gf_imass_apply(gf_mesh *m, const double *in, double *out) {
 int gf_integrate(gf_mesh *m, const double *u, double *result) {
   if (!m || !u || !result) return GF_ERR_ARG;
   return gf_dot_t(m, m->mass, u, m->ndof, result);
+}
+
+// gf_integrate's kernels and block count with the partial sums in this
+// device's scratch array; the result stays on the device; asynchronous.
+int gf_integrate_noalloc(gf_mesh *m, const double *u, double *result) {
+  if (!m || !u || !result) return GF_ERR_ARG;
+  double *part = m->ctx->scratch;
+  if (!part) return gf_set_err(m->ctx, GF_ERR_ARG, "integrate_noalloc: context has no scratch");
+  const int nb = std::min(nblocks(m->ndof), 1024);
+  if (nb > kNoAllocParts)
+    return gf_set_err(m->ctx, GF_ERR_ARG, "integrate_noalloc: more partial sums than scratch slots");
+  hipLaunchKernelGGL(k_partial_dot, dim3(nb), dim3(kThreads), 0, m->ctx->stream, m->mass, u, part,
+                     m->ndof);
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kThreads), 0, m->ctx->stream, part, nb, result);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
 }
 
 int gf_helmholtz_apply(gf_mesh *m, double p_scalar, const double *p_field, double q_scalar,

@@ -281,6 +281,17 @@ int gf_mass_apply(gf_mesh *mesh, const double *in, double *out);
 int gf_imass_apply(gf_mesh *mesh, const double *in, double *out);
 /* GGrid::integrate (src/cdg/grid/ggrid.ipp:1077-1148).  Synchronous. */
 int gf_integrate(gf_mesh *mesh, const double *u, double *result);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gf_integrate with its partial sums in a fixed 8 KiB
+ * per-device scratch array (1024 doubles, held until the process ends)
+ * instead of the mesh's reduction buffer.  result is a DEVICE pointer (one
+ * double); asynchronous on the context stream.  Calls no hipMalloc/hipFree.
+ * Any mesh size is accepted: the partial-sum kernel strides over the mesh
+ * with at most 1024 blocks, one scratch slot each.  Every call on a device
+ * shares that device's array, so calls that overlap on one device (two
+ * streams, two host threads) race; calls ordered on one stream do not. */
+int gf_integrate_noalloc(gf_mesh *mesh, const double *u, double *result);
 
 /* ---- Helmholtz ------------------------------------------------------------
  * GHelmholtz::opVec_prod -> reg_prod / embed_prod
