@@ -18,6 +18,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -66,52 +67,131 @@ def burgers_ic(torch, xyz, seed=12345, nmodes=12):
     return comps
 
 
-def cpu_baseline(seconds=12.0):
-    """Oracle (scalar C++ restatement of the reference) on a bounded sample of
-    the same workload: 8^3 elements, p=7, periodic, 3 components; 1 thread
-    (one MPI rank of the reference), then all of the host's cores (OpenMP
-    over contiguous element ranges, as the reference's element decomposition
-    over ranks would split the work)."""
+def host_info():
+    """Host CPU as lscpu and nproc report it."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                             "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                           timeout=20).stdout.strip())
+    except (OSError, subprocess.SubprocessError, ValueError):
+        info["nproc"] = os.cpu_count() or 1
+    return info
+
+
+def median_time(fn, warm=3, runs=5):
+    """Wall clock of fn(): median of `runs` runs after `warm` warm-up runs."""
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline():
+    """Oracle (C++ restatement of the reference, -O3 -march=native
+    -ffp-contract=off) on bounded samples of configs 1-2 and 4 of the
+    workload set, on 1 thread (one MPI rank of the reference) and on the
+    host's threads (OpenMP over contiguous index ranges of every
+    element-local and pointwise pass, as the reference's element
+    decomposition over ranks splits the work; dot products stay serial).
+    Each figure is the median of 5 runs after 3 warm-up runs; set-up (grid,
+    GGFX maps, basis) is excluded."""
     import oracle
     import geoflow_amd as ga
+    from geoflow_amd import workloads
+    info = host_info()
+    nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["nproc"]
+    # RK4 Burgers: 8^3 elements, p=7, periodic, 3 components; a run is 10 steps
     og = oracle.Grid.box(3, 7, [8, 8, 8])
     bg = ga.BoxGrid(3, 7, [8, 8, 8])
+    us = [np.cos(2 * np.pi * (bg.xyz[0] + c * bg.xyz[1])) + 0.3 * np.sin(2 * np.pi * bg.xyz[2])
+          for c in range(3)]
+    # FP64 Jacobi-PCG: 4^3 elements, p=7, Dirichlet, (M + 0.1 L) u = DSS(M f)
+    dirich = [ga.GF_BDY_DIRICHLET] * 6
+    ocg = oracle.Grid.box(3, 7, [4, 4, 4], bdy=dirich)
+    bcg = ga.BoxGrid(3, 7, [4, 4, 4], bdy=dirich)
+    bvec = ocg.mass_apply(workloads.unique_normal(bcg, seed=67890))
+    ones, zero = np.ones(ocg.ndof), np.zeros(ocg.ndof)
+    cg_it = [0]
+    # forward + adjoint: 4^3 elements, p=6, periodic, nu=0.02, dt=5e-5, a run
+    # is 10 RK4 steps forward (states kept) and their 10 adjoint steps
+    oad = oracle.Grid.box(3, 6, [4, 4, 4])
+    bad = ga.BoxGrid(3, 6, [4, 4, 4])
+    u0 = workloads.fourier_field_3d(*bad.xyz, seed=12345, kmax=6)
 
-    def run(threads, secs):
-        oracle.set_threads(threads)
-        us = [np.cos(2 * np.pi * (bg.xyz[0] + c * bg.xyz[1])) + 0.3 * np.sin(2 * np.pi * bg.xyz[2])
-              for c in range(3)]
-        og.burgers_step(us, 0.02, 1e-5)  # warm-up
-        t0 = time.perf_counter()
-        n = 0
-        while time.perf_counter() - t0 < secs:
+    def burgers():
+        for _ in range(10):
             og.burgers_step(us, 0.02, 1e-5)
-            n += 1
-        return n, time.perf_counter() - t0
 
-    n, dt = run(1, seconds)
-    try:
-        ncores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncores = os.cpu_count() or 1
-    ncores = max(1, min(ncores, 16))
-    nm, dtm = run(ncores, max(2.0, seconds / 3))
+    def cg():
+        _, _, it, _ = ocg.cg_solve(bvec, xb=zero, p=0.1, q=ones, maxit=5000, tol=1e-10,
+                                   normtype=4, precond=1)
+        cg_it[0] = it
+
+    def adjoint():
+        u = [a.copy() for a in u0]
+        states = []
+        for _ in range(10):
+            states.append([a.copy() for a in u])
+            oad.burgers_step(u, 0.02, 5e-5)
+        lam = [a.copy() for a in u]
+        for st in reversed(states):
+            oad.burgers_step_adjoint(st, lam, 0.02, 5e-5)
+
+    res = {}
+    for thr in (1, nthr):
+        oracle.set_threads(thr)
+        tb = median_time(burgers)
+        tc = median_time(cg)
+        ta = median_time(adjoint)
+        res[thr] = {"burgers": og.ndof * 10 / tb, "cg_s": tc, "cg_iters": cg_it[0],
+                    "cg_dof_iters": ocg.ndof * cg_it[0] / tc, "adjoint": bad.ndof * 10 / ta}
+        log("CPU baseline, %d thread(s): Burgers %.3e DOF-steps/s, CG %.3f s (%d it), "
+            "forward+adjoint %.3e DOF-steps/s" % (thr, res[thr]["burgers"], tc, cg_it[0],
+                                                  res[thr]["adjoint"]))
     oracle.set_threads(1)
-    return {"value": og.ndof * n / dt, "unit": "DOF-steps/s", "cores": 1, "kind": "port",
+    r1, rn = res[1], res[nthr]
+    return {"value": r1["burgers"], "unit": "DOF-steps/s", "cores": 1, "kind": "port",
             "sample": "oracle/gf_oracle.cpp (-O3 -march=native -ffp-contract=off), 1 thread: "
-                      "%d RK4 Burgers steps of an 8^3-element p=7 periodic box (%d local DOF, "
-                      "3 components) in %.1f s" % (n, og.ndof, dt),
-            "all_cores": {"value": og.ndof * nm / dtm, "cores": ncores,
-                          "sample": "same, OpenMP over %d threads: %d steps in %.1f s"
-                                    % (ncores, nm, dtm)}}
+                      "RK4 Burgers steps of an 8^3-element p=7 periodic box (%d local DOF, "
+                      "3 components), median of 5 runs of 10 steps after 3 warm-up runs"
+                      % og.ndof,
+            "host": info,
+            "all_cores": {"value": rn["burgers"], "cores": nthr,
+                          "sample": "same, OpenMP over %d threads" % nthr},
+            "cg": {"sample": "FP64 Jacobi-PCG (M + 0.1 L) u = DSS(M f), 4^3 elements, p=7, "
+                             "Dirichlet, to assembled rel-res 1e-10 (%d local DOF)" % ocg.ndof,
+                   "iters": r1["cg_iters"], "solve_s": r1["cg_s"],
+                   "dof_iters_per_s": r1["cg_dof_iters"],
+                   "all_cores": {"cores": nthr, "solve_s": rn["cg_s"],
+                                 "dof_iters_per_s": rn["cg_dof_iters"]}},
+            "adjoint": {"sample": "10 RK4 steps forward (nu=0.02, dt=5e-5) and their adjoint, "
+                                  "4^3 elements, p=6, periodic (%d local DOF)" % bad.ndof,
+                        "dof_steps_per_s": r1["adjoint"],
+                        "all_cores": {"cores": nthr, "dof_steps_per_s": rn["adjoint"]}}}
 
 
 def sphere_advection(ctx, level, p, nsteps, full=False, seed=12345, nu=0.0):
     """Cosine bell (radius R/3, seed-chosen centre) under Williamson test-1
     solid-body rotation tilted by pi/4 on the icosahedral sphere (GICOS
-    bisection, GE_2DEMBEDDED quads), RK4 pure advection (nu = 0); one
-    revolution takes nsteps steps.  short: times 100 steps; full: runs the
-    revolution and reports L2/Linf errors against the exact (rotated) bell."""
+    bisection, GE_2DEMBEDDED quads), RK4 advection with viscosity nu; one
+    revolution takes nsteps steps.  FP64 and mixed precision (FP32 field and
+    stage fields, FP64 metric terms).  short: times 100 steps of each; full:
+    runs the revolution in each precision and reports L2/Linf errors against
+    the exact (rotated, inviscid) bell and the mixed run's difference from
+    the FP64 run."""
     import torch
     import geoflow_amd as ga
     from geoflow_amd.device import Mesh
@@ -125,36 +205,47 @@ def sphere_advection(ctx, level, p, nsteps, full=False, seed=12345, nu=0.0):
     alpha, T = np.pi / 4, 1.0
     c0 = 2 * np.pi / T
     h0 = W.cosine_bell(*g.xyz, lat0, lon0)
-    st = [torch.from_numpy(a).cuda() for a in [h0] + W.williamson1_velocity(*g.xyz, c0, alpha)]
+    vel = [torch.from_numpy(a).cuda() for a in W.williamson1_velocity(*g.xyz, c0, alpha)]
     dt = T / nsteps
-    m.burgers_step(st, nu, dt, mode=1)  # warm-up
-    st[0].copy_(torch.from_numpy(h0))
-    torch.cuda.synchronize()
     run = nsteps if full else min(100, nsteps)
-    t0 = time.perf_counter()
-    done = 0
-    while done < run:
-        k = min(1000, run - done)
-        m.burgers_steps(st, nu, dt, k, t=done * dt, mode=1)
-        done += k
-        torch.cuda.synchronize()
-        if full:
-            log("sphere: %d / %d steps, %.1f s" % (done, run, time.perf_counter() - t0))
-    el = time.perf_counter() - t0
     res = {"level": level, "p": p, "nu": nu, "quads": g.nelems, "ndof": g.ndof,
-           "steps_per_revolution": nsteps,
-           "dt": dt, "steps_timed": run, "ms_per_step": 1e3 * el / run,
-           "dof_steps_per_s": g.ndof * run / el, "dtype": "f64",
+           "steps_per_revolution": nsteps, "dt": dt, "steps_timed": run,
            "bell_centre_deg": [float(np.rad2deg(lat0)), float(np.rad2deg(lon0))]}
-    if full:
+    mass = m.get("mass")
+    ex = W.rotated_bell(*g.xyz, lat0, lon0, alpha, c0 * T) if full else None
+    finals = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        st = [torch.from_numpy(h0).cuda()] + vel
+        m.burgers_steps(st, nu, dt, 1, mode=1, precision=prec)  # warm-up
+        st[0].copy_(torch.from_numpy(h0))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while done < run:
+            k = min(1000, run - done)
+            m.burgers_steps(st, nu, dt, k, t=done * dt, mode=1, precision=prec)
+            done += k
+            torch.cuda.synchronize()
+            if full:
+                log("sphere %s: %d / %d steps, %.1f s" % (name, done, run, time.perf_counter() - t0))
+        el = time.perf_counter() - t0
+        r = {"ms_per_step": 1e3 * el / run, "dof_steps_per_s": g.ndof * run / el,
+             "dtype": "f64" if prec == 0 else "f32 field / f64 metric"}
         h = st[0].cpu().numpy()
-        ex = W.rotated_bell(*g.xyz, lat0, lon0, alpha, c0 * T)
-        err = h - ex
-        mass = m.get("mass")
-        res["l2_error"] = float(np.sqrt(np.dot(mass, err * err) / np.dot(mass, ex * ex)))
-        res["linf_error"] = float(np.abs(err).max() / np.abs(ex).max())
-        log("sphere: L2 %.3e  Linf %.3e" % (res["l2_error"], res["linf_error"]))
-    log("sphere advection: %.3f ms/step, %.3e DOF-steps/s" % (res["ms_per_step"], res["dof_steps_per_s"]))
+        finals[name] = h
+        if full:
+            err = h - ex
+            r["l2_error"] = float(np.sqrt(np.dot(mass, err * err) / np.dot(mass, ex * ex)))
+            r["linf_error"] = float(np.abs(err).max() / np.abs(ex).max())
+            log("sphere %s: L2 %.3e  Linf %.3e" % (name, r["l2_error"], r["linf_error"]))
+        log("sphere advection %s: %.3f ms/step, %.3e DOF-steps/s"
+            % (name, r["ms_per_step"], r["dof_steps_per_s"]))
+        res[name] = r
+    d = finals["mixed"] - finals["fp64"]
+    res["mixed_vs_fp64"] = {"linf_rel": float(np.abs(d).max() / np.abs(finals["fp64"]).max()),
+                            "l2_rel": float(np.sqrt(np.dot(mass, d * d) /
+                                                    np.dot(mass, finals["fp64"] ** 2)))}
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
     return res
 
 
@@ -168,14 +259,22 @@ def ensemble_run(ctx, nmembers, counts, nsteps=50, seed=12345, nu=0.02, dt=5e-5)
     from geoflow_amd.workloads import fourier_field_3d
     members = draw_members(nmembers, seed=seed, counts=counts)
     t = time.perf_counter()
-    ens = Ensemble(ctx, members)
+    ens = Ensemble(ctx, members, device_maps=True)
     log("ensemble: %d members, %d local DOF, orders %s, set-up %.1f s"
         % (nmembers, ens.ndof, sorted(ens.mesh), time.perf_counter() - t))
 
+    cache = {}
+
     def fill(i, g):
-        base = fourier_field_3d(*g.xyz, seed=seed, kmax=3)
-        rng = np.random.default_rng(seed + 1 + i)
-        return [b + 1e-3 * rng.standard_normal(g.ndof) for b in base]
+        # the config-4 Fourier field (formed on the device) plus 1e-3 N(0,1)
+        # noise per member, drawn once and reused by both runs
+        if i not in cache:
+            xyz = [torch.from_numpy(a).cuda() for a in g.xyz]
+            base = fourier_field_3d(*xyz, seed=seed, kmax=3, xp=torch)
+            gen = torch.Generator(device="cuda").manual_seed(seed + 1 + i)
+            cache[i] = [b + 1e-3 * torch.randn(g.ndof, dtype=torch.float64, device="cuda",
+                                               generator=gen) for b in base]
+        return cache[i]
 
     out = {"members": nmembers, "ndof": ens.ndof, "steps": nsteps, "nu": nu, "dt": dt,
            "orders": {str(p): len(ix) for p, ix in ens.groups.items()},
@@ -193,7 +292,9 @@ def ensemble_run(ctx, nmembers, counts, nsteps=50, seed=12345, nu=0.02, dt=5e-5)
             iters += sum(sum(v) for v in its.values())
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        out[name] = {"seconds": el, "dof_steps_per_s": ens.ndof * nsteps / el, "cg_iters": iters}
+        mi = [int(a.max()) for v in ens.last_member_iters.values() for a in v]
+        out[name] = {"seconds": el, "dof_steps_per_s": ens.ndof * nsteps / el, "cg_iters": iters,
+                     "last_step_member_iters_max": max(mi) if mi else 0}
         finals[name] = st
         log("ensemble %s: %.2f s, %.3e DOF-steps/s, %d CG iterations"
             % (name, el, ens.ndof * nsteps / el, iters))
@@ -203,6 +304,123 @@ def ensemble_run(ctx, nmembers, counts, nsteps=50, seed=12345, nu=0.02, dt=5e-5)
         errs.append(max(float(np.abs(x - y).max() / np.abs(y).max()) for x, y in zip(a, b)))
     out["mixed_rel_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs))}
     out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    return out
+
+
+def copy_peak(torch, nbytes=2 * 134217728, reps=20):
+    """Measured HBM bandwidth of a plain device copy (torch copy_, read +
+    write counted), timed with HIP events on the current stream."""
+    a = torch.ones(nbytes // 16, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2 * (nbytes // 2) / (ms * 1e-3) / 1e9
+
+
+def adjoint_run(ctx, ne=16, p=6, nsteps=200, ckpt=20, nu=0.02, dt=5e-5):
+    """Config 4 on the GPU: forward RK4 Burgers and the checkpointed adjoint
+    gradient of 0.5 ||u(T) - u_target||_M^2 with respect to u(0) on a 16^3
+    periodic box at p=6, 200 steps of dt=5e-5, checkpoints every 20 steps."""
+    import torch
+    import geoflow_amd as ga
+    from geoflow_amd.device import Mesh
+    from geoflow_amd.workloads import fourier_field_3d
+    g = ga.BoxGrid(3, p, [ne] * 3)
+    m = Mesh(ctx, g)
+    u0 = [torch.from_numpy(a).cuda() for a in fourier_field_3d(*g.xyz, seed=12345, kmax=6)]
+    ut = [torch.from_numpy(a).cuda() for a in fourier_field_3d(*g.xyz, seed=2, kmax=6)]
+    mass = torch.from_numpy(m.get("mass")).cuda()
+
+    def fwd():
+        u = [a.clone() for a in u0]
+        m.burgers_steps(u, nu, dt, nsteps)
+        return u
+
+    def grad():
+        # dJ/du(T) = M (u(T) - u_target) needs u(T): one forward sweep, then
+        # gf_burgers_adjoint (its checkpointed forward sweep and the reverse
+        # sweep that recomputes each segment)
+        u = fwd()
+        lam = [mass * (a - b) for a, b in zip(u, ut)]
+        m.burgers_adjoint(u0, lam, nu, dt, nsteps, ckpt=ckpt)
+        return lam
+
+    grad()  # warm-up (workspace, code objects)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd()
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lam = grad()
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    finite = all(bool(torch.isfinite(a).all()) for a in lam)
+    log("adjoint 16^3 p=6: forward %.3f s, forward + gradient %.3f s" % (tf, tg))
+    return {"elements": [ne] * 3, "p": p, "ndof": g.ndof, "steps": nsteps, "ckpt": ckpt,
+            "nu": nu, "dt": dt, "forward_s": tf, "forward_and_gradient_s": tg,
+            "dof_steps_per_s_forward": g.ndof * nsteps / tf, "finite": finite}
+
+
+def node_metric_cg(ctx, runs=3):
+    """FP64 versus mixed-precision Jacobi-PCG of (M + 0.1 L) u = DSS(M f) to
+    assembled rel-res 1e-10 on meshes with per-node metrics: the icosahedral
+    sphere (level 5, p=5) and the 3-D terrain-following box (16x16x8, p=7).
+    Median of `runs` solves after one warm-up solve; the true FP64 assembled
+    relative residual of each returned solution."""
+    import torch
+    import geoflow_amd as ga
+    from geoflow_amd.device import Mesh
+    from geoflow_amd import workloads
+    out = {}
+    for name, mk in (("sphere_l5_p5", lambda: ga.IcosGrid(5, 5, radius=1.0)),
+                     ("terrain3d_16x16x8_p7", lambda: ga.TerrainBoxGrid3D(7, [16, 16, 8],
+                                                                          amp=0.1))):
+        g = mk()
+        m = Mesh(ctx, g)
+        b = m.mass_apply(torch.from_numpy(workloads.unique_normal(g, seed=67890)).cuda())
+        has_mask = getattr(g, "mask", None) is not None and float(np.min(g.mask)) == 0.0
+        mask = torch.from_numpy(g.mask).cuda() if has_mask else None
+        xb = torch.zeros_like(b) if has_mask else None
+        imult = m.imult()
+        bA = m.dss(b.clone(), "sum")
+        if mask is not None:
+            bA *= mask
+        bn = float(torch.sqrt((bA * bA * imult).sum()))
+        res = {"ndof": g.ndof}
+        for pname, prec in (("fp64", 0), ("mixed", 1)):
+            def solve():
+                x, info = m.cg_solve(b, xb=xb, p=0.1, q=1.0, maxit=20000, tol=1e-10, normtype=4,
+                                     precond=1, precision=prec)
+                torch.cuda.synchronize()
+                return x, info
+            solve()
+            ts = []
+            for _ in range(runs):
+                t0 = time.perf_counter()
+                x, info = solve()
+                ts.append(time.perf_counter() - t0)
+            r = b - m.helmholtz(x, 0.1, 1.0)
+            m.dss(r, "sum")
+            if mask is not None:
+                r *= mask
+            rr = float(torch.sqrt((r * r * imult).sum())) / bn
+            res[pname] = {"solve_s": float(np.median(ts)), "iters": info["iters"],
+                          "outer": info["outer"], "status": info["status"], "true_relres": rr}
+        res["speedup_mixed_vs_fp64"] = res["fp64"]["solve_s"] / res["mixed"]["solve_s"]
+        log("node-metric CG %s: fp64 %.4f s (%d it), mixed %.4f s (%d it), %.2fx"
+            % (name, res["fp64"]["solve_s"], res["fp64"]["iters"], res["mixed"]["solve_s"],
+               res["mixed"]["iters"], res["speedup_mixed_vs_fp64"]))
+        out[name] = res
+        del m
+        torch.cuda.empty_cache()
     return out
 
 
@@ -241,18 +459,19 @@ def main():
     ap.add_argument("--p", type=int, default=7)
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--sphere", choices=["off", "short", "full"], default="short",
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false")
+    ap.add_argument("--sphere", choices=["off", "short", "full"], default="full",
                     help="icosahedral sphere advection: short = timed steps only, "
                          "full = one revolution with L2/Linf errors")
     ap.add_argument("--sphere-level", type=int, default=6)
     ap.add_argument("--sphere-steps", type=int, default=10000, help="RK4 steps per revolution")
-    ap.add_argument("--sphere-nu", type=float, default=0.0,
-                    help="viscosity of the advected field (0: pure advection)")
-    ap.add_argument("--ensemble", type=int, default=8,
+    ap.add_argument("--sphere-nu", type=float, default=5e-5,
+                    help="viscosity of the advected field (0: pure advection, which the "
+                         "collocation scheme does not keep bounded on the sphere)")
+    ap.add_argument("--ensemble", type=int, default=48,
                     help="members of the batched ensemble (0: off)")
-    ap.add_argument("--ensemble-counts", default="8,12",
-                    help="element counts per direction to draw from (full workload: 8,12,16,24)")
+    ap.add_argument("--ensemble-counts", default="8,12,16,24",
+                    help="element counts per direction to draw from")
     ap.add_argument("--kernels", type=int, default=0, metavar="REPS",
                     help="only time the path's kernels REPS times each on the 32^3 p=7 boxes "
                          "(for profiler counter passes) and print their average times")
@@ -278,9 +497,10 @@ def main():
 
     # ---------------- Burgers RK4 ------------------------------------------
     t = time.perf_counter()
-    bg = ga.BoxGrid(3, p, [ne] * 3)
-    log("periodic box grid + GGFX maps: %.1f s, ndof=%d" % (time.perf_counter() - t, bg.ndof))
+    bg = ga.BoxGrid(3, p, [ne] * 3, device_maps=True)
     m = Mesh(ctx, bg)
+    log("periodic box grid + mesh (GGFX maps on the device): %.1f s, ndof=%d"
+        % (time.perf_counter() - t, bg.ndof))
     ndof = m.ndof
     xyz = [torch.from_numpy(a).cuda() for a in bg.xyz]
     us = burgers_ic(torch, xyz)
@@ -309,22 +529,29 @@ def main():
     step_gbs = step_bytes / (el / args.steps) / 1e9
     out.update({"value": dofsteps, "unit": "DOF-steps/s", "ms_per_step": 1e3 * el / args.steps})
     traffic = measured_traffic("k_burgers_stage3b<%d, true, 2, false>" % (p + 1)) if boxed else None
+    peak_meas = copy_peak(torch)
+    log("measured copy bandwidth %.0f GB/s" % peak_meas)
     out["roofline"] = {"kernel": stage_name,
                        "bound": "hbm", "achieved": stage_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": stage_gbs / HBM_PEAK_GBS, "traffic": traffic,
                        "traffic_source": "profiles/pmc_32cube_p7_kernels.md" if traffic else None,
                        "avg_launch_ms": stage_ms, "alg_bytes_per_launch": stage_bytes,
                        "step_alg_bytes": step_bytes, "step_achieved_gbs": step_gbs,
-                       "step_frac": step_gbs / HBM_PEAK_GBS}
+                       "step_frac": step_gbs / HBM_PEAK_GBS,
+                       "measured_peak": peak_meas,
+                       "measured_peak_source": "torch copy_ of 134 MB (read + write), HIP events",
+                       "frac_of_measured": stage_gbs / peak_meas,
+                       "step_frac_of_measured": step_gbs / peak_meas}
     del us, m
     torch.cuda.empty_cache()
 
     # ---------------- Helmholtz CG -------------------------------------------
     if not args.no_cg:
         t = time.perf_counter()
-        cg_grid = ga.BoxGrid(3, p, [ne] * 3, bdy=[ga.GF_BDY_DIRICHLET] * 6)
-        log("Dirichlet box grid + GGFX maps: %.1f s" % (time.perf_counter() - t))
+        cg_grid = ga.BoxGrid(3, p, [ne] * 3, bdy=[ga.GF_BDY_DIRICHLET] * 6, device_maps=True)
         m = Mesh(ctx, cg_grid)
+        log("Dirichlet box grid + mesh (GGFX maps on the device): %.1f s"
+            % (time.perf_counter() - t))
         f = torch.from_numpy(workloads.unique_normal(cg_grid, seed=67890)).cuda()
         b = m.mass_apply(f)  # local RHS; GCG assembles b - A x (so the system's RHS is DSS_sum(M f))
         q = 1.0  # (M + 0.1 L): a one-entry q of 1, as GHelmholtz::set_mass_scalar allows
@@ -385,6 +612,10 @@ def main():
         out["cg"] = res
         del m
 
+    if not args.no_cg:
+        out["cg_node_metric"] = node_metric_cg(ctx)
+    if args.adjoint:
+        out["adjoint"] = adjoint_run(ctx)
     if args.ensemble > 0:
         out["ensemble"] = ensemble_run(ctx, args.ensemble,
                                        tuple(int(c) for c in args.ensemble_counts.split(",")))
@@ -400,7 +631,7 @@ def main():
                      "parallelism": "single GPU"}
     if not args.no_cpu:
         log("CPU baseline (oracle, 1 thread) ...")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline()
         log("CPU baseline %.3e DOF-steps/s" % out["cpu_baseline"]["value"])
     print(json.dumps(out), flush=True)
 

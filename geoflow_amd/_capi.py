@@ -50,7 +50,9 @@ class MeshDesc(ctypes.Structure):
                 ("elem_dxidx", _dp), ("elem_jac", _dp), ("node_dxidx", _dp * 9),
                 ("node_jac", _dp), ("xyz", _dp * 3), ("mask", _dp),
                 ("ggfx_offs", _i64p), ("ggfx_members", _i64p),
-                ("box_ne", ctypes.c_int * 3), ("box_periodic", ctypes.c_int * 3)]
+                ("box_ne", ctypes.c_int * 3), ("box_periodic", ctypes.c_int * 3),
+                ("box_device_maps", ctypes.c_int), ("box_count", ctypes.c_int),
+                ("box_list", ctypes.POINTER(ctypes.c_int))]
 
 
 class GioInfo(ctypes.Structure):
@@ -127,6 +129,13 @@ SIGNATURES = {
     "gf_mass_apply": (ctypes.c_int, [_vp, _vp, _vp]),
     "gf_imass_apply": (ctypes.c_int, [_vp, _vp, _vp]),
     "gf_integrate": (ctypes.c_int, [_vp, _vp, _dp]),
+    "gf_dealias": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
+    "gf_mesh_set_segments": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "gf_cg_member_results": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                            _dp, _dp]),
+    "gf_box_ggfx_device": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), _vp, _vp,
+                                          ctypes.POINTER(ctypes.c_size_t)]),
     "gf_integrate_noalloc": (ctypes.c_int, [_vp, _vp, _vp]),
     "gf_helmholtz_apply": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp,
                                           _vp]),
@@ -142,6 +151,9 @@ SIGNATURES = {
     "gf_burgers_steps": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, ctypes.POINTER(_vp)]),
+    "gf_burgers_steps_mixed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, ctypes.POINTER(_vp)]),
     "gf_burgers_rhs_adjoint": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int,
                                               ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                               ctypes.POINTER(_vp)]),
@@ -152,6 +164,10 @@ SIGNATURES = {
                                           ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                           ctypes.POINTER(_vp)]),
+    "gf_burgers_adjoint_gio": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                              ctypes.POINTER(_vp), ctypes.c_char_p]),
     "gf_cg_solve": (ctypes.c_int, [_vp, ctypes.POINTER(CGParams), _vp, _vp, _vp,
                                    ctypes.POINTER(CGResult)]),
     "gf_helmholtz_diag_inv": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "gf_solve_common.hpp"
@@ -296,6 +297,93 @@ int gf_burgers_adjoint(gf_mesh *m, double nu, int order, int nstate, double dt, 
     return GF_OK;
   };
   return run();
+}
+
+// gf_burgers_adjoint with the checkpoints written as GIO state files
+// (GIO::write_state_impl layout, gio.ipp:588-690) on the forward sweep and
+// read back on the reverse sweep: only one segment (ckpt + 1 states) stays in
+// device memory.  Checkpoint c (the state at step c*ckpt) is the file
+// "<prefix>.<c>.gio": ivers 0, the mesh's dim, element count, orders and
+// element type, cycle = step, time = step*dt, multivar with one field per
+// state component.
+int gf_burgers_adjoint_gio(gf_mesh *m, double nu, int order, int nstate, double dt, int nsteps,
+                           int ckpt, const double *const *u0, double *const *lam,
+                           double *const *uT, const char *prefix) {
+  if (!m || !u0 || !lam || !prefix || order < 1 || order > 4 || nsteps < 0 || ckpt < 1)
+    return GF_ERR_ARG;
+  if (!m->Niso || m->gtype != GF_GE_REGULAR)
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "adjoint: regular isotropic meshes");
+  if (nstate != nvel_of(m)) return gf_set_err(m->ctx, GF_ERR_ARG, "adjoint: nonlinear mode needs nstate == velocity components");
+  const size_t n = m->ndof, bytes = n * sizeof(double);
+  const int ncp = (nsteps + ckpt - 1) / ckpt;
+  std::vector<double *> store((size_t)(ckpt + 1) * nstate, nullptr);
+  for (size_t i = 0; i < store.size(); i++) {
+    store[i] = (double *)gf_ws(m, WS_ADJ_CKPT + (int)i, bytes);
+    if (!store[i]) return gf_set_err(m->ctx, GF_ERR_ALLOC, "adjoint segment workspace");
+  }
+  auto seg = [&](int i) { return &store[(size_t)i * nstate]; };
+  std::vector<double> host((size_t)nstate * n);
+  std::vector<double *> hf(nstate);
+  for (int k = 0; k < nstate; k++) hf[k] = host.data() + (size_t)k * n;
+  const int porder[3] = {m->N[0] - 1, m->N[1] - 1, m->dim == 3 ? m->N[2] - 1 : 0};
+  auto path = [&](int c) { return std::string(prefix) + "." + std::to_string(c) + ".gio"; };
+  auto save = [&](int c, int step, double *const *st) -> int {
+    for (int k = 0; k < nstate; k++)
+      GF_HIP(m->ctx, hipMemcpyAsync(hf[k], st[k], bytes, hipMemcpyDeviceToHost, m->ctx->stream));
+    GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+    gf_gio_info info;
+    memset(&info, 0, sizeof(info));
+    info.ivers = 0;
+    info.dim = m->dim;
+    info.nelems = m->nelems;
+    info.gtype = m->gtype;
+    info.cycle = (uint64_t)step;
+    info.time = step * dt;
+    info.multivar = nstate > 1 ? 1 : 0;
+    if (gf_gio_write(path(c).c_str(), &info, porder, nullptr, hf.data(), nstate, n) != GF_OK)
+      return gf_set_err(m->ctx, GF_ERR_ARG, "adjoint: cannot write a GIO checkpoint");
+    return GF_OK;
+  };
+  auto load = [&](int c, int step, double *const *st) -> int {
+    gf_gio_info info;
+    memset(&info, 0, sizeof(info));
+    if (gf_gio_read(path(c).c_str(), &info, nullptr, 0, nullptr, 0, hf.data(), nstate, n) != GF_OK)
+      return gf_set_err(m->ctx, GF_ERR_ARG, "adjoint: cannot read a GIO checkpoint");
+    if (info.cycle != (uint64_t)step || info.nelems != m->nelems)
+      return gf_set_err(m->ctx, GF_ERR_ARG, "adjoint: GIO checkpoint of another run");
+    for (int k = 0; k < nstate; k++)
+      GF_HIP(m->ctx, hipMemcpyAsync(st[k], hf[k], bytes, hipMemcpyHostToDevice, m->ctx->stream));
+    return GF_OK;
+  };
+  double *cur[3] = {nullptr, nullptr, nullptr};
+  for (int k = 0; k < nstate; k++) {
+    cur[k] = seg(0)[k];
+    RCA(copyd(m, cur[k], u0[k]));
+  }
+  for (int s = 0; s < nsteps; s++) {
+    if (s % ckpt == 0) RCA(save(s / ckpt, s, cur));
+    RCA(gf_burgers_step(m, 0, nu, order, nstate, s * dt, dt, cur, 0));
+  }
+  if (uT)
+    for (int k = 0; k < nstate; k++)
+      if (uT[k]) RCA(copyd(m, uT[k], cur[k]));
+  for (int c = ncp - 1; c >= 0; c--) {
+    const int s0 = c * ckpt, s1 = std::min(nsteps, s0 + ckpt);
+    RCA(load(c, s0, seg(0)));
+    for (int s = s0 + 1; s < s1; s++) {
+      for (int k = 0; k < nstate; k++) RCA(copyd(m, seg(s - s0)[k], seg(s - s0 - 1)[k]));
+      double *st[3] = {seg(s - s0)[0], nstate > 1 ? seg(s - s0)[1] : nullptr,
+                       nstate > 2 ? seg(s - s0)[2] : nullptr};
+      RCA(gf_burgers_step(m, 0, nu, order, nstate, (s - 1) * dt, dt, st, 0));
+    }
+    for (int s = s1 - 1; s >= s0; s--) {
+      const double *un[3] = {seg(s - s0)[0], nstate > 1 ? seg(s - s0)[1] : nullptr,
+                             nstate > 2 ? seg(s - s0)[2] : nullptr};
+      RCA(step_adjoint(m, nu, order, dt, un, lam));
+    }
+  }
+  GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+  return GF_OK;
 }
 
 }  // extern "C"

@@ -74,10 +74,28 @@ struct gf_mesh_s {
   // mixed-precision CG on embedded and deformed meshes (made at creation)
   float *G32[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   float *mass32 = nullptr;
+  // members of a batched mesh (gf_mesh_set_segments): element ranges of
+  // independent sub-meshes whose GGFX groups stay inside them; CG solves
+  // give every member its own scalars and stopping test
+  int nseg = 0;
+  std::vector<int64_t> h_seg_eoff;  // nseg+1 element offsets (host)
+  std::vector<int> h_seg_boff;      // nseg+1 update-block offsets (host)
+  int64_t *seg_eoff = nullptr;      // device copies of the two
+  int *seg_boff = nullptr;
+  double *sscal = nullptr;          // per-member solver scalars (device)
+  double *hsscal = nullptr, *hsscal_dev = nullptr;  // mapped host mirror of sscal
+  std::vector<int> seg_iters;       // last solve: iterations per member
+  std::vector<double> seg_rnorm, seg_rnorm0;
+  // quadratic dealiasing (gf_dealias): the last order pq's interpolation
+  // matrix and weights on the device
+  int dealias_pq = -1;
+  double *dl_I = nullptr, *dl_wq = nullptr;
   double *red = nullptr;    // reduction partials, red_cap doubles
   size_t red_cap = 0;
   double *scal = nullptr;   // solver scalars
   double *hscal = nullptr;  // pinned host mirror of scal
+  double *hscal_dev = nullptr;  // device address of hscal (mapped)
+  hipEvent_t cg_ev[2] = {nullptr, nullptr};  // CG batch completion (polling)
 };
 
 int gf_set_err(gf_ctx *ctx, int code, const char *msg);
@@ -85,6 +103,13 @@ int gf_check_hip(gf_ctx *ctx, hipError_t e, const char *what);
 
 // Internal (C++ linkage) helpers shared across translation units.
 int gf_dss_t(gf_mesh *m, int op, double *u);
+// GGFX groups of a mesh of nbox boxes in sequence built on the device
+// (gf_ggfx_dev.hip): fills m->goff, m->gmem, m->ngroups, m->maxgrp (and
+// m->box for a single box).
+int gf_box_groups_device(gf_mesh *m, int nbox, const int *ne, const int *periodic);
+// Interpolation matrix p -> pq GLL nodes (column-major (pq+1) x (p+1)) and the
+// order-pq weights, gf_setup.cpp.
+int gf_dealias_matrix(int p, int pq, double *I, double *wq);
 int gf_dss_f32(gf_mesh *m, int op, float *u);
 int gf_helm_t(gf_mesh *m, double p, const double *pf, const double *qf, double qs,
               const double *u, double *uo);

@@ -5,6 +5,8 @@
 // element orders (GF_K3D_INSTANTIATE).
 #pragma once
 
+#include <type_traits>
+
 #include "gf_solve_common.hpp"
 
 namespace gfs {
@@ -41,6 +43,16 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_helm3p(
     }
     uo[col + k * S::N2] = acc;
   };
+  if constexpr (N == 8 && std::is_same<T, float>::value) {
+    if (!pf) {
+      // FP32 at p = 7: the contractions on the matrix cores
+      gf::MfmaD8 Dm;
+      Dm.load(D.R[0], D.C[0], (int)threadIdx.x & 63);
+      gf::pen_lap3_mfma8(ru, su[t.el], sd[t.el], Dm, W, t.i, t.j, W[t.i], W[t.j], jac, f0 * f0,
+                         f1 * f1, f2 * f2, post);
+      return;
+    }
+  }
   gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, D, W, t.i, t.j, W[t.i], W[t.j],
                            jac, f0 * f0,
                            f1 * f1, f2 * f2, pf ? pf + t.e * S::NN : (const T *)nullptr,
@@ -53,14 +65,25 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     gf::PenD<T> Dp, const T *__restrict__ W, const T *__restrict__ edx,
     const T *__restrict__ ejac, const T *__restrict__ qf, T qs, const T *__restrict__ mask, double pd,
     const double *__restrict__ scal, double *__restrict__ part, size_t nelems, gf::BoxTopo topo,
-    int tmask) {
+    int tmask, const int64_t *__restrict__ seg_eoff, int nseg, const double *__restrict__ sscal) {
   using S = gf::Pen3<N>;
   __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
-  __shared__ double sred[4];
-  if (scal[S_DONE] != 0.0) return;
+  __shared__ double sred[S::THREADS > 64 ? S::THREADS : 64];
+  ColIdx<N> t(nelems);
+  // batched mesh: each element's member scalars; an element of a member
+  // that has converged computes nothing it stores, and the partial sums are
+  // kept per element so that each member's are one contiguous range
+  if (nseg > 0) {
+    scal = sscal + kSegScal * seg_of(seg_eoff, nseg, t.e);
+    if (scal[S_DONE] != 0.0) {
+      if (S::EPB == 1) return;  // the whole workgroup
+      t.active = false;
+    }
+  } else if (scal[S_DONE] != 0.0) {
+    return;
+  }
   const bool first = scal[S_FIRST] != 0.0;
   const T beta = (T)scal[S_BETA];
-  ColIdx<N> t(nelems);
   const size_t col = t.e * S::NN + t.r;
   const T jac = ejac[t.e];
   const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
@@ -105,10 +128,28 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     qv[g] = acc;
     dot += (double)ru[k] * (double)acc;
   };
-  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, Dp, W, t.i, t.j, W[t.i], W[t.j],
-                           jac, f0 * f0,
-                           f1 * f1, f2 * f2, (const T *)nullptr, gf::NoPenCB(), gf::NoPenCB(),
-                           gf::NoPenCB(), post);
+  if constexpr (N == 8 && std::is_same<T, float>::value) {
+    // FP32 at p = 7: the contractions on the matrix cores
+    gf::MfmaD8 Dm;
+    Dm.load(Dp.R[0], Dp.C[0], (int)threadIdx.x & 63);
+    gf::pen_lap3_mfma8(ru, su[t.el], sd[t.el], Dm, W, t.i, t.j, W[t.i], W[t.j], jac, f0 * f0,
+                       f1 * f1, f2 * f2, post);
+  } else {
+    gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, Dp, W, t.i, t.j, W[t.i],
+                             W[t.j], jac, f0 * f0, f1 * f1, f2 * f2, (const T *)nullptr,
+                             gf::NoPenCB(), gf::NoPenCB(), gf::NoPenCB(), post);
+  }
+  if (nseg > 0) {
+    // per-element partials: the element's lanes, summed in lane order
+    sred[threadIdx.x] = dot;
+    __syncthreads();
+    if (t.r == 0 && t.active) {
+      double v = 0.0;
+      for (int q2 = 0; q2 < S::N2; q2++) v += sred[t.el * S::N2 + q2];
+      part[t.e] = v;
+    }
+    return;
+  }
   double bs = gf::block_reduce<false>(dot, sred);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
@@ -471,7 +512,8 @@ int launch_cg_apply3p(gf_mesh *m, const CGApply3Args &a) {
   hipLaunchKernelGGL((k_cg_apply3p<N, T>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream,
                      (const T *)a.r, (const T *)a.dinv, (T *)a.w, (T *)a.q, Dp, (const T *)a.W,
                      (const T *)a.edx, (const T *)a.ejac, (const T *)a.qf, (T)a.qs, (const T *)a.mask, a.p,
-                     (const double *)a.scal, a.part, a.nelems, a.topo, a.tmask);
+                     (const double *)a.scal, a.part, a.nelems, a.topo, a.tmask, a.seg_eoff, a.nseg,
+                     (const double *)a.sscal);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
 }
@@ -604,8 +646,178 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
   }
 }
 
+// 16-byte vector of T (4 floats or 2 doubles).
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using V = float4;
+  static constexpr int R = 4;
+  __device__ static float get(const V &v, int n) {
+    return n == 0 ? v.x : (n == 1 ? v.y : (n == 2 ? v.z : v.w));
+  }
+  __device__ static void set(V &v, int n, float a) {
+    if (n == 0) v.x = a;
+    else if (n == 1) v.y = a;
+    else if (n == 2) v.z = a;
+    else v.w = a;
+  }
+};
+template <>
+struct Vec16<double> {
+  using V = double2;
+  static constexpr int R = 2;
+  __device__ static double get(const V &v, int n) { return n == 0 ? v.x : v.y; }
+  __device__ static void set(V &v, int n, double a) {
+    if (n == 0) v.x = a;
+    else v.y = a;
+  }
+};
+
+// True when k_cg_update_boxv handles order N in precision T: a node row
+// splits into at least two 16-byte runs and a workgroup holds whole elements.
+template <typename T, int N>
+constexpr bool boxv_ok() {
+  return N % Vec16<T>::R == 0 && N > Vec16<T>::R &&
+         kThreads % (N * N * N / Vec16<T>::R) == 0;
+}
+
+// k_cg_update_box with each lane working a run of R = 16/sizeof(T)
+// consecutive nodes along x (one 16-byte load or store per vector), so the
+// wave issues a quarter (FP32) or half (FP64) of the memory instructions.
+// The group of a node is the product of its partner choices per direction:
+// the run's y, z and yz partners are runs of the neighbouring elements at the
+// same x offset (16-byte loads), and only the run's first or last node can
+// have an x partner (scalar loads).  Members are summed in the order
+// (z side, y side, x side), lower side first, where the lower side of a
+// direction is the copy with the smaller global index; because the element
+// index dominates the global index, that order is the ascending order of the
+// group's global indices, GGFX's summation order (ggfx.hpp:346-443), so the
+// result equals k_cg_update_box's bit for bit.
+template <typename T, int NORM, int N, typename I>
+__global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
+    T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r, const T *__restrict__ qv,
+    const T *__restrict__ dinv, const double *__restrict__ mass, gf::BoxTopo topo, double *scal,
+    double *__restrict__ part, size_t nelems) {
+  using VT = Vec16<T>;
+  using V = typename VT::V;
+  constexpr int R = VT::R;
+  constexpr int NN = N * N * N;
+  constexpr int RPR = N / R;           // runs per node row
+  constexpr int RPE = NN / R;          // runs per element
+  constexpr int EPP = kThreads / RPE;  // elements per workgroup pass
+  __shared__ double sred[kThreads / 64];
+  if (scal[S_DONE] != 0.0) return;
+  const T alpha = (T)scal[S_ALPHA];
+  double rho = 0.0, nrm = 0.0;
+  const int el = (int)threadIdx.x / RPE, rr = (int)threadIdx.x - el * RPE;
+  const int ir = rr % RPR, j = (rr / RPR) % N, k = rr / (RPR * N);
+  const int i0 = ir * R;
+  // the run node that can have an x partner: the row's first or last node
+  const int xn = ir == 0 ? 0 : (ir == RPR - 1 ? R - 1 : -1);
+  const size_t npass = (nelems + EPP - 1) / EPP;
+  const size_t ppb = (npass + gridDim.x - 1) / gridDim.x;
+  const size_t c0 = xcd_block() * ppb, c1 = c0 + ppb < npass ? c0 + ppb : npass;
+  for (size_t c = c0; c < c1; c++) {
+    const size_t e = c * EPP + el;
+    if (e >= nelems) continue;
+    int ec[3];
+    gf::box_elem_coords(topo, e, ec);
+    const gf::BoxElemDeltas d = gf::box_elem_deltas(topo, N, ec);
+    const I g0 = (I)(e * NN) + (I)(k * N * N + j * N + i0);
+    const I dy = j == 0 ? (I)d.lo[1] : (j == N - 1 ? (I)d.hi[1] : (I)0);
+    const I dz = k == 0 ? (I)d.lo[2] : (k == N - 1 ? (I)d.hi[2] : (I)0);
+    const I dx = xn == 0 ? (I)d.lo[0] : (xn == R - 1 ? (I)d.hi[0] : (I)0);
+    const bool py = dy != 0, pz = dz != 0, px = dx != 0;
+    // every load of the run before its first store
+    const V xo = *(const V *)(x + g0);
+    const V wo = *(const V *)(w + g0);
+    const V ro = *(const V *)(r + g0);
+    V dv;
+    if (dinv) dv = *(const V *)(dinv + g0);
+    const V q0 = *(const V *)(qv + g0);
+    V qy = {}, qz = {}, qyz = {};
+    if (py) qy = *(const V *)(qv + g0 + dy);
+    if (pz) qz = *(const V *)(qv + g0 + dz);
+    if (py && pz) qyz = *(const V *)(qv + g0 + dy + dz);
+    T sx = T(0), sxy = T(0), sxz = T(0), sxyz = T(0);
+    if (px) {
+      const I gx = g0 + (I)xn + dx;
+      sx = qv[gx];
+      if (py) sxy = qv[gx + dy];
+      if (pz) sxz = qv[gx + dz];
+      if (py && pz) sxyz = qv[gx + dy + dz];
+    }
+    // the partner side is the lower one when its offset is negative
+    const bool ysw = dy < 0, zsw = dz < 0, xsw = dx < 0;
+    V xv, rv;
+#pragma unroll
+    for (int n = 0; n < R; n++) {
+      // t[side z][side y][side x], side 0 = own copy, 1 = partner
+      T t[2][2][2];
+      t[0][0][0] = VT::get(q0, n);
+      t[0][1][0] = VT::get(qy, n);
+      t[1][0][0] = VT::get(qz, n);
+      t[1][1][0] = VT::get(qyz, n);
+      const bool hx = (n == 0 || n == R - 1) && n == xn && px;
+      t[0][0][1] = hx ? sx : T(0);
+      t[0][1][1] = hx ? sxy : T(0);
+      t[1][0][1] = hx ? sxz : T(0);
+      t[1][1][1] = hx ? sxyz : T(0);
+      const bool xs = hx && xsw;
+      // canonical order: lower side first in z, then y, then x; absent
+      // partners contribute +0 (see box_sum8z)
+      T sum = T(0);
+#pragma unroll
+      for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+        for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+          for (int cx = 0; cx < 2; cx++) {
+            const int sz = cz ^ (int)zsw, sy = cy ^ (int)ysw, sxi = cx ^ (int)xs;
+            sum = sum + t[sz][sy][sxi];
+          }
+      const int cnt = (1 + (int)py) * (1 + (int)pz) * (1 + (int)hx);
+      const T qa = cnt == 1 ? t[0][0][0] : sum;
+      const T xn1 = (T)1 * VT::get(xo, n) + alpha * VT::get(wo, n);
+      VT::set(xv, n, xn1);
+      const T rn1 = (T)1 * VT::get(ro, n) + (-alpha) * qa;
+      VT::set(rv, n, rn1);
+      const T z = dinv ? VT::get(dv, n) * rn1 : rn1;
+      const double im = gf::box_inv(cnt);
+      rho += (double)rn1 * (double)z * im;
+      const double rd = (double)rn1;
+      if (NORM == GF_NORM_INF) nrm = fmax(nrm, fabs(rd));
+      else if (NORM == GF_NORM_EUC) nrm += rd * rd;
+      else if (NORM == GF_NORM_L2) nrm += mass[g0 + n] * rd * rd;
+      else if (NORM == GF_NORM_L1) nrm += mass[g0 + n] * fabs(rd);
+      else nrm += rd * rd * im;
+    }
+    *(V *)(x + g0) = xv;
+    *(V *)(r + g0) = rv;
+  }
+  double a = gf::block_reduce<false>(rho, sred);
+  double b = NORM == GF_NORM_INF ? gf::block_reduce<true>(nrm, sred)
+                                 : gf::block_reduce<false>(nrm, sred);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
 template <typename T, int NORM, int N>
 int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part) {
+  if constexpr (boxv_ok<T, N>()) {
+    if (m->ndof < ((size_t)1 << 31))
+      hipLaunchKernelGGL((k_cg_update_boxv<T, NORM, N, int32_t>), dim3(kUpdBlocks), dim3(kThreads),
+                         0, m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal,
+                         part, m->nelems);
+    else
+      hipLaunchKernelGGL((k_cg_update_boxv<T, NORM, N, int64_t>), dim3(kUpdBlocks), dim3(kThreads),
+                         0, m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal,
+                         part, m->nelems);
+    return GF_OK;
+  }
   // 32-bit node indices whenever every index of the mesh fits
   if (m->ndof < ((size_t)1 << 31))
     hipLaunchKernelGGL((k_cg_update_box<T, NORM, N, int32_t>), dim3(kUpdBlocks), dim3(kThreads), 0,

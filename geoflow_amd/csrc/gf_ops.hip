@@ -520,6 +520,40 @@ __global__ void k_dss(T *__restrict__ u, const int *__restrict__ goff,
                       const int *__restrict__ gmem, int ngroups) {
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += gridDim.x * blockDim.x) {
     int b = goff[g], e = goff[g + 1];
+    if (e - b <= 8) {
+      // groups of at most 8 (boxes, the sphere): every member index and
+      // value is loaded before the first use, so a thread waits for memory
+      // twice instead of once per member; same order, same result
+      const int cnt = e - b;
+      int id[8];
+      T v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) id[k] = k < cnt ? gmem[b + k] : 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = k < cnt ? u[id[k]] : T(0);
+      T r;
+      if (OP == GF_DSS_SUM || OP == GF_DSS_SMOOTH) {
+        r = T(0);
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          if (k < cnt) r = r + v[k];
+        if (OP == GF_DSS_SMOOTH) r = r / (T)cnt;
+      } else if (OP == GF_DSS_MAX) {
+        r = std::numeric_limits<T>::lowest();
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          if (k < cnt) r = fmax(r, v[k]);
+      } else {
+        r = std::numeric_limits<T>::max();
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          if (k < cnt) r = fmin(r, v[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k < cnt) u[id[k]] = r;
+      continue;
+    }
     T r;
     if (OP == GF_DSS_SUM || OP == GF_DSS_SMOOTH) {
       r = T(0);
@@ -746,7 +780,15 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     return gf_set_err(ctx, GF_ERR_ARG, "unknown element type");
   if (d->gtype == GF_GE_2DEMBEDDED && d->dim != 2)
     return gf_set_err(ctx, GF_ERR_ARG, "embedded meshes are 2-D");
-  if (!d->ggfx_offs || !d->ggfx_members) return gf_set_err(ctx, GF_ERR_ARG, "missing GGFX maps");
+  // device-built maps: a 3-D regular isotropic box with its structure given
+  const bool devmaps = d->box_device_maps != 0;
+  const bool boxunion = devmaps && d->box_count >= 1 && d->box_list;
+  if (devmaps && !(d->dim == 3 && d->gtype == GF_GE_REGULAR &&
+                   (boxunion ? d->box_list != nullptr : d->box_ne[0] > 0) &&
+                   d->N[0] == d->N[1] && d->N[1] == d->N[2]))
+    return gf_set_err(ctx, GF_ERR_ARG, "device GGFX maps need 3-D regular isotropic boxes");
+  if (!devmaps && (!d->ggfx_offs || !d->ggfx_members))
+    return gf_set_err(ctx, GF_ERR_ARG, "missing GGFX maps");
   GF_HIP(ctx, hipSetDevice(ctx->device));
   gf_mesh *m = new gf_mesh_s();
   m->ctx = ctx;
@@ -895,7 +937,23 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
   }
   // DSS groups with more than one member, each stored once, in order of
   // their smallest member (which is the node at which GGFX::init formed it).
-  {
+  if (devmaps) {
+    if (boxunion) {
+      std::vector<int> ne(3 * (size_t)d->box_count), per(3 * (size_t)d->box_count);
+      for (int b = 0; b < d->box_count; b++)
+        for (int c = 0; c < 3; c++) {
+          ne[3 * b + c] = d->box_list[6 * b + c];
+          per[3 * b + c] = d->box_list[6 * b + 3 + c];
+        }
+      rc = gf_box_groups_device(m, d->box_count, ne.data(), per.data());
+    } else {
+      rc = gf_box_groups_device(m, 1, d->box_ne, d->box_periodic);
+    }
+    if (rc) {
+      gf_mesh_destroy(m);
+      return rc;
+    }
+  } else {
     std::vector<int> goff(1, 0), gmem;
     int maxg = 1;
     for (size_t q = 0; q < n; q++) {
@@ -917,7 +975,7 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
   }
   // Structured box: enable gather assembly only if every node's GGFX group
   // is exactly the group the box structure implies.
-  if (d->box_ne[0] > 0 && d->dim == 3 && d->gtype == GF_GE_REGULAR && m->Niso) {
+  if (d->box_ne[0] > 0 && !boxunion && d->dim == 3 && d->gtype == GF_GE_REGULAR && m->Niso) {
     gf::BoxTopo t;
     size_t ne_tot = 1;
     for (int k = 0; k < 3; k++) {
@@ -944,7 +1002,9 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
       }
     }
     bool mask_ok = ok;
-    for (size_t q = 0; ok && q < n; q++) {
+    // device-built maps are the box's groups by construction; only the mask
+    // is checked node by node
+    for (size_t q = 0; ok && q < n && !(devmaps && !mask_ok); q++) {
       const size_t e = q / m->nn, r = q - e * m->nn;
       const int i = (int)(r % Nn), j = (int)((r / Nn) % Nn), k = (int)(r / ((size_t)Nn * Nn));
       // the construction the kernels use: per-element partner offsets
@@ -952,10 +1012,12 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
       int ec[3];
       gf::box_elem_coords(t, e, ec);
       const gf::BoxElemDeltas de = gf::box_elem_deltas(t, Nn, ec);
-      const int c = gf::box_members<int64_t>(de, Nn, (int64_t)q, i, j, k, g);
-      const int64_t b = d->ggfx_offs[q], eo = d->ggfx_offs[q + 1];
-      ok = (eo - b) == c;
-      for (int s2 = 0; ok && s2 < c; s2++) ok = d->ggfx_members[b + s2] == g[s2];
+      if (!devmaps) {
+        const int c = gf::box_members<int64_t>(de, Nn, (int64_t)q, i, j, k, g);
+        const int64_t b = d->ggfx_offs[q], eo = d->ggfx_offs[q + 1];
+        ok = (eo - b) == c;
+        for (int s2 = 0; ok && s2 < c; s2++) ok = d->ggfx_members[b + s2] == g[s2];
+      }
       if (mask_ok) {
         const double mv = d->mask ? d->mask[q] : 1.0;
         mask_ok = (mv == 0.0 || mv == 1.0) && ((mv == 0.0) == gf::box_masked(t, Nn, ec, i, j, k));
@@ -968,7 +1030,11 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
   m->red_cap = gf_red_need(m->nelems);
   GF_HIP(ctx, hipMalloc((void **)&m->red, m->red_cap * sizeof(double)));
   GF_HIP(ctx, hipMalloc((void **)&m->scal, 64 * sizeof(double)));
-  GF_HIP(ctx, hipHostMalloc((void **)&m->hscal, 64 * sizeof(double), hipHostMallocDefault));
+  GF_HIP(ctx, hipHostMalloc((void **)&m->hscal, 64 * sizeof(double),
+                            hipHostMallocMapped | hipHostMallocCoherent));
+  GF_HIP(ctx, hipHostGetDevicePointer((void **)&m->hscal_dev, m->hscal, 0));
+  for (int k = 0; k < 2; k++)
+    GF_HIP(ctx, hipEventCreateWithFlags(&m->cg_ev[k], hipEventDisableTiming));
   GF_HIP(ctx, hipMalloc((void **)&m->mass, n * sizeof(double)));
   GF_HIP(ctx, hipMalloc((void **)&m->imass, n * sizeof(double)));
   GF_HIP(ctx, hipMalloc((void **)&m->imult, n * sizeof(double)));
@@ -1047,6 +1113,10 @@ int gf_mesh_destroy(gf_mesh *m) {
   for (int k = 0; k < 6; k++) F(m->G32[k]);
   F(m->mass32);
   if (m->hscal) (void)hipHostFree(m->hscal);
+  F(m->seg_eoff); F(m->seg_boff); F(m->sscal); F(m->dl_I); F(m->dl_wq);
+  if (m->hsscal) (void)hipHostFree(m->hsscal);
+  for (int k = 0; k < 2; k++)
+    if (m->cg_ev[k]) (void)hipEventDestroy(m->cg_ev[k]);
   for (size_t k = 0; k < m->ws.size(); k++) F(m->ws[k]);
   delete m;
   return GF_OK;
@@ -1162,6 +1232,115 @@ int gf_imass_apply(gf_mesh *m, const double *in, double *out) {
 int gf_integrate(gf_mesh *m, const double *u, double *result) {
   if (!m || !u || !result) return GF_ERR_ARG;
   return gf_dot_t(m, m->mass, u, m->ndof, result);
+}
+
+// GGrid::dealias (ggrid.ipp:2424-2503): one workgroup per element; v1 and v2
+// are interpolated to the order-pq GLL grid direction by direction in LDS
+// (x, then y, then z), multiplied pointwise and by the q tensor weights
+// ((Wx Wy) Wz), projected back with the transposed interpolation in the same
+// direction order and multiplied by iWp = Jac * imass (init_qdealias,
+// ggrid.ipp:2402-2407).  I is column-major Q x N: I[a + Q i] = h_i(xq_a).
+namespace {
+__device__ void dl_pass(const double *__restrict__ in, double *__restrict__ out, int n0, int n1,
+                        int n2, int d, int lout, const double *__restrict__ I, int Q, bool back) {
+  const int lin = d == 0 ? n0 : (d == 1 ? n1 : n2);
+  const int m0 = d == 0 ? lout : n0, m1 = d == 1 ? lout : n1, m2 = d == 2 ? lout : n2;
+  const int tot = m0 * m1 * m2;
+  for (int o = threadIdx.x; o < tot; o += blockDim.x) {
+    const int a = o % m0, b = (o / m0) % m1, c = o / (m0 * m1);
+    const int oi = d == 0 ? a : (d == 1 ? b : c);
+    double acc = 0.0;
+    for (int k = 0; k < lin; k++) {
+      const int ia = d == 0 ? k : a, ib = d == 1 ? k : b, ic = d == 2 ? k : c;
+      const double op = back ? I[k + Q * oi] : I[oi + Q * k];
+      acc = fma(op, in[ia + n0 * (ib + n1 * ic)], acc);
+    }
+    out[o] = acc;
+  }
+  __syncthreads();
+}
+
+__global__ void k_dealias(const double *__restrict__ v1, const double *__restrict__ v2,
+                          double *__restrict__ prod, const double *__restrict__ I,
+                          const double *__restrict__ wq, const double *__restrict__ ejac,
+                          const double *__restrict__ imass, int dim, int N, int Q) {
+  extern __shared__ double sh[];
+  const int z = dim == 3 ? 1 : 0;
+  const int NN = z ? N * N * N : N * N, QQ = z ? Q * Q * Q : Q * Q;
+  double *sI = sh, *sw = sI + Q * N, *sv = sw + Q, *a1 = sv + NN, *a2 = a1 + QQ, *b1 = a2 + QQ,
+         *b2 = b1 + QQ;
+  const size_t e = blockIdx.x;
+  for (int q = threadIdx.x; q < Q * N; q += blockDim.x) sI[q] = I[q];
+  for (int q = threadIdx.x; q < Q; q += blockDim.x) sw[q] = wq[q];
+  for (int f = 0; f < 2; f++) {
+    const double *v = (f == 0 ? v1 : v2) + e * NN;
+    for (int q = threadIdx.x; q < NN; q += blockDim.x) sv[q] = v[q];
+    __syncthreads();
+    double *dst = f == 0 ? b1 : b2;
+    dl_pass(sv, a1, N, N, z ? N : 1, 0, Q, sI, Q, false);
+    if (z) {
+      dl_pass(a1, a2, Q, N, N, 1, Q, sI, Q, false);
+      dl_pass(a2, dst, Q, Q, N, 2, Q, sI, Q, false);
+    } else {
+      dl_pass(a1, dst, Q, N, 1, 1, Q, sI, Q, false);
+    }
+  }
+  for (int o = threadIdx.x; o < QQ; o += blockDim.x) {
+    const int a = o % Q, b = (o / Q) % Q, c = o / (Q * Q);
+    double w = sw[a] * sw[b];
+    if (z) w = w * sw[c];
+    const double t = b1[o] * b2[o];
+    b1[o] = t * w;
+  }
+  __syncthreads();
+  dl_pass(b1, a1, Q, Q, z ? Q : 1, 0, N, sI, Q, true);
+  double *res = z ? b2 : a2;
+  if (z) {
+    dl_pass(a1, a2, N, Q, Q, 1, N, sI, Q, true);
+    dl_pass(a2, res, N, N, Q, 2, N, sI, Q, true);
+  } else {
+    dl_pass(a1, res, N, Q, 1, 1, N, sI, Q, true);
+  }
+  const double jac = ejac[e];
+  for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+    const size_t g = e * NN + q;
+    prod[g] = res[q] * (jac * imass[g]);
+  }
+}
+}  // namespace
+
+int gf_dealias(gf_mesh *m, int pq, const double *v1, const double *v2, double *prod) {
+  if (!m || !v1 || !v2 || !prod) return GF_ERR_ARG;
+  if (m->gtype != GF_GE_REGULAR || !m->Niso)
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "dealias: regular meshes of constant p");
+  const int N = m->Niso, p = N - 1, Q = pq + 1;
+  if (pq < p || pq > 31) return gf_set_err(m->ctx, GF_ERR_ARG, "dealias: need p <= pq <= 31");
+  const int z = m->dim == 3 ? 1 : 0;
+  const size_t NN = z ? (size_t)N * N * N : (size_t)N * N, QQ = z ? (size_t)Q * Q * Q : (size_t)Q * Q;
+  const size_t lds = ((size_t)Q * N + Q + NN + 4 * QQ) * sizeof(double);
+  if (lds > 160 * 1024)
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "dealias: element does not fit in LDS");
+  if (m->dealias_pq != pq) {
+    std::vector<double> I((size_t)Q * N), wq(Q);
+    if (gf_dealias_matrix(p, pq, I.data(), wq.data()) != GF_OK)
+      return gf_set_err(m->ctx, GF_ERR_ARG, "dealias: basis");
+    if (m->dl_I) (void)hipFree(m->dl_I);
+    if (m->dl_wq) (void)hipFree(m->dl_wq);
+    m->dl_I = m->dl_wq = nullptr;
+    m->dealias_pq = -1;
+    GF_HIP(m->ctx, hipMalloc((void **)&m->dl_I, I.size() * sizeof(double)));
+    GF_HIP(m->ctx, hipMalloc((void **)&m->dl_wq, wq.size() * sizeof(double)));
+    GF_HIP(m->ctx, hipMemcpy(m->dl_I, I.data(), I.size() * sizeof(double), hipMemcpyHostToDevice));
+    GF_HIP(m->ctx, hipMemcpy(m->dl_wq, wq.data(), wq.size() * sizeof(double), hipMemcpyHostToDevice));
+    m->dealias_pq = pq;
+  }
+  if (lds > 64 * 1024)
+    GF_HIP(m->ctx, hipFuncSetAttribute((const void *)k_dealias,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_dealias, dim3(m->nelems), dim3(kThreads), lds, m->ctx->stream, v1, v2, prod,
+                     m->dl_I, m->dl_wq, m->ejac, m->imass, m->dim, N, Q);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
 }
 
 // gf_integrate's kernels and block count with the partial sums in this

@@ -190,6 +190,135 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 Laplacian of a p=7 element (N = 8, one wave per element) with the
+// tensor-product contractions on the matrix cores.  Same data flow and
+// thread roles as pen_op3 (lane a + 8b owns the z-column (a,b) and the x-
+// and y-pencils (., a, b), (a, ., b)); each directional gradient and each
+// D^T is a batch of 64 length-8 contractions, done as 16 rank-1 block
+// updates of v_mfma_f32_4x4x1f32 (16 independent 4x4 blocks per wave):
+//   grad  g(l) = sum_i D(l,i) v(i):  block b covers pencils 4b..4b+3 (one
+//         per lane of the block, the B operand is the lane's own v(i)) and
+//         4 consecutive l (the A operand D(4t + lane%4, i)), accumulated
+//         over i = 0..7, for t = 0, 1; the result lands in the lane that
+//         owns the pencil, l = 4t + register;
+//   D^T   s(i) = sum_l D(l,i) f(l):  the same with A = D(l, 4t + lane%4).
+// Every block product is used (no padding), and the results stay in the
+// pencil owner's registers, so no data moves between lanes.  The scaling
+// M (D_d u) G_d between the two passes and the sums of the three parts are
+// those of pen_op3; only the rounding of the contractions differs (FP32
+// accumulation in the matrix core).
+// ---------------------------------------------------------------------------
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+struct MfmaD8 {
+  float g[2][8];  // g[t][i] = D(4t + lane%4, i)
+  float d[2][8];  // d[t][l] = D(l, 4t + lane%4)
+  // R: row-major D (R[l*8 + m] = D(l,m)), C: column-major (C[l*8 + m] =
+  // D(m,l)), both FP32; each lane reads two rows of each with 16-byte loads
+  __device__ __forceinline__ void load(const float *__restrict__ R, const float *__restrict__ C,
+                                       int lane) {
+    const int r = lane & 3;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const float4 *rr = (const float4 *)(R + (4 * t + r) * 8);
+      const float4 *cc = (const float4 *)(C + (4 * t + r) * 8);
+      const float4 a0 = rr[0], a1 = rr[1], b0 = cc[0], b1 = cc[1];
+      g[t][0] = a0.x; g[t][1] = a0.y; g[t][2] = a0.z; g[t][3] = a0.w;
+      g[t][4] = a1.x; g[t][5] = a1.y; g[t][6] = a1.z; g[t][7] = a1.w;
+      d[t][0] = b0.x; d[t][1] = b0.y; d[t][2] = b0.z; d[t][3] = b0.w;
+      d[t][4] = b1.x; d[t][5] = b1.y; d[t][6] = b1.z; d[t][7] = b1.w;
+    }
+  }
+};
+
+// out(l) = sum_m A_t(l, m) v(m) for the lane's own pencil (A = D or D^T).
+// Four independent accumulation chains (two row blocks t, two halves of
+// the sum over m), interleaved so that consecutive matrix instructions do
+// not wait on each other's results; the halves are added at the end.
+__device__ __forceinline__ void mfma_contract8(const float (&A)[2][8], const float (&v)[8],
+                                               float (&out)[8]) {
+  mfma_f4 c00 = {0.f, 0.f, 0.f, 0.f}, c01 = c00, c10 = c00, c11 = c00;
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    c00 = __builtin_amdgcn_mfma_f32_4x4x1f32(A[0][m], v[m], c00, 0, 0, 0);
+    c10 = __builtin_amdgcn_mfma_f32_4x4x1f32(A[1][m], v[m], c10, 0, 0, 0);
+    c01 = __builtin_amdgcn_mfma_f32_4x4x1f32(A[0][m + 4], v[m + 4], c01, 0, 0, 0);
+    c11 = __builtin_amdgcn_mfma_f32_4x4x1f32(A[1][m + 4], v[m + 4], c11, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    out[r] = c00[r] + c01[r];
+    out[4 + r] = c10[r] + c11[r];
+  }
+}
+
+// pen_op3 for N = 8, T = float, no p field and no hooks, on the matrix
+// cores; post(k, Lu_k, M_k) as in pen_op3.  One element per wave: lane =
+// a + 8 b.  All lanes of the workgroup must call this (barriers).
+template <typename POST>
+__device__ __forceinline__ void pen_lap3_mfma8(const float (&ru)[8], float *su, float *sd,
+                                               const MfmaD8 &Dm, const float *__restrict__ W,
+                                               int a, int b, float Wa, float Wb, float jac,
+                                               float g0, float g1, float g2, POST &&post) {
+  using S = Pen3<8>;
+  const float cab = (Wb * Wa) * jac;
+#pragma unroll
+  for (int k = 0; k < 8; k++) su[S::at(a, b, k)] = ru[k];
+  float dz[8];
+  {
+    float gz[8], f[8];
+    mfma_contract8(Dm.g, ru, gz);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const float M = W[k] * cab;
+      f[k] = M * (gz[k] * g2);
+    }
+    mfma_contract8(Dm.d, f, dz);
+  }
+  __syncthreads();
+  {
+    float xp[8], gx[8], f[8], s[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) xp[m] = su[S::at(m, a, b)];
+    mfma_contract8(Dm.g, xp, gx);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const float M = W[i] * cab;
+      f[i] = M * (gx[i] * g0);
+    }
+    mfma_contract8(Dm.d, f, s);
+#pragma unroll
+    for (int i = 0; i < 8; i++) sd[S::at(i, a, b)] = s[i];
+  }
+  float dy[8];
+  {
+    float yp[8], gy[8], f[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) yp[m] = su[S::at(a, m, b)];
+    mfma_contract8(Dm.g, yp, gy);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const float M = W[j] * cab;
+      f[j] = M * (gy[j] * g1);
+    }
+    mfma_contract8(Dm.d, f, dy);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int q = S::at(a, j, b);
+    sd[q] = sd[q] + dy[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int q = S::at(a, b, k);
+    const float M = W[k] * cab;
+    post(k, sd[q] + dz[k], M, 0.f, 0.f);
+  }
+}
+
 struct NoPenCB {
   template <typename T>
   __device__ __forceinline__ T operator()(int, T, T) const {

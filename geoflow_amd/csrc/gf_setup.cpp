@@ -467,6 +467,24 @@ double eval_basis(const LDBasis &b, int i, double eta) {
   return f;
 }
 
+// h_j(eta) of GLLBasis::evalBasis (3), the matrix form used by
+// init_qdealias (gllbasis.ipp:1341-1377): no ttiny shifts.
+double eval_basis_m(const LDBasis &b, int j, double eta) {
+  const int Np = b.p;
+  const LD fact = -1.0 / (Np * (Np + 1.0));
+  const LD tetiny = 100.0L * std::numeric_limits<double>::epsilon();
+  const LD xi = (LD)eta;
+  double f = 1.0;
+  if (xi < -1.0L || xi > 1.0L) f = 0.0;
+  else if (std::fabs(xi - b.x[j]) > tetiny) {
+    LD P, dP;
+    jacobi_ld(Np, xi, P, dP);
+    const LD g = (1.0L - xi * xi) / (xi - b.x[j]);
+    f = (double)(fact * g * dP / b.Pn[j]);
+  }
+  return f;
+}
+
 // h_j'(eta) of GLLBasis::evalDBasis (4) (gllbasis.ipp:1628-1665).
 double eval_dbasis(const LDBasis &b, int j, double eta) {
   const int Np = b.p;
@@ -1038,3 +1056,17 @@ int gf_terrain_box_grid(int p, const int *ne, const double *P0, const double *P1
 }
 
 }  // extern "C"
+
+// Interpolation from the order-p GLL nodes to the order-pq GLL nodes,
+// IQP(a, i) = h_i(xq_a) at I[a + (pq+1) i], and the order-pq weights
+// (GGrid::init_qdealias, ggrid.ipp:2368-2383).  Host-side.
+int gf_dealias_matrix(int p, int pq, double *I, double *wq) {
+  if (p < 1 || pq < p || pq > 63 || !I || !wq) return GF_ERR_ARG;
+  const int N = p + 1, Q = pq + 1;
+  const LDBasis lb = make_ldbasis(p);
+  std::vector<double> xq(Q), Dq((size_t)Q * Q);
+  if (gf_gll_basis(pq, xq.data(), wq, Dq.data()) != GF_OK) return GF_ERR_ARG;
+  for (int a = 0; a < Q; a++)
+    for (int i = 0; i < N; i++) I[a + (size_t)Q * i] = eval_basis_m(lb, i, xq[a]);
+  return GF_OK;
+}

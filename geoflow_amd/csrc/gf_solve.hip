@@ -340,6 +340,24 @@ __global__ void k_dss_cg(T *__restrict__ u, const int *__restrict__ goff,
   if (scal[S_DONE] != 0.0) return;
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += gridDim.x * blockDim.x) {
     int b = goff[g], e = goff[g + 1];
+    if (e - b <= 8) {
+      // all member loads issued before the sum (as in k_dss)
+      const int cnt = e - b;
+      int id[8];
+      T v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) id[k] = k < cnt ? gmem[b + k] : 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = k < cnt ? u[id[k]] : T(0);
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k < cnt) s = s + v[k];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k < cnt) u[id[k]] = s;
+      continue;
+    }
     T s = T(0);
     for (int k = b; k < e; k++) s = s + u[gmem[k]];
     for (int k = b; k < e; k++) u[gmem[k]] = s;
@@ -423,10 +441,14 @@ __device__ double finish_norm(int normtype, double v, const double *scal) {
 }
 
 // rho_new, beta, rnorm, iteration count and the stopping rule of
-// gcg.ipp:297-335: continue while iter < maxit and rnorm > rtol.
-__global__ void k_cg_beta(double *scal, const double *__restrict__ part, int np, int normtype,
-                          double *resid) {
-  __shared__ double sred[kThreads / 64];
+// gcg.ipp:297-335: continue while iter < maxit and rnorm > rtol.  The
+// scalars are also written to hmirror (pinned host memory mapped into the
+// device), from which the host polls the stopping flag without a copy.
+__global__ void k_cg_beta(double *scal,
+                                                         const double *__restrict__ part, int np,
+                                                         int normtype, double *resid,
+                                                         double *hmirror) {
+  __shared__ double sred[kRedThreads / 64];
   if (scal[S_DONE] != 0.0) return;
   double v = 0.0, w = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) {
@@ -449,6 +471,173 @@ __global__ void k_cg_beta(double *scal, const double *__restrict__ part, int np,
     scal[S_ITER] = it;
     scal[S_FIRST] = 0.0;
     if (!(it < (int)scal[S_MAXIT] && rn > scal[S_RTOL])) scal[S_DONE] = 1.0;
+    if (hmirror)
+      for (int k = 0; k < S_COUNT; k++) hmirror[k] = scal[k];
+  }
+}
+
+// ---- CG on a batched mesh: every member its own scalars ------------------
+// Update blocks are assigned to members in proportion to their size
+// (boff[s] .. boff[s+1]); each block works its member's node range only.
+__device__ __forceinline__ int seg_of_block(const int *__restrict__ boff, int nseg, int b) {
+  int lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (boff[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename T, int NORM>
+__global__ void k_cg_update_seg(T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r,
+                                const T *__restrict__ qv, const T *__restrict__ mask,
+                                const T *__restrict__ dinv, const T *__restrict__ imult,
+                                const double *__restrict__ mass, const double *sscal,
+                                const int64_t *__restrict__ eoff, const int *__restrict__ boff,
+                                int nseg, size_t nn, double *__restrict__ part) {
+  __shared__ double sred[kThreads / 64];
+  const int s = seg_of_block(boff, nseg, blockIdx.x);
+  const double *scal = sscal + kSegScal * s;
+  double rho = 0.0, nrm = 0.0;
+  if (scal[S_DONE] == 0.0) {
+    const T alpha = (T)scal[S_ALPHA];
+    const size_t n0 = (size_t)eoff[s] * nn, n1 = (size_t)eoff[s + 1] * nn;
+    const size_t lb = blockIdx.x - boff[s], nbs = boff[s + 1] - boff[s];
+    for (size_t g = n0 + lb * blockDim.x + threadIdx.x; g < n1; g += nbs * blockDim.x) {
+      const T dv = dinv ? dinv[g] : T(1);
+      const double im = (double)imult[g];
+      T xv = (T)1 * x[g] + alpha * w[g];
+      T qm = mask ? qv[g] * mask[g] : qv[g];
+      T rv = (T)1 * r[g] + (-alpha) * qm;
+      x[g] = xv;
+      r[g] = rv;
+      T z = dinv ? dv * rv : rv;
+      rho += (double)rv * (double)z * im;
+      double rd = (double)rv;
+      if (NORM == GF_NORM_INF) nrm = fmax(nrm, fabs(rd));
+      else if (NORM == GF_NORM_EUC) nrm += rd * rd;
+      else if (NORM == GF_NORM_L2) nrm += mass[g] * rd * rd;
+      else if (NORM == GF_NORM_L1) nrm += mass[g] * fabs(rd);
+      else nrm += rd * rd * im;
+    }
+  }
+  double a = gf::block_reduce<false>(rho, sred);
+  double b = NORM == GF_NORM_INF ? gf::block_reduce<true>(nrm, sred)
+                                 : gf::block_reduce<false>(nrm, sred);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// alpha of every member: one wave per member, over the member's apply
+// partials (logical blocks eoff[s]/epb .. eoff[s+1]/epb).
+__global__ __launch_bounds__(1024) void k_cg_alpha_seg(double *sscal, const double *__restrict__ part,
+                                                       const int64_t *__restrict__ eoff, int nseg,
+                                                       int epb) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int s = wid; s < nseg; s += nw) {
+    double *sc = sscal + kSegScal * s;
+    if (sc[S_DONE] != 0.0) continue;
+    const int64_t b0 = eoff[s] / epb, b1 = eoff[s + 1] / epb;
+    double v = 0.0;
+    for (int64_t i = b0 + lane; i < b1; i += 64) v += part[i];
+    v = gf::wave_sum(v);
+    if (lane == 0) {
+      sc[S_WQ] = v;
+      sc[S_ALPHA] = sc[S_RHO] / v;
+    }
+  }
+}
+
+// rho, beta, norm, iteration count and stopping test of every member (the
+// rule of k_cg_beta per member), one wave per member; then the batch is
+// done when every member is, which scal[S_DONE] and the host mirror record.
+__global__ __launch_bounds__(1024) void k_cg_beta_seg(double *scal, double *sscal,
+                                                      const double *__restrict__ part,
+                                                      const int *__restrict__ boff, int nseg,
+                                                      int normtype, double *hmirror) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int s = wid; s < nseg; s += nw) {
+    double *sc = sscal + kSegScal * s;
+    if (sc[S_DONE] != 0.0) continue;
+    double v = 0.0, w = 0.0;
+    for (int i = boff[s] + lane; i < boff[s + 1]; i += 64) {
+      v += part[2 * i];
+      w = normtype == GF_NORM_INF ? fmax(w, part[2 * i + 1]) : w + part[2 * i + 1];
+    }
+    v = gf::wave_sum(v);
+    w = normtype == GF_NORM_INF ? gf::wave_max(w) : gf::wave_sum(w);
+    if (lane == 0) {
+      const double rhom = sc[S_RHO];
+      sc[S_RHOM] = rhom;
+      sc[S_RHO] = v;
+      sc[S_BETA] = v / rhom;
+      const double rn = finish_norm(normtype, w, sc);
+      sc[S_RNORM] = rn;
+      const int it = (int)sc[S_ITER] + 1;
+      sc[S_ITER] = it;
+      sc[S_FIRST] = 0.0;
+      if (!(it < (int)sc[S_MAXIT] && rn > sc[S_RTOL])) sc[S_DONE] = 1.0;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    bool all = true;
+    for (int s = 0; s < nseg; s++) all = all && sscal[kSegScal * s + S_DONE] != 0.0;
+    scal[S_DONE] = all ? 1.0 : 0.0;
+    if (hmirror) hmirror[S_DONE] = scal[S_DONE];
+  }
+}
+
+// Set-up reductions per member (rho = sum r z imult and the norm's sum),
+// blocks assigned as in k_cg_update_seg; k_setup_fin_seg finishes them,
+// out[2s], out[2s+1].
+template <typename T>
+__global__ void k_setup_red_seg(const T *__restrict__ r, const T *__restrict__ z,
+                                const T *__restrict__ imult, const double *__restrict__ mass,
+                                int normtype, const int64_t *__restrict__ eoff,
+                                const int *__restrict__ boff, int nseg, size_t nn,
+                                double *__restrict__ part) {
+  __shared__ double sred[kThreads / 64];
+  const int s = seg_of_block(boff, nseg, blockIdx.x);
+  const size_t n0 = (size_t)eoff[s] * nn, n1 = (size_t)eoff[s + 1] * nn;
+  const size_t lb = blockIdx.x - boff[s], nbs = boff[s + 1] - boff[s];
+  double rho = 0.0, nrm = 0.0;
+  for (size_t g = n0 + lb * blockDim.x + threadIdx.x; g < n1; g += nbs * blockDim.x) {
+    double rv = (double)r[g];
+    if (z) rho += rv * (double)z[g] * (double)imult[g];
+    if (normtype == GF_NORM_INF) nrm = fmax(nrm, fabs(rv));
+    else if (normtype == GF_NORM_EUC) nrm += rv * rv;
+    else if (normtype == GF_NORM_L2) nrm += mass[g] * rv * rv;
+    else if (normtype == GF_NORM_L1) nrm += mass[g] * fabs(rv);
+    else nrm += rv * rv * (double)imult[g];
+  }
+  double a = gf::block_reduce<false>(rho, sred);
+  double b = normtype == GF_NORM_INF ? gf::block_reduce<true>(nrm, sred)
+                                     : gf::block_reduce<false>(nrm, sred);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+__global__ __launch_bounds__(1024) void k_setup_fin_seg(const double *__restrict__ part,
+                                                        const int *__restrict__ boff, int nseg,
+                                                        int normtype, double *out) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int s = wid; s < nseg; s += nw) {
+    double v = 0.0, w = 0.0;
+    for (int i = boff[s] + lane; i < boff[s + 1]; i += 64) {
+      v += part[2 * i];
+      w = normtype == GF_NORM_INF ? fmax(w, part[2 * i + 1]) : w + part[2 * i + 1];
+    }
+    v = gf::wave_sum(v);
+    w = normtype == GF_NORM_INF ? gf::wave_max(w) : gf::wave_sum(w);
+    if (lane == 0) {
+      out[2 * s] = v;
+      out[2 * s + 1] = w;
+    }
   }
 }
 
@@ -502,6 +691,9 @@ __global__ void k_fill_f32(float *__restrict__ o, float v, size_t n) {
 }
 __global__ void k_to_f32(float *__restrict__ o, const double *__restrict__ i, double s, size_t n) {
   GRID_LOOP(g, n) o[g] = (float)(i[g] * s);
+}
+__global__ void k_from_f32(double *__restrict__ o, const float *__restrict__ i, size_t n) {
+  GRID_LOOP(g, n) o[g] = (double)i[g];
 }
 __global__ void k_add_f32(double *__restrict__ x, const float *__restrict__ d, double s, size_t n) {
   GRID_LOOP(g, n) x[g] = x[g] + s * (double)d[g];
@@ -971,29 +1163,35 @@ int rk_steps_box(gf_mesh *m, double nu, int order, double dt, int nsteps, double
 // stages read the two w_i instead of three velocity and six metric fields,
 //   K = imass (-nu L us) - ((D1 us) w_0 + (D2 us) w_1).
 // The regrouping changes rounding only (relative 1e-16 per term).
+template <typename T>
 struct EmbAdvArgs {
-  const double *uin;    // stage input base (state at s = 0, step-start copy after)
-  const double *kprev;  // smoothed K of the previous stage, or null
-  double *ust;          // state, advanced by ch * kprev when kprev is set
-  double *ucopy;        // receives uin when kprev is null (step-start copy)
+  const T *uin;         // stage input base (state at s = 0, step-start copy after)
+  const T *kprev;       // smoothed K of the previous stage, or null
+  T *ust;               // state, advanced by ch * kprev when kprev is set
+  T *ucopy;             // receives uin when kprev is null (step-start copy)
   double bh, ch, nu;
   const double *c[3];   // velocity (read when wform is set)
   const double *ndx[6]; // dXi_i/dX_j at [i*3+j]
   const double *G[3];
-  const double *mass, *imass, *D;
+  const double *mass, *imass;
+  const T *D;
   double *w[2];         // imass M chat_i: written when wform is set, else read
   int wform;
-  double *kout;
+  T *kout;
   size_t nelems;
 };
 
-template <int N>
-__global__ __launch_bounds__(kThreads) void k_adv_stage_embed(EmbAdvArgs a) {
+// T is the precision of the advected field, of its stage fields and of the
+// two tensor-product contractions (FP64, or FP32 in the mixed-precision
+// stepper); the metric terms (dXi/dX, G, mass, the velocity products w_i)
+// stay FP64 and the products with them are formed in FP64.
+template <int N, typename T>
+__global__ __launch_bounds__(kThreads) void k_adv_stage_embed(EmbAdvArgs<T> a) {
   using S = Shape<2, N>;
   static_assert(S::NODES <= kThreads, "one node per thread");
-  __shared__ double su[S::NODES];
-  __shared__ double sg[2][S::NODES];
-  __shared__ double sD[N * N];
+  __shared__ T su[S::NODES];
+  __shared__ T sg[2][S::NODES];
+  __shared__ T sD[N * N];
   const size_t e0 = (size_t)blockIdx.x * S::EPB;
   const int nel = (int)min((size_t)S::EPB, a.nelems - e0);
   const size_t base = e0 * S::NN;
@@ -1002,11 +1200,11 @@ __global__ __launch_bounds__(kThreads) void k_adv_stage_embed(EmbAdvArgs a) {
   const bool act = q < nel * S::NN;
   const size_t g = base + q;
   if (act) {
-    double v = a.uin[g];
+    T v = a.uin[g];
     if (a.kprev) {
-      const double kp = a.kprev[g];
-      a.ust[g] = a.ust[g] + a.ch * kp;
-      v = v + a.bh * kp;
+      const T kp = a.kprev[g];
+      a.ust[g] = a.ust[g] + (T)a.ch * kp;
+      v = v + (T)a.bh * kp;
     } else if (a.ucopy) {
       a.ucopy[g] = v;
     }
@@ -1018,7 +1216,7 @@ __global__ __launch_bounds__(kThreads) void k_adv_stage_embed(EmbAdvArgs a) {
   gf::node_ijk<2, N>(r, i, j, k);
   double po = 0.0, im = 0.0;
   if (act) {
-    const double *se = su + el * S::NN;
+    const T *se = su + el * S::NN;
     const double d0 = gf::contract<2, N, false>(se, sD, 0, i, j, k);
     const double d1 = gf::contract<2, N, false>(se, sD, 1, i, j, k);
     im = a.imass[g];
@@ -1040,28 +1238,64 @@ __global__ __launch_bounds__(kThreads) void k_adv_stage_embed(EmbAdvArgs a) {
     g0 += d1 * G01;
     double g1 = 0.0 + d0 * G01;
     g1 += d1 * G11;
-    sg[0][q] = g0;
-    sg[1][q] = g1;
+    sg[0][q] = (T)g0;
+    sg[1][q] = (T)g1;
   }
   __syncthreads();
   if (act) {
-    double lap = 0.0;
-    lap += gf::contract<2, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
-    lap += gf::contract<2, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
+    T lt = T(0);
+    lt += gf::contract<2, N, true>(&sg[0][el * S::NN], sD, 0, i, j, k);
+    lt += gf::contract<2, N, true>(&sg[1][el * S::NN], sD, 1, i, j, k);
+    double lap = lt;
     if (a.nu != 1.0) lap *= a.nu;
-    a.kout[g] = im * (-lap) - po;
+    a.kout[g] = (T)(im * (-lap) - po);
   }
 }
 
+template <typename T>
+struct AdvStageLaunch {
+  template <int DIM, int N>
+  static int run(gf_mesh *m, const EmbAdvArgs<T> &a) {
+    if (DIM != 2) return GF_ERR_UNSUPPORTED;
+    using S = Shape<2, N>;
+    const size_t nb = (m->nelems + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL((k_adv_stage_embed<N, T>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
+    GF_LAUNCH_CHECK(m->ctx);
+    return GF_OK;
+  }
+};
 template <int DIM, int N>
-int launch_adv_stage_embed(gf_mesh *m, const EmbAdvArgs &a) {
-  if (DIM != 2) return GF_ERR_UNSUPPORTED;
-  using S = Shape<2, N>;
-  const size_t nb = (m->nelems + S::EPB - 1) / S::EPB;
-  hipLaunchKernelGGL((k_adv_stage_embed<N>), dim3(nb), dim3(kThreads), 0, m->ctx->stream, a);
-  GF_LAUNCH_CHECK(m->ctx);
-  return GF_OK;
+int launch_adv_stage_embed_d(gf_mesh *m, const EmbAdvArgs<double> &a) {
+  return AdvStageLaunch<double>::run<DIM, N>(m, a);
 }
+template <int DIM, int N>
+int launch_adv_stage_embed_f(gf_mesh *m, const EmbAdvArgs<float> &a) {
+  return AdvStageLaunch<float>::run<DIM, N>(m, a);
+}
+int launch_adv_stage_embed_t(gf_mesh *m, const EmbAdvArgs<double> &a) {
+  return GF_DISPATCH_N(2, m->Niso, launch_adv_stage_embed_d, m, a);
+}
+int launch_adv_stage_embed_t(gf_mesh *m, const EmbAdvArgs<float> &a) {
+  return GF_DISPATCH_N(2, m->Niso, launch_adv_stage_embed_f, m, a);
+}
+
+template <typename T>
+__global__ void k_final_input_t(T *__restrict__ u3, const T *__restrict__ uin,
+                                const T *__restrict__ k, T bh, T *acc, const T *accsrc, T ch,
+                                size_t n) {
+  GRID_LOOP(g, n) {
+    T kv = k[g];
+    u3[g] = uin[g] + bh * kv;
+    if (acc) acc[g] = accsrc[g] + kv * ch;
+  }
+}
+template <typename T>
+__global__ void k_axpby_t(T *__restrict__ z, T a, const T *__restrict__ x, T b, const T *y,
+                          size_t n) {
+  GRID_LOOP(g, n) z[g] = a * x[g] + b * y[g];
+}
+int dss_smooth(gf_mesh *m, double *u) { return gf_dss_t(m, GF_DSS_SMOOTH, u); }
+int dss_smooth(gf_mesh *m, float *u) { return gf_dss_f32(m, GF_DSS_SMOOTH, u); }
 
 // Pure-advection RK step on a 2-D embedded isotropic mesh with three velocity
 // components (mode 1, nstate 4): the arithmetic of rk_step_general with each
@@ -1072,65 +1306,74 @@ int launch_adv_stage_embed(gf_mesh *m, const EmbAdvArgs &a) {
 //              ust += c h K;  Smooth ust
 // where the ust accumulation of K_{s-1} is done by stage s's kernel.  The
 // Butcher tables of orders 1-4 have only sub-diagonal b, so the reference's
-// isum = sum_j b_{sj} h K_j equals b_{s,s-1} h K_{s-1} exactly.
-int rk_step_embed_adv(gf_mesh *m, double nu, int order, double dt, double *const *ust) {
-  const size_t n = m->ndof, bytes = n * sizeof(double);
+// isum = sum_j b_{sj} h K_j equals b_{s,s-1} h K_{s-1} exactly.  T = float
+// is the mixed-precision form: the advected field and its stage fields in
+// FP32, the metric terms in FP64 (k_adv_stage_embed).
+template <typename T>
+int rk_step_embed_adv_t(gf_mesh *m, double nu, int order, double dt, T *ust0,
+                        const double *const *vel) {
+  const size_t n = m->ndof, bytes = n * sizeof(T), dbytes = n * sizeof(double);
   double alpha[4], beta[16], c[4];
   butcher(order, alpha, beta, c);
   const int ns = order;
   const double h = dt;
-  double *ucopy = (double *)gf_ws(m, WS_RK_BEGIN + 0, bytes);
-  double *us = (double *)gf_ws(m, WS_RK_BEGIN + 3, bytes);
-  double *K[2] = {(double *)gf_ws(m, WS_RK_BEGIN + 6, bytes),
-                  (double *)gf_ws(m, WS_RK_BEGIN + 9, bytes)};
-  double *w0 = (double *)gf_ws(m, WS_RK_BEGIN + 1, bytes);
-  double *w1 = (double *)gf_ws(m, WS_RK_BEGIN + 2, bytes);
+  T *ucopy = (T *)gf_ws(m, WS_RK_BEGIN + 0, bytes);
+  T *us = (T *)gf_ws(m, WS_RK_BEGIN + 3, bytes);
+  T *K[2] = {(T *)gf_ws(m, WS_RK_BEGIN + 6, bytes), (T *)gf_ws(m, WS_RK_BEGIN + 9, bytes)};
+  double *w0 = (double *)gf_ws(m, WS_RK_BEGIN + 1, dbytes);
+  double *w1 = (double *)gf_ws(m, WS_RK_BEGIN + 2, dbytes);
   if (!ucopy || !us || !K[0] || !K[1] || !w0 || !w1)
     return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
-  EmbAdvArgs a;
+  EmbAdvArgs<T> a;
   memset(&a, 0, sizeof(a));
   a.nu = nu;
-  for (int k = 0; k < 3; k++) a.c[k] = ust[1 + k];
+  for (int k = 0; k < 3; k++) a.c[k] = vel[k];
   for (int k = 0; k < 6; k++) a.ndx[k] = m->ndx[(k / 3) * 3 + k % 3];
   for (int k = 0; k < 3; k++) a.G[k] = m->G[k];
   a.mass = m->mass;
   a.imass = m->imass;
-  a.D = m->D[0];
+  a.D = sizeof(T) == 8 ? (const T *)m->D[0] : (const T *)m->D32;
   a.w[0] = w0;
   a.w[1] = w1;
   a.nelems = m->nelems;
   for (int s = 0; s < ns - 1; s++) {
     a.wform = s == 0;
-    a.uin = s == 0 ? ust[0] : ucopy;
+    a.uin = s == 0 ? ust0 : ucopy;
     a.kprev = s == 0 ? nullptr : K[(s - 1) % 2];
-    a.ust = ust[0];
+    a.ust = ust0;
     a.ucopy = s == 0 ? ucopy : nullptr;
     a.bh = s == 0 ? 0.0 : beta[s * 4 + (s - 1)] * h;
     a.ch = s == 0 ? 0.0 : c[s - 1] * h;
     a.kout = K[s % 2];
-    RC(GF_DISPATCH_N(2, m->Niso, launch_adv_stage_embed, m, a));
-    RC(gf_dss_t(m, GF_DSS_SMOOTH, K[s % 2]));
+    RC(launch_adv_stage_embed_t(m, a));
+    RC(dss_smooth(m, K[s % 2]));
   }
   if (ns >= 2) {
-    hipLaunchKernelGGL(k_final_input, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, us,
-                       ucopy, K[(ns - 2) % 2], beta[(ns - 1) * 4 + (ns - 2)] * h, ust[0], ust[0],
-                       c[ns - 2] * h, n);
+    hipLaunchKernelGGL(k_final_input_t<T>, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, us,
+                       ucopy, K[(ns - 2) % 2], (T)(beta[(ns - 1) * 4 + (ns - 2)] * h), ust0, ust0,
+                       (T)(c[ns - 2] * h), n);
     GF_LAUNCH_CHECK(m->ctx);
   } else {
-    RC(copy(m, us, ust[0]));
+    GF_HIP(m->ctx, hipMemcpyAsync(us, ust0, bytes, hipMemcpyDeviceToDevice, m->ctx->stream));
   }
-  RC(gf_dss_t(m, GF_DSS_SMOOTH, us));
+  RC(dss_smooth(m, us));
   a.uin = us;
   a.kprev = nullptr;
   a.ucopy = nullptr;
   a.wform = ns == 1;
   a.bh = a.ch = 0.0;
   a.kout = K[(ns - 1) % 2];
-  RC(GF_DISPATCH_N(2, m->Niso, launch_adv_stage_embed, m, a));
-  double *kl = K[(ns - 1) % 2];
-  RC(gf_dss_t(m, GF_DSS_SMOOTH, kl));
-  RC(axpby(m, ust[0], 1.0, ust[0], c[ns - 1] * h, kl));
-  return gf_dss_t(m, GF_DSS_SMOOTH, ust[0]);
+  RC(launch_adv_stage_embed_t(m, a));
+  T *kl = K[(ns - 1) % 2];
+  RC(dss_smooth(m, kl));
+  hipLaunchKernelGGL(k_axpby_t<T>, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, ust0, (T)1,
+                     ust0, (T)(c[ns - 1] * h), kl, n);
+  GF_LAUNCH_CHECK(m->ctx);
+  return dss_smooth(m, ust0);
+}
+
+int rk_step_embed_adv(gf_mesh *m, double nu, int order, double dt, double *const *ust) {
+  return rk_step_embed_adv_t<double>(m, nu, order, dt, ust[0], ust + 1);
 }
 
 // General RK step composed from the operators, following step_b literally
@@ -1253,6 +1496,26 @@ int gf_burgers_steps(gf_mesh *m, int mode, double nu, int order, int nstate, dou
   return GF_OK;
 }
 
+int gf_burgers_steps_mixed(gf_mesh *m, int mode, double nu, int order, int nstate, double t,
+                           double dt, int nsteps, double *const *u) {
+  if (!m || !u || nsteps < 0 || order < 1 || order > 4) return GF_ERR_ARG;
+  if (!(mode == 1 && nstate == 4 && m->gtype == GF_GE_2DEMBEDDED && m->Niso && m->D32))
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED,
+                      "mixed-precision steps: pure advection (mode 1, nstate 4) on embedded meshes");
+  for (int k = 0; k < nstate; k++)
+    if (!u[k]) return GF_ERR_ARG;
+  (void)t;
+  const size_t n = m->ndof;
+  float *h32 = (float *)gf_ws(m, WS_RK_BEGIN + 4, n * sizeof(float));
+  if (!h32) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
+  hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, h32, u[0], 1.0,
+                     n);
+  for (int s = 0; s < nsteps; s++) RC(rk_step_embed_adv_t<float>(m, nu, order, dt, h32, u + 1));
+  hipLaunchKernelGGL(k_from_f32, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, u[0], h32, n);
+  GF_LAUNCH_CHECK(m->ctx);
+  return GF_OK;
+}
+
 int gf_helmholtz_diag_inv(gf_mesh *m, double p, double qs, const double *qf, double *dinv) {
   if (!m || !dinv) return GF_ERR_ARG;
   if (!m->Niso) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "Jacobi diagonal: isotropic meshes");
@@ -1365,6 +1628,9 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a.part = partA;
   a.nelems = m->nelems;
   CGApply3Args a3;
+  a3.seg_eoff = nullptr;
+  a3.nseg = 0;
+  a3.sscal = nullptr;
   a3.r = v.r;
   a3.dinv = v.dinv;
   a3.w = v.w;
@@ -1387,6 +1653,17 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a3.scal = m->scal;
   a3.part = partA;
   a3.nelems = m->nelems;
+  // the host mirror starts from the scalars prepared by the caller (a
+  // solve that is already done queues nothing that would update it)
+  volatile double *hm = m->hscal;
+  GF_HIP(m->ctx, hipMemcpyAsync(m->hscal, m->scal, S_COUNT * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  GF_HIP(m->ctx, hipStreamSynchronize(s));
+  if (hm[S_DONE] != 0.0) {
+    *iters = (int)m->hscal[S_ITER];
+    return GF_OK;
+  }
+  int cur = 0, nq = 0;
   for (;;) {
     for (int b = 0; b < kBatch; b++) {
       int rc;
@@ -1420,13 +1697,24 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
         }
       }
       hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kThreads), 0, s, m->scal, partU, kUpdBlocks,
-                         normtype, d_resid);
+                         normtype, d_resid, m->hscal_dev);
     }
     GF_LAUNCH_CHECK(m->ctx);
-    GF_HIP(m->ctx, hipMemcpyAsync(m->hscal, m->scal, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
-    GF_HIP(m->ctx, hipStreamSynchronize(s));
-    if (m->hscal[S_DONE] != 0.0) break;
+    // One batch of look-ahead: batch b is queued before the host waits for
+    // batch b-1, whose last k_cg_beta has written the stopping flag into
+    // the mapped host mirror.  Iterations queued after convergence return
+    // at once (S_DONE).
+    GF_HIP(m->ctx, hipEventRecord(m->cg_ev[cur], s));
+    if (nq++ > 0) {
+      GF_HIP(m->ctx, hipEventSynchronize(m->cg_ev[cur ^ 1]));
+      if (hm[S_DONE] != 0.0) break;
+    }
+    cur ^= 1;
   }
+  // the final scalars, after everything queued has run
+  GF_HIP(m->ctx, hipMemcpyAsync(m->hscal, m->scal, S_COUNT * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  GF_HIP(m->ctx, hipStreamSynchronize(s));
   *iters = (int)m->hscal[S_ITER];
   return GF_OK;
 }
@@ -1453,6 +1741,313 @@ int cg_check_args(gf_mesh *m, const gf_cg_params *prm, const double *b, double *
   if (prm->normtype < GF_NORM_INF || prm->normtype > GF_NORM_ASSEMBLED_L2) return GF_ERR_ARG;
   if (prm->p_field) return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED, "CG: p field not supported");
   return GF_OK;
+}
+
+// ---- CG on a batched mesh (members set by gf_mesh_set_segments) -----------
+__global__ void k_fill_d(double *__restrict__ o, double v, size_t n) {
+  GRID_LOOP(g, n) o[g] = v;
+}
+
+// Per-member (rho, norm sum) of r (and z): out[2s], out[2s+1]; synchronous.
+template <typename T>
+int seg_red(gf_mesh *m, const T *r, const T *z, const T *imult, int normtype,
+            std::vector<double> &out) {
+  const int ns = m->nseg, nbU = m->h_seg_boff[ns];
+  double *dout = m->red + 2 * (size_t)nbU + 64;
+  hipLaunchKernelGGL(k_setup_red_seg<T>, dim3(nbU), dim3(kThreads), 0, m->ctx->stream, r, z, imult,
+                     m->mass, normtype, m->seg_eoff, m->seg_boff, ns, m->nn, m->red);
+  hipLaunchKernelGGL(k_setup_fin_seg, dim3(1), dim3(1024), 0, m->ctx->stream, m->red, m->seg_boff,
+                     ns, normtype, dout);
+  GF_LAUNCH_CHECK(m->ctx);
+  out.resize(2 * (size_t)ns);
+  GF_HIP(m->ctx, hipMemcpyAsync(out.data(), dout, 2 * ns * sizeof(double), hipMemcpyDeviceToHost,
+                                m->ctx->stream));
+  GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+  return GF_OK;
+}
+
+// CG iterations of every member of a batched 3-D mesh (pencil apply, CSR
+// assembly) until all members are done; per-member scalars in m->sscal,
+// copied to m->hsscal at the end.
+template <typename T>
+int cg_loop_seg(gf_mesh *m, CGVecs<T> &v, int normtype) {
+  hipStream_t s = m->ctx->stream;
+  const int ns = m->nseg, nbU = m->h_seg_boff[ns];
+  const int nbA = col_blocks(m);
+  double *partA = m->red;  // one partial per element
+  double *partU = m->red + std::max((size_t)nbA, m->nelems) + 64;
+  CGApply3Args a3;
+  a3.r = v.r;
+  a3.dinv = v.dinv;
+  a3.w = v.w;
+  a3.q = v.q;
+  a3.D = v.D;
+  for (int c = 0; c < 3; c++) {
+    a3.DR[c] = sizeof(T) == 8 ? (const void *)m->pDR[c] : (const void *)m->pDR32[c];
+    a3.DC[c] = sizeof(T) == 8 ? (const void *)m->pDC[c] : (const void *)m->pDC32[c];
+  }
+  a3.W = v.W;
+  a3.edx = v.edx;
+  a3.ejac = v.ejac;
+  a3.qf = v.qf;
+  a3.qs = v.qs;
+  a3.tmask = 0;
+  a3.topo = m->box;
+  a3.mask = m->has_mask ? v.mask : nullptr;
+  a3.p = v.p;
+  a3.scal = m->scal;
+  a3.part = partA;
+  a3.nelems = m->nelems;
+  a3.seg_eoff = m->seg_eoff;
+  a3.nseg = ns;
+  a3.sscal = m->sscal;
+  volatile double *hm = m->hscal;
+  GF_HIP(m->ctx, hipMemcpyAsync(m->hscal, m->scal, S_COUNT * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  GF_HIP(m->ctx, hipStreamSynchronize(s));
+  int cur = 0, nq = 0;
+  while (hm[S_DONE] == 0.0) {
+    for (int bb = 0; bb < kBatch; bb++) {
+      int rc = sizeof(T) == 8 ? GF_DISPATCH_N3(m->Niso, cgapply3c_d, m, a3)
+                              : GF_DISPATCH_N3(m->Niso, cgapply3c_f, m, a3);
+      if (rc) return rc;
+      if (m->ngroups)
+        hipLaunchKernelGGL(k_dss_cg<T>, dim3(nblocks((size_t)m->ngroups)), dim3(kThreads), 0, s,
+                           v.q, m->goff, m->gmem, m->ngroups, (const double *)m->scal);
+      hipLaunchKernelGGL(k_cg_alpha_seg, dim3(1), dim3(1024), 0, s, m->sscal, partA, m->seg_eoff,
+                         ns, 1);
+      const T *mk = m->has_mask ? v.mask : nullptr;
+      switch (normtype) {
+#define GF_SEG_UPD(NT)                                                                        \
+  hipLaunchKernelGGL((k_cg_update_seg<T, NT>), dim3(nbU), dim3(kThreads), 0, s, v.x, v.w, v.r, \
+                     v.q, mk, v.dinv, v.imult, m->mass, (const double *)m->sscal, m->seg_eoff,  \
+                     m->seg_boff, ns, m->nn, partU);                                           \
+  break;
+        case GF_NORM_INF: GF_SEG_UPD(GF_NORM_INF)
+        case GF_NORM_EUC: GF_SEG_UPD(GF_NORM_EUC)
+        case GF_NORM_L2: GF_SEG_UPD(GF_NORM_L2)
+        case GF_NORM_L1: GF_SEG_UPD(GF_NORM_L1)
+        default: GF_SEG_UPD(GF_NORM_ASSEMBLED_L2)
+#undef GF_SEG_UPD
+      }
+      hipLaunchKernelGGL(k_cg_beta_seg, dim3(1), dim3(1024), 0, s, m->scal, m->sscal, partU,
+                         m->seg_boff, ns, normtype, m->hscal_dev);
+    }
+    GF_LAUNCH_CHECK(m->ctx);
+    GF_HIP(m->ctx, hipEventRecord(m->cg_ev[cur], s));
+    if (nq++ > 0) {
+      GF_HIP(m->ctx, hipEventSynchronize(m->cg_ev[cur ^ 1]));
+      if (hm[S_DONE] != 0.0) break;
+    }
+    cur ^= 1;
+  }
+  GF_HIP(m->ctx, hipMemcpyAsync(m->hsscal, m->sscal, (size_t)ns * kSegScal * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  GF_HIP(m->ctx, hipStreamSynchronize(s));
+  return GF_OK;
+}
+
+// gf_cg_solve on a batched mesh: the GCG recurrence (FP64) or the
+// mixed-precision solver of cg_solve_impl, with every member's own scalars,
+// norm of b, stopping test and reliable updates.
+int cg_solve_seg(gf_mesh *m, const gf_cg_params *prm, const double *b, const double *xb,
+                 double *x, gf_cg_result *res, const CGWork &wk) {
+  if (!(m->dim == 3 && m->dconst && !m->has_box))
+    return gf_set_err(m->ctx, GF_ERR_UNSUPPORTED,
+                      "batched CG: 3-D regular meshes with the standard D, CSR assembly");
+  hipStream_t s = m->ctx->stream;
+  const int ns = m->nseg;
+  const size_t n = m->ndof;
+  const bool bbv = xb != nullptr;
+  const int normtype = prm->normtype;
+  double *r = wk.r, *w = wk.w, *q = wk.q, *dinv = wk.dinv;
+  const double qs = prm->q_field ? 0.0 : prm->q_scalar;
+  std::vector<double> o, vol(ns), nseg_n(ns), rn0(ns), rtol(ns);
+  for (int k = 0; k < ns; k++)
+    nseg_n[k] = (double)((m->h_seg_eoff[k + 1] - m->h_seg_eoff[k]) * (int64_t)m->nn);
+  // member volumes: sum of the mass (the L1 norm of ones)
+  hipLaunchKernelGGL(k_fill_d, dim3(nblocks(n)), dim3(kThreads), 0, s, w, 1.0, n);
+  RC(seg_red<double>(m, w, nullptr, m->imult, GF_NORM_L1, o));
+  for (int k = 0; k < ns; k++) vol[k] = o[2 * k + 1];
+  auto fin = [&](int k, double v) {
+    switch (normtype) {
+      case GF_NORM_INF: return v;
+      case GF_NORM_EUC: return std::sqrt(v / nseg_n[k]);
+      case GF_NORM_L2: return std::sqrt(v / vol[k]);
+      case GF_NORM_L1: return v / vol[k];
+      default: return std::sqrt(v);
+    }
+  };
+  if (prm->precond) RC(gf_helmholtz_diag_inv(m, prm->p_scalar, qs, prm->q_field, dinv));
+  if (bbv) {
+    RC(gf_mul_t(m, x, m->mask, x));
+    RC(gf_axpy_t(m, 1.0, xb, x));
+  }
+  if (normtype == GF_NORM_ASSEMBLED_L2) {
+    RC(copy(m, q, b));
+    RC(gf_dss_t(m, GF_DSS_SUM, q));
+    if (bbv) RC(gf_mul_t(m, q, m->mask, q));
+    RC(seg_red<double>(m, q, nullptr, m->imult, normtype, o));
+    for (int k = 0; k < ns; k++) {
+      rn0[k] = fin(k, o[2 * k + 1]);
+      rtol[k] = prm->tol * rn0[k];
+    }
+  } else {
+    RC(seg_red<double>(m, b, nullptr, m->imult, normtype, o));
+    for (int k = 0; k < ns; k++) {
+      rn0[k] = fin(k, o[2 * k + 1]);
+      rtol[k] = rn0[k] < 1.0 ? prm->tol : prm->tol * rn0[k];
+    }
+  }
+  auto residual = [&](double *rr) -> int {
+    RC(gf_helm_t(m, prm->p_scalar, nullptr, prm->q_field, qs, x, q));
+    hipLaunchKernelGGL(k_residual, dim3(nblocks(n)), dim3(kThreads), 0, s, rr, b, q, n);
+    RC(gf_dss_t(m, GF_DSS_SUM, rr));
+    if (bbv) RC(gf_mul_t(m, rr, m->mask, rr));
+    return GF_OK;
+  };
+  std::vector<double> hs((size_t)ns * kSegScal, 0.0);
+  auto upload = [&]() -> int {
+    bool all = true;
+    for (int k = 0; k < ns; k++) all = all && hs[(size_t)k * kSegScal + S_DONE] != 0.0;
+    double g[S_COUNT] = {0};
+    g[S_DONE] = all ? 1.0 : 0.0;
+    GF_HIP(m->ctx, hipMemcpyAsync(m->sscal, hs.data(), hs.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, s));
+    GF_HIP(m->ctx, hipMemcpyAsync(m->scal, g, sizeof(g), hipMemcpyHostToDevice, s));
+    GF_HIP(m->ctx, hipStreamSynchronize(s));
+    return GF_OK;
+  };
+  m->seg_iters.assign(ns, 0);
+  m->seg_rnorm.assign(ns, 0.0);
+  m->seg_rnorm0 = rn0;
+  int status = GF_OK, outer = 0;
+  if (prm->precision == 0) {
+    RC(residual(r));
+    double *z = q;
+    if (dinv) RC(gf_mul_t(m, dinv, r, z));
+    RC(seg_red<double>(m, r, dinv ? z : r, m->imult, normtype, o));
+    for (int k = 0; k < ns; k++) {
+      double *h = &hs[(size_t)k * kSegScal];
+      h[S_RHO] = o[2 * k];
+      h[S_RTOL] = rtol[k];
+      h[S_RNORM] = 10.0 * rtol[k];
+      h[S_MAXIT] = prm->maxit;
+      h[S_DONE] = (0 < prm->maxit && h[S_RNORM] > rtol[k]) ? 0.0 : 1.0;
+      h[S_FIRST] = 1.0;
+      h[S_VOL] = vol[k];
+      h[S_N] = nseg_n[k];
+    }
+    RC(upload());
+    CGVecs<double> v;
+    v.x = x;
+    v.r = r;
+    v.w = w;
+    v.q = q;
+    v.dinv = dinv;
+    v.mask = bbv ? m->mask : nullptr;
+    v.imult = m->imult;
+    v.qf = prm->q_field;
+    v.qs = qs;
+    v.D = m->D[0];
+    v.W = m->W[0];
+    v.edx = m->edx;
+    v.ejac = m->ejac;
+    for (int c = 0; c < 6; c++) v.G[c] = m->G[c];
+    v.mass = m->mass;
+    v.p = prm->p_scalar;
+    RC(cg_loop_seg<double>(m, v, normtype));
+    for (int k = 0; k < ns; k++) {
+      const double *h = &m->hsscal[(size_t)k * kSegScal];
+      m->seg_iters[k] = (int)h[S_ITER];
+      m->seg_rnorm[k] = h[S_RNORM];
+      if (m->seg_iters[k] >= prm->maxit && h[S_RNORM] > prm->tol) status = GF_ERR_NOCONVERGE;
+    }
+  } else {
+    float *r32 = wk.r32, *w32 = wk.w32, *q32 = wk.q32, *x32 = wk.x32;
+    float *dinv32 = dinv ? wk.dinv32 : nullptr;
+    float *qf32 = prm->q_field ? wk.qf32 : nullptr;
+    if (dinv) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, dinv32, dinv, 1.0, n);
+    if (qf32) hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, qf32, prm->q_field, 1.0, n);
+    const double delta = prm->inner_tol > 0 ? prm->inner_tol : 0.01;
+    CGVecs<float> v;
+    v.x = x32;
+    v.r = r32;
+    v.w = w32;
+    v.q = q32;
+    v.dinv = dinv32;
+    v.mask = bbv ? m->mask32 : nullptr;
+    v.imult = m->imult32;
+    v.qf = qf32;
+    v.qs = qs;
+    v.D = m->D32;
+    v.W = m->W32;
+    v.edx = m->edx32;
+    v.ejac = m->ejac32;
+    for (int c = 0; c < 6; c++) v.G[c] = nullptr;
+    v.mass = nullptr;
+    v.p = prm->p_scalar;
+    std::vector<int> first(ns, 1), conv(ns, 0);
+    std::vector<double> rhom(ns, 0.0);
+    for (outer = 0;; outer++) {
+      if (outer > 0)
+        hipLaunchKernelGGL(k_add_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, x, x32, 1.0, n);
+      RC(residual(r));
+      RC(seg_red<double>(m, r, nullptr, m->imult, normtype, o));
+      bool all = true;
+      for (int k = 0; k < ns; k++) {
+        m->seg_rnorm[k] = fin(k, o[2 * k + 1]);
+        conv[k] = m->seg_rnorm[k] <= rtol[k] || m->seg_iters[k] >= prm->maxit;
+        all = all && conv[k];
+      }
+      if (all || outer >= 10000) break;
+      hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, r32, r, 1.0, n);
+      hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(n)), dim3(kThreads), 0, s, x32, 0.0f, n);
+      float *z32 = q32;
+      if (dinv32) hipLaunchKernelGGL(k_mul_t<float>, dim3(nblocks(n)), dim3(kThreads), 0, s, dinv32, r32, z32, n);
+      RC(seg_red<float>(m, r32, dinv32 ? z32 : r32, m->imult32, normtype, o));
+      for (int k = 0; k < ns; k++) {
+        double *h = &hs[(size_t)k * kSegScal];
+        std::fill(h, h + kSegScal, 0.0);
+        if (conv[k]) {
+          h[S_DONE] = 1.0;
+          continue;
+        }
+        const double rn32 = fin(k, o[2 * k + 1]);
+        h[S_RHO] = o[2 * k];
+        h[S_BETA] = first[k] ? 0.0 : o[2 * k] / rhom[k];
+        h[S_RHOM] = first[k] ? 0.0 : rhom[k];
+        h[S_RTOL] = std::max(delta * rn32, 0.5 * rtol[k]);
+        h[S_RNORM] = rn32;
+        h[S_MAXIT] = std::max(1, prm->maxit - m->seg_iters[k]);
+        h[S_FIRST] = first[k] ? 1.0 : 0.0;
+        h[S_VOL] = vol[k];
+        h[S_N] = nseg_n[k];
+      }
+      RC(upload());
+      RC(cg_loop_seg<float>(m, v, normtype));
+      for (int k = 0; k < ns; k++) {
+        if (conv[k]) continue;
+        const double *h = &m->hsscal[(size_t)k * kSegScal];
+        m->seg_iters[k] += (int)h[S_ITER];
+        rhom[k] = h[S_RHOM];
+        first[k] = 0;
+      }
+    }
+    for (int k = 0; k < ns; k++)
+      if (m->seg_rnorm[k] > rtol[k]) status = GF_ERR_NOCONVERGE;
+  }
+  if (bbv) {
+    RC(gf_mul_t(m, x, m->mask, x));
+    RC(gf_axpy_t(m, 1.0, xb, x));
+  }
+  GF_HIP(m->ctx, hipStreamSynchronize(s));
+  res->status = status;
+  res->iters = *std::max_element(m->seg_iters.begin(), m->seg_iters.end());
+  res->outer = outer;
+  res->rnorm = *std::max_element(m->seg_rnorm.begin(), m->seg_rnorm.end());
+  res->rnorm0 = *std::max_element(rn0.begin(), rn0.end());
+  return status;
 }
 
 int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const double *xb,
@@ -1631,6 +2226,92 @@ int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const do
 
 }  // namespace
 
+__global__ void k_groups_in_segments(const int *__restrict__ goff, const int *__restrict__ gmem,
+                                     int ngroups, const int64_t *__restrict__ eoff, int nseg,
+                                     size_t nn, double *bad) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += gridDim.x * blockDim.x) {
+    const int a = gmem[goff[g]], z = gmem[goff[g + 1] - 1];
+    if (seg_of(eoff, nseg, (size_t)a / nn) != seg_of(eoff, nseg, (size_t)z / nn)) *bad = 1.0;
+  }
+}
+
+extern "C" int gf_mesh_set_segments(gf_mesh *m, int nseg, const int64_t *eoff) {
+  if (!m || nseg < 0 || (nseg > 0 && !eoff)) return GF_ERR_ARG;
+  (void)hipFree(m->seg_eoff);
+  (void)hipFree(m->seg_boff);
+  (void)hipFree(m->sscal);
+  if (m->hsscal) (void)hipHostFree(m->hsscal);
+  m->seg_eoff = nullptr;
+  m->seg_boff = nullptr;
+  m->sscal = m->hsscal = m->hsscal_dev = nullptr;
+  m->nseg = 0;
+  m->h_seg_eoff.clear();
+  m->h_seg_boff.clear();
+  if (nseg == 0) return GF_OK;
+  if (nseg > 1000) return gf_set_err(m->ctx, GF_ERR_ARG, "segments: at most 1000 members");
+  if (eoff[0] != 0 || eoff[nseg] != (int64_t)m->nelems)
+    return gf_set_err(m->ctx, GF_ERR_ARG, "segments: offsets must run from 0 to nelems");
+  for (int k = 0; k < nseg; k++)
+    if (eoff[k + 1] <= eoff[k]) return gf_set_err(m->ctx, GF_ERR_ARG, "segments: empty member");
+  m->h_seg_eoff.assign(eoff, eoff + nseg + 1);
+  // update blocks in proportion to the members' sizes, at least one each
+  const int budget = std::max(nseg, 1024 - nseg);
+  m->h_seg_boff.assign(nseg + 1, 0);
+  for (int k = 0; k < nseg; k++) {
+    const double frac = (double)(eoff[k + 1] - eoff[k]) / (double)m->nelems;
+    m->h_seg_boff[k + 1] = m->h_seg_boff[k] + std::max(1, (int)(frac * budget + 0.5));
+  }
+  // the reduction buffer holds the apply partials (one per logical block),
+  // then the update blocks' pairs, and the set-up pairs plus one pair per
+  // member for the per-member reductions
+  {
+    const int nbU = m->h_seg_boff[nseg];
+    const size_t need = std::max(std::max((size_t)col_blocks(m), m->nelems) + 64 + 2 * (size_t)nbU,
+                                 2 * (size_t)nbU + 64 + 2 * (size_t)nseg) + 64;
+    if (m->red_cap < need) {
+      GF_HIP(m->ctx, hipFree(m->red));
+      m->red = nullptr;
+      GF_HIP(m->ctx, hipMalloc((void **)&m->red, need * sizeof(double)));
+      m->red_cap = need;
+    }
+  }
+  GF_HIP(m->ctx, hipMalloc((void **)&m->seg_eoff, (nseg + 1) * sizeof(int64_t)));
+  GF_HIP(m->ctx, hipMalloc((void **)&m->seg_boff, (nseg + 1) * sizeof(int)));
+  GF_HIP(m->ctx, hipMalloc((void **)&m->sscal, (size_t)nseg * kSegScal * sizeof(double)));
+  GF_HIP(m->ctx, hipHostMalloc((void **)&m->hsscal, (size_t)nseg * kSegScal * sizeof(double),
+                               hipHostMallocDefault));
+  GF_HIP(m->ctx, hipMemcpy(m->seg_eoff, eoff, (nseg + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+  GF_HIP(m->ctx, hipMemcpy(m->seg_boff, m->h_seg_boff.data(), (nseg + 1) * sizeof(int),
+                           hipMemcpyHostToDevice));
+  m->nseg = nseg;
+  // no GGFX group may join two members
+  double bad = 0.0;
+  double *dbad = m->scal + 63;
+  GF_HIP(m->ctx, hipMemcpy(dbad, &bad, sizeof(double), hipMemcpyHostToDevice));
+  if (m->ngroups)
+    hipLaunchKernelGGL(k_groups_in_segments, dim3(nblocks((size_t)m->ngroups)), dim3(kThreads), 0,
+                       m->ctx->stream, m->goff, m->gmem, m->ngroups, m->seg_eoff, nseg, m->nn, dbad);
+  GF_LAUNCH_CHECK(m->ctx);
+  GF_HIP(m->ctx, hipMemcpyAsync(&bad, dbad, sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+  GF_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+  if (bad != 0.0) {
+    gf_mesh_set_segments(m, 0, nullptr);
+    return gf_set_err(m->ctx, GF_ERR_ARG, "segments: a GGFX group joins two members");
+  }
+  return GF_OK;
+}
+
+extern "C" int gf_cg_member_results(gf_mesh *m, int nseg, int *iters, double *rnorm,
+                                    double *rnorm0) {
+  if (!m || nseg != m->nseg || (int)m->seg_iters.size() != nseg) return GF_ERR_ARG;
+  for (int k = 0; k < nseg; k++) {
+    if (iters) iters[k] = m->seg_iters[k];
+    if (rnorm) rnorm[k] = m->seg_rnorm[k];
+    if (rnorm0) rnorm0[k] = m->seg_rnorm0[k];
+  }
+  return GF_OK;
+}
+
 extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b, const double *xb,
                            double *x, gf_cg_result *res) {
   RC(cg_check_args(m, prm, b, x, res));
@@ -1655,6 +2336,7 @@ extern "C" int gf_cg_solve(gf_mesh *m, const gf_cg_params *prm, const double *b,
         (prm->q_field && !wk.qf32))
       return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
   }
+  if (m->nseg > 0) return cg_solve_seg(m, prm, b, xb, x, res, wk);
   return cg_solve_impl(m, prm, b, xb, x, res, wk);
 }
 
@@ -1736,6 +2418,9 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
   memset(&sa, 0, sizeof(sa));
   memset(&ca, 0, sizeof(ca));
   CGApply3Args ca3;
+  ca3.seg_eoff = nullptr;
+  ca3.nseg = 0;
+  ca3.sscal = nullptr;
   memset(&ca3, 0, sizeof(ca3));
   const bool pencil = w == "burgers_stage" || w == "cg_apply" || w == "cg_apply_f32" ||
                       w == "burgers_stage_box";

@@ -110,7 +110,27 @@ struct CGApply3Args {
   size_t nelems;
   gf::BoxTopo topo;  // with tmask: mask formed from the box faces (mask NULL)
   int tmask;
+  // members of a batched mesh (nseg > 0): element offsets and per-member
+  // scalars; partial sums are then indexed by logical block
+  const int64_t *seg_eoff;
+  int nseg;
+  double *sscal;
 };
+
+// Solver scalars per member of a batched mesh: kSegScal doubles each.
+constexpr int kSegScal = 16;
+
+// Member of element e: the s with eoff[s] <= e < eoff[s+1] (binary search
+// over the nseg+1 offsets; e past the end maps to the last member).
+__device__ __forceinline__ int seg_of(const int64_t *__restrict__ eoff, int nseg, size_t e) {
+  int lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if ((size_t)eoff[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 inline int col_blocks(gf_mesh *m) {
   int N = m->Niso, N2 = N * N, epb = N2 >= 64 ? 1 : 64 / N2;

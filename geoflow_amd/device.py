@@ -69,6 +69,23 @@ def _q_args(q):
     return None, float(q)
 
 
+def box_ggfx_device(ctx, N, ne, periodic):
+    """gf_box_ggfx_device: the GGFX CSR maps of a 3-D box built on the
+    device, returned as host int64 arrays (offs, members)."""
+    nea = (ctypes.c_int * 3)(*[int(q) for q in ne])
+    pa = (ctypes.c_int * 3)(*[int(q) for q in periodic])
+    ndof = int(np.prod(ne)) * N ** 3
+    offs = torch.empty(ndof + 1, dtype=torch.int64, device="cuda")
+    nm = ctypes.c_size_t(0)
+    check(lib().gf_box_ggfx_device(ctx.h, int(N), nea, pa, _ptr(offs), None, ctypes.byref(nm)),
+          ctx.h)
+    mem = torch.empty(max(nm.value, 1), dtype=torch.int64, device="cuda")
+    check(lib().gf_box_ggfx_device(ctx.h, int(N), nea, pa, _ptr(offs), _ptr(mem),
+                                   ctypes.byref(nm)), ctx.h)
+    torch.cuda.synchronize()
+    return offs.cpu().numpy(), mem[:nm.value].cpu().numpy()
+
+
 class Mesh:
     """Device-resident mesh: GLL basis, metrics, mask and GGFX groups.
 
@@ -118,12 +135,22 @@ class Mesh:
                 d.xyz[k] = host(coords[k])
         if use_mask and getattr(grid, "mask", None) is not None:
             d.mask = host(grid.mask)
-        offs = np.ascontiguousarray(grid.ggfx_offs, dtype=np.int64)
-        mem = np.ascontiguousarray(grid.ggfx_members, dtype=np.int64)
-        keep += [offs, mem]
-        d.ggfx_offs = offs.ctypes.data_as(_i64p)
-        d.ggfx_members = mem.ctypes.data_as(_i64p)
-        if (structured and gtype == GF_GE_REGULAR and grid.dim == 3
+        devmaps = getattr(grid, "device_maps", False)
+        if devmaps:
+            d.box_device_maps = 1
+            bl = getattr(grid, "box_list", None)
+            if bl is not None:
+                arr = (ctypes.c_int * len(bl))(*[int(v) for v in bl])
+                keep.append(arr)
+                d.box_count = len(bl) // 6
+                d.box_list = arr
+        else:
+            offs = np.ascontiguousarray(grid.ggfx_offs, dtype=np.int64)
+            mem = np.ascontiguousarray(grid.ggfx_members, dtype=np.int64)
+            keep += [offs, mem]
+            d.ggfx_offs = offs.ctypes.data_as(_i64p)
+            d.ggfx_members = mem.ctypes.data_as(_i64p)
+        if ((structured or devmaps) and gtype == GF_GE_REGULAR and grid.dim == 3
                 and getattr(grid, "ne", None) is not None):
             for k in range(3):
                 d.box_ne[k] = grid.ne[k]
@@ -212,6 +239,12 @@ class Mesh:
         self._c(lib().gf_helmholtz_apply_f32(self.h, float(p), qs, _ptr(qf), _ptr(u), _ptr(out)))
         return out
 
+    def dealias(self, v1, v2, pq, out=None):
+        """GGrid::dealias: v1 v2 through the order-pq GLL grid."""
+        out = self.new() if out is None else out
+        self._c(lib().gf_dealias(self.h, int(pq), _ptr(v1), _ptr(v2), _ptr(out)))
+        return out
+
     def dss(self, u, op="sum"):
         ops = {"sum": 0, "smooth": 1, "max": 2, "min": 3}
         self._c(lib().gf_dss(self.h, ops[op], _ptr(u)))
@@ -240,9 +273,11 @@ class Mesh:
                                       int(check_finite)))
         return us
 
-    def burgers_steps(self, us, nu, dt, nsteps, t=0.0, mode=0, order=4):
-        self._c(lib().gf_burgers_steps(self.h, mode, nu, order, len(us), t, dt, nsteps,
-                                       _ptrs(us)))
+    def burgers_steps(self, us, nu, dt, nsteps, t=0.0, mode=0, order=4, precision=0):
+        """nsteps RK steps in place; precision 1 selects the mixed FP32/FP64
+        stepper (pure advection on embedded meshes, gf_burgers_steps_mixed)."""
+        fn = lib().gf_burgers_steps_mixed if precision else lib().gf_burgers_steps
+        self._c(fn(self.h, mode, nu, order, len(us), t, dt, nsteps, _ptrs(us)))
         return us
 
     def burgers_rhs_adjoint(self, us, ws, lams, nu):
@@ -257,13 +292,40 @@ class Mesh:
                                               _ptrs(lams)))
         return lams
 
-    def burgers_adjoint(self, u0, lams, nu, dt, nsteps, order=4, ckpt=16, uT=None):
+    def burgers_adjoint(self, u0, lams, nu, dt, nsteps, order=4, ckpt=16, uT=None,
+                        gio_prefix=None):
         """lams (dJ/du(T)) becomes dJ/du(0) for nsteps steps from u0; uT, when
-        given, receives u(T)."""
-        self._c(lib().gf_burgers_adjoint(self.h, nu, order, len(u0), dt, nsteps, ckpt,
-                                         _ptrs(u0), _ptrs(lams),
-                                         _ptrs(uT) if uT is not None else None))
+        given, receives u(T).  gio_prefix: keep the checkpoints as GIO state
+        files <prefix>.<c>.gio instead of device memory."""
+        if gio_prefix is None:
+            self._c(lib().gf_burgers_adjoint(self.h, nu, order, len(u0), dt, nsteps, ckpt,
+                                             _ptrs(u0), _ptrs(lams),
+                                             _ptrs(uT) if uT is not None else None))
+        else:
+            self._c(lib().gf_burgers_adjoint_gio(self.h, nu, order, len(u0), dt, nsteps, ckpt,
+                                                 _ptrs(u0), _ptrs(lams),
+                                                 _ptrs(uT) if uT is not None else None,
+                                                 str(gio_prefix).encode()))
         return lams
+
+    def set_segments(self, elem_offsets):
+        """gf_mesh_set_segments: the mesh is a union of independent members
+        (element ranges elem_offsets[k] .. elem_offsets[k+1]); CG solves then
+        give every member its own scalars and stopping test."""
+        eo = np.ascontiguousarray(elem_offsets, dtype=np.int64)
+        self._c(lib().gf_mesh_set_segments(self.h, int(eo.size - 1),
+                                           eo.ctypes.data_as(_i64p) if eo.size > 1 else None))
+        self.nseg = int(eo.size - 1)
+
+    def member_results(self):
+        """Per-member (iterations, final norm, initial norm) of the last CG
+        solve on a batched mesh."""
+        n = self.nseg
+        it = np.zeros(n, dtype=np.int32)
+        rn, r0 = np.zeros(n), np.zeros(n)
+        self._c(lib().gf_cg_member_results(self.h, n, it.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                           _hp(rn), _hp(r0)))
+        return it, rn, r0
 
     def time_kernel(self, which, reps=20):
         """Average ms per launch of one kernel, timed with HIP events."""

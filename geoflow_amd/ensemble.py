@@ -12,8 +12,10 @@ stepper; this is the build's addition for the ensemble workload):
     M u*   = M u^n - dt M (u^n . grad) u^n           (GAdvect, explicit)
     (M + nu dt L) u^{n+1} = M u*                     (Helmholtz, Jacobi PCG)
 per velocity component, the solve in FP64 or with FP32 inner iterations and
-FP64 reliable updates.  The CG of a group runs on the block-diagonal system
-of its members with one set of CG scalars.
+FP64 reliable updates.  The CG of a group runs all its members in the same
+kernel launches, each member with its own CG scalars, norm of b and stopping
+test (gf_mesh_set_segments), so every member reaches the relative residual
+on its own system.
 
 The ensemble gradient of a misfit J = 1/2 sum ||u(T) - u_target||_M^2 is
 taken through the explicit RK4 stepper's adjoint (gf_burgers_adjoint) on the
@@ -38,9 +40,11 @@ def draw_members(nmembers=48, seed=12345, counts=(8, 12, 16, 24), orders=(4, 6, 
 
 
 class _UnionGrid:
-    """Host arrays of several periodic boxes of one order, concatenated."""
+    """Host arrays of several periodic boxes of one order, concatenated.
+    With device_maps the boxes' GGFX groups are built on the device at mesh
+    creation (box_list describes the boxes) and no host maps are formed."""
 
-    def __init__(self, grids):
+    def __init__(self, grids, device_maps=False):
         g0 = grids[0]
         self.dim = 3
         self.gtype = g0.gtype
@@ -49,10 +53,19 @@ class _UnionGrid:
         self.basis = g0.basis
         self.nelems = sum(g.nelems for g in grids)
         self.ndof = sum(g.ndof for g in grids)
-        self.xyz = [np.concatenate([g.xyz[d] for g in grids]) for d in range(3)]
+        # coordinates only where the host matching needs them (regular boxes'
+        # operators do not read them)
+        self.xyz = None if device_maps else [np.concatenate([g.xyz[d] for g in grids])
+                                             for d in range(3)]
         self.edx = np.concatenate([g.edx for g in grids])
         self.ejac = np.concatenate([g.ejac for g in grids])
         self.mask = None
+        self.ne = None  # no single box structure: CSR DSS path
+        self.device_maps = bool(device_maps)
+        if self.device_maps:
+            self.box_list = [v for g in grids for v in list(g.ne) + list(g.periodic)]
+            self.ggfx_offs = self.ggfx_members = None
+            return
         offs, mem, base = [np.zeros(1, dtype=np.int64)], [], 0
         for g in grids:
             offs.append(g.ggfx_offs[1:] + (offs[-1][-1] if len(offs) > 1 else 0))
@@ -60,13 +73,14 @@ class _UnionGrid:
             base += g.ndof
         self.ggfx_offs = np.concatenate(offs)
         self.ggfx_members = np.concatenate(mem)
-        self.ne = None  # no single box structure: CSR DSS path
 
 
 class Ensemble:
     """Members grouped by order; one device mesh per group."""
 
-    def __init__(self, ctx, members):
+    def __init__(self, ctx, members, device_maps=False):
+        """device_maps: build the members' GGFX groups on the device (no host
+        coordinate matching; the fast set-up for large ensembles)."""
         from .device import Mesh
         self.ctx = ctx
         self.members = list(members)
@@ -75,9 +89,12 @@ class Ensemble:
             self.groups.setdefault(ms.p, []).append(idx)
         self.mesh, self.slices, self.grids = {}, {}, {}
         for p, idxs in sorted(self.groups.items()):
-            grids = [BoxGrid(3, p, [self.members[i].ne] * 3) for i in idxs]
-            ug = _UnionGrid(grids)
+            grids = [BoxGrid(3, p, [self.members[i].ne] * 3, device_maps=device_maps)
+                     for i in idxs]
+            ug = _UnionGrid(grids, device_maps=device_maps)
             self.mesh[p] = Mesh(ctx, ug, structured=False)
+            # each member its own CG scalars and stopping test
+            self.mesh[p].set_segments(np.cumsum([0] + [g.nelems for g in grids]))
             self.grids[p] = ug
             off = 0
             for i, g in zip(idxs, grids):
@@ -90,17 +107,19 @@ class Ensemble:
 
     def new_state(self, fill):
         """State per group: 3 device fields; fill(i, grid) gives member i's
-        3 host arrays."""
+        3 fields (host arrays or device tensors)."""
         import torch
         st = {}
         for p, idxs in self.groups.items():
-            comps = [np.zeros(self.mesh[p].ndof) for _ in range(3)]
+            comps = [torch.zeros(self.mesh[p].ndof, dtype=torch.float64, device="cuda")
+                     for _ in range(3)]
             for i in idxs:
                 _, off, n, g = self.slices[i]
                 vals = fill(i, g)
                 for c in range(3):
-                    comps[c][off:off + n] = vals[c]
-            st[p] = [torch.from_numpy(a).cuda() for a in comps]
+                    comps[c][off:off + n] = torch.as_tensor(vals[c], dtype=torch.float64,
+                                                            device="cuda")
+            st[p] = comps
         return st
 
     def member(self, st, i):
@@ -116,6 +135,7 @@ class Ensemble:
         solve; the solver's default 0.01 would need five)."""
         import torch
         its = {}
+        self.last_member_iters = {}
         for p, m in self.mesh.items():
             us = st[p]
             new = []
@@ -128,6 +148,7 @@ class Ensemble:
                 if info["status"] != 0:
                     raise RuntimeError("ensemble CG did not converge (order %d, comp %d)" % (p, c))
                 its.setdefault(p, []).append(info["iters"])
+                self.last_member_iters.setdefault(p, []).append(m.member_results()[0])
                 new.append(x)
             for c in range(3):
                 us[c].copy_(new[c])

@@ -261,7 +261,10 @@ class BoxGrid:
     gtype = GF_GE_REGULAR
 
     def __init__(self, dim, p, ne, P0=(0.0, 0.0, 0.0), P1=(1.0, 1.0, 1.0), bdy=None,
-                 tol_fac=0.001):
+                 tol_fac=0.001, device_maps=False):
+        """device_maps: leave the GGFX maps to the device (gf_box_ggfx_device
+        at mesh creation; 3-D boxes periodic in every direction or in none),
+        so no host matching runs; ggfx_offs/ggfx_members are then None."""
         if bdy is None:
             bdy = [GF_BDY_PERIODIC] * (2 * dim)
         self.dim = dim
@@ -277,6 +280,13 @@ class BoxGrid:
         self.N = [q + 1 for q in self.p]
         self.basis = [gll_basis(q) for q in self.p]
         self.tol = tol_fac * self.minnodedist
-        # GGFX::init uses 3 coordinates; missing ones are zero (geoflow_cdg.cpp:745-753)
-        self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyzp, self.tol,
-                                                       max_dup=2 ** dim)
+        self.device_maps = bool(device_maps)
+        if self.device_maps:
+            if dim != 3 or len(set(self.p)) != 1 or sum(self.periodic) not in (0, 3):
+                raise ValueError("device GGFX maps: 3-D isotropic boxes, periodic in every "
+                                 "direction or in none")
+            self.ggfx_offs = self.ggfx_members = None
+        else:
+            # GGFX::init uses 3 coordinates; missing ones are zero (geoflow_cdg.cpp:745-753)
+            self.ggfx_offs, self.ggfx_members = ggfx_build(self.xyzp, self.tol,
+                                                           max_dup=2 ** dim)

@@ -54,10 +54,45 @@ def fourier_field_3d(x, y, z, seed=12345, kmin=1, kmax=6, ncomp=3, L=1.0, xp=np)
     return comps
 
 
+def box_first_member(grid):
+    """Smallest member of every node's GGFX group on a 3-D box whose maps
+    are left to the device (BoxGrid(..., device_maps=True)): in each
+    direction where the node has a partner (index 0 or N-1 on a shared or
+    periodic face) the group offers two copies, and the global index
+    e*N^3 + local is smallest for the copy with the smaller element
+    coordinate (the element index dominates), or the smaller node index
+    when the partner is in the same element (one periodic element)."""
+    N = grid.N[0]
+    ne = grid.ne
+    NN = N ** 3
+    q = np.arange(grid.ndof, dtype=np.int64)
+    e, r = q // NN, q % NN
+    node = [r % N, (r // N) % N, r // (N * N)]
+    ec = [e % ne[0], (e // ne[0]) % ne[1], e // (ne[0] * ne[1])]
+    best_c, best_n = [], []
+    for d in range(3):
+        c, n = ec[d].copy(), node[d].copy()
+        per = bool(grid.periodic[d])
+        lo = (node[d] == 0) & ((ec[d] > 0) | per)
+        hi = (node[d] == N - 1) & ((ec[d] < ne[d] - 1) | per)
+        pc = np.where(lo, np.where(ec[d] > 0, ec[d] - 1, ne[d] - 1),
+                      np.where(hi, np.where(ec[d] < ne[d] - 1, ec[d] + 1, 0), ec[d]))
+        pn = np.where(lo, N - 1, np.where(hi, 0, node[d]))
+        take = (pc < c) | ((pc == c) & (pn < n))
+        best_c.append(np.where(take, pc, c))
+        best_n.append(np.where(take, pn, n))
+    eb = best_c[0] + ne[0] * (best_c[1] + ne[1] * best_c[2])
+    return eb * NN + best_n[0] + N * (best_n[1] + N * best_n[2])
+
+
 def unique_normal(grid, seed=67890):
     """N(0,1) per unique node, scattered to all duplicates through the GGFX
     groups (each group takes the draw of its smallest member)."""
     rng = np.random.default_rng(seed)
+    if getattr(grid, "ggfx_offs", None) is None and getattr(grid, "device_maps", False):
+        first = box_first_member(grid)
+        draw = rng.standard_normal(grid.ndof)
+        return draw[first]
     offs, mem = grid.ggfx_offs, grid.ggfx_members
     n = offs.size - 1
     draw = rng.standard_normal(n)

@@ -180,6 +180,17 @@ int gf_terrain_box_grid3d(int p, const int *ne, const double *P0, const double *
 int gf_ggfx_build(size_t ndof, int ncoord, const double *const *xyz, double tol,
                   size_t max_dup, int64_t *offs, int64_t *members, size_t *nmembers);
 
+/* GGFX::init local matching (src/cdg/comm/ggfx.hpp:167-342) for a 3-D box of
+ * regular elements (elements x-fastest, N nodes per direction, ne[3]
+ * elements, periodic[3] flags) on the device: the same CSR as gf_ggfx_build
+ * on the box's coordinates, formed from the element lattice (no coordinate
+ * matching).  offs (device, ndof+1 int64) and members (device int64, or NULL
+ * to obtain *nmembers).  Every direction periodic or none: a box with both
+ * kinds returns GF_ERR_UNSUPPORTED (periodize joins its groups differently).
+ * Synchronous. */
+int gf_box_ggfx_device(gf_ctx *ctx, int N, const int *ne, const int *periodic, int64_t *offs,
+                       int64_t *members, size_t *nmembers);
+
 /* ---- mesh (device-resident operator data) ------------------------------ */
 typedef struct {
   int dim;               /* 2 or 3 (GDIM is a runtime value here)                  */
@@ -208,6 +219,19 @@ typedef struct {
    * assemble by gather instead of a separate DSS pass.  Zero = unstructured. */
   int box_ne[3];
   int box_periodic[3];
+  /* Non-zero: build the GGFX groups on the device from the box structure
+   * above (3-D GE_REGULAR isotropic box, every direction periodic or none)
+   * instead of reading ggfx_offs/ggfx_members, which may then be NULL.  The
+   * groups are those gf_ggfx_build forms from the box's coordinates
+   * (gf_box_ggfx_device). */
+  int box_device_maps;
+  /* With box_device_maps: box_count >= 1 boxes in sequence (a batched mesh of
+   * members, elements of box b following those of box b-1), box_list[6b ..
+   * 6b+5] = ne0, ne1, ne2 and the three periodic flags of box b; the
+   * groups never join two boxes.  box_ne/box_periodic are then unused and
+   * the mesh has no single box structure. */
+  int box_count;
+  const int *box_list;
 } gf_mesh_desc;
 
 int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *desc, gf_mesh **out);
@@ -293,6 +317,18 @@ int gf_integrate(gf_mesh *mesh, const double *u, double *result);
  * streams, two host threads) race; calls ordered on one stream do not. */
 int gf_integrate_noalloc(gf_mesh *mesh, const double *u, double *result);
 
+/* GGrid::dealias (src/cdg/grid/ggrid.ipp:2351-2503): the quadratic
+ * dealiasing of the product v1 v2 -- both interpolated to the order-pq GLL
+ * grid of each element, multiplied there with the order-pq weights and
+ * projected back (prod = Jac imass IQP^T W_q (IQP v1 . IQP v2)); used by
+ * GDiv (src/cdg/sem/gdiv.ipp:91-128).  Regular meshes of constant p,
+ * p <= pq <= 31; 2-D and 3-D.  The 3-D non-square tensor products are
+ * those GMTK::D3_X_D2_X_D1 intends (its plane loop uses the output
+ * plane stride, gmtk.ipp:1967-1968, so for pq > p the reference reads past
+ * its first pass's result).  The first call for an order pq builds its
+ * interpolation matrix on the device.  Asynchronous. */
+int gf_dealias(gf_mesh *mesh, int pq, const double *v1, const double *v2, double *prod);
+
 /* ---- Helmholtz ------------------------------------------------------------
  * GHelmholtz::opVec_prod -> reg_prod / embed_prod
  * (src/cdg/sem/ghelmholtz.ipp:76-378): uo = p L u + q M u, unassembled.
@@ -335,6 +371,16 @@ int gf_burgers_step(gf_mesh *mesh, int mode, double nu, int order, int nstate, d
 /* nsteps consecutive steps (one host call; identical arithmetic). */
 int gf_burgers_steps(gf_mesh *mesh, int mode, double nu, int order, int nstate, double t,
                      double dt, int nsteps, double *const *u);
+/* Mixed-precision form of gf_burgers_steps for pure advection (mode 1,
+ * nstate 4: u[0] advected by the velocity u[1..3]) on GE_2DEMBEDDED meshes:
+ * the advected field, its RK stage fields, their DSS and the tensor-product
+ * contractions in FP32; the metric terms (dXi/dX, G, mass, inverse mass and
+ * the velocity-metric products) in FP64.  u[0] is FP64 at the boundary,
+ * converted to FP32 at the start of the call and back at its end.  No
+ * reference counterpart (GeoFLOW is FP64 throughout).  Other modes and
+ * meshes return GF_ERR_UNSUPPORTED. */
+int gf_burgers_steps_mixed(gf_mesh *mesh, int mode, double nu, int order, int nstate, double t,
+                           double dt, int nsteps, double *const *u);
 
 /* ---- adjoint (no reference counterpart: hand-written adjoint of the two
  * functions above, for gradients of final-time functionals) ---------------
@@ -357,6 +403,17 @@ int gf_burgers_step_adjoint(gf_mesh *mesh, double nu, int order, int nstate, dou
 int gf_burgers_adjoint(gf_mesh *mesh, double nu, int order, int nstate, double dt, int nsteps,
                        int ckpt, const double *const *u0, double *const *lam,
                        double *const *uT);
+/* gf_burgers_adjoint with the checkpoints persisted as GIO state files (the
+ * GIO::write_state_impl layout of gf_gio_write, src/cdg/io/gio.ipp:588-690)
+ * instead of device memory: checkpoint c, the state at step c*ckpt, is the
+ * file "<prefix>.<c>.gio" (cycle = step, time = step*dt, one field per state
+ * component); the reverse sweep reads them back.  Device memory holds one
+ * segment ((ckpt + 1) nstate fields).  Results equal gf_burgers_adjoint's
+ * bit for bit.  Synchronous.  No reference counterpart (the reference has
+ * no adjoint); the files are GeoFLOW state files its tools can read. */
+int gf_burgers_adjoint_gio(gf_mesh *mesh, double nu, int order, int nstate, double dt,
+                           int nsteps, int ckpt, const double *const *u0, double *const *lam,
+                           double *const *uT, const char *prefix);
 
 /* ---- conjugate gradient -----------------------------------------------------
  * GCG::solve_impl (src/cdg/solvers/gcg.ipp:239-345, 361-382) applied to the
@@ -390,6 +447,21 @@ typedef struct {
  * precision keeps FP32 copies of the node metrics in the mesh workspace). */
 int gf_cg_solve(gf_mesh *mesh, const gf_cg_params *prm, const double *b, const double *xb,
                 double *x, gf_cg_result *res);
+
+/* Batched meshes (no reference counterpart): declare the mesh a union of
+ * nseg independent members, member k being elements eoff[k] .. eoff[k+1]
+ * (host int64, eoff[0] = 0, eoff[nseg] = nelems); no GGFX group may join two
+ * members (checked).  gf_cg_solve then runs the GCG recurrence (or the
+ * mixed-precision solver) with every member's own scalars, norm of b and
+ * stopping test, all members in the same kernel launches; the result's
+ * iters/rnorm/rnorm0 are the maxima over members, status GF_ERR_NOCONVERGE
+ * if any member did not converge.  Supported on 3-D regular meshes with the
+ * standard D and CSR assembly (the ensembles' concatenated meshes).  nseg 0
+ * makes the mesh one system again. */
+int gf_mesh_set_segments(gf_mesh *mesh, int nseg, const int64_t *eoff);
+/* Per-member iterations and final/initial residual norms of the last
+ * gf_cg_solve on a batched mesh (arrays of nseg, any may be NULL). */
+int gf_cg_member_results(gf_mesh *mesh, int nseg, int *iters, double *rnorm, double *rnorm0);
 
 /* Assembled Jacobi diagonal inverse of (p L + q M), q as in gf_helmholtz_apply;
  * on embedded and deformed meshes the diagonal includes the off-diagonal

@@ -127,7 +127,7 @@ class BoxGrid {
           std::array<double, 3> P0 = {0, 0, 0}, std::array<double, 3> P1 = {1, 1, 1},
           std::array<int, 6> bdy = {GF_BDY_PERIODIC, GF_BDY_PERIODIC, GF_BDY_PERIODIC,
                                     GF_BDY_PERIODIC, GF_BDY_PERIODIC, GF_BDY_PERIODIC},
-          double tol_fac = 0.001)
+          double tol_fac = 0.001, bool device_maps = false)
       : ctx_(ctx), dim_(dim) {
     if (dim != 2 && dim != 3) throw Error(GF_ERR_ARG, "BoxGrid: dim must be 2 or 3");
     size_t nn = 1, nel = 1;
@@ -153,15 +153,19 @@ class BoxGrid {
                       mask_.data(), edx_.data(), ejac_.data(), &minnodedist_),
           nullptr, "gf_box_grid");
     const double *cx[3] = {xpp[0], xpp[1], xpp[2]};
-    offs_.resize(ndof_ + 1);
-    size_t nm = 0;
-    const double tol = tol_fac * minnodedist_;
-    check(gf_ggfx_build(ndof_, dim, cx, tol, (size_t)1 << dim, offs_.data(), nullptr, &nm),
-          nullptr, "gf_ggfx_build");
-    members_.resize(nm);
-    check(gf_ggfx_build(ndof_, dim, cx, tol, (size_t)1 << dim, offs_.data(), members_.data(),
-                        &nm),
-          nullptr, "gf_ggfx_build");
+    // device_maps: the GGFX groups are formed on the device from the box
+    // lattice at mesh creation (3-D, every direction periodic or none)
+    if (!device_maps) {
+      offs_.resize(ndof_ + 1);
+      size_t nm = 0;
+      const double tol = tol_fac * minnodedist_;
+      check(gf_ggfx_build(ndof_, dim, cx, tol, (size_t)1 << dim, offs_.data(), nullptr, &nm),
+            nullptr, "gf_ggfx_build");
+      members_.resize(nm);
+      check(gf_ggfx_build(ndof_, dim, cx, tol, (size_t)1 << dim, offs_.data(), members_.data(),
+                          &nm),
+            nullptr, "gf_ggfx_build");
+    }
     for (int d = 0; d < dim; d++) {
       std::vector<double> x(p[d] + 1), w(p[d] + 1), D((p[d] + 1) * (p[d] + 1));
       check(gf_gll_basis(p[d], x.data(), w.data(), D.data()), nullptr, "gf_gll_basis");
@@ -181,8 +185,9 @@ class BoxGrid {
     md.elem_dxidx = edx_.data();
     md.elem_jac = ejac_.data();
     md.mask = mask_.data();
-    md.ggfx_offs = offs_.data();
-    md.ggfx_members = members_.data();
+    md.ggfx_offs = device_maps ? nullptr : offs_.data();
+    md.ggfx_members = device_maps ? nullptr : members_.data();
+    md.box_device_maps = device_maps ? 1 : 0;
     if (dim == 3)
       for (int d = 0; d < 3; d++) {
         md.box_ne[d] = ne[d];
@@ -205,6 +210,29 @@ class BoxGrid {
   const std::vector<double> &xyz(int d) const { return xyz_[d]; }
   const std::vector<double> &mask() const { return mask_; }
   Field field() { return Field(ctx_, ndof_); }
+
+  // GGrid::deriv (src/cdg/grid/ggrid.ipp:1163-1191): du = d u / dx_idir
+  // (idir 1..dim).
+  void deriv(const Field &u, int idir, State & /*utmp*/, Field &du) {
+    check(gf_deriv(mesh_, idir, u.data(), du.data()), ctx_.handle(), "gf_deriv");
+  }
+  // GGrid::compute_grefderiv (src/cdg/grid/ggrid.ipp:1954-2265): reference-
+  // space derivative along idir, D^T when dotrans.
+  void grefderiv(const Field &u, int idir, bool dotrans, Field &du) {
+    check(gf_grefderiv(mesh_, idir, dotrans ? 1 : 0, u.data(), du.data()), ctx_.handle(),
+          "gf_grefderiv");
+  }
+  // GGrid::dealias (src/cdg/grid/ggrid.ipp:2424-2503) through the order-pq
+  // GLL grid (the reference's qdealias_order).
+  void dealias(const Field &v1, const Field &v2, Field &prod, int pq) {
+    check(gf_dealias(mesh_, pq, v1.data(), v2.data(), prod.data()), ctx_.handle(), "gf_dealias");
+  }
+  // GGrid::integrate (src/cdg/grid/ggrid.ipp:1077-1148).
+  double integrate(const Field &u, State & /*utmp*/) {
+    double r = 0.0;
+    check(gf_integrate(mesh_, u.data(), &r), ctx_.handle(), "gf_integrate");
+    return r;
+  }
 
  private:
   Context &ctx_;

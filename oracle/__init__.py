@@ -8,8 +8,8 @@ It wraps ``liboracle.so`` (built from ``gf_oracle.cpp`` by ``oracle/Makefile``),
 a C++ restatement of the reference algorithms in the reference's operation
 order; see ``gf_oracle.h`` for the per-function citations.
 
-Parity status of each oracle function (what pins it) is listed in
-``oracle/PINNING.md``.
+What pins each oracle function (closed forms, the reference's own test
+criteria, brute force) is checked in ``tests/test_oracle_pins.py``.
 """
 import ctypes
 import os
@@ -89,6 +89,7 @@ def lib():
         L.or_integrate.argtypes = [ctypes.c_void_p, _dp]
         L.or_mass_apply.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.or_imass_apply.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.or_dealias.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
         L.or_helmholtz_apply.argtypes = [ctypes.c_void_p, _dp, ctypes.c_size_t, _dp, _dp, _dp]
         L.or_dss.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.or_imult.argtypes = [ctypes.c_void_p, _dp]
@@ -292,6 +293,13 @@ class Grid:
     def imass_apply(self, u):
         o = np.zeros(self.ndof)
         lib().or_imass_apply(self.h, _p(u), _p(o))
+        return o
+
+    def dealias(self, v1, v2, pq):
+        """GGrid::dealias: v1 v2 through the order-pq GLL grid."""
+        o = np.zeros(self.ndof)
+        lib().or_dealias(self.h, int(pq), _p(np.ascontiguousarray(v1)),
+                         _p(np.ascontiguousarray(v2)), _p(o))
         return o
 
     def helmholtz(self, u, p=1.0, q=None):

@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 #ifdef _OPENMP
+#include <functional>
 #include <omp.h>
 #endif
 
@@ -180,6 +181,8 @@ void dmxm(double *C, const double *A, size_t nai, const double *B, size_t nbi, s
 }
 
 int g_threads = 1;
+#define OR_PFOR _Pragma("omp parallel for num_threads(g_threads) schedule(static)")
+#define OR_PFOR_EL OR_PFOR
 
 }  // namespace
 
@@ -286,6 +289,11 @@ int or_gll_basis(int p, double *xi, double *w, double *D) {
 }
 
 void or_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
+// Element-local and pointwise passes run on g_threads OpenMP threads over
+// contiguous index ranges (the reference's element decomposition over ranks
+// splits the same work); every output entry is computed by the same
+// operations as with one thread, so results do not depend on g_threads.
 
 void or_dmxm(double *C, const double *A, size_t nai, size_t naj, const double *B, size_t nbi,
              size_t nbj) {
@@ -533,6 +541,7 @@ void or_grefdiv(OrGrid *g, const double *const *u, int dotrans, double *divu) {
   for (int k = 0; k < g->dim; k++) {
     if (!u[k]) continue;
     or_grefderiv(g, u[k], k + 1, dotrans, et.data());
+    OR_PFOR
     for (size_t j = 0; j < n; j++) divu[j] += et[j];
   }
 }
@@ -543,16 +552,20 @@ void or_deriv(OrGrid *g, const double *u, int idir, double *du) {
   if (g->gtype == 0) {
     or_grefderiv(g, u, idir, 0, du);
     const std::vector<double> &f = g->dxidx[idir - 1][0];
+    OR_PFOR
     for (size_t j = 0; j < n; j++) du[j] *= f[j];
   } else {
     std::vector<double> tmp(n);
     or_grefderiv(g, u, 1, 0, du);
     const std::vector<double> &f0 = g->dxidx[0][idir - 1];
+    OR_PFOR
     for (size_t j = 0; j < n; j++) du[j] *= f0[j];
     for (int jd = 1; jd < g->dim; jd++) {
       or_grefderiv(g, u, jd + 1, 0, tmp.data());
       const std::vector<double> &f = g->dxidx[jd][idir - 1];
+      OR_PFOR
       for (size_t j = 0; j < n; j++) tmp[j] *= f[j];
+      OR_PFOR
       for (size_t j = 0; j < n; j++) du[j] += tmp[j];
     }
   }
@@ -588,10 +601,12 @@ double or_integrate(OrGrid *g, const double *u) {
 
 void or_mass_apply(OrGrid *g, const double *in, double *out) {
   // GMass::opVec_prod: mass_.pointProd(input, output) (gmass.ipp:225-232)
+  OR_PFOR
   for (size_t j = 0; j < g->ndof; j++) out[j] = g->mass[j] * in[j];
 }
 
 void or_imass_apply(OrGrid *g, const double *in, double *out) {
+  OR_PFOR
   for (size_t j = 0; j < g->ndof; j++) out[j] = g->imass[j] * in[j];
 }
 
@@ -603,6 +618,7 @@ void or_imass_apply(OrGrid *g, const double *in, double *out) {
 void or_dss(OrGrid *g, int op, double *u) {
   size_t n = g->ndof;
   std::vector<double> out(n);
+  OR_PFOR
   for (size_t i = 0; i < n; i++) {
     const std::vector<size_t> &grp = g->buf[i];
     double r;
@@ -639,12 +655,16 @@ void or_helmholtz_apply(OrGrid *g, const double *p, size_t np, const double *q, 
     for (int k = 0; k < dim; k++) or_grefderiv(g, u, k + 1, 0, gdu[k].data());
     for (int k = 0; k < dim; k++) {
       const std::vector<double> &dx = g->dxidx[k][0];
+      OR_PFOR
       for (size_t j = 0; j < n; j++) {
         double G = dx[j] * dx[j];  // reg_init (ghelmholtz.ipp:571-575)
         gdu[k][j] *= G;
       }
-      if (pfull)
+      if (pfull) {
+        OR_PFOR
         for (size_t j = 0; j < n; j++) gdu[k][j] *= p[j];
+      }
+      OR_PFOR
       for (size_t j = 0; j < n; j++) gdu[k][j] = g->mass[j] * gdu[k][j];
     }
   } else {
@@ -657,18 +677,24 @@ void or_helmholtz_apply(OrGrid *g, const double *p, size_t np, const double *q, 
       std::fill(gdu[jj].begin(), gdu[jj].end(), 0.0);
       for (int ii = 0; ii < dim; ii++) {
         or_embed_G(g, ii, jj, G);
+        OR_PFOR
         for (size_t j = 0; j < n; j++) gdu[jj][j] += du[ii][j] * G[j];
       }
     }
     if (pfull)
-      for (int k = 0; k < dim; k++)
+      for (int k = 0; k < dim; k++) {
+        OR_PFOR
         for (size_t j = 0; j < n; j++) gdu[k][j] *= p[j];
+      }
   }
   const double *up[3] = {gdu[0].data(), gdu[1].data(), dim == 3 ? gdu[2].data() : nullptr};
   or_grefdiv(g, up, 1, uo);
-  if (p && np < n && p[0] != 1.0)
+  if (p && np < n && p[0] != 1.0) {
+    OR_PFOR
     for (size_t j = 0; j < n; j++) uo[j] *= p[0];
+  }
   if (q) {
+    OR_PFOR
     for (size_t j = 0; j < n; j++) {
       double t = g->mass[j] * u[j];
       t *= q[j];
@@ -759,12 +785,106 @@ void or_helmholtz_diag_inv(OrGrid *g, const double *p, size_t np, const double *
   for (size_t j = 0; j < n; j++) dinv[j] = 1.0 / d[j];
 }
 
+// GLLBasis::evalBasis (3), the matrix form (gllbasis.ipp:1341-1377):
+// phi_j(eta) without the ttiny shifts of the vector form.
+double evalBasisM(const Basis &b, int j, double eta) {
+  LD tetiny = 100.0L * std::numeric_limits<double>::epsilon();
+  int Np = b.p;
+  LD fact = -1.0 / (Np * (Np + 1.0)), xi = static_cast<LD>(eta);
+  double fRet = 1.0;
+  if (xi < -1.0L || xi > 1.0L) fRet = 0.0;
+  else if (std::fabs(xi - b.xl[j]) > tetiny) {
+    LD ppn_j = b.Pnl[j], ppn_xi, pder_xi, pm1, pdm1, pm2, pdm2;
+    jacobi(Np, 0.0L, 0.0L, ppn_xi, pder_xi, pm1, pdm1, pm2, pdm2, xi);
+    LD gfact = (1.0L - xi * xi) / (xi - b.xl[j]);
+    fRet = static_cast<double>(fact * gfact * pder_xi / ppn_j);
+  }
+  return fRet;
+}
+
+// GGrid::dealias (ggrid.ipp:2424-2503) with init_qdealias (2351-2414), for
+// regular isotropic grids: v1 and v2 interpolated to the order-pq GLL grid of
+// each element by IQP(a, i) = phi_i(xq_a) applied direction by direction
+// (x, then y, then z, each a dmxm-order contraction), multiplied pointwise
+// and by the q tensor weights, projected back with IQP^T in the same
+// direction order, and multiplied by iWp = Jac * imass.  The contractions
+// follow GMTK::D2_X_D1 / D3_X_D2_X_D1 (gmtk.ipp:1570-1600, 1930-1992) as those
+// routines intend for non-square factors: in 3-D the middle (y) pass runs
+// over every z plane of its input with that input's plane stride.
+void or_dealias(OrGrid *g, int pq, const double *v1, const double *v2, double *prod) {
+  const int dim = g->dim, N = g->N[0], Q = pq + 1;
+  Basis bq;
+  gll(pq, bq);
+  std::vector<double> I((size_t)Q * N);  // I[a + Q*i] = phi_i(xq_a)
+  for (int a = 0; a < Q; a++)
+    for (int i = 0; i < N; i++) I[a + (size_t)Q * i] = evalBasisM(g->basis[0], i, bq.xi[a]);
+  const int NQd = dim == 3 ? Q * Q * Q : Q * Q, NNd = dim == 3 ? N * N * N : N * N;
+  // contraction along one direction: in is (n0 x n1 x n2) x-fastest, the
+  // direction d has length lin in and lout out, op(o, k) the matrix entry
+  auto pass = [](const double *in, double *out, int n0, int n1, int n2, int d, int lout,
+                 const std::function<double(int, int)> &op) {
+    const int lin = d == 0 ? n0 : (d == 1 ? n1 : n2);
+    const int m0 = d == 0 ? lout : n0, m1 = d == 1 ? lout : n1, m2 = d == 2 ? lout : n2;
+    for (int c = 0; c < m2; c++)
+      for (int b = 0; b < m1; b++)
+        for (int a = 0; a < m0; a++) {
+          double acc = 0.0;
+          for (int k = 0; k < lin; k++) {
+            const int ia = d == 0 ? k : a, ib = d == 1 ? k : b, ic = d == 2 ? k : c;
+            acc += op(d == 0 ? a : (d == 1 ? b : c), k) * in[ia + n0 * (ib + n1 * ic)];
+          }
+          out[a + m0 * (b + m1 * c)] = acc;
+        }
+  };
+  auto fwd = [&](int o, int k) { return I[o + (size_t)Q * k]; };  // IQP(o, k)
+  auto bwd = [&](int o, int k) { return I[k + (size_t)Q * o]; };  // IQP^T(o, k)
+  const int z = dim == 3 ? 1 : 0;
+  OR_PFOR_EL
+  for (long e = 0; e < (long)g->nelems; e++) {
+    std::vector<double> a1((size_t)NQd), a2((size_t)NQd), b1((size_t)NQd), b2((size_t)NQd);
+    for (int f = 0; f < 2; f++) {
+      const double *v = (f == 0 ? v1 : v2) + (size_t)e * NNd;
+      std::vector<double> &dst = f == 0 ? b1 : b2;
+      pass(v, a1.data(), N, N, z ? N : 1, 0, Q, fwd);
+      if (z) {
+        pass(a1.data(), a2.data(), Q, N, N, 1, Q, fwd);
+        pass(a2.data(), dst.data(), Q, Q, N, 2, Q, fwd);
+      } else {
+        pass(a1.data(), dst.data(), Q, N, 1, 1, Q, fwd);
+      }
+    }
+    // pointwise product times the q tensor weights ((Wx Wy) Wz)
+    for (int c = 0; c < (z ? Q : 1); c++)
+      for (int b = 0; b < Q; b++)
+        for (int a = 0; a < Q; a++) {
+          const size_t n = a + (size_t)Q * (b + (size_t)Q * c);
+          double w = bq.w[a] * bq.w[b];
+          if (z) w = w * bq.w[c];
+          b1[n] = b1[n] * b2[n];
+          b1[n] = b1[n] * w;
+        }
+    double *out = prod + (size_t)e * NNd;
+    pass(b1.data(), a1.data(), Q, Q, z ? Q : 1, 0, N, bwd);
+    if (z) {
+      pass(a1.data(), a2.data(), N, Q, Q, 1, N, bwd);
+      pass(a2.data(), out, N, N, Q, 2, N, bwd);
+    } else {
+      pass(a1.data(), out, N, Q, 1, 1, N, bwd);
+    }
+    for (int q = 0; q < NNd; q++) {
+      const size_t gi = (size_t)e * NNd + q;
+      out[q] *= g->jac[gi] * g->imass[gi];
+    }
+  }
+}
+
 // GAdvect::apply (gadvect.ipp:57-99) for ivec = -2 (no Neumann hack).
 void or_advect(OrGrid *g, const double *p, int nu, const double *const *u, double *po) {
   size_t n = g->ndof;
   std::vector<double> t0(n);
   if (u[0]) {
     or_deriv(g, p, 1, po);
+    OR_PFOR
     for (size_t j = 0; j < n; j++) po[j] *= u[0][j];
   } else {
     std::fill(po, po + n, 0.0);
@@ -772,9 +892,12 @@ void or_advect(OrGrid *g, const double *p, int nu, const double *const *u, doubl
   for (int jv = 1; jv < nu; jv++) {
     if (!u[jv]) continue;
     or_deriv(g, p, jv + 1, t0.data());
+    OR_PFOR
     for (size_t j = 0; j < n; j++) t0[j] *= u[jv][j];
+    OR_PFOR
     for (size_t j = 0; j < n; j++) po[j] += t0[j];
   }
+  OR_PFOR
   for (size_t j = 0; j < n; j++) po[j] *= g->mass[j];
 }
 
@@ -787,6 +910,7 @@ void or_burgers_rhs(OrGrid *g, int mode, double nu, int nstate, const double *co
   if (mode == 2) {
     for (int k = 0; k < nstate; k++) {
       or_helmholtz_apply(g, &pnu, 1, nullptr, u[k], urhs.data());
+      OR_PFOR
       for (size_t j = 0; j < n; j++) urhs[j] *= -1.0;
       or_imass_apply(g, urhs.data(), dudt[k]);
     }
@@ -795,6 +919,7 @@ void or_burgers_rhs(OrGrid *g, int mode, double nu, int nstate, const double *co
   if (mode == 1) {
     or_advect(g, u[0], nstate - 1, u + 1, dudt[0]);
     or_helmholtz_apply(g, &pnu, 1, nullptr, u[0], urhs.data());
+    OR_PFOR
     for (size_t j = 0; j < n; j++) urhs[j] = -1.0 * urhs[j] + -1.0 * dudt[0][j];
     or_imass_apply(g, urhs.data(), dudt[0]);
     return;
@@ -802,6 +927,7 @@ void or_burgers_rhs(OrGrid *g, int mode, double nu, int nstate, const double *co
   for (int k = 0; k < g->dim; k++) {
     or_advect(g, u[k], nstate, u, dudt[k]);
     or_helmholtz_apply(g, &pnu, 1, nullptr, u[k], urhs.data());
+    OR_PFOR
     for (size_t j = 0; j < n; j++) urhs[j] = -1.0 * urhs[j] + -1.0 * dudt[k][j];
     or_imass_apply(g, urhs.data(), dudt[k]);
   }
@@ -910,8 +1036,10 @@ void or_burgers_step(OrGrid *g, int mode, double nu, int order, int nstate, doub
       std::fill(isum.begin(), isum.end(), 0.0);
       for (int j = 0; j < m; j++) {
         double b = beta[(size_t)m * order + j] * h;
+        OR_PFOR
         for (size_t q = 0; q < n; q++) isum[q] = 1.0 * isum[q] + b * K[j][k][q];
       }
+      OR_PFOR
       for (size_t q = 0; q < n; q++) u[k][q] = uin[k][q] + isum[q];
     }
     {
@@ -922,7 +1050,9 @@ void or_burgers_step(OrGrid *g, int mode, double nu, int order, int nstate, doub
     for (int k = 0; k < nev; k++) {
       or_dss(g, 1, K[m][k].data());
       double cm = c[m] * h;
+      OR_PFOR
       for (size_t q = 0; q < n; q++) isum[q] = K[m][k][q] * cm;
+      OR_PFOR
       for (size_t q = 0; q < n; q++) uout[k][q] += isum[q];
     }
     {
@@ -935,8 +1065,10 @@ void or_burgers_step(OrGrid *g, int mode, double nu, int order, int nstate, doub
     std::fill(isum.begin(), isum.end(), 0.0);
     for (int j = 0; j < nstage - 1; j++) {
       double b = beta[(size_t)(nstage - 1) * order + j] * h;
+      OR_PFOR
       for (size_t q = 0; q < n; q++) isum[q] = 1.0 * isum[q] + b * K[j][k][q];
     }
+    OR_PFOR
     for (size_t q = 0; q < n; q++) u[k][q] = uin[k][q] + isum[q];
     or_dss(g, 1, u[k].data());
   }
@@ -948,7 +1080,9 @@ void or_burgers_step(OrGrid *g, int mode, double nu, int order, int nstate, doub
   for (int k = 0; k < nev; k++) or_dss(g, 1, K[0][k].data());
   for (int k = 0; k < nev; k++) {
     double cm = c[nstage - 1] * h;
+    OR_PFOR
     for (size_t q = 0; q < n; q++) isum[q] = K[0][k][q] * cm;
+    OR_PFOR
     for (size_t q = 0; q < n; q++) uout[k][q] += isum[q];
     or_dss(g, 1, uout[k]);
   }

@@ -99,6 +99,9 @@ void or_deriv(OrGrid *g, const double *u, int idir, double *du);
 double or_integrate(OrGrid *g, const double *u);
 void or_mass_apply(OrGrid *g, const double *in, double *out);
 void or_imass_apply(OrGrid *g, const double *in, double *out);
+/* GGrid::dealias (src/cdg/grid/ggrid.ipp:2351-2503): quadratic dealiasing of
+ * the product v1 v2 through the order-pq GLL grid (regular isotropic grids). */
+void or_dealias(OrGrid *g, int pq, const double *v1, const double *v2, double *prod);
 /* GHelmholtz::opVec_prod (src/cdg/sem/ghelmholtz.ipp:76-378):
  * p: Laplacian factor, np == 1 (scalar) or np == ndof (field);
  * q: mass factor field (length ndof) or NULL (pure Laplacian). */

@@ -1,5 +1,6 @@
 // Exercises the C++ adapter (include/geoflow_hip.hpp) the way reference code
-// drives GHelmholtz, GMass, GGFX, GCG and the Burgers RK stepper, and writes
+// drives GHelmholtz, GMass, GGFX, GCG, the Burgers RK stepper and GGrid's
+// deriv, dealias and integrate, and writes
 // the device results as raw little-endian doubles into the directory given as
 // argv[1].  tests/test_cpp_adapter.py runs it on a GPU and compares the files
 // with the CPU oracle.
@@ -94,6 +95,26 @@ int main(int argc, char **argv) {
       std::snprintf(name, sizeof name, "burgers_u%d.bin", c);
       save(name, us[c].download());
     }
+
+    // A periodic box whose GGFX groups are built on the device: GGrid::deriv,
+    // GGrid::dealias (order 8 through p = 5) and GGrid::integrate.
+    BoxGrid gd(ctx, 3, {5, 5, 5}, {2, 3, 2}, {0, 0, 0}, {1, 1, 1},
+               {GF_BDY_PERIODIC, GF_BDY_PERIODIC, GF_BDY_PERIODIC, GF_BDY_PERIODIC,
+                GF_BDY_PERIODIC, GF_BDY_PERIODIC},
+               0.001, true);
+    const size_t nd = gd.ndof();
+    std::vector<double> hv(nd), hw(nd);
+    for (size_t i = 0; i < nd; i++) {
+      const double X = gd.xyz(0)[i], Y = gd.xyz(1)[i], Z = gd.xyz(2)[i];
+      hv[i] = std::sin(2 * pi * X) * std::cos(2 * pi * Y) + Z;
+      hw[i] = std::cos(2 * pi * (X + Z)) + 0.5 * Y;
+    }
+    Field v(ctx, hv), w(ctx, hw), dv = gd.field(), pr = gd.field();
+    gd.deriv(v, 1, tmp, dv);
+    save("deriv_x.bin", dv.download());
+    gd.dealias(v, w, pr, 8);
+    save("dealias.bin", pr.download());
+    save("integrate.bin", {gd.integrate(v, tmp)});
     ctx.synchronize();
   } catch (const Error &e) {
     std::fprintf(stderr, "adapter_driver: %s\n", e.what());

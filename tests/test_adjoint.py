@@ -293,3 +293,41 @@ def test_adjoint_gradient_fd_16cube_p6_200_steps(ctx):
         print("direction %d: fd %.12e adjoint %.12e rel %.2e" % (k, fd, ad, rel), flush=True)
         worst = max(worst, rel)
     assert worst <= 1e-6
+
+
+@pytest.mark.gpu
+def test_gio_checkpointed_gradient_equals_device_checkpoints(ctx, tmp_path):
+    """gf_burgers_adjoint_gio keeps its checkpoints as GIO state files: the
+    gradient and u(T) equal gf_burgers_adjoint's bit for bit (7 steps,
+    checkpoints every 3, so the last segment is short), and checkpoint c is
+    a GeoFLOW state file of the state at step 3c (header cycle/time/orders,
+    one field per component) equal to the forward steps bit for bit."""
+    import torch
+    import geoflow_amd as ga
+    from geoflow_amd import gio
+    from geoflow_amd.device import Mesh
+    print("adjoint gio checkpoints", flush=True)
+    bg = ga.BoxGrid(3, 4, [3, 2, 2])
+    m = Mesh(ctx, bg)
+    u0 = [torch.from_numpy(a).cuda() for a in fourier_field_3d(*bg.xyz, seed=3, kmax=2)]
+    lam0 = [torch.from_numpy(a).cuda() for a in fourier_field_3d(*bg.xyz, seed=4, kmax=2)]
+    nu, dt, n, ck = 0.02, 1e-3, 7, 3
+    la = [a.clone() for a in lam0]
+    lb = [a.clone() for a in lam0]
+    ua = [torch.empty_like(a) for a in u0]
+    ub = [torch.empty_like(a) for a in u0]
+    m.burgers_adjoint(u0, la, nu, dt, n, ckpt=ck, uT=ua)
+    prefix = tmp_path / "ck"
+    m.burgers_adjoint(u0, lb, nu, dt, n, ckpt=ck, uT=ub, gio_prefix=prefix)
+    for a, b in zip(la + ua, lb + ub):
+        assert torch.equal(a, b)
+    st = [a.clone() for a in u0]
+    for c in range(3):
+        meta, rows, keys, fields = gio.read_state(str(prefix) + ".%d.gio" % c, nfields=3)
+        assert meta["cycle"] == ck * c and abs(meta["time"] - ck * c * dt) < 1e-15
+        assert meta["multivar"] == 1 and meta["nelems"] == bg.nelems
+        assert list(rows[0]) == [4, 4, 4]
+        assert np.array_equal(keys, np.arange(bg.nelems))
+        for f, s in zip(fields, st):
+            assert np.array_equal(f, s.cpu().numpy())
+        m.burgers_steps(st, nu, dt, ck)

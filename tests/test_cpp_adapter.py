@@ -6,7 +6,9 @@ files; here they are compared with the CPU oracle on the same inputs:
   * Helmholtz (p=0.5, q=1 as a field and q=2.5 as a one-entry q), its DSS
     sum and the mass apply: 1e-12 relative;
   * 10 identity-preconditioned CG iterations: 1e-11 relative;
-  * 3 RK4 steps of nonlinear Burgers (nu=0.05, dt=1e-3): 1e-12 relative.
+  * 3 RK4 steps of nonlinear Burgers (nu=0.05, dt=1e-3): 1e-12 relative;
+  * on a box whose GGFX groups are built on the device: GGrid::deriv and
+    GGrid::dealias (pq = 8) at 1e-12 relative, GGrid::integrate to 1e-13.
 """
 import os
 import subprocess
@@ -71,3 +73,12 @@ def test_adapter_matches_oracle(ctx, tmp_path):
         gp.burgers_step(us, 0.05, 1e-3, t=s * 1e-3)
     for c in range(3):
         assert relerr(_load(tmp_path, "burgers_u%d.bin" % c), us[c]) <= 1e-12
+
+    # device-built GGFX maps; GGrid::deriv, dealias (pq = 8) and integrate
+    gd = oracle.Grid.box(3, 5, [2, 3, 2])
+    X, Y, Z = gd.get("x"), gd.get("y"), gd.get("z")
+    v = np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) + Z
+    w = np.cos(2 * np.pi * (X + Z)) + 0.5 * Y
+    assert relerr(_load(tmp_path, "deriv_x.bin"), gd.deriv(v, 1)) <= 1e-12
+    assert relerr(_load(tmp_path, "dealias.bin"), gd.dealias(v, w, 8)) <= 1e-12
+    assert abs(_load(tmp_path, "integrate.bin")[0] - gd.integrate(v)) <= 1e-13

@@ -16,6 +16,7 @@ import geoflow_amd as ga
 from geoflow_amd.ensemble import MemberSpec, _UnionGrid, draw_members
 from geoflow_amd.workloads import fourier_field_3d
 from conftest import relerr
+import oracle
 
 MEMBERS = [MemberSpec(3, 2), MemberSpec(4, 3), MemberSpec(3, 3), MemberSpec(4, 2),
            MemberSpec(5, 2)]
@@ -105,3 +106,77 @@ def test_batched_rk4_gradient_matches_members_alone(ctx):
         p, off, nd, _ = ens.slices[i]
         for c in range(3):
             assert relerr(grads[p][c][off:off + nd].cpu().numpy(), lam[c].cpu().numpy()) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batched_members_converge_each_and_match_oracle(ctx, precision):
+    """Members of orders 4, 6 and 8 in one batch, each member with its own
+    CG scalars and stopping test: every member's assembled residual reaches
+    1e-10 of its own right-hand side, and every member's implicit step equals
+    the oracle's step (the oracle's GAdvect for the explicit part, its GCG
+    with the Jacobi diagonal for (M + nu dt L) x = b, solved to 1e-13) to
+    1e-8, the accuracy a 1e-10 residual gives on this well-conditioned
+    system."""
+    from geoflow_amd.ensemble import Ensemble
+    import torch
+    print("ensemble per-member CG", precision, flush=True)
+    members = [MemberSpec(4, 2), MemberSpec(6, 2), MemberSpec(8, 2), MemberSpec(4, 4),
+               MemberSpec(6, 2)]
+    ens = Ensemble(ctx, members)
+    st = ens.new_state(_ic(21))
+    nu, dt = 0.05, 2e-3
+    ens.imex_step(st, nu, dt, precision=precision, tol=1e-10)
+    for p, m in ens.mesh.items():
+        its, rn, r0 = m.member_results()
+        assert (rn <= 1e-10 * r0).all(), (p, rn / r0)
+    for i, ms in enumerate(members):
+        og = oracle.Grid.box(3, ms.p, [ms.ne] * 3)
+        g = ga.BoxGrid(3, ms.p, [ms.ne] * 3)
+        us = _ic(21)(i, g)
+        got = ens.member(st, i)
+        for c in range(3):
+            b = og.mass_apply(us[c]) - dt * og.advect(us[c], us)
+            x, ret, it, _ = og.cg_solve(b, x0=us[c], p=nu * dt, q=np.ones(og.ndof), maxit=2000,
+                                        tol=1e-13, normtype=4, precond=1)
+            assert ret == 0
+            assert relerr(got[c], x) <= 1e-8, (i, c, relerr(got[c], x))
+
+
+@pytest.mark.gpu
+def test_member_offsets_must_not_split_groups(ctx):
+    """Members may start at any element (at p=4 the CG apply works two
+    elements per workgroup, and a workgroup may then hold two members), but
+    no GGFX group may join two members."""
+    from geoflow_amd import _capi
+    from geoflow_amd.device import Mesh
+    grids = [ga.BoxGrid(3, 4, [3, 3, 3]), ga.BoxGrid(3, 4, [2, 2, 2])]
+    m = Mesh(ctx, _UnionGrid(grids), structured=False)
+    m.set_segments([0, 27, 35])
+    g2 = [ga.BoxGrid(3, 4, [2, 2, 2]), ga.BoxGrid(3, 4, [2, 2, 2])]
+    m2 = Mesh(ctx, _UnionGrid(g2), structured=False)
+    with pytest.raises(_capi.GeoflowError):
+        m2.set_segments([0, 4, 16])
+    m2.set_segments([0, 8, 16])
+
+
+@pytest.mark.gpu
+def test_device_built_member_maps_equal_host_maps(ctx):
+    """An ensemble whose members' GGFX groups are built on the device (the
+    union of boxes described to gf_mesh_create) steps bit for bit like the
+    one built from the host's coordinate matching."""
+    from geoflow_amd.ensemble import Ensemble
+    import torch
+    print("ensemble device maps", flush=True)
+    members = [MemberSpec(4, 2), MemberSpec(4, 4), MemberSpec(6, 2), MemberSpec(8, 2)]
+    eh = Ensemble(ctx, members)
+    ed = Ensemble(ctx, members, device_maps=True)
+    for p in eh.mesh:
+        for w in ("mass", "imass", "imult"):
+            assert np.array_equal(eh.mesh[p].get(w), ed.mesh[p].get(w)), (p, w)
+    sh, sd = eh.new_state(_ic(31)), ed.new_state(_ic(31))
+    eh.imex_step(sh, 0.05, 2e-3, precision=1)
+    ed.imex_step(sd, 0.05, 2e-3, precision=1)
+    for p in sh:
+        for a, b in zip(sh[p], sd[p]):
+            assert torch.equal(a, b)

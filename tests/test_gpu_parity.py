@@ -365,6 +365,55 @@ def test_cg_gather_path_matches_csr_path(ctx, bdy):
     assert relerr(outs[1], outs[3]) <= 1e-8
 
 
+@pytest.mark.parametrize("p", [3, 7])
+@pytest.mark.parametrize("bdy", [[0] * 6, [6] * 6], ids=["dirichlet", "periodic"])
+def test_cg_vector_update_matches_csr_path(ctx, bdy, p):
+    """At p=7 (FP32 and FP64) and p=3 (FP64) the structured-box CG update
+    works 16-byte runs of nodes and sums each group by lattice side, which
+    is GGFX's ascending order: 20 FP64 iterations agree with the CSR DSS
+    path to rounding, the mixed solves converge to the same solution, and
+    every solution stays continuous bit for bit (all copies of a node sum
+    their group identically)."""
+    print("cg vector update", p, bdy[0], flush=True)
+    bg = ga.BoxGrid(3, p, [3, 2, 3], bdy=bdy)
+    f = workloads.unique_normal(bg, seed=12)
+    outs = []
+    for structured in (True, False):
+        m = Mesh(ctx, bg, structured=structured)
+        b = m.mass_apply(dev(f))
+        xb = torch.zeros_like(b) if bdy[0] == 0 else None
+        xg, info = m.cg_solve(b, xb=xb, p=0.1, q=1.0, maxit=20, tol=1e-300, normtype=0, precond=1)
+        assert info["iters"] == 20
+        outs.append(host(xg))
+        xm, info = m.cg_solve(b, xb=xb, p=0.1, q=1.0, maxit=2000, tol=1e-10, normtype=4, precond=1,
+                              precision=1)
+        assert info["status"] == 0
+        outs.append(host(xm))
+        if structured:
+            for h in outs[-2:]:
+                assert np.array_equal(host(m.dss(dev(h), "max")), h)
+                assert np.array_equal(host(m.dss(dev(h), "min")), h)
+    assert relerr(outs[0], outs[2]) <= 1e-12
+    assert relerr(outs[1], outs[3]) <= 1e-8
+
+
+@pytest.mark.parametrize("p", [5, 7])
+def test_helmholtz_f32_against_oracle(ctx, p):
+    """The FP32 Helmholtz apply of the mixed-precision CG (at p = 7 its
+    contractions run on the matrix cores, v_mfma_f32_4x4x1f32) against the
+    FP64 oracle on the same (FP32-representable) input: FP32 tolerance
+    3e-6 of the result's maximum (three directions of two 8-term
+    contractions each, about 30 FP32 roundings per entry)."""
+    print("helmholtz f32", p, flush=True)
+    c = dict(dim=3, p=p, ne=[3, 2, 2], bdy=[6] * 6)
+    og, bg, m = make(ctx, c)
+    u = rand(og.ndof, 7).astype(np.float32).astype(np.float64)
+    h = host(m.helmholtz_f32(dev(u).float(), 0.1, 1.0)).astype(np.float64)
+    hr = og.helmholtz(u, 0.1, np.ones(og.ndof))
+    err = np.abs(h - hr).max() / np.abs(hr).max()
+    assert err <= 3e-6, err
+
+
 def test_helmholtz_scalar_q_matches_field_q(ctx):
     """A one-entry q (set_mass_scalar with a scalar) multiplies M u only when
     it differs from 1 (ghelmholtz.ipp:184-186); against the oracle with the
@@ -454,3 +503,18 @@ def test_nonstandard_d_falls_back_to_generic_kernels(ctx):
         us.append([host(a) for a in st])
     for k in range(3):
         assert relerr(us[1][k], us[0][k]) <= 1e-12
+
+
+@pytest.mark.parametrize("dim,p,ne,pqs", [(2, 4, [3, 2], (4, 6, 8)), (3, 3, [2, 2, 2], (3, 5)),
+                                          (3, 7, [2, 1, 1], (7, 11))])
+def test_dealias_against_oracle(ctx, dim, p, ne, pqs):
+    """gf_dealias (GGrid::dealias) against the oracle's restatement, which
+    tests/test_oracle_pins.py pins by dealias(1, 1) = 1, the pointwise product
+    at pq = p and the exact Galerkin projection of low-degree products."""
+    print("dealias", dim, p, pqs, flush=True)
+    c = dict(dim=dim, p=p, ne=ne, bdy=[6] * (2 * dim))
+    og, bg, m = make(ctx, c)
+    u, v = rand(og.ndof, 3), rand(og.ndof, 4)
+    for pq in pqs:
+        got = host(m.dealias(dev(u), dev(v), pq))
+        assert relerr(got, og.dealias(u, v, pq)) <= 1e-12, pq

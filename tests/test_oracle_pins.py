@@ -577,3 +577,45 @@ def test_advection_spectrum_flat_box_neutral_sphere_growing():
         lam = np.linalg.eigvals(_assembled_advection_matrix(og, c))
         res.append(lam.real.max())
     assert res[0] > 0.3 and res[1] > 1.3 * res[0], res
+
+
+# ---------------------------------------------------------------------------
+# Quadratic dealiasing (GGrid::dealias, ggrid.ipp:2351-2503)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,p,pqs", [(2, 4, (4, 6, 8)), (3, 3, (3, 5))])
+def test_dealias_identities(dim, p, pqs):
+    """dealias(1, 1) = 1 for every pq (the GLL weights are the exact
+    integrals of the Lagrange basis); with pq = p the interpolation is the
+    identity and dealias is the pointwise product."""
+    og = oracle.Grid.box(dim, p, [3] * dim)
+    one = np.ones(og.ndof)
+    for pq in pqs:
+        assert np.abs(og.dealias(one, one, pq) - 1.0).max() <= 1e-14
+    x, y = og.get("x"), og.get("y")
+    u, v = np.sin(3 * x) + y, np.cos(2 * y) * x
+    assert np.abs(og.dealias(u, v, p) - u * v).max() <= 1e-14
+
+
+def test_dealias_is_the_exact_galerkin_projection():
+    """For a product f = v1 v2 of degree <= p on each element and pq high
+    enough that the order-pq GLL quadrature integrates phi_i f exactly,
+    dealias returns (M f)_i / W_i per direction: the exact 1-D mass matrix
+    M_im = int phi_i phi_m, formed here independently with numpy's Lagrange
+    interpolation and 20-point Gauss-Legendre quadrature."""
+    p, pq = 4, 6
+    og = oracle.Grid.box(2, p, [1, 1], bdy=[0] * 4)
+    xi, w, _ = oracle.gll_basis(p)
+    gx, gw = np.polynomial.legendre.leggauss(20)
+    N = p + 1
+    phi = np.zeros((N, gx.size))
+    for i in range(N):
+        others = [xi[m] for m in range(N) if m != i]
+        phi[i] = np.prod([(gx - o) / (xi[i] - o) for o in others], axis=0)
+    M = (phi * gw) @ phi.T
+    x, y = og.get("x"), og.get("y")
+    v1, v2 = x, y * y
+    got = og.dealias(v1, v2, pq).reshape(N, N)  # [j, i]
+    fx = (xi + 1) / 2              # x on the element's nodes
+    fy = ((xi + 1) / 2) ** 2       # y^2 on the element's nodes
+    want = np.outer(M @ fy / w, M @ fx / w)
+    assert np.abs(got - want).max() <= 1e-13
