@@ -44,12 +44,14 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_helm3p(
     uo[col + k * S::N2] = acc;
   };
   if constexpr (N == 8 && std::is_same<T, float>::value) {
-    if (!pf) {
-      // FP32 at p = 7: the contractions on the matrix cores
-      gf::MfmaD8 Dm;
-      Dm.load(D.R[0], D.C[0], (int)threadIdx.x & 63);
-      gf::pen_lap3_mfma8(ru, su[t.el], sd[t.el], Dm, W, t.i, t.j, W[t.i], W[t.j], jac, f0 * f0,
-                         f1 * f1, f2 * f2, post);
+    if (!pf && D.K) {
+      // FP32 at p = 7: stiffness form on the matrix cores
+      __shared__ float sK[64];
+      gf::MfmaK8 Km;
+      Km.init(sK, D.K, (int)threadIdx.x & 63);
+      __syncthreads();
+      gf::pen_lap3k_mfma8(ru, su[t.el], sd[t.el], Km, W, W[t.i], W[t.j], t.i, t.j, jac, f0 * f0,
+                          f1 * f1, f2 * f2, post);
       return;
     }
   }
@@ -65,7 +67,8 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     gf::PenD<T> Dp, const T *__restrict__ W, const T *__restrict__ edx,
     const T *__restrict__ ejac, const T *__restrict__ qf, T qs, const T *__restrict__ mask, double pd,
     const double *__restrict__ scal, double *__restrict__ part, size_t nelems, gf::BoxTopo topo,
-    int tmask, const int64_t *__restrict__ seg_eoff, int nseg, const double *__restrict__ sscal) {
+    int tmask, const int64_t *__restrict__ seg_eoff, int nseg, const double *__restrict__ sscal,
+    const T *__restrict__ jA, int jdiag, CGTail tail, double *scalw) {
   using S = gf::Pen3<N>;
   __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
   __shared__ double sred[S::THREADS > 64 ? S::THREADS : 64];
@@ -87,18 +90,29 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
   const size_t col = t.e * S::NN + t.r;
   const T jac = ejac[t.e];
   const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
+  const T p = (T)pd;
+  int ec[3] = {0, 0, 0};
+  if (tmask || jdiag) gf::box_elem_coords(topo, t.e, ec);
   // all loads of the column first, then the stores of w
   T ru[N], dv[N], wv[N];
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const size_t g = col + k * S::N2;
     ru[k] = r[g];
-    dv[k] = dinv ? dinv[g] : T(1);
+    dv[k] = (dinv && !jdiag) ? dinv[g] : T(1);
     wv[k] = first ? T(0) : w[g];
   }
+  if (jdiag) {
+    // the Jacobi diagonal formed here instead of read (uniform box)
+#pragma unroll
+    for (int k = 0; k < N; k++)
+      dv[k] = box_jacobi_inv<T>(jA, W, t.i, t.j, k, f0 * f0, f1 * f1, f2 * f2, jac, p, qs,
+                                box_count(topo, N, ec, t.i, t.j, k));
+  }
+  const bool pre = dinv || jdiag;
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    const T z = dinv ? dv[k] * ru[k] : ru[k];
+    const T z = pre ? dv[k] * ru[k] : ru[k];
     ru[k] = first ? z : beta * wv[k] + T(1) * z;
   }
   if (t.active) {
@@ -106,9 +120,6 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     for (int k = 0; k < N; k++) w[col + k * S::N2] = ru[k];
   }
   double dot = 0.0;
-  const T p = (T)pd;
-  int ec[3] = {0, 0, 0};
-  if (tmask) gf::box_elem_coords(topo, t.e, ec);
   auto post = [&](int k, T Lu, T M, T, T) {
     if (!t.active) return;
     const size_t g = col + k * S::N2;
@@ -128,13 +139,20 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     qv[g] = acc;
     dot += (double)ru[k] * (double)acc;
   };
+  bool mf = false;
   if constexpr (N == 8 && std::is_same<T, float>::value) {
-    // FP32 at p = 7: the contractions on the matrix cores
-    gf::MfmaD8 Dm;
-    Dm.load(Dp.R[0], Dp.C[0], (int)threadIdx.x & 63);
-    gf::pen_lap3_mfma8(ru, su[t.el], sd[t.el], Dm, W, t.i, t.j, W[t.i], W[t.j], jac, f0 * f0,
-                       f1 * f1, f2 * f2, post);
-  } else {
+    if (Dp.K) {
+      // FP32 at p = 7: stiffness form on the matrix cores
+      __shared__ float sK[64];
+      gf::MfmaK8 Km;
+      Km.init(sK, Dp.K, (int)threadIdx.x & 63);
+      __syncthreads();
+      gf::pen_lap3k_mfma8(ru, su[t.el], sd[t.el], Km, W, W[t.i], W[t.j], t.i, t.j, jac, f0 * f0,
+                          f1 * f1, f2 * f2, post);
+      mf = true;
+    }
+  }
+  if (!mf) {
     gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, Dp, W, t.i, t.j, W[t.i],
                              W[t.j], jac, f0 * f0, f1 * f1, f2 * f2, (const T *)nullptr,
                              gf::NoPenCB(), gf::NoPenCB(), gf::NoPenCB(), post);
@@ -152,6 +170,12 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
   }
   double bs = gf::block_reduce<false>(dot, sred);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  if (tail.cnt) {
+    // alpha here instead of in k_cg_alpha (scalw: the scalars, written)
+    const bool mx[1] = {false};
+    double tot[1];
+    if (grid_tail<1>(part, tail.gpart, tail.cnt, mx, tot)) cg_alpha_finish(scalw, tot[0]);
+  }
 }
 
 // Fused Burgers stage, 3-D pencil form (see k_burgers_stage for the contract).
@@ -163,7 +187,7 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_p
 void k_burgers_stage3p(
     StageArgs a, gf::PenD<double> Dp) {
   using S = gf::Pen3<N>;
-  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZ];
+  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZH];
   ColIdx<N> t(a.nelems);
   const size_t col = t.e * S::NN + t.r;
   // the inverse mass of the column, read once for all components
@@ -248,7 +272,7 @@ int launch_stage3p(gf_mesh *m, const StageArgs &a) {
 //            and is this stage's input
 // ---------------------------------------------------------------------------
 template <int N, bool KP, int ACC, bool SMV>
-__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict__ ui1,
                        const double *__restrict__ ui2, const double *__restrict__ kp0,
                        const double *__restrict__ kp1, const double *__restrict__ kp2,
@@ -259,7 +283,7 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
                        gf::PenD<double> Dp, double bh, double ch, double nu, gf::BoxTopo topo,
                        size_t nelems) {
   using S = gf::Pen3<N>;
-  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZ];
+  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZH];
   const double *uin[3] = {ui0, ui1, ui2};
   const double *kp[3] = {kp0, kp1, kp2};
   double *acc[3] = {ac0, ac1, ac2};
@@ -267,20 +291,13 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
   ColIdx<N> t(nelems);
   const size_t col = t.e * S::NN + t.r;
   gf::BoxElemDeltas de = {{0, 0, 0}, {0, 0, 0}};
-  if (KP) {
-    int ec[3];
-    gf::box_elem_coords(topo, t.e, ec);
-    de = gf::box_elem_deltas(topo, N, ec);
-  }
-  // The inverse mass of the column, read once for all components.
-  double im[N];
-#pragma unroll
-  for (int k = 0; k < N; k++) im[k] = imass[col + (size_t)k * S::N2];
-  // Stage input (and accumulator) per node; all loads are issued before any
-  // store, the accumulator stores follow the loop.  Absent group members are
-  // not read (box_load8m).
+  int ec[3] = {0, 0, 0};
+  if (KP || !imass) gf::box_elem_coords(topo, t.e, ec);
+  if (KP) de = gf::box_elem_deltas(topo, N, ec);
+  // Stage input (and accumulator) per node; each accumulator value is
+  // stored as soon as it is formed.  Absent group members are not read
+  // (box_load8m).
   double ru[3][N];
-  double av[ACC ? 3 : 1][ACC ? N : 1];
   constexpr bool GU = SMV || ACC == 1;  // the stage reads uin's group sums
 #pragma unroll
   for (int k = 0; k < N; k++) {
@@ -313,13 +330,15 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       // GGFX Smooth: Sum / count (an exact multiply for counts 1, 2, 4, 8)
       const double ks = cnt == 1 ? kv[0] : gf::box_sum8z<double>(kv, cnt) * gf::box_inv(cnt);
       double v = ui + bh * ks;
-      if (ACC == 2) av[c][k] = ao + ks * ch;
-      if (ACC == 3) av[c][k] = v;
+      double av = 0.0;
+      if (ACC == 2) av = ao + ks * ch;
+      if (ACC == 3) av = v;
       if (ACC == 1) {
         // the accumulator starts from Smooth(uin), so it stays continuous
         const double us = cnt == 1 ? uv[0] : gf::box_sum8z<double>(uv, cnt) * gf::box_inv(cnt);
-        av[c][k] = us + ks * ch;
+        av = us + ks * ch;
       }
+      if (ACC && t.active) acc[c][g] = av;
       if (SMV) {
 #pragma unroll
         for (int q = 0; q < 8; q++) uv[q] = uv[q] + bh * ks;
@@ -327,12 +346,6 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       }
       ru[c][k] = v;
     }
-  }
-  if (ACC && t.active) {
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-      for (int k = 0; k < N; k++) acc[c][col + (size_t)k * S::N2] = av[ACC ? c : 0][ACC ? k : 0];
   }
   const double jac = ejac[t.e];
   const double wa = W[t.i], wb = W[t.j];
@@ -356,7 +369,10 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       const double adv = po * M;
       if (nu != 1.0) L *= nu;
       const double urhs = -1.0 * L + -1.0 * adv;
-      ko[g] = im[k] * urhs;
+      // the inverse mass, read or (imass NULL) formed from the group size
+      const double im = imass ? imass[g]
+                              : box_imass(W[k], wb, wa, jac, box_count(topo, N, ec, t.i, t.j, k));
+      ko[g] = im * urhs;
     };
     gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, W, t.i, t.j, wa, wb, jac,
                                  rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
@@ -442,8 +458,8 @@ void launch_stage3b_v(gf_mesh *m, int nb, double *const *uin, double *const *kp,
                      m->ctx->stream, uin[0], uin[1], uin[2], kp ? kp[0] : nullptr,
                      kp ? kp[1] : nullptr, kp ? kp[2] : nullptr, acc ? acc[0] : nullptr,
                      acc ? acc[1] : nullptr, acc ? acc[2] : nullptr, kout[0], kout[1], kout[2],
-                     m->imass, m->W[0], m->edx, m->ejac, pend_d(m), bh, ch, nu, m->box,
-                     m->nelems);
+                     m->imass_box ? nullptr : m->imass, m->W[0], m->edx, m->ejac, pend_d(m), bh,
+                     ch, nu, m->box, m->nelems);
 }
 
 template <int N>
@@ -509,11 +525,12 @@ int launch_cg_apply3p(gf_mesh *m, const CGApply3Args &a) {
     Dp.R[c] = (const T *)a.DR[c];
     Dp.C[c] = (const T *)a.DC[c];
   }
+  Dp.K = (const T *)a.K;
   hipLaunchKernelGGL((k_cg_apply3p<N, T>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream,
                      (const T *)a.r, (const T *)a.dinv, (T *)a.w, (T *)a.q, Dp, (const T *)a.W,
                      (const T *)a.edx, (const T *)a.ejac, (const T *)a.qf, (T)a.qs, (const T *)a.mask, a.p,
                      (const double *)a.scal, a.part, a.nelems, a.topo, a.tmask, a.seg_eoff, a.nseg,
-                     (const double *)a.sscal);
+                     (const double *)a.sscal, (const T *)a.jA, a.jdiag, a.tail, a.scal);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
 }
@@ -536,6 +553,32 @@ int cgapply3c_f(gf_mesh *m, const CGApply3Args &a) { return launch_cg_apply3p<N,
 
 // u3 = uin + bh*k ; acc = accsrc + ch*k   (final-stage input of step_b, :315-320)
 
+// Jacobi diagonal inverse formed in the update kernels of a uniform box
+// (box_jacobi_inv) from the element's metrics and the node's group size.
+template <typename T>
+struct JacobiBox {
+  int on;
+  const T *A, *W, *edx, *ejac;
+  T p, q;
+  __device__ __forceinline__ T inv(size_t e, int i, int j, int k, int cnt) const {
+    const T f0 = edx[e * 3], f1 = edx[e * 3 + 1], f2 = edx[e * 3 + 2];
+    return box_jacobi_inv<T>(A, W, i, j, k, f0 * f0, f1 * f1, f2 * f2, ejac[e], p, q, cnt);
+  }
+};
+template <typename T>
+JacobiBox<T> jacobi_box(gf_mesh *m, const CGVecs<T> &v) {
+  JacobiBox<T> jb;
+  jb.on = v.jdiag;
+  jb.A = v.jA;
+  jb.W = (const T *)v.W;
+  jb.edx = (const T *)v.edx;
+  jb.ejac = (const T *)v.ejac;
+  jb.p = (T)v.p;
+  jb.q = (T)v.qs;
+  (void)m;
+  return jb;
+}
+
 // k_cg_update on a structured box: q arrives unassembled (straight from the
 // apply) and is assembled here by gather (gf_boxdss.hpp), in the GGFX order,
 // so no DSS pass runs between the apply and the update.  The multiplicity
@@ -546,7 +589,7 @@ template <typename T, int NORM, int N, typename I>
 __global__ __launch_bounds__(kThreads) void k_cg_update_box(
     T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r, const T *__restrict__ qv,
     const T *__restrict__ dinv, const double *__restrict__ mass, gf::BoxTopo topo, double *scal,
-    double *__restrict__ part, size_t nelems) {
+    double *__restrict__ part, size_t nelems, JacobiBox<T> jb, CGTail tail) {
   constexpr int NN = N * N * N;
   constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;  // elements per chunk
   constexpr int CN = CE * NN;                              // nodes per chunk
@@ -610,7 +653,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
 #pragma unroll
           for (int m2 = 0; m2 < 8; m2++) mem[m2] = (I)0;
         }
-        dv[s] = dinv ? dinv[g] : T(1);
+        dv[s] = (dinv && !jb.on) ? dinv[g] : T(1);
+        if (jb.on && ok[s]) dv[s] = jb.inv(e, i, j, k, cnt[s]);
         xo[s] = x[g];
         wo[s] = w[g];
         ro[s] = r[g];
@@ -626,7 +670,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
       x[g] = xv;
       const T rv = (T)1 * ro[s] + (-alpha) * qa;
       r[g] = rv;
-      const T z = dinv ? dv[s] * rv : rv;
+      const T z = (dinv || jb.on) ? dv[s] * rv : rv;
       const double im = gf::box_inv(cnt[s]);
       rho += (double)rv * (double)z * im;
       const double rd = (double)rv;
@@ -643,6 +687,13 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = a;
     part[2 * blockIdx.x + 1] = b;
+  }
+  if (tail.cnt) {
+    // rho, beta and the stopping test here instead of in k_cg_beta
+    const bool mx[2] = {false, NORM == GF_NORM_INF};
+    double tot[2];
+    if (grid_tail<2>(part, tail.gpart, tail.cnt, mx, tot))
+      cg_beta_finish(scal, tot[0], tot[1], NORM, tail.resid, tail.hmirror);
   }
 }
 
@@ -698,7 +749,7 @@ template <typename T, int NORM, int N, typename I>
 __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
     T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r, const T *__restrict__ qv,
     const T *__restrict__ dinv, const double *__restrict__ mass, gf::BoxTopo topo, double *scal,
-    double *__restrict__ part, size_t nelems) {
+    double *__restrict__ part, size_t nelems, JacobiBox<T> jb, CGTail tail) {
   using VT = Vec16<T>;
   using V = typename VT::V;
   constexpr int R = VT::R;
@@ -734,7 +785,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
     const V wo = *(const V *)(w + g0);
     const V ro = *(const V *)(r + g0);
     V dv;
-    if (dinv) dv = *(const V *)(dinv + g0);
+    if (dinv && !jb.on) dv = *(const V *)(dinv + g0);
     const V q0 = *(const V *)(qv + g0);
     V qy = {}, qz = {}, qyz = {};
     if (py) qy = *(const V *)(qv + g0 + dy);
@@ -783,7 +834,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
       VT::set(xv, n, xn1);
       const T rn1 = (T)1 * VT::get(ro, n) + (-alpha) * qa;
       VT::set(rv, n, rn1);
-      const T z = dinv ? VT::get(dv, n) * rn1 : rn1;
+      const T z = jb.on ? jb.inv(e, i0 + n, j, k, cnt) * rn1
+                        : (dinv ? VT::get(dv, n) * rn1 : rn1);
       const double im = gf::box_inv(cnt);
       rho += (double)rn1 * (double)z * im;
       const double rd = (double)rn1;
@@ -803,41 +855,48 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
     part[2 * blockIdx.x] = a;
     part[2 * blockIdx.x + 1] = b;
   }
+  if (tail.cnt) {
+    // rho, beta and the stopping test here instead of in k_cg_beta
+    const bool mx[2] = {false, NORM == GF_NORM_INF};
+    double tot[2];
+    if (grid_tail<2>(part, tail.gpart, tail.cnt, mx, tot))
+      cg_beta_finish(scal, tot[0], tot[1], NORM, tail.resid, tail.hmirror);
+  }
 }
 
 template <typename T, int NORM, int N>
-int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part) {
+int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part, const CGTail &tail) {
   if constexpr (boxv_ok<T, N>()) {
     if (m->ndof < ((size_t)1 << 31))
       hipLaunchKernelGGL((k_cg_update_boxv<T, NORM, N, int32_t>), dim3(kUpdBlocks), dim3(kThreads),
                          0, m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal,
-                         part, m->nelems);
+                         part, m->nelems, jacobi_box(m, v), tail);
     else
       hipLaunchKernelGGL((k_cg_update_boxv<T, NORM, N, int64_t>), dim3(kUpdBlocks), dim3(kThreads),
                          0, m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal,
-                         part, m->nelems);
+                         part, m->nelems, jacobi_box(m, v), tail);
     return GF_OK;
   }
   // 32-bit node indices whenever every index of the mesh fits
   if (m->ndof < ((size_t)1 << 31))
     hipLaunchKernelGGL((k_cg_update_box<T, NORM, N, int32_t>), dim3(kUpdBlocks), dim3(kThreads), 0,
                        m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal, part,
-                       m->nelems);
+                       m->nelems, jacobi_box(m, v), tail);
   else
     hipLaunchKernelGGL((k_cg_update_box<T, NORM, N, int64_t>), dim3(kUpdBlocks), dim3(kThreads), 0,
                        m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal, part,
-                       m->nelems);
+                       m->nelems, jacobi_box(m, v), tail);
   return GF_OK;
 }
 
 }  // namespace gfs
 
 #define GF_K3D_UPD(T, N)                                                              \
-  template int launch_update_box_n<T, GF_NORM_INF, N>(gf_mesh *, CGVecs<T> &, double *); \
-  template int launch_update_box_n<T, GF_NORM_EUC, N>(gf_mesh *, CGVecs<T> &, double *); \
-  template int launch_update_box_n<T, GF_NORM_L2, N>(gf_mesh *, CGVecs<T> &, double *);  \
-  template int launch_update_box_n<T, GF_NORM_L1, N>(gf_mesh *, CGVecs<T> &, double *);  \
-  template int launch_update_box_n<T, GF_NORM_ASSEMBLED_L2, N>(gf_mesh *, CGVecs<T> &, double *);
+  template int launch_update_box_n<T, GF_NORM_INF, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &); \
+  template int launch_update_box_n<T, GF_NORM_EUC, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &); \
+  template int launch_update_box_n<T, GF_NORM_L2, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &);  \
+  template int launch_update_box_n<T, GF_NORM_L1, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &);  \
+  template int launch_update_box_n<T, GF_NORM_ASSEMBLED_L2, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &);
 
 #define GF_K3D_INSTANTIATE(N)                                                                \
   template int launch_stage3p<N>(gf_mesh *, const StageArgs &);                              \
