@@ -289,4 +289,194 @@ __device__ __forceinline__ T box_gather_sum(const BoxElemDeltas &d, const T *__r
   return box_sum8<T>(v, n);
 }
 
+// ---------------------------------------------------------------------------
+// Element halo in LDS for a gather Smooth at N = 8 (one wave of 64 lanes per
+// element, lane = a + 8 b owning the column (a, b, .)).  Instead of each node
+// loading its group members from global memory -- loads under per-lane
+// conditions, each waited for in turn -- the wave loads the element's own
+// values and every partner value it can need with 17 unconditional loads
+// per lane, all in flight together, and the group sums read LDS.
+// Layout (doubles):
+//   own values       [0, 512)     i + 8 j + 64 k
+//   faces            [512, 896)   512 + 64 f + p + 8 q, f = 2 d + s (d the
+//                                 face direction, s = 1 for the upper face),
+//                                 (p, q) the node's other two indices in
+//                                 increasing direction order
+//   edges            [896, 992)   896 + 8 e + r, e = 4 pd + 2 s1 + s2 for the
+//                                 direction pair pd (0: xy, 1: xz, 2: yz) with
+//                                 sides s1, s2, r the node's third index
+//   corners          [992, 1000)  992 + sx + 2 sy + 4 sz
+//   zero             1000         +0, read for absent partners
+// A halo slot holds the value at the partner node across that face, edge
+// or corner (the offsets of box_elem_deltas added up); where the element has
+// no partner the offset is 0, the slot holds an own value and is never read.
+// ---------------------------------------------------------------------------
+constexpr int kHalo8 = 1001;
+
+// f(std::integral_constant<int, B>), ..., f(std::integral_constant<int, E-1>):
+// a loop whose index is a compile-time constant in the body.
+template <int B, int E, typename F>
+__host__ __device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+constexpr int kHalo8Zero = 1000;
+
+__device__ __forceinline__ int64_t halo8_delta(const BoxElemDeltas &de, int d, int s) {
+  return s ? de.hi[d] : de.lo[d];
+}
+
+// Offset from the element's first node read by load q (0..8) of lane `lane`
+// and the LDS slot it fills (-1: the lane has nothing to load in this round;
+// it reads the element's first node instead, so every load is
+// unconditional and in bounds).
+__device__ __forceinline__ int64_t halo8_src(const BoxElemDeltas &de, int lane, int q, int *slot) {
+  if (q < 6) {  // faces: the node's own offset is a lane function per face
+    const int d = q >> 1, s = q & 1, p = lane & 7, r = lane >> 3;
+    const int own = d == 0 ? 8 * lane + 7 * s : (d == 1 ? p + 64 * r + 56 * s : lane + 448 * s);
+    *slot = 512 + 64 * q + lane;
+    return own + halo8_delta(de, d, s);
+  }
+  if (q < 8) {  // edges: 12 x 8 values over two rounds
+    const int e = (q - 6) * 8 + (lane >> 3), r = lane & 7;
+    if (e >= 12) {
+      *slot = -1;
+      return 0;
+    }
+    // pair pd: 0 (x, y), 1 (x, z), 2 (y, z), sides s1, s2; selects only,
+    // so no array is indexed by a lane-dependent value
+    const int pd = e >> 2, s1 = (e >> 1) & 1, s2 = e & 1;
+    const int e1 = s1 ? 7 : 0, e2 = s2 ? 7 : 0;
+    const int c0 = pd == 2 ? r : e1;
+    const int c1 = pd == 0 ? e2 : (pd == 1 ? r : e1);
+    const int c2 = pd == 0 ? r : e2;
+    const int64_t dx = halo8_delta(de, 0, s1), dy1 = halo8_delta(de, 1, s1),
+                  dy2 = halo8_delta(de, 1, s2), dz = halo8_delta(de, 2, s2);
+    *slot = 896 + 8 * e + r;
+    return c0 + 8 * c1 + 64 * c2 + (pd == 0 ? dx + dy2 : (pd == 1 ? dx + dz : dy1 + dz));
+  }
+  if (lane >= 8) {  // corners
+    *slot = -1;
+    return 0;
+  }
+  const int sx = lane & 1, sy = (lane >> 1) & 1, sz = lane >> 2;
+  *slot = 992 + lane;
+  return 7 * sx + 56 * sy + 448 * sz + halo8_delta(de, 0, sx) + halo8_delta(de, 1, sy) +
+         halo8_delta(de, 2, sz);
+}
+
+// Issues the 17 loads of field u for the element at ebase (8 own column
+// values of the lane, 9 halo values) into registers; store() writes them to
+// the LDS halo hl (kHalo8 doubles, zero slot included).
+struct Halo8Load {
+  double own[8], hv[9];
+  int slot[9];
+  __device__ __forceinline__ void issue(const double *__restrict__ u, const BoxElemDeltas &de,
+                                        int64_t ebase, int lane) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) own[k] = u[ebase + lane + 64 * k];
+#pragma unroll
+    for (int q = 0; q < 9; q++) hv[q] = u[ebase + halo8_src(de, lane, q, &slot[q])];
+  }
+  __device__ __forceinline__ void store(double *hl, int lane) const {
+#pragma unroll
+    for (int k = 0; k < 8; k++) hl[lane + 64 * k] = own[k];
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+      if (slot[q] >= 0) hl[slot[q]] = hv[q];
+    if (lane == 0) hl[kHalo8Zero] = 0.0;
+  }
+};
+
+// True when halo8_order's canonical order is GGFX's on this box at N = 8:
+// every partner offset in x smaller than every one in y, and the x and y ones
+// together smaller than every z one (superincreasing in x, y, z).  An offset
+// is the element stride less 7 node strides between neighbours, (ne - 1)
+// element strides plus 7 node strides across a periodic wrap, and 7 node
+// strides within an element that is its own periodic neighbour (ne = 1).
+__host__ __device__ __forceinline__ bool box_halo8_ok(const BoxTopo &t) {
+  const int64_t NN = 512, ns[3] = {1, 8, 64};
+  const int64_t es[3] = {NN, (int64_t)t.ne[0] * NN, (int64_t)t.ne[0] * t.ne[1] * NN};
+  int64_t below = 0;  // sum of the largest offsets of the lower directions
+  bool ok = true;
+  for (int d = 0; d < 3; d++) {
+    int64_t mn, mx;
+    if (t.ne[d] == 1) {
+      if (!t.per[d]) continue;  // no partners in this direction
+      mn = mx = 7 * ns[d];
+    } else {
+      mn = es[d] - 7 * ns[d];
+      mx = t.per[d] ? (int64_t)(t.ne[d] - 1) * es[d] + 7 * ns[d] : mn;
+    }
+    ok = ok && below < mn;
+    below += mx;
+  }
+  return ok;
+}
+
+// Per-lane constants of the group sums of the column (i, j, .): which x and
+// y partners exist, whether they lie below, and the k-independent parts of
+// their halo addresses (the zero slot where a partner is absent).
+struct Halo8Col {
+  int i, j, xon, yon, nx, ny;
+  int ax0, ay0, axy0;  // + k * (8 xon), + k * (8 yon), + k * (xon & yon)
+  __device__ __forceinline__ Halo8Col(const BoxElemDeltas &de, int i_, int j_) : i(i_), j(j_) {
+    const int sx = i == 7, sy = j == 7;
+    const int64_t dx = i == 0 ? de.lo[0] : (i == 7 ? de.hi[0] : 0);
+    const int64_t dy = j == 0 ? de.lo[1] : (j == 7 ? de.hi[1] : 0);
+    xon = dx != 0;
+    yon = dy != 0;
+    nx = dx < 0;
+    ny = dy < 0;
+    ax0 = xon ? 512 + 64 * sx + j : kHalo8Zero;
+    ay0 = yon ? 640 + 64 * sy + i : kHalo8Zero;
+    axy0 = (xon & yon) ? 896 + 16 * sx + 8 * sy : kHalo8Zero;
+  }
+};
+
+// The group of node (i, j, K) as halo addresses in GGFX order: the partner
+// offsets' magnitudes are superincreasing x < y < z for N = 8 (an element
+// has 512 nodes, more than any in-element offset), so ascending global
+// index is the canonical order -- lower side first in z, then in y, then in
+// x -- over the 2 x 2 (x 2 on the z faces) crossings, an absent partner's
+// terms reading the zero slot, which adds +0 and leaves every sum as it is
+// (box_sum8z).  Returns the count of the group.
+template <int K>
+__device__ __forceinline__ int halo8_order(const Halo8Col &c, const BoxElemDeltas &de,
+                                           int (&a)[8]) {
+  const int own = c.i + 8 * c.j + 64 * K;
+  const int ax = c.ax0 + K * 8 * c.xon, ay = c.ay0 + K * 8 * c.yon;
+  const int axy = c.axy0 + K * (c.xon & c.yon);
+  // within a y row, lower x first; then lower y row first
+  const int l0 = c.nx ? ax : own, u0 = c.nx ? own : ax;
+  const int l1 = c.nx ? axy : ay, u1 = c.nx ? ay : axy;
+  int p[4] = {c.ny ? l1 : l0, c.ny ? u1 : u0, c.ny ? l0 : l1, c.ny ? u0 : u1};
+  int zon = 0, nz = 0;
+  int q[4] = {kHalo8Zero, kHalo8Zero, kHalo8Zero, kHalo8Zero};
+  if (K == 0 || K == 7) {
+    const int64_t dz = K == 0 ? de.lo[2] : de.hi[2];
+    zon = dz != 0;
+    nz = dz < 0;
+    const int sz = K == 7, sx = c.i == 7, sy = c.j == 7;
+    const int bz = zon ? 768 + 64 * sz + c.i + 8 * c.j : kHalo8Zero;
+    const int bxz = (zon & c.xon) ? 928 + 16 * sx + 8 * sz + c.j : kHalo8Zero;
+    const int byz = (zon & c.yon) ? 960 + 16 * sy + 8 * sz + c.i : kHalo8Zero;
+    const int bxyz = (zon & c.xon & c.yon) ? 992 + sx + 2 * sy + 4 * sz : kHalo8Zero;
+    const int m0 = c.nx ? bxz : bz, n0 = c.nx ? bz : bxz;
+    const int m1 = c.nx ? bxyz : byz, n1 = c.nx ? byz : bxyz;
+    q[0] = c.ny ? m1 : m0;
+    q[1] = c.ny ? n1 : n0;
+    q[2] = c.ny ? m0 : m1;
+    q[3] = c.ny ? n0 : n1;
+  }
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    a[m] = nz ? q[m] : p[m];
+    a[4 + m] = nz ? p[m] : q[m];
+  }
+  return (1 + c.xon) * (1 + c.yon) * (1 + zon);
+}
+
 }  // namespace gf

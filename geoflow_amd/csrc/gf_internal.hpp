@@ -23,12 +23,6 @@ struct gf_ctx_s {
 
 // Reduction slot layout on the device (see gf_reduce.hpp).
 #define GF_RED_MAX_BLOCKS 4096
-// Fused CG reductions (grid_tail): blocks per group, and the counters (one
-// per group plus one) that a grid of nblocks needs.
-#define GF_TAIL_GRP 256
-inline size_t gf_tail_counters(size_t nblocks) {
-  return (nblocks + GF_TAIL_GRP - 1) / GF_TAIL_GRP + 1;
-}
 
 struct gf_mesh_s {
   gf_ctx *ctx = nullptr;
@@ -44,7 +38,6 @@ struct gf_mesh_s {
   double *D[3] = {nullptr, nullptr, nullptr};
   double *W[3] = {nullptr, nullptr, nullptr};
   float *D32 = nullptr, *W32 = nullptr;  // isotropic FP32 copies
-  float *K32 = nullptr;  // FP32 1-D stiffness matrix D^T diag(W) D (N = 8 only)
   // Per-direction row-major / column-major copies of D for the pencil kernels
   double *pDR[3] = {nullptr, nullptr, nullptr}, *pDC[3] = {nullptr, nullptr, nullptr};
   float *pDR32[3] = {nullptr, nullptr, nullptr}, *pDC32[3] = {nullptr, nullptr, nullptr};
@@ -64,15 +57,6 @@ struct gf_mesh_s {
   int ngroups = 0;
   int *goff = nullptr;
   int *gmem = nullptr;
-  // The same groups by size, for the standalone GGFX operations (k_dss_bk:
-  // one launch, one thread per group, members read with vector loads):
-  // bucket c = 2..8 holds bk_n[c] groups of c members, group-major with
-  // ascending members, from bk_mem + bk_moff[c] (16-byte aligned).  NULL
-  // when a group has more than 8 members or the maps were built on the
-  // device.
-  int *bk_mem = nullptr;
-  int64_t bk_moff[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  int bk_n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int maxgrp = 1;
   // Structured box: when has_box, the groups above are exactly those of
   // gf::box_group for `box`, and fused kernels may assemble by gather.
@@ -81,16 +65,10 @@ struct gf_mesh_s {
   // has_box_mask: the boundary mask equals gf::box_masked for `box`, so
   // kernels may form it from the node position instead of reading it.
   bool has_box_mask = false;
-  // box_uniform: a structured box whose elements all have the same metrics,
-  // so the assembled Jacobi diagonal of (p L + q M) follows from the node's
-  // position and group size; jdA[i] = sum_m D(m,i)^2 W(m) (FP64 and FP32)
-  bool box_uniform = false;
-  // imass_box: box_uniform with a symmetric weight table (W_i == W_{N-1-i}),
-  // so the Burgers stage kernels form the inverse mass from the group size
-  // (box_imass) instead of reading it
-  bool imass_box = false;
-  double *jdA = nullptr;
-  float *jdA32 = nullptr;
+  // halo8: N = 8 structured box whose partner offsets are superincreasing in
+  // the order x < y < z, so the stage kernels gather through the LDS halo
+  // (gf::halo8_order's canonical order is GGFX's)
+  bool halo8 = false;
   // reusable device workspace; ws_frozen: gf_mesh_reserve(GF_RESERVE_FREEZE)
   // was called, so gf_ws hands out only what is already allocated
   std::vector<void *> ws;
@@ -118,13 +96,6 @@ struct gf_mesh_s {
   double *dl_I = nullptr, *dl_wq = nullptr;
   double *red = nullptr;    // reduction partials, red_cap doubles
   size_t red_cap = 0;
-  // CG reductions fused into the apply and update kernels: block counters
-  // and group results, the apply's in the first tail_na entries, the
-  // update's after them (zero-filled at creation; the kernels leave the
-  // counters at zero)
-  unsigned *tail_cnt = nullptr;
-  double *tail_gpart = nullptr;
-  size_t tail_na = 0;
   double *scal = nullptr;   // solver scalars
   double *hscal = nullptr;  // pinned host mirror of scal
   double *hscal_dev = nullptr;  // device address of hscal (mapped)

@@ -43,18 +43,6 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_helm3p(
     }
     uo[col + k * S::N2] = acc;
   };
-  if constexpr (N == 8 && std::is_same<T, float>::value) {
-    if (!pf && D.K) {
-      // FP32 at p = 7: stiffness form on the matrix cores
-      __shared__ float sK[64];
-      gf::MfmaK8 Km;
-      Km.init(sK, D.K, (int)threadIdx.x & 63);
-      __syncthreads();
-      gf::pen_lap3k_mfma8(ru, su[t.el], sd[t.el], Km, W, W[t.i], W[t.j], t.i, t.j, jac, f0 * f0,
-                          f1 * f1, f2 * f2, post);
-      return;
-    }
-  }
   gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, D, W, t.i, t.j, W[t.i], W[t.j],
                            jac, f0 * f0,
                            f1 * f1, f2 * f2, pf ? pf + t.e * S::NN : (const T *)nullptr,
@@ -67,8 +55,7 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     gf::PenD<T> Dp, const T *__restrict__ W, const T *__restrict__ edx,
     const T *__restrict__ ejac, const T *__restrict__ qf, T qs, const T *__restrict__ mask, double pd,
     const double *__restrict__ scal, double *__restrict__ part, size_t nelems, gf::BoxTopo topo,
-    int tmask, const int64_t *__restrict__ seg_eoff, int nseg, const double *__restrict__ sscal,
-    const T *__restrict__ jA, int jdiag, CGTail tail, double *scalw) {
+    int tmask, const int64_t *__restrict__ seg_eoff, int nseg, const double *__restrict__ sscal) {
   using S = gf::Pen3<N>;
   __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
   __shared__ double sred[S::THREADS > 64 ? S::THREADS : 64];
@@ -90,29 +77,18 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
   const size_t col = t.e * S::NN + t.r;
   const T jac = ejac[t.e];
   const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
-  const T p = (T)pd;
-  int ec[3] = {0, 0, 0};
-  if (tmask || jdiag) gf::box_elem_coords(topo, t.e, ec);
   // all loads of the column first, then the stores of w
   T ru[N], dv[N], wv[N];
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const size_t g = col + k * S::N2;
     ru[k] = r[g];
-    dv[k] = (dinv && !jdiag) ? dinv[g] : T(1);
+    dv[k] = dinv ? dinv[g] : T(1);
     wv[k] = first ? T(0) : w[g];
   }
-  if (jdiag) {
-    // the Jacobi diagonal formed here instead of read (uniform box)
-#pragma unroll
-    for (int k = 0; k < N; k++)
-      dv[k] = box_jacobi_inv<T>(jA, W, t.i, t.j, k, f0 * f0, f1 * f1, f2 * f2, jac, p, qs,
-                                box_count(topo, N, ec, t.i, t.j, k));
-  }
-  const bool pre = dinv || jdiag;
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    const T z = pre ? dv[k] * ru[k] : ru[k];
+    const T z = dinv ? dv[k] * ru[k] : ru[k];
     ru[k] = first ? z : beta * wv[k] + T(1) * z;
   }
   if (t.active) {
@@ -120,6 +96,9 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     for (int k = 0; k < N; k++) w[col + k * S::N2] = ru[k];
   }
   double dot = 0.0;
+  const T p = (T)pd;
+  int ec[3] = {0, 0, 0};
+  if (tmask) gf::box_elem_coords(topo, t.e, ec);
   auto post = [&](int k, T Lu, T M, T, T) {
     if (!t.active) return;
     const size_t g = col + k * S::N2;
@@ -139,24 +118,9 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     qv[g] = acc;
     dot += (double)ru[k] * (double)acc;
   };
-  bool mf = false;
-  if constexpr (N == 8 && std::is_same<T, float>::value) {
-    if (Dp.K) {
-      // FP32 at p = 7: stiffness form on the matrix cores
-      __shared__ float sK[64];
-      gf::MfmaK8 Km;
-      Km.init(sK, Dp.K, (int)threadIdx.x & 63);
-      __syncthreads();
-      gf::pen_lap3k_mfma8(ru, su[t.el], sd[t.el], Km, W, W[t.i], W[t.j], t.i, t.j, jac, f0 * f0,
-                          f1 * f1, f2 * f2, post);
-      mf = true;
-    }
-  }
-  if (!mf) {
-    gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, Dp, W, t.i, t.j, W[t.i],
-                             W[t.j], jac, f0 * f0, f1 * f1, f2 * f2, (const T *)nullptr,
-                             gf::NoPenCB(), gf::NoPenCB(), gf::NoPenCB(), post);
-  }
+  gf::pen_op3<N, T, false>(ru, su[t.el], sd[t.el], (T *)nullptr, Dp, W, t.i, t.j, W[t.i],
+                           W[t.j], jac, f0 * f0, f1 * f1, f2 * f2, (const T *)nullptr,
+                           gf::NoPenCB(), gf::NoPenCB(), gf::NoPenCB(), post);
   if (nseg > 0) {
     // per-element partials: the element's lanes, summed in lane order
     sred[threadIdx.x] = dot;
@@ -170,12 +134,6 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
   }
   double bs = gf::block_reduce<false>(dot, sred);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (tail.cnt) {
-    // alpha here instead of in k_cg_alpha (scalw: the scalars, written)
-    const bool mx[1] = {false};
-    double tot[1];
-    if (grid_tail<1>(part, tail.gpart, tail.cnt, mx, tot)) cg_alpha_finish(scalw, tot[0]);
-  }
 }
 
 // Fused Burgers stage, 3-D pencil form (see k_burgers_stage for the contract).
@@ -187,7 +145,7 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_p
 void k_burgers_stage3p(
     StageArgs a, gf::PenD<double> Dp) {
   using S = gf::Pen3<N>;
-  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZH];
+  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZ];
   ColIdx<N> t(a.nelems);
   const size_t col = t.e * S::NN + t.r;
   // the inverse mass of the column, read once for all components
@@ -271,8 +229,8 @@ int launch_stage3p(gf_mesh *m, const StageArgs &a) {
 //            in the same operations -- is stored through `acc` (the state)
 //            and is this stage's input
 // ---------------------------------------------------------------------------
-template <int N, bool KP, int ACC, bool SMV>
-__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+template <int N, bool KP, int ACC, bool SMV, bool HL>
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict__ ui1,
                        const double *__restrict__ ui2, const double *__restrict__ kp0,
                        const double *__restrict__ kp1, const double *__restrict__ kp2,
@@ -283,7 +241,16 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
                        gf::PenD<double> Dp, double bh, double ch, double nu, gf::BoxTopo topo,
                        size_t nelems) {
   using S = gf::Pen3<N>;
-  __shared__ double su[S::EPB][S::SZ], sd[S::EPB][S::SZ], sx[S::EPB][S::SZH];
+  // N = 8 with a previous stage: the three LDS element buffers share memory
+  // with the halos of the gather (gf::Halo8Load; the kernel is held to two
+  // waves per SIMD by its registers, so 16 KB of LDS per wave costs nothing)
+  constexpr bool HALO = N == 8 && KP && HL;
+  constexpr int NSL = HALO ? 2 * gf::kHalo8 : 3 * S::EPB * S::SZ;
+  static_assert(!HALO || 3 * S::SZ <= NSL, "element buffers fit the halo area");
+  __shared__ double sl[NSL];
+  double(*su)[S::SZ] = (double(*)[S::SZ])sl;
+  double(*sd)[S::SZ] = (double(*)[S::SZ])(sl + S::EPB * S::SZ);
+  double(*sx)[S::SZ] = (double(*)[S::SZ])(sl + 2 * S::EPB * S::SZ);
   const double *uin[3] = {ui0, ui1, ui2};
   const double *kp[3] = {kp0, kp1, kp2};
   double *acc[3] = {ac0, ac1, ac2};
@@ -291,14 +258,84 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
   ColIdx<N> t(nelems);
   const size_t col = t.e * S::NN + t.r;
   gf::BoxElemDeltas de = {{0, 0, 0}, {0, 0, 0}};
-  int ec[3] = {0, 0, 0};
-  if (KP || !imass) gf::box_elem_coords(topo, t.e, ec);
-  if (KP) de = gf::box_elem_deltas(topo, N, ec);
-  // Stage input (and accumulator) per node; each accumulator value is
-  // stored as soon as it is formed.  Absent group members are not read
-  // (box_load8m).
+  if (KP) {
+    int ec[3];
+    gf::box_elem_coords(topo, t.e, ec);
+    de = gf::box_elem_deltas(topo, N, ec);
+  }
+  // The inverse mass of the column, read once for all components.
+  double im[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) im[k] = imass[col + (size_t)k * S::N2];
   double ru[3][N];
   constexpr bool GU = SMV || ACC == 1;  // the stage reads uin's group sums
+  if constexpr (HALO) {
+    // Per component: every load of the previous stage derivative (and of
+    // uin's halo where its group sums are needed) is issued at once, the
+    // values go to LDS, and each node sums its group from there in GGFX
+    // order (halo8_order) -- the operations and order of the gather path
+    // below, so the same result bit for bit.
+    const int lane = (int)threadIdx.x;
+    const int64_t ebase = (int64_t)t.e * 512;
+    double *hk = sl, *hu = sl + gf::kHalo8;
+    const gf::Halo8Col hc(de, t.i, t.j);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      gf::Halo8Load Lk, Lu;
+      Lk.issue(kp[c], de, ebase, lane);
+      if (GU) {
+        Lu.issue(uin[c], de, ebase, lane);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) Lu.own[k] = uin[c][col + (size_t)k * 64];
+      }
+      double ao[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) ao[k] = ACC == 2 ? acc[c][col + (size_t)k * 64] : 0.0;
+      __syncthreads();  // the previous component's group sums have read LDS
+      Lk.store(hk, lane);
+      if (GU) Lu.store(hu, lane);
+      __syncthreads();
+      gf::static_for<0, 8>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr int NT = (k == 0 || k == 7) ? 8 : 4;  // terms that can be non-zero
+        int ad[8];
+        const int cnt = gf::halo8_order<k>(hc, de, ad);
+        double sk = 0.0;
+#pragma unroll
+        for (int m = 0; m < NT; m++) sk = sk + hk[ad[m]];
+        // GGFX Smooth: Sum / count (an exact multiply for counts 1, 2, 4, 8)
+        const double ks = cnt == 1 ? Lk.own[k] : sk * gf::box_inv(cnt);
+        double v = Lu.own[k] + bh * ks;
+        double av = 0.0;
+        if (ACC == 2) av = ao[k] + ks * ch;
+        if (ACC == 3) av = v;
+        if (ACC == 1) {
+          // the accumulator starts from Smooth(uin), so it stays continuous
+          double su1 = 0.0;
+#pragma unroll
+          for (int m = 0; m < NT; m++) su1 = su1 + hu[ad[m]];
+          const double us = cnt == 1 ? Lu.own[k] : su1 * gf::box_inv(cnt);
+          av = us + ks * ch;
+        }
+        if (SMV && cnt > 1) {
+          // the members only: an absent partner's zero slot would add bh ks
+          double sv = 0.0;
+#pragma unroll
+          for (int m = 0; m < NT; m++)
+            if (ad[m] != gf::kHalo8Zero) sv = sv + (hu[ad[m]] + bh * ks);
+          v = sv * gf::box_inv(cnt);
+        }
+        if (ACC && t.active) acc[c][col + (size_t)k * 64] = av;
+        ru[c][k] = v;
+      });
+    }
+    __syncthreads();  // the element buffers below reuse the halo area
+  } else {
+  // Stage input (and accumulator) per node; all loads are issued before any
+  // store, the accumulator stores follow the loop.  Absent group members are
+  // not read (box_load8m).
+  double av[ACC ? 3 : 1][ACC ? N : 1];
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const size_t g = col + (size_t)k * S::N2;
@@ -330,15 +367,13 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       // GGFX Smooth: Sum / count (an exact multiply for counts 1, 2, 4, 8)
       const double ks = cnt == 1 ? kv[0] : gf::box_sum8z<double>(kv, cnt) * gf::box_inv(cnt);
       double v = ui + bh * ks;
-      double av = 0.0;
-      if (ACC == 2) av = ao + ks * ch;
-      if (ACC == 3) av = v;
+      if (ACC == 2) av[c][k] = ao + ks * ch;
+      if (ACC == 3) av[c][k] = v;
       if (ACC == 1) {
         // the accumulator starts from Smooth(uin), so it stays continuous
         const double us = cnt == 1 ? uv[0] : gf::box_sum8z<double>(uv, cnt) * gf::box_inv(cnt);
-        av = us + ks * ch;
+        av[c][k] = us + ks * ch;
       }
-      if (ACC && t.active) acc[c][g] = av;
       if (SMV) {
 #pragma unroll
         for (int q = 0; q < 8; q++) uv[q] = uv[q] + bh * ks;
@@ -346,6 +381,13 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       }
       ru[c][k] = v;
     }
+  }
+  if (ACC && t.active) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int k = 0; k < N; k++) acc[c][col + (size_t)k * S::N2] = av[ACC ? c : 0][ACC ? k : 0];
+  }
   }
   const double jac = ejac[t.e];
   const double wa = W[t.i], wb = W[t.j];
@@ -369,10 +411,7 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
       const double adv = po * M;
       if (nu != 1.0) L *= nu;
       const double urhs = -1.0 * L + -1.0 * adv;
-      // the inverse mass, read or (imass NULL) formed from the group size
-      const double im = imass ? imass[g]
-                              : box_imass(W[k], wb, wa, jac, box_count(topo, N, ec, t.i, t.j, k));
-      ko[g] = im * urhs;
+      ko[g] = im[k] * urhs;
     };
     gf::pen_op3<N, double, true>(ru[c], su[t.el], sd[t.el], sx[t.el], Dp, W, t.i, t.j, wa, wb, jac,
                                  rx * rx, ry * ry, rz * rz, (const double *)nullptr, xcb, ycb, zcb,
@@ -450,16 +489,30 @@ __global__ __launch_bounds__(kThreads) void k_rk_out_box(
   }
 }
 
-template <int N, bool KP, int ACC, bool SMV>
-void launch_stage3b_v(gf_mesh *m, int nb, double *const *uin, double *const *kp,
+template <int N, bool KP, int ACC, bool SMV, bool HL>
+void launch_stage3b_h(gf_mesh *m, int nb, double *const *uin, double *const *kp,
                       double *const *acc, double *const *kout, double bh, double ch, double nu) {
   using S = gf::Pen3<N>;
-  hipLaunchKernelGGL((k_burgers_stage3b<N, KP, ACC, SMV>), dim3(nb), dim3(S::THREADS), 0,
+  hipLaunchKernelGGL((k_burgers_stage3b<N, KP, ACC, SMV, HL>), dim3(nb), dim3(S::THREADS), 0,
                      m->ctx->stream, uin[0], uin[1], uin[2], kp ? kp[0] : nullptr,
                      kp ? kp[1] : nullptr, kp ? kp[2] : nullptr, acc ? acc[0] : nullptr,
                      acc ? acc[1] : nullptr, acc ? acc[2] : nullptr, kout[0], kout[1], kout[2],
-                     m->imass_box ? nullptr : m->imass, m->W[0], m->edx, m->ejac, pend_d(m), bh,
-                     ch, nu, m->box, m->nelems);
+                     m->imass, m->W[0], m->edx, m->ejac, pend_d(m), bh, ch, nu, m->box,
+                     m->nelems);
+}
+
+// The N = 8 gather through the LDS halo where the box's partner offsets
+// order x < y < z (gf_mesh::halo8), else the per-node gather.
+template <int N, bool KP, int ACC, bool SMV>
+void launch_stage3b_v(gf_mesh *m, int nb, double *const *uin, double *const *kp,
+                      double *const *acc, double *const *kout, double bh, double ch, double nu) {
+  if constexpr (N == 8 && KP) {
+    if (m->halo8) {
+      launch_stage3b_h<N, KP, ACC, SMV, true>(m, nb, uin, kp, acc, kout, bh, ch, nu);
+      return;
+    }
+  }
+  launch_stage3b_h<N, KP, ACC, SMV, false>(m, nb, uin, kp, acc, kout, bh, ch, nu);
 }
 
 template <int N>
@@ -525,12 +578,11 @@ int launch_cg_apply3p(gf_mesh *m, const CGApply3Args &a) {
     Dp.R[c] = (const T *)a.DR[c];
     Dp.C[c] = (const T *)a.DC[c];
   }
-  Dp.K = (const T *)a.K;
   hipLaunchKernelGGL((k_cg_apply3p<N, T>), dim3(nb), dim3(S::THREADS), 0, m->ctx->stream,
                      (const T *)a.r, (const T *)a.dinv, (T *)a.w, (T *)a.q, Dp, (const T *)a.W,
                      (const T *)a.edx, (const T *)a.ejac, (const T *)a.qf, (T)a.qs, (const T *)a.mask, a.p,
                      (const double *)a.scal, a.part, a.nelems, a.topo, a.tmask, a.seg_eoff, a.nseg,
-                     (const double *)a.sscal, (const T *)a.jA, a.jdiag, a.tail, a.scal);
+                     (const double *)a.sscal);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
 }
@@ -553,32 +605,6 @@ int cgapply3c_f(gf_mesh *m, const CGApply3Args &a) { return launch_cg_apply3p<N,
 
 // u3 = uin + bh*k ; acc = accsrc + ch*k   (final-stage input of step_b, :315-320)
 
-// Jacobi diagonal inverse formed in the update kernels of a uniform box
-// (box_jacobi_inv) from the element's metrics and the node's group size.
-template <typename T>
-struct JacobiBox {
-  int on;
-  const T *A, *W, *edx, *ejac;
-  T p, q;
-  __device__ __forceinline__ T inv(size_t e, int i, int j, int k, int cnt) const {
-    const T f0 = edx[e * 3], f1 = edx[e * 3 + 1], f2 = edx[e * 3 + 2];
-    return box_jacobi_inv<T>(A, W, i, j, k, f0 * f0, f1 * f1, f2 * f2, ejac[e], p, q, cnt);
-  }
-};
-template <typename T>
-JacobiBox<T> jacobi_box(gf_mesh *m, const CGVecs<T> &v) {
-  JacobiBox<T> jb;
-  jb.on = v.jdiag;
-  jb.A = v.jA;
-  jb.W = (const T *)v.W;
-  jb.edx = (const T *)v.edx;
-  jb.ejac = (const T *)v.ejac;
-  jb.p = (T)v.p;
-  jb.q = (T)v.qs;
-  (void)m;
-  return jb;
-}
-
 // k_cg_update on a structured box: q arrives unassembled (straight from the
 // apply) and is assembled here by gather (gf_boxdss.hpp), in the GGFX order,
 // so no DSS pass runs between the apply and the update.  The multiplicity
@@ -589,7 +615,7 @@ template <typename T, int NORM, int N, typename I>
 __global__ __launch_bounds__(kThreads) void k_cg_update_box(
     T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r, const T *__restrict__ qv,
     const T *__restrict__ dinv, const double *__restrict__ mass, gf::BoxTopo topo, double *scal,
-    double *__restrict__ part, size_t nelems, JacobiBox<T> jb, CGTail tail) {
+    double *__restrict__ part, size_t nelems) {
   constexpr int NN = N * N * N;
   constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;  // elements per chunk
   constexpr int CN = CE * NN;                              // nodes per chunk
@@ -653,8 +679,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
 #pragma unroll
           for (int m2 = 0; m2 < 8; m2++) mem[m2] = (I)0;
         }
-        dv[s] = (dinv && !jb.on) ? dinv[g] : T(1);
-        if (jb.on && ok[s]) dv[s] = jb.inv(e, i, j, k, cnt[s]);
+        dv[s] = dinv ? dinv[g] : T(1);
         xo[s] = x[g];
         wo[s] = w[g];
         ro[s] = r[g];
@@ -670,7 +695,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
       x[g] = xv;
       const T rv = (T)1 * ro[s] + (-alpha) * qa;
       r[g] = rv;
-      const T z = (dinv || jb.on) ? dv[s] * rv : rv;
+      const T z = dinv ? dv[s] * rv : rv;
       const double im = gf::box_inv(cnt[s]);
       rho += (double)rv * (double)z * im;
       const double rd = (double)rv;
@@ -687,13 +712,6 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_box(
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = a;
     part[2 * blockIdx.x + 1] = b;
-  }
-  if (tail.cnt) {
-    // rho, beta and the stopping test here instead of in k_cg_beta
-    const bool mx[2] = {false, NORM == GF_NORM_INF};
-    double tot[2];
-    if (grid_tail<2>(part, tail.gpart, tail.cnt, mx, tot))
-      cg_beta_finish(scal, tot[0], tot[1], NORM, tail.resid, tail.hmirror);
   }
 }
 
@@ -749,7 +767,7 @@ template <typename T, int NORM, int N, typename I>
 __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
     T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r, const T *__restrict__ qv,
     const T *__restrict__ dinv, const double *__restrict__ mass, gf::BoxTopo topo, double *scal,
-    double *__restrict__ part, size_t nelems, JacobiBox<T> jb, CGTail tail) {
+    double *__restrict__ part, size_t nelems) {
   using VT = Vec16<T>;
   using V = typename VT::V;
   constexpr int R = VT::R;
@@ -785,7 +803,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
     const V wo = *(const V *)(w + g0);
     const V ro = *(const V *)(r + g0);
     V dv;
-    if (dinv && !jb.on) dv = *(const V *)(dinv + g0);
+    if (dinv) dv = *(const V *)(dinv + g0);
     const V q0 = *(const V *)(qv + g0);
     V qy = {}, qz = {}, qyz = {};
     if (py) qy = *(const V *)(qv + g0 + dy);
@@ -834,8 +852,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
       VT::set(xv, n, xn1);
       const T rn1 = (T)1 * VT::get(ro, n) + (-alpha) * qa;
       VT::set(rv, n, rn1);
-      const T z = jb.on ? jb.inv(e, i0 + n, j, k, cnt) * rn1
-                        : (dinv ? VT::get(dv, n) * rn1 : rn1);
+      const T z = dinv ? VT::get(dv, n) * rn1 : rn1;
       const double im = gf::box_inv(cnt);
       rho += (double)rn1 * (double)z * im;
       const double rd = (double)rn1;
@@ -855,48 +872,41 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
     part[2 * blockIdx.x] = a;
     part[2 * blockIdx.x + 1] = b;
   }
-  if (tail.cnt) {
-    // rho, beta and the stopping test here instead of in k_cg_beta
-    const bool mx[2] = {false, NORM == GF_NORM_INF};
-    double tot[2];
-    if (grid_tail<2>(part, tail.gpart, tail.cnt, mx, tot))
-      cg_beta_finish(scal, tot[0], tot[1], NORM, tail.resid, tail.hmirror);
-  }
 }
 
 template <typename T, int NORM, int N>
-int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part, const CGTail &tail) {
+int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part) {
   if constexpr (boxv_ok<T, N>()) {
     if (m->ndof < ((size_t)1 << 31))
       hipLaunchKernelGGL((k_cg_update_boxv<T, NORM, N, int32_t>), dim3(kUpdBlocks), dim3(kThreads),
                          0, m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal,
-                         part, m->nelems, jacobi_box(m, v), tail);
+                         part, m->nelems);
     else
       hipLaunchKernelGGL((k_cg_update_boxv<T, NORM, N, int64_t>), dim3(kUpdBlocks), dim3(kThreads),
                          0, m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal,
-                         part, m->nelems, jacobi_box(m, v), tail);
+                         part, m->nelems);
     return GF_OK;
   }
   // 32-bit node indices whenever every index of the mesh fits
   if (m->ndof < ((size_t)1 << 31))
     hipLaunchKernelGGL((k_cg_update_box<T, NORM, N, int32_t>), dim3(kUpdBlocks), dim3(kThreads), 0,
                        m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal, part,
-                       m->nelems, jacobi_box(m, v), tail);
+                       m->nelems);
   else
     hipLaunchKernelGGL((k_cg_update_box<T, NORM, N, int64_t>), dim3(kUpdBlocks), dim3(kThreads), 0,
                        m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal, part,
-                       m->nelems, jacobi_box(m, v), tail);
+                       m->nelems);
   return GF_OK;
 }
 
 }  // namespace gfs
 
 #define GF_K3D_UPD(T, N)                                                              \
-  template int launch_update_box_n<T, GF_NORM_INF, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &); \
-  template int launch_update_box_n<T, GF_NORM_EUC, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &); \
-  template int launch_update_box_n<T, GF_NORM_L2, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &);  \
-  template int launch_update_box_n<T, GF_NORM_L1, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &);  \
-  template int launch_update_box_n<T, GF_NORM_ASSEMBLED_L2, N>(gf_mesh *, CGVecs<T> &, double *, const CGTail &);
+  template int launch_update_box_n<T, GF_NORM_INF, N>(gf_mesh *, CGVecs<T> &, double *); \
+  template int launch_update_box_n<T, GF_NORM_EUC, N>(gf_mesh *, CGVecs<T> &, double *); \
+  template int launch_update_box_n<T, GF_NORM_L2, N>(gf_mesh *, CGVecs<T> &, double *);  \
+  template int launch_update_box_n<T, GF_NORM_L1, N>(gf_mesh *, CGVecs<T> &, double *);  \
+  template int launch_update_box_n<T, GF_NORM_ASSEMBLED_L2, N>(gf_mesh *, CGVecs<T> &, double *);
 
 #define GF_K3D_INSTANTIATE(N)                                                                \
   template int launch_stage3p<N>(gf_mesh *, const StageArgs &);                              \

@@ -570,81 +570,6 @@ __global__ void k_dss(T *__restrict__ u, const int *__restrict__ goff,
   }
 }
 
-// GGFX Sum / Smooth / Max / Min over the size buckets (gf_mesh::bk_mem): one
-// thread per group, block ranges per bucket (bstart[c] = first block of
-// bucket c, non-decreasing in c), the c members of a group read with one or
-// two vector loads.  Same operations in the same order as k_dss, so the
-// same result bit for bit.
-struct DssBuckets {
-  const int *mem;
-  int64_t moff[9];
-  int n[9];
-  int bstart[10];
-};
-
-template <int OP, typename T, int C>
-__device__ __forceinline__ void dss_group_c(T *__restrict__ u, const int *__restrict__ gm) {
-  int id[C];
-  if constexpr (C == 2) {
-    const int2 v = *(const int2 *)gm;
-    id[0] = v.x;
-    id[1] = v.y;
-  } else if constexpr (C == 4 || C == 8) {
-#pragma unroll
-    for (int h = 0; h < C / 4; h++) {
-      const int4 v = ((const int4 *)gm)[h];
-      id[4 * h] = v.x;
-      id[4 * h + 1] = v.y;
-      id[4 * h + 2] = v.z;
-      id[4 * h + 3] = v.w;
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < C; k++) id[k] = gm[k];
-  }
-  T v[C];
-#pragma unroll
-  for (int k = 0; k < C; k++) v[k] = u[id[k]];
-  T r;
-  if (OP == GF_DSS_SUM || OP == GF_DSS_SMOOTH) {
-    r = T(0);
-#pragma unroll
-    for (int k = 0; k < C; k++) r = r + v[k];
-    if (OP == GF_DSS_SMOOTH) r = r / (T)C;
-  } else if (OP == GF_DSS_MAX) {
-    r = std::numeric_limits<T>::lowest();
-#pragma unroll
-    for (int k = 0; k < C; k++) r = fmax(r, v[k]);
-  } else {
-    r = std::numeric_limits<T>::max();
-#pragma unroll
-    for (int k = 0; k < C; k++) r = fmin(r, v[k]);
-  }
-#pragma unroll
-  for (int k = 0; k < C; k++) u[id[k]] = r;
-}
-
-template <int OP, typename T>
-__global__ __launch_bounds__(kThreads) void k_dss_bk(T *__restrict__ u, DssBuckets bk) {
-  const int b = (int)blockIdx.x;
-  int c = 2;
-#pragma unroll
-  for (int q = 3; q <= 8; q++)
-    if (b >= bk.bstart[q]) c = q;
-  const int gi = (b - bk.bstart[c]) * (int)blockDim.x + (int)threadIdx.x;
-  if (gi >= bk.n[c]) return;
-  const int *gm = bk.mem + bk.moff[c] + (int64_t)gi * c;
-  switch (c) {
-    case 2: dss_group_c<OP, T, 2>(u, gm); break;
-    case 3: dss_group_c<OP, T, 3>(u, gm); break;
-    case 4: dss_group_c<OP, T, 4>(u, gm); break;
-    case 5: dss_group_c<OP, T, 5>(u, gm); break;
-    case 6: dss_group_c<OP, T, 6>(u, gm); break;
-    case 7: dss_group_c<OP, T, 7>(u, gm); break;
-    default: dss_group_c<OP, T, 8>(u, gm); break;
-  }
-}
-
 // mult/imult from groups: start from 1, overwrite group members.
 __global__ void k_fill(double *__restrict__ a, double v, size_t n) {
   for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < n;
@@ -763,43 +688,8 @@ int launch_mass_init(gf_mesh *m) {
 }  // namespace gfl
 
 // Internal entry points used by other translation units.
-// Block ranges of the size buckets (k_dss_bk); returns the block count.
-static int dss_buckets(gf_mesh *m, DssBuckets &bk) {
-  bk.mem = m->bk_mem;
-  int nb = 0;
-  bk.bstart[0] = bk.bstart[1] = 0;
-  for (int c = 0; c <= 8; c++) {
-    bk.moff[c] = m->bk_moff[c];
-    bk.n[c] = m->bk_n[c];
-  }
-  for (int c = 2; c <= 8; c++) {
-    bk.bstart[c] = nb;
-    nb += (m->bk_n[c] + kThreads - 1) / kThreads;
-  }
-  bk.bstart[9] = nb;
-  return nb;
-}
-
-template <typename T>
-static int dss_bk_launch(gf_mesh *m, int op, T *u) {
-  DssBuckets bk;
-  const int nb = dss_buckets(m, bk);
-  if (nb == 0) return GF_OK;
-  hipStream_t s = m->ctx->stream;
-  switch (op) {
-    case GF_DSS_SUM: hipLaunchKernelGGL((k_dss_bk<GF_DSS_SUM, T>), dim3(nb), dim3(kThreads), 0, s, u, bk); break;
-    case GF_DSS_SMOOTH: hipLaunchKernelGGL((k_dss_bk<GF_DSS_SMOOTH, T>), dim3(nb), dim3(kThreads), 0, s, u, bk); break;
-    case GF_DSS_MAX: hipLaunchKernelGGL((k_dss_bk<GF_DSS_MAX, T>), dim3(nb), dim3(kThreads), 0, s, u, bk); break;
-    case GF_DSS_MIN: hipLaunchKernelGGL((k_dss_bk<GF_DSS_MIN, T>), dim3(nb), dim3(kThreads), 0, s, u, bk); break;
-    default: return GF_ERR_ARG;
-  }
-  GF_LAUNCH_CHECK(m->ctx);
-  return GF_OK;
-}
-
 int gf_dss_t(gf_mesh *m, int op, double *u) {
   if (m->ngroups == 0) return GF_OK;
-  if (m->bk_mem) return dss_bk_launch<double>(m, op, u);
   int nb = nblocks((size_t)m->ngroups);
   hipStream_t s = m->ctx->stream;
   switch (op) {
@@ -815,10 +705,6 @@ int gf_dss_t(gf_mesh *m, int op, double *u) {
 
 int gf_dss_f32(gf_mesh *m, int op, float *u) {
   if (m->ngroups == 0) return GF_OK;
-  if (m->bk_mem) {
-    if (op != GF_DSS_SUM && op != GF_DSS_SMOOTH) return GF_ERR_ARG;
-    return dss_bk_launch<float>(m, op, u);
-  }
   int nb = nblocks((size_t)m->ngroups);
   hipStream_t s = m->ctx->stream;
   if (op == GF_DSS_SUM)
@@ -964,17 +850,6 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
       UP(m->pDR32[c], Rf.data(), Rf.size());
       UP(m->pDC32[c], Cf.data(), Cf.size());
     }
-    if (Nn == 8) {
-      // K(i,l) = sum_m D(m,i) W_m D(m,l) in FP64, ascending m, then rounded
-      std::vector<float> Kf((size_t)Nn * Nn);
-      for (int i = 0; i < Nn; i++)
-        for (int l = 0; l < Nn; l++) {
-          double s = 0.0;
-          for (int q = 0; q < Nn; q++) s += Rm[q * Nn + i] * d->W[0][q] * Rm[q * Nn + l];
-          Kf[i * Nn + l] = (float)s;
-        }
-      UP(m->K32, Kf.data(), Kf.size());
-    }
   }
   size_t n = m->ndof;
   if (d->gtype == GF_GE_REGULAR) {
@@ -1097,24 +972,6 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     m->maxgrp = maxg;
     UP(m->goff, goff.data(), goff.size());
     UP(m->gmem, gmem.data(), std::max<size_t>(gmem.size(), 1));
-    if (m->ngroups > 0 && maxg <= 8) {
-      // the size buckets, each 16-byte aligned, groups in GGFX order
-      int64_t off = 0;
-      for (int c = 2; c <= 8; c++) m->bk_n[c] = 0;
-      for (int g = 0; g < m->ngroups; g++) m->bk_n[goff[g + 1] - goff[g]]++;
-      for (int c = 2; c <= 8; c++) {
-        m->bk_moff[c] = off;
-        off += ((int64_t)m->bk_n[c] * c + 3) / 4 * 4;
-      }
-      std::vector<int> bm((size_t)std::max<int64_t>(off, 4), 0);
-      int64_t pos[9];
-      for (int c = 2; c <= 8; c++) pos[c] = m->bk_moff[c];
-      for (int g = 0; g < m->ngroups; g++) {
-        const int c = goff[g + 1] - goff[g];
-        for (int k = 0; k < c; k++) bm[pos[c]++] = gmem[goff[g] + k];
-      }
-      UP(m->bk_mem, bm.data(), bm.size());
-    }
   }
   // Structured box: enable gather assembly only if every node's GGFX group
   // is exactly the group the box structure implies.
@@ -1170,47 +1027,10 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     m->has_box = ok;
     m->has_box_mask = ok && mask_ok;
   }
-  // a box of identical elements: the Jacobi diagonal of the CG follows from
-  // the node position and group size, from the 1-D table
-  // A(i) = sum_m D(m,i)^2 W(m) (k_helm_diag's line sums)
-  if (m->has_box && m->dconst && m->Niso) {
-    bool uni = true;
-    for (size_t e = 1; uni && e < m->nelems; e++) {
-      uni = d->elem_jac[e] == d->elem_jac[0];
-      for (int c = 0; uni && c < 3; c++) uni = d->elem_dxidx[e * 3 + c] == d->elem_dxidx[c];
-    }
-    if (uni) {
-      const int Nn = m->Niso;
-      std::vector<double> A(Nn);
-      std::vector<float> A32(Nn);
-      for (int i = 0; i < Nn; i++) {
-        double acc = 0.0;
-        for (int q = 0; q < Nn; q++) {
-          const double Dm = d->D[0][q + (size_t)i * Nn];
-          acc += Dm * Dm * d->W[0][q];
-        }
-        A[i] = acc;
-        A32[i] = (float)acc;
-      }
-      UP(m->jdA, A.data(), A.size());
-      UP(m->jdA32, A32.data(), A32.size());
-      m->box_uniform = true;
-      bool wsym = true;
-      for (int i = 0; i < Nn; i++) wsym = wsym && d->W[0][i] == d->W[0][Nn - 1 - i];
-      m->imass_box = wsym;
-    }
-  }
+  // the LDS-halo gather of the N = 8 stage kernels (gf::box_halo8_ok)
+  m->halo8 = m->has_box && m->Niso == 8 && gf::box_halo8_ok(m->box);
   m->red_cap = gf_red_need(m->nelems);
   GF_HIP(ctx, hipMalloc((void **)&m->red, m->red_cap * sizeof(double)));
-  // apply tail: at most one block per element (plus the XCD rounding)
-  // (the update kernels run GF_RED_MAX_BLOCKS blocks at most)
-  m->tail_na = gf_tail_counters(m->nelems + 8);
-  {
-    const size_t nc = m->tail_na + gf_tail_counters(GF_RED_MAX_BLOCKS);
-    GF_HIP(ctx, hipMalloc((void **)&m->tail_cnt, nc * sizeof(unsigned)));
-    GF_HIP(ctx, hipMalloc((void **)&m->tail_gpart, 2 * nc * sizeof(double)));
-    GF_HIP(ctx, hipMemset(m->tail_cnt, 0, nc * sizeof(unsigned)));
-  }
   GF_HIP(ctx, hipMalloc((void **)&m->scal, 64 * sizeof(double)));
   GF_HIP(ctx, hipHostMalloc((void **)&m->hscal, 64 * sizeof(double),
                             hipHostMallocMapped | hipHostMallocCoherent));
@@ -1268,8 +1088,7 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
 int gf_mesh_flags(const gf_mesh *m) {
   if (!m) return 0;
   return (m->has_box ? GF_MESH_BOX_GATHER : 0) | (m->has_box_mask ? GF_MESH_BOX_MASK : 0) |
-         (m->dconst ? GF_MESH_CONST_D : 0) | (m->imass_box ? GF_MESH_BOX_UNIFORM : 0) |
-         (m->bk_mem ? GF_MESH_SIZE_BUCKETS : 0);
+         (m->dconst ? GF_MESH_CONST_D : 0);
 }
 
 int gf_mesh_destroy(gf_mesh *m) {
@@ -1290,13 +1109,13 @@ int gf_mesh_destroy(gf_mesh *m) {
     F(m->pDR32[k]);
     F(m->pDC32[k]);
   }
-  F(m->D32); F(m->W32); F(m->K32); F(m->edx); F(m->ejac); F(m->edx32); F(m->ejac32); F(m->njac);
+  F(m->D32); F(m->W32); F(m->edx); F(m->ejac); F(m->edx32); F(m->ejac32); F(m->njac);
   F(m->mask); F(m->mass); F(m->imass); F(m->imult); F(m->mask32); F(m->imult32);
-  F(m->goff); F(m->gmem); F(m->bk_mem); F(m->red); F(m->scal); F(m->tail_cnt); F(m->tail_gpart);
+  F(m->goff); F(m->gmem); F(m->red); F(m->scal);
   for (int k = 0; k < 6; k++) F(m->G32[k]);
   F(m->mass32);
   if (m->hscal) (void)hipHostFree(m->hscal);
-  F(m->seg_eoff); F(m->seg_boff); F(m->sscal); F(m->dl_I); F(m->dl_wq); F(m->jdA); F(m->jdA32);
+  F(m->seg_eoff); F(m->seg_boff); F(m->sscal); F(m->dl_I); F(m->dl_wq);
   if (m->hsscal) (void)hipHostFree(m->hsscal);
   for (int k = 0; k < 2; k++)
     if (m->cg_ev[k]) (void)hipEventDestroy(m->cg_ev[k]);

@@ -29,9 +29,6 @@ template <typename T>
 struct PenD {
   const T *R[3];
   const T *C[3];
-  // 1-D stiffness matrix K = D^T diag(W) D, row-major (FP32 meshes of
-  // order 7 only; else null)
-  const T *K = nullptr;
 };
 
 // Returns p unchanged, but hidden from the optimiser: loads through the
@@ -57,16 +54,6 @@ struct Pen3 {
   static constexpr int EPB = N2 >= 64 ? 1 : 64 / N2;  // elements per workgroup
   static constexpr int THREADS = EPB * N2;
   __device__ static constexpr int at(int i, int j, int k) { return i + j * P + k * SK; }
-  // The hook buffer is written along x-pencils and read along columns only.
-  // For N = 8 it is unpadded (512 doubles) with the x index XOR-swizzled by
-  // (y ^ z): both access patterns then touch 16 distinct 8-byte banks per
-  // 16 lanes, and the three LDS buffers of a stage kernel fit 12 waves per
-  // CU (3 per SIMD); other orders use the padded layout.
-  static constexpr bool HSW = N == 8;
-  static constexpr int SZH = HSW ? NN : SZ;
-  __device__ static constexpr int hat(int i, int j, int k) {
-    return HSW ? (i ^ ((j ^ k) & (N - 1))) + j * N + k * N2 : at(i, j, k);
-  }
 };
 
 // Computes, for the column (i=a, j=b) owned by the calling thread,
@@ -78,7 +65,7 @@ struct Pen3 {
 // rounds differently from the stored mass by at most an ulp or two).
 //   ru    the thread's column of u (nodes (a,b,0..N-1))
 //   su,sd two LDS element buffers (Pen3<N>::SZ each) owned by this element
-//   sx    a third LDS buffer (Pen3<N>::SZH) when HOOK is true, else unused
+//   sx    a third LDS element buffer when HOOK is true, else unused
 //   pe    p field restricted to this element, or NULL
 //   Wa,Wb the GLL weights W[a], W[b] (loaded once by the caller)
 // Hooks see the reference derivatives where they are formed:
@@ -143,7 +130,7 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
       for (int m = 0; m < N; m++) ux = fma(dR(i, m), xp[m], ux);
       const T M = W[i] * cab;
       const T hx = xcb(i, ux, M);
-      if (HOOK) sx[S::hat(i, a, b)] = hx;
+      if (HOOK) sx[S::at(i, a, b)] = hx;
       T g = ux * g0;
       if (pe) g *= pe[i + a * N + b * N * N];
       f[i] = M * g;
@@ -199,117 +186,7 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
   for (int k = 0; k < N; k++) {
     const int q = S::at(a, b, k);
     const T M = W[k] * cab;
-    post(k, sd[q] + dz[k], M, HOOK ? sx[S::hat(a, b, k)] : T(0), HOOK ? su[q] : T(0));
-  }
-}
-
-// ---------------------------------------------------------------------------
-// FP32 Laplacian of a p=7 element (N = 8, one wave per element) in stiffness
-// form on the matrix cores.  On a regular element the d-part of L u,
-// D_d^T (M G_dd (D_d u)), equals at node (i,j,k), for d = x,
-//     (W_j W_k Jac G_xx) sum_l K(i,l) u(l,j,k),   K = D^T diag(W) D,
-// the 1-D stiffness matrix (symmetric), and likewise along y and z.  The
-// pencils a lane owns -- the z-column (a,b,.), the x-pencil (.,a,b) and
-// the y-pencil (a,.,b) -- all have (W_a W_b Jac) as that factor, so each
-// direction is ONE batch of 64 length-8 contractions per element (instead
-// of a gradient and a D^T batch) followed by one multiply.  Each batch is 16
-// rank-1 block updates of v_mfma_f32_4x4x1f32 (16 independent 4x4 blocks
-// per wave): block b covers pencils 4b..4b+3 (one per lane of the block;
-// the B operand is the lane's own u(l)) and rows 4t..4t+3 of K (the A
-// operand K(4t + lane%4, l)), accumulated over l, for t = 0, 1; the result
-// lands in the lane that owns the pencil, row 4t + register.  Every block
-// product is used and no data moves between lanes.  This regroups the
-// reference's sums (ghelmholtz.ipp:308-378), so it serves the FP32 applies
-// only, whose results are FP32-accurate either way.
-// ---------------------------------------------------------------------------
-typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-
-// K in LDS for the whole kernel (64 floats): a lane reads rows 4t + lane%4
-// right before the contraction that needs them, so the A operands occupy
-// registers only while it runs.
-struct MfmaK8 {
-  const float *sK;
-  int r;  // lane % 4
-  // Copies K (global, row-major) into the LDS table; every lane of the
-  // wave takes part.  The caller synchronises before the first contraction.
-  __device__ __forceinline__ void init(float *lds, const float *__restrict__ K, int lane) {
-    lds[lane] = K[lane];
-    sK = lds;
-    r = lane & 3;
-  }
-};
-
-// out(l) = sum_m K(l, m) v(m) for the lane's own pencil.  Four independent
-// accumulation chains (two row blocks t, two halves of the sum over m),
-// interleaved so that consecutive matrix instructions do not wait on each
-// other's results; the halves are added at the end.
-__device__ __forceinline__ void mfma_contract8(const MfmaK8 &Km, const float (&v)[8],
-                                               float (&out)[8]) {
-  const float *row0 = Km.sK + Km.r * 8, *row1 = Km.sK + (4 + Km.r) * 8;
-  asm volatile("" : "+v"(row0), "+v"(row1));  // keep the reads here
-  const float4 a0 = *(const float4 *)row0, a1 = *(const float4 *)(row0 + 4);
-  const float4 b0 = *(const float4 *)row1, b1 = *(const float4 *)(row1 + 4);
-  const float A0[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-  const float A1[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-  mfma_f4 c00 = {0.f, 0.f, 0.f, 0.f}, c01 = c00, c10 = c00, c11 = c00;
-#pragma unroll
-  for (int m = 0; m < 4; m++) {
-    c00 = __builtin_amdgcn_mfma_f32_4x4x1f32(A0[m], v[m], c00, 0, 0, 0);
-    c10 = __builtin_amdgcn_mfma_f32_4x4x1f32(A1[m], v[m], c10, 0, 0, 0);
-    c01 = __builtin_amdgcn_mfma_f32_4x4x1f32(A0[m + 4], v[m + 4], c01, 0, 0, 0);
-    c11 = __builtin_amdgcn_mfma_f32_4x4x1f32(A1[m + 4], v[m + 4], c11, 0, 0, 0);
-  }
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    out[q] = c00[q] + c01[q];
-    out[4 + q] = c10[q] + c11[q];
-  }
-}
-
-// post(k, Lu_k, M_k, 0, 0) for the lane's column, as in pen_op3, with M_k =
-// W_k ((W_b W_a) Jac).  One element per wave: lane = a + 8 b.  All lanes of
-// the workgroup must call this (barriers).
-template <typename POST>
-__device__ __forceinline__ void pen_lap3k_mfma8(const float (&ru)[8], float *su, float *sd,
-                                                const MfmaK8 &Km, const float *__restrict__ W,
-                                                float Wa, float Wb, int a, int b, float jac,
-                                                float g0, float g1, float g2, POST &&post) {
-  using S = Pen3<8>;
-  const float cab = (Wb * Wa) * jac;
-#pragma unroll
-  for (int k = 0; k < 8; k++) su[S::at(a, b, k)] = ru[k];
-  float kz[8];
-  mfma_contract8(Km, ru, kz);
-  __syncthreads();
-  {
-    float xp[8], kx[8];
-#pragma unroll
-    for (int m = 0; m < 8; m++) xp[m] = su[S::at(m, a, b)];
-    mfma_contract8(Km, xp, kx);
-    const float fx = cab * g0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) sd[S::at(i, a, b)] = fx * kx[i];
-  }
-  float ky[8];
-  {
-    float yp[8];
-#pragma unroll
-    for (int m = 0; m < 8; m++) yp[m] = su[S::at(a, m, b)];
-    mfma_contract8(Km, yp, ky);
-  }
-  __syncthreads();
-  const float fy = cab * g1;
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const int q = S::at(a, j, b);
-    sd[q] = sd[q] + fy * ky[j];
-  }
-  __syncthreads();
-  const float fz = cab * g2;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const int q = S::at(a, b, k);
-    post(k, sd[q] + fz * kz[k], W[k] * cab, 0.f, 0.f);
+    post(k, sd[q] + dz[k], M, HOOK ? sx[q] : T(0), HOOK ? su[q] : T(0));
   }
 }
 

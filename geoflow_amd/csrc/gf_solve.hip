@@ -1539,17 +1539,17 @@ double finish_norm_host(int normtype, double v, gf_mesh *m, double volume) {
 }
 
 template <typename T, int NORM>
-int launch_update_box(gf_mesh *m, CGVecs<T> &v, double *part, const CGTail &tail) {
+int launch_update_box(gf_mesh *m, CGVecs<T> &v, double *part) {
   switch (m->Niso) {
-    case 2: return launch_update_box_n<T, NORM, 2>(m, v, part, tail);
-    case 3: return launch_update_box_n<T, NORM, 3>(m, v, part, tail);
-    case 4: return launch_update_box_n<T, NORM, 4>(m, v, part, tail);
-    case 5: return launch_update_box_n<T, NORM, 5>(m, v, part, tail);
-    case 6: return launch_update_box_n<T, NORM, 6>(m, v, part, tail);
-    case 7: return launch_update_box_n<T, NORM, 7>(m, v, part, tail);
-    case 8: return launch_update_box_n<T, NORM, 8>(m, v, part, tail);
-    case 9: return launch_update_box_n<T, NORM, 9>(m, v, part, tail);
-    case 10: return launch_update_box_n<T, NORM, 10>(m, v, part, tail);
+    case 2: return launch_update_box_n<T, NORM, 2>(m, v, part);
+    case 3: return launch_update_box_n<T, NORM, 3>(m, v, part);
+    case 4: return launch_update_box_n<T, NORM, 4>(m, v, part);
+    case 5: return launch_update_box_n<T, NORM, 5>(m, v, part);
+    case 6: return launch_update_box_n<T, NORM, 6>(m, v, part);
+    case 7: return launch_update_box_n<T, NORM, 7>(m, v, part);
+    case 8: return launch_update_box_n<T, NORM, 8>(m, v, part);
+    case 9: return launch_update_box_n<T, NORM, 9>(m, v, part);
+    case 10: return launch_update_box_n<T, NORM, 10>(m, v, part);
     default: return GF_ERR_UNSUPPORTED;
   }
 }
@@ -1610,7 +1610,6 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
     a3.DR[c] = sizeof(T) == 8 ? (const void *)m->pDR[c] : (const void *)m->pDR32[c];
     a3.DC[c] = sizeof(T) == 8 ? (const void *)m->pDC[c] : (const void *)m->pDC32[c];
   }
-  a3.K = sizeof(T) == 4 ? (const void *)m->K32 : nullptr;
   a3.W = v.W;
   a3.edx = v.edx;
   a3.ejac = v.ejac;
@@ -1624,22 +1623,6 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a3.scal = m->scal;
   a3.part = partA;
   a3.nelems = m->nelems;
-  a3.jdiag = v.jdiag;
-  a3.jA = v.jA;
-  // 3-D column applies form alpha at their end, and the structured-box
-  // updates beta and the stopping test at theirs (grid_tail): two launches
-  // per iteration instead of four
-  CGTail tailU;
-  if (col) {
-    a3.tail.cnt = m->tail_cnt;
-    a3.tail.gpart = m->tail_gpart;
-  }
-  if (gather) {
-    tailU.cnt = m->tail_cnt + m->tail_na;
-    tailU.gpart = m->tail_gpart + 2 * m->tail_na;
-    tailU.resid = d_resid;
-    tailU.hmirror = m->hscal_dev;
-  }
   // the host mirror starts from the scalars prepared by the caller (a
   // solve that is already done queues nothing that would update it)
   volatile double *hm = m->hscal;
@@ -1664,15 +1647,14 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
       if (m->ngroups && !gather)
         hipLaunchKernelGGL(k_dss_cg<T>, dim3(nblocks((size_t)m->ngroups)), dim3(kThreads), 0, s,
                            v.q, m->goff, m->gmem, m->ngroups, (const double *)m->scal);
-      if (!col)
-        hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kRedThreads), 0, s, m->scal, partA, nbA);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kRedThreads), 0, s, m->scal, partA, nbA);
       if (gather) {
         switch (normtype) {
-          case GF_NORM_INF: rc = launch_update_box<T, GF_NORM_INF>(m, v, partU, tailU); break;
-          case GF_NORM_EUC: rc = launch_update_box<T, GF_NORM_EUC>(m, v, partU, tailU); break;
-          case GF_NORM_L2: rc = launch_update_box<T, GF_NORM_L2>(m, v, partU, tailU); break;
-          case GF_NORM_L1: rc = launch_update_box<T, GF_NORM_L1>(m, v, partU, tailU); break;
-          default: rc = launch_update_box<T, GF_NORM_ASSEMBLED_L2>(m, v, partU, tailU); break;
+          case GF_NORM_INF: rc = launch_update_box<T, GF_NORM_INF>(m, v, partU); break;
+          case GF_NORM_EUC: rc = launch_update_box<T, GF_NORM_EUC>(m, v, partU); break;
+          case GF_NORM_L2: rc = launch_update_box<T, GF_NORM_L2>(m, v, partU); break;
+          case GF_NORM_L1: rc = launch_update_box<T, GF_NORM_L1>(m, v, partU); break;
+          default: rc = launch_update_box<T, GF_NORM_ASSEMBLED_L2>(m, v, partU); break;
         }
         if (rc) return rc;
       } else {
@@ -1684,9 +1666,8 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
           default: launch_update<T, GF_NORM_ASSEMBLED_L2>(m, v, partU, col); break;
         }
       }
-      if (!gather)
-        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kThreads), 0, s, m->scal, partU, kUpdBlocks,
-                           normtype, d_resid, m->hscal_dev);
+      hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kThreads), 0, s, m->scal, partU, kUpdBlocks,
+                         normtype, d_resid, m->hscal_dev);
     }
     GF_LAUNCH_CHECK(m->ctx);
     // One batch of look-ahead: batch b is queued before the host waits for
@@ -1775,7 +1756,6 @@ int cg_loop_seg(gf_mesh *m, CGVecs<T> &v, int normtype) {
     a3.DR[c] = sizeof(T) == 8 ? (const void *)m->pDR[c] : (const void *)m->pDR32[c];
     a3.DC[c] = sizeof(T) == 8 ? (const void *)m->pDC[c] : (const void *)m->pDC32[c];
   }
-  a3.K = sizeof(T) == 4 ? (const void *)m->K32 : nullptr;
   a3.W = v.W;
   a3.edx = v.edx;
   a3.ejac = v.ejac;
@@ -1788,8 +1768,6 @@ int cg_loop_seg(gf_mesh *m, CGVecs<T> &v, int normtype) {
   a3.scal = m->scal;
   a3.part = partA;
   a3.nelems = m->nelems;
-  a3.jdiag = v.jdiag;
-  a3.jA = v.jA;
   a3.seg_eoff = m->seg_eoff;
   a3.nseg = ns;
   a3.sscal = m->sscal;
@@ -2060,10 +2038,6 @@ int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const do
   // ghelmholtz.ipp:184-186, 370-373)
   const double qs = prm->q_field ? 0.0 : prm->q_scalar;
   if (prm->precond) RC(gf_helmholtz_diag_inv(m, prm->p_scalar, qs, prm->q_field, dinv));
-  // on a uniform structured box the iteration kernels form the Jacobi
-  // diagonal themselves (the array above serves the set-up only)
-  const int jdiag = (prm->precond && m->box_uniform && m->dim == 3 && m->dconst && m->has_box &&
-                     !prm->q_field && m->jdA) ? 1 : 0;
   if (bbv) {  // x = x*mask + xb (gcg.ipp:371-375)
     RC(gf_mul_t(m, x, m->mask, x));
     RC(gf_axpy_t(m, 1.0, xb, x));
@@ -2129,8 +2103,6 @@ int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const do
     for (int c = 0; c < 6; c++) v.G[c] = m->G[c];
     v.mass = m->mass;
     v.p = prm->p_scalar;
-    v.jdiag = jdiag;
-    v.jA = m->jdA;
     int it = 0;
     RC(cg_loop<double>(m, v, normtype, d_resid, &it));
     total_iters = it;
@@ -2172,8 +2144,6 @@ int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const do
     for (int c = 0; c < 6; c++) v.G[c] = m->G32[c];
     v.mass = m->gtype != GF_GE_REGULAR ? m->mass32 : nullptr;
     v.p = prm->p_scalar;
-    v.jdiag = jdiag;
-    v.jA = m->jdA32;
     int hist = 0;
     bool first = true;
     for (outer = 0;; outer++) {
@@ -2421,8 +2391,6 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
   ca3.seg_eoff = nullptr;
   ca3.nseg = 0;
   ca3.sscal = nullptr;
-  ca3.jdiag = 0;
-  ca3.jA = nullptr;
   memset(&ca3, 0, sizeof(ca3));
   const bool pencil = w == "burgers_stage" || w == "cg_apply" || w == "cg_apply_f32" ||
                       w == "burgers_stage_box";
@@ -2472,7 +2440,6 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
       ca3.DR[c] = f32 ? (const void *)m->pDR32[c] : (const void *)m->pDR[c];
       ca3.DC[c] = f32 ? (const void *)m->pDC32[c] : (const void *)m->pDC[c];
     }
-    ca3.K = f32 ? (const void *)m->K32 : nullptr;
     ca3.W = f32 ? (const void *)m->W32 : (const void *)m->W[0];
     ca3.edx = f32 ? (const void *)m->edx32 : (const void *)m->edx;
     ca3.ejac = f32 ? (const void *)m->ejac32 : (const void *)m->ejac;
@@ -2485,10 +2452,6 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
     ca3.scal = m->scal;
     ca3.part = m->red;
     ca3.nelems = m->nelems;
-    // as the solver runs it: the Jacobi diagonal formed in the kernel on a
-    // uniform box (buf[1] is then not read)
-    ca3.jdiag = (m->box_uniform && m->has_box && m->jdA) ? 1 : 0;
-    ca3.jA = f32 ? (const void *)m->jdA32 : (const void *)m->jdA;
     if (f32)
       launch = [&]() { return GF_DISPATCH_N3(m->Niso, cgapply3c_f, m, ca3); };
     else
@@ -2534,9 +2497,7 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
         v.ejac = m->ejac32;
         v.p = 0.1;
         v.qs = 1.0;
-        v.jdiag = (m->box_uniform && m->jdA32) ? 1 : 0;
-        v.jA = m->jdA32;
-        return launch_update_box<float, GF_NORM_ASSEMBLED_L2>(m, v, m->red, CGTail());
+        return launch_update_box<float, GF_NORM_ASSEMBLED_L2>(m, v, m->red);
       };
     } else {
       launch = [&]() {
@@ -2551,9 +2512,7 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
         v.ejac = m->ejac;
         v.p = 0.1;
         v.qs = 1.0;
-        v.jdiag = (m->box_uniform && m->jdA) ? 1 : 0;
-        v.jA = m->jdA;
-        return launch_update_box<double, GF_NORM_ASSEMBLED_L2>(m, v, m->red, CGTail());
+        return launch_update_box<double, GF_NORM_ASSEMBLED_L2>(m, v, m->red);
       };
     }
   } else if (w == "helmholtz") {

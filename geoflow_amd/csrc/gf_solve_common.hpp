@@ -63,7 +63,6 @@ inline gf::PenD<float> pend_f(gf_mesh *m) {
     p.R[c] = m->pDR32[c];
     p.C[c] = m->pDC32[c];
   }
-  p.K = m->K32;
   return p;
 }
 
@@ -100,37 +99,23 @@ struct ColIdx {
 // CG direction update + operator apply (see k_cg_apply for the contract),
 // 3-D pencil form; the Dirichlet mask is applied to q here (the mask is
 // constant over each coincidence group, so DSS(q) * mask = DSS(q * mask)).
-// Where the fused tails of the CG apply and update kernels keep their group
-// results and counters (tail_counters(nblocks) of each), or NULL pointers
-// for a separate k_cg_alpha / k_cg_beta launch.
-struct CGTail {
-  double *gpart = nullptr;
-  unsigned *cnt = nullptr;
-  double *resid = nullptr, *hmirror = nullptr;  // update tail: as k_cg_beta
-};
 
 struct CGApply3Args {
   const void *r, *dinv;
   void *w, *q;
   const void *D, *W, *edx, *ejac, *qf, *mask;
   const void *DR[3], *DC[3];  // pencil copies of D (same precision as T)
-  const void *K;              // FP32 stiffness matrix (FP32 applies at N = 8) or NULL
   double p;
   double qs;  // mass-term scalar when qf is NULL (0: no mass term)
   double *scal, *part;
   size_t nelems;
   gf::BoxTopo topo;  // with tmask: mask formed from the box faces (mask NULL)
   int tmask;
-  // jdiag: Jacobi diagonal inverse formed in the kernel (box_jacobi_inv)
-  int jdiag;
-  const void *jA;
   // members of a batched mesh (nseg > 0): element offsets and per-member
   // scalars; partial sums are then indexed by logical block
   const int64_t *seg_eoff;
   int nseg;
   double *sscal;
-  // alpha formed at the end of the apply kernel (grid_tail) when tail.cnt
-  CGTail tail;
 };
 
 // Solver scalars per member of a batched mesh: kSegScal doubles each.
@@ -170,86 +155,7 @@ inline int col_blocks(gf_mesh *m) {
   }()
 
 constexpr int kUpdBlocks = 1024;
-static_assert(kUpdBlocks <= GF_RED_MAX_BLOCKS, "update tail counters are sized for GF_RED_MAX_BLOCKS");
 constexpr int kBatch = 16;
-
-// ---------------------------------------------------------------------------
-// Grid-wide reduction of per-block partials at the end of the kernel that
-// produced them, instead of in a second launch.  Blocks are taken in groups
-// of kTailGrp consecutive indices.  The last block of a group to finish (a
-// device-scope counter per group) reduces the group's partials in index
-// order; the last group to finish reduces the group results in index order.
-// The total therefore does not depend on the order in which blocks finish.
-// Each counter is set back to zero by the block that consumes it, ready for
-// the next launch.  part[NV*b + v] holds value v of block b (written by
-// thread 0 of block b before the call); mx[v] selects max instead of sum.
-// All threads of every block call this; it returns true in thread 0 of the
-// one block that holds the totals.  Nothing waits on another block.
-// ---------------------------------------------------------------------------
-constexpr int kTailGrp = GF_TAIL_GRP;
-
-template <int NV>
-__device__ __forceinline__ bool grid_tail(const double *part, double *gpart, unsigned *cnt,
-                                          const bool (&mx)[NV], double (&tot)[NV]) {
-  __shared__ int s_grp_last, s_all_last;
-  const int nb = (int)gridDim.x, ng = (nb + kTailGrp - 1) / kTailGrp;
-  const int grp = (int)blockIdx.x / kTailGrp, b0 = grp * kTailGrp;
-  const int gs = nb - b0 < kTailGrp ? nb - b0 : kTailGrp;
-  // lanes of the first wave (blocks of fewer than 64 threads: all of them;
-  // the wave reductions see zeros from absent lanes)
-  const int nl = blockDim.x < 64 ? (int)blockDim.x : 64;
-  if (threadIdx.x == 0) {
-    __threadfence();  // this block's partial before its count
-    s_grp_last = atomicAdd(&cnt[grp], 1u) == (unsigned)(gs - 1);
-  }
-  __syncthreads();
-  if (!s_grp_last) return false;
-  if (threadIdx.x < 64) {
-    __threadfence();  // every partial of the group after the count
-    const int lane = (int)threadIdx.x;
-    double v[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) v[q] = 0.0;
-    for (int b = lane; b < gs; b += nl)
-#pragma unroll
-      for (int q = 0; q < NV; q++) {
-        const double x = part[NV * (b0 + b) + q];
-        v[q] = mx[q] ? fmax(v[q], x) : v[q] + x;
-      }
-#pragma unroll
-    for (int q = 0; q < NV; q++) v[q] = mx[q] ? gf::wave_max(v[q]) : gf::wave_sum(v[q]);
-    if (lane == 0) {
-#pragma unroll
-      for (int q = 0; q < NV; q++) gpart[NV * grp + q] = v[q];
-      cnt[grp] = 0u;
-      __threadfence();
-      s_all_last = atomicAdd(&cnt[ng], 1u) == (unsigned)(ng - 1);
-    }
-  }
-  __syncthreads();
-  if (!s_all_last) return false;
-  if (threadIdx.x < 64) {
-    __threadfence();
-    const int lane = (int)threadIdx.x;
-    double v[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) v[q] = 0.0;
-    for (int g = lane; g < ng; g += nl)
-#pragma unroll
-      for (int q = 0; q < NV; q++) {
-        const double x = gpart[NV * g + q];
-        v[q] = mx[q] ? fmax(v[q], x) : v[q] + x;
-      }
-#pragma unroll
-    for (int q = 0; q < NV; q++) v[q] = mx[q] ? gf::wave_max(v[q]) : gf::wave_sum(v[q]);
-    if (lane == 0) {
-#pragma unroll
-      for (int q = 0; q < NV; q++) tot[q] = v[q];
-      cnt[ng] = 0u;
-    }
-  }
-  return threadIdx.x == 0;
-}
 
 // alpha = rho / (w.q) from the total w.q.
 __device__ __forceinline__ void cg_alpha_finish(double *scal, double wq) {
@@ -301,56 +207,10 @@ struct CGVecs {
   const void *mass;
   double p;
   double qs;  // mass-term scalar when qf is NULL
-  // jdiag: the Jacobi diagonal inverse is formed in the kernels (uniform
-  // structured box, scalar p and q) from jA and the node's group size
-  int jdiag;
-  const T *jA;
 };
 
-// Assembled Jacobi diagonal of (p L + q M) at node (i,j,k) of a uniform
-// structured box whose group has cnt members: every copy contributes the
-// same element diagonal (k_helm_diag's terms: p (sum_d G_dd A(n_d) prod_o W)
-// Jac + q W_i W_j W_k Jac), so the sum is cnt times it.
-template <typename T>
-__device__ __forceinline__ T box_jacobi_inv(const T *__restrict__ A, const T *__restrict__ W,
-                                            int i, int j, int k, T g0, T g1, T g2, T jac, T p,
-                                            T q, int cnt) {
-  T lap = g0 * A[i] * (W[j] * W[k]) * jac;
-  lap += g1 * A[j] * (W[i] * W[k]) * jac;
-  lap += g2 * A[k] * (W[i] * W[j]) * jac;
-  T val = p * lap;
-  if (q != T(0)) val += q * (W[k] * (W[j] * W[i]) * jac);
-  return T(1) / (val * (T)cnt);
-}
 
-// Group size of node (i,j,k) of the element at box coordinates ec.
-__device__ __forceinline__ int box_count(const gf::BoxTopo &t, int N, const int (&ec)[3], int i,
-                                         int j, int k) {
-  const int nc[3] = {i, j, k};
-  int c = 1;
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const bool part = (nc[d] == 0 && (ec[d] > 0 || t.per[d])) ||
-                      (nc[d] == N - 1 && (ec[d] < t.ne[d] - 1 || t.per[d]));
-    c *= part ? 2 : 1;
-  }
-  return c;
-}
 
-// Inverse lumped mass at a node of a box of identical elements whose weight
-// table is symmetric (W_i == W_{N-1-i} bit for bit): every member of the
-// node's group then has the node's own mass, so the stored inverse mass --
-// k_mass_init's ((W_k W_j) W_i) Jac, GGFX Smooth (a sum from 0 over the
-// group, then / count) and k_recip's reciprocal -- follows from the group
-// size alone, in the same operations and with the same result.
-__device__ __forceinline__ double box_imass(double wk, double wj, double wi, double jac, int cnt) {
-  const double mn = ((wk * wj) * wi) * jac;
-  double s = mn;
-#pragma unroll
-  for (int q = 1; q < 8; q++)
-    if (q < cnt) s = s + mn;
-  return 1.0 / (s * gf::box_inv(cnt));
-}
 
 // Launchers of the 3-D element kernels (gf_k3d.hpp), one instantiation per
 // element order N = 2..10 in the gf_k3d_n*.hip translation units.
@@ -373,6 +233,6 @@ int cgapply3c_d(gf_mesh *m, const CGApply3Args &a);
 template <int N>
 int cgapply3c_f(gf_mesh *m, const CGApply3Args &a);
 template <typename T, int NORM, int N>
-int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part, const CGTail &tail);
+int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part);
 
 }  // namespace gfs

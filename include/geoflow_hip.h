@@ -241,14 +241,8 @@ size_t gf_mesh_ndof(const gf_mesh *mesh);
  * GF_MESH_BOX_GATHER  the structured-box gather assembly is in use;
  * GF_MESH_BOX_MASK    the boundary mask is formed from the box faces;
  * GF_MESH_CONST_D     3-D regular mesh whose D is the standard GLL matrix, so the
- *                     pencil kernels (D as compile-time constants) are used;
- * GF_MESH_BOX_UNIFORM structured box of bit-identical elements with a symmetric
- *                     weight table: the CG's Jacobi diagonal and the Burgers
- *                     stages' inverse mass are formed in the kernels;
- * GF_MESH_SIZE_BUCKETS the standalone GGFX operations run over the groups
- *                     bucketed by size (host maps, groups of at most 8). */
-enum { GF_MESH_BOX_GATHER = 1, GF_MESH_BOX_MASK = 2, GF_MESH_CONST_D = 4,
-       GF_MESH_BOX_UNIFORM = 8, GF_MESH_SIZE_BUCKETS = 16 };
+ *                     pencil kernels (D as compile-time constants) are used. */
+enum { GF_MESH_BOX_GATHER = 1, GF_MESH_BOX_MASK = 2, GF_MESH_CONST_D = 4 };
 int gf_mesh_flags(const gf_mesh *mesh);
 /* Copy a device-resident mesh array to host: "mass", "imass", "imult",
  * "mask" (ndof doubles each).  Synchronous. */

@@ -6,7 +6,9 @@ compiles that header for the host together with the product's grid and GGFX
 builders (gf_setup.cpp) and checks, node by node over periodic, Dirichlet and
 degenerate (one element per direction) boxes at orders 1-7, that the members
 equal the GGFX group in ascending order with 64- and 32-bit indices, and that
-the reported own slot holds the node itself.
+the reported own slot holds the node itself; at p = 7 it also fills the N = 8
+stage kernels' LDS halo (Halo8Load) lane by lane with global indices and checks
+that halo8_group's slots give every node's group in the same order.
 """
 import os
 import subprocess
@@ -24,4 +26,6 @@ def test_box_members_match_ggfx_groups(tmp_path):
     subprocess.run(cmd, check=True, capture_output=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count(" ok") == 13, r.stdout
+    assert r.stdout.count(" ok") == 20, r.stdout
+    # both kinds of N = 8 box occur: the halo order holds, and it does not
+    assert "halo_ok=1" in r.stdout and "p=7 ne=2,1,3" in r.stdout

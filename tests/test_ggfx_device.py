@@ -46,18 +46,13 @@ def test_device_maps_refuse_mixed_periodicity(ctx):
 @pytest.mark.gpu
 @pytest.mark.parametrize("bdy", [PER, DIR], ids=["periodic", "dirichlet"])
 def test_mesh_from_device_maps_matches_host_maps(ctx, bdy):
-    """Mass, inverse mass, multiplicity, DSS (size buckets against CSR), a CG
-    solve and (periodic) RK4 steps equal bit for bit between meshes built
-    from host and device maps."""
+    """Mass, inverse mass, multiplicity, DSS, a CG solve and (periodic) RK4
+    steps equal bit for bit between meshes built from host and device maps."""
     print("mesh device maps", bdy[0], flush=True)
     hg = ga.BoxGrid(3, 7, [3, 2, 2], bdy=bdy)
     dg = ga.BoxGrid(3, 7, [3, 2, 2], bdy=bdy, device_maps=True)
     mh, md = Mesh(ctx, hg), Mesh(ctx, dg)
-    # the host maps are also bucketed by group size (k_dss_bk), the device
-    # maps keep the CSR kernel (k_dss): the DSS results below must agree
-    bk = _capi.GF_MESH_SIZE_BUCKETS
-    assert mh.flags & bk and not md.flags & bk
-    assert mh.flags & ~bk == md.flags
+    assert mh.flags == md.flags
     for w in ("mass", "imass", "imult", "mask"):
         assert np.array_equal(mh.get(w), md.get(w)), w
     u = torch.from_numpy(workloads.unique_normal(hg, seed=3) + 0.1 * np.arange(hg.ndof) / hg.ndof)

@@ -318,21 +318,21 @@ def test_box_multistep_equals_single_steps(ctx, order):
         assert np.array_equal(host(a[c]), host(b[c])), c
 
 
-@pytest.mark.parametrize("ne", [[4, 3, 3], [4, 2, 2]], ids=["stored_imass", "uniform"])
+@pytest.mark.parametrize("ne", [[4, 3, 3], [4, 2, 2], [2, 1, 3]],
+                         ids=["halo_4x3x3", "halo_4x2x2", "per_node_gather"])
 def test_box_multistep_p7_against_oracle(ctx, ne):
     """The benchmark's own kernels at N=8 (p=7) against the oracle: one
     gf_burgers_steps call of n steps (first stage, middle stages, final-stage
     form, and for n >= 2 the output fused with the next step's first stage)
     on a periodic box, for n = 1..6, each compared with n RK4 steps of the
     oracle (GExRKStepper::step_b, gexrk_stepper.ipp:251-341) at 1e-12.  The
-    4x2x2 box has bit-identical elements (power-of-two sizes), so its stage
-    kernels form the inverse mass from the group size, as on the 32^3
-    benchmark box; the 4x3x3 box's read the stored one."""
+    4x3x3 and 4x2x2 boxes gather the previous stage derivative through the
+    LDS halo (partner offsets ordered x < y < z); the periodic 2x1x3 box,
+    whose y partners lie within the element, takes the per-node gather."""
     print("box multistep p7 vs oracle", ne, flush=True)
     c = dict(dim=3, p=7, ne=ne, bdy=[6] * 6)
     og, bg, m = make(ctx, c)
     assert m.flags & 1
-    assert bool(m.flags & _capi.GF_MESH_BOX_UNIFORM) == (ne == [4, 2, 2])
     us = workloads.fourier_field_3d(*bg.xyz, seed=21, kmax=3)
     ref = [u.copy() for u in us]
     nu, dt = 0.02, 2e-4
@@ -342,37 +342,6 @@ def test_box_multistep_p7_against_oracle(ctx, ne):
         m.burgers_steps(dus, nu, dt, n)
         for k in range(3):
             assert relerr(host(dus[k]), ref[k]) <= TOL, (n, k)
-
-
-@pytest.mark.parametrize("p,ne,bdy", [(7, [4, 2, 2], [6] * 6), (4, [2, 4, 2], [0] * 6),
-                                      (5, [1, 2, 4], [6] * 6), (3, [2, 2, 2], [0] * 6)])
-def test_box_imass_formula_equals_stored_imass(ctx, p, ne, bdy):
-    """On a box of identical elements the Burgers stage kernels form the
-    inverse mass from the group size (box_imass): ((W_k W_j) W_i) Jac summed
-    count times from 0, divided by the count, reciprocal.  The same
-    operations in numpy (IEEE double, no contraction possible: products,
-    sums and divisions only) equal the stored inverse mass -- k_mass_init,
-    GGFX Smooth, k_recip on the device -- bit for bit at every node."""
-    print("box imass", p, ne, flush=True)
-    bg = ga.BoxGrid(3, p, ne, bdy=bdy)
-    m = Mesh(ctx, bg)
-    # element sizes that are powers of two: bit-identical elements, so the
-    # kernels take the formula path
-    assert m.flags & _capi.GF_MESH_BOX_UNIFORM
-    W = bg.basis[0][1]
-    N = p + 1
-    cnt = np.rint(1.0 / m.get("imult")).astype(int)
-    jac = bg.ejac[0]
-    r = np.arange(bg.ndof) % (N ** 3)
-    i, j, k = r % N, (r // N) % N, r // (N * N)
-    mn = ((W[k] * W[j]) * W[i]) * jac
-    s = mn.copy()
-    for q in range(1, 8):
-        s = np.where(q < cnt, s + mn, s)
-    inv = {1: 1.0, 2: 0.5, 4: 0.25, 8: 0.125}
-    assert set(np.unique(cnt)) <= set(inv)
-    im = 1.0 / (s * np.vectorize(inv.get)(cnt))
-    assert np.array_equal(im, m.get("imass"))
 
 
 @pytest.mark.parametrize("bdy", [[0] * 6, [6] * 6], ids=["dirichlet", "periodic"])
