@@ -531,6 +531,8 @@ def main():
     traffic = measured_traffic("k_burgers_stage3b<%d, true, 2, false>" % (p + 1)) if boxed else None
     peak_meas = copy_peak(torch)
     log("measured copy bandwidth %.0f GB/s" % peak_meas)
+    peaks_tf = {w: ctx.peak(w) for w in ("mfma_f32", "mfma_f64", "valu_f64")}
+    log("measured peaks: " + ", ".join("%s %.1f TFLOP/s" % kv for kv in peaks_tf.items()))
     out["roofline"] = {"kernel": stage_name,
                        "bound": "hbm", "achieved": stage_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": stage_gbs / HBM_PEAK_GBS, "traffic": traffic,
@@ -540,6 +542,10 @@ def main():
                        "step_frac": step_gbs / HBM_PEAK_GBS,
                        "measured_peak": peak_meas,
                        "measured_peak_source": "torch copy_ of 134 MB (read + write), HIP events",
+                       "measured_compute_peaks_tflops": peaks_tf,
+                       "measured_compute_peaks_source": "gf_peak_probe: 4 waves/SIMD of "
+                       "independent v_mfma_f32_32x32x2f32 / v_mfma_f64_16x16x4f64 / v_fma_f64 "
+                       "chains on every CU, HIP events",
                        "frac_of_measured": stage_gbs / peak_meas,
                        "step_frac_of_measured": step_gbs / peak_meas}
     del us, m

@@ -137,6 +137,7 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_int), _vp, _vp,
                                           ctypes.POINTER(ctypes.c_size_t)]),
     "gf_integrate_noalloc": (ctypes.c_int, [_vp, _vp, _vp]),
+    "gf_peak_probe": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
     "gf_helmholtz_apply": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp,
                                           _vp]),
     "gf_helmholtz_apply_f32": (ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]),

@@ -369,24 +369,25 @@ __device__ __forceinline__ int64_t halo8_src(const BoxElemDeltas &de, int lane, 
 
 // Issues the 17 loads of field u for the element at ebase (8 own column
 // values of the lane, 9 halo values) into registers; store() writes them to
-// the LDS halo hl (kHalo8 doubles, zero slot included).
+// the LDS halo hl (kHalo8 values, zero slot included).
+template <typename T>
 struct Halo8Load {
-  double own[8], hv[9];
+  T own[8], hv[9];
   int slot[9];
-  __device__ __forceinline__ void issue(const double *__restrict__ u, const BoxElemDeltas &de,
+  __device__ __forceinline__ void issue(const T *__restrict__ u, const BoxElemDeltas &de,
                                         int64_t ebase, int lane) {
 #pragma unroll
     for (int k = 0; k < 8; k++) own[k] = u[ebase + lane + 64 * k];
 #pragma unroll
     for (int q = 0; q < 9; q++) hv[q] = u[ebase + halo8_src(de, lane, q, &slot[q])];
   }
-  __device__ __forceinline__ void store(double *hl, int lane) const {
+  __device__ __forceinline__ void store(T *hl, int lane) const {
 #pragma unroll
     for (int k = 0; k < 8; k++) hl[lane + 64 * k] = own[k];
 #pragma unroll
     for (int q = 0; q < 9; q++)
       if (slot[q] >= 0) hl[slot[q]] = hv[q];
-    if (lane == 0) hl[kHalo8Zero] = 0.0;
+    if (lane == 0) hl[kHalo8Zero] = T(0);
   }
 };
 

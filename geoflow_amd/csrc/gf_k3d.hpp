@@ -281,7 +281,7 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
     const gf::Halo8Col hc(de, t.i, t.j);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      gf::Halo8Load Lk, Lu;
+      gf::Halo8Load<double> Lk, Lu;
       Lk.issue(kp[c], de, ebase, lane);
       if (GU) {
         Lu.issue(uin[c], de, ebase, lane);

@@ -52,6 +52,12 @@ class Context:
     def sync(self):
         check(lib().gf_ctx_synchronize(self.h), self.h)
 
+    def peak(self, which):
+        """Measured TFLOP/s of "mfma_f32", "mfma_f64" or "valu_f64"."""
+        out = ctypes.c_double()
+        check(lib().gf_peak_probe(self.h, which.encode(), ctypes.byref(out)), self.h)
+        return out.value
+
     def __del__(self):
         try:
             lib().gf_ctx_destroy(self.h)

@@ -317,6 +317,13 @@ int gf_integrate(gf_mesh *mesh, const double *u, double *result);
  * streams, two host threads) race; calls ordered on one stream do not. */
 int gf_integrate_noalloc(gf_mesh *mesh, const double *u, double *result);
 
+/* Measured throughput peak (no reference counterpart; for the roofline
+ * report): which = "mfma_f32" (v_mfma_f32_32x32x2f32), "mfma_f64"
+ * (v_mfma_f64_16x16x4f64) or "valu_f64" (vector FMA), four waves per SIMD of
+ * independent chains on every CU; *tflops receives the achieved TFLOP/s.
+ * Synchronous; allocates a small output buffer. */
+int gf_peak_probe(gf_ctx *ctx, const char *which, double *tflops);
+
 /* GGrid::dealias (src/cdg/grid/ggrid.ipp:2351-2503): the quadratic
  * dealiasing of the product v1 v2 -- both interpolated to the order-pq GLL
  * grid of each element, multiplied there with the order-pq weights and

@@ -102,7 +102,7 @@ int main() {
         const int64_t ebase = (int64_t)e * 512;
         for (double &h : hl) h = -1.0;
         for (int lane = 0; lane < 64; lane++) {
-          gf::Halo8Load L;
+          gf::Halo8Load<double> L;
           L.issue(gid.data(), de, ebase, lane);
           L.store(hl.data(), lane);
         }
