@@ -519,16 +519,18 @@ def main():
     # the stage kernel the step runs: the structured-box form (previous K
     # assembled by gather inside the stage) when the mesh allows it
     boxed = bool(m.flags & ga._capi.GF_MESH_BOX_GATHER)
+    halo = bool(m.flags & ga._capi.GF_MESH_BOX_HALO)
     stage_ms = m.time_kernel("burgers_stage_box" if boxed else "burgers_stage", reps=20)
-    stage_name = ("k_burgers_stage3b<8> (structured-box RK stage, K gathered in-kernel; "
-                  "middle-stage form)" if boxed else
+    stage_name = ("k_burgers_stage3b<8> (structured-box RK stage, K gathered in-kernel%s; "
+                  "middle-stage form)" % (" through the LDS halo" if halo else "") if boxed else
                   "k_burgers_stage3p<8> (fused RK stage; middle-stage form)")
     stage_bytes = 5 * 8 * 3 * ndof  # middle-stage form: read uin,kprev,acc; write acc,k (per comp)
     stage_gbs = stage_bytes / (stage_ms * 1e-3) / 1e9
     step_bytes = 16 * 8 * 3 * ndof
     step_gbs = step_bytes / (el / args.steps) / 1e9
     out.update({"value": dofsteps, "unit": "DOF-steps/s", "ms_per_step": 1e3 * el / args.steps})
-    traffic = measured_traffic("k_burgers_stage3b<%d, true, 2, false>" % (p + 1)) if boxed else None
+    traffic = (measured_traffic("k_burgers_stage3b<%d, true, 2, false, %s>"
+                                % (p + 1, "true" if halo else "false")) if boxed else None)
     peak_meas = copy_peak(torch)
     log("measured copy bandwidth %.0f GB/s" % peak_meas)
     peaks_tf = {w: ctx.peak(w) for w in ("mfma_f32", "mfma_f64", "valu_f64")}

@@ -1088,7 +1088,7 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
 int gf_mesh_flags(const gf_mesh *m) {
   if (!m) return 0;
   return (m->has_box ? GF_MESH_BOX_GATHER : 0) | (m->has_box_mask ? GF_MESH_BOX_MASK : 0) |
-         (m->dconst ? GF_MESH_CONST_D : 0);
+         (m->dconst ? GF_MESH_CONST_D : 0) | (m->halo8 ? GF_MESH_BOX_HALO : 0);
 }
 
 int gf_mesh_destroy(gf_mesh *m) {

@@ -241,8 +241,12 @@ size_t gf_mesh_ndof(const gf_mesh *mesh);
  * GF_MESH_BOX_GATHER  the structured-box gather assembly is in use;
  * GF_MESH_BOX_MASK    the boundary mask is formed from the box faces;
  * GF_MESH_CONST_D     3-D regular mesh whose D is the standard GLL matrix, so the
- *                     pencil kernels (D as compile-time constants) are used. */
-enum { GF_MESH_BOX_GATHER = 1, GF_MESH_BOX_MASK = 2, GF_MESH_CONST_D = 4 };
+ *                     pencil kernels (D as compile-time constants) are used;
+ * GF_MESH_BOX_HALO    p = 7 structured box whose RK stages gather the previous
+ *                     stage derivative through an LDS halo of each element
+ *                     (same results as the per-node gather). */
+enum { GF_MESH_BOX_GATHER = 1, GF_MESH_BOX_MASK = 2, GF_MESH_CONST_D = 4,
+       GF_MESH_BOX_HALO = 8 };
 int gf_mesh_flags(const gf_mesh *mesh);
 /* Copy a device-resident mesh array to host: "mass", "imass", "imult",
  * "mask" (ndof doubles each).  Synchronous. */

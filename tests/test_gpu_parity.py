@@ -333,6 +333,7 @@ def test_box_multistep_p7_against_oracle(ctx, ne):
     c = dict(dim=3, p=7, ne=ne, bdy=[6] * 6)
     og, bg, m = make(ctx, c)
     assert m.flags & 1
+    assert bool(m.flags & _capi.GF_MESH_BOX_HALO) == (ne != [2, 1, 3])
     us = workloads.fourier_field_3d(*bg.xyz, seed=21, kmax=3)
     ref = [u.copy() for u in us]
     nu, dt = 0.02, 2e-4
