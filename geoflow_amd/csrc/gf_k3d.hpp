@@ -489,6 +489,55 @@ __global__ __launch_bounds__(kThreads) void k_rk_out_box(
   }
 }
 
+// Step output on an N = 8 box through the LDS halo (gf::Halo8Load): one
+// element per 64-thread block; per component the element's K and all its
+// partner values are loaded at once (17 loads per lane) with the
+// accumulator, and each node sums its group from LDS in canonical order
+// (gf::halo8_order) -- k_rk_out_box's operations, the same result bit for
+// bit, with every K value fetched once instead of once per group copy.
+template <int N>
+__global__ __launch_bounds__(64) void k_rk_out_halo8(
+    double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2,
+    const double *__restrict__ a0, const double *__restrict__ a1, const double *__restrict__ a2,
+    const double *__restrict__ k0, const double *__restrict__ k1, const double *__restrict__ k2,
+    double cl, gf::BoxTopo topo, size_t nelems) {
+  __shared__ double hk[gf::kHalo8];
+  double *u[3] = {u0, u1, u2};
+  const double *acc[3] = {a0, a1, a2};
+  const double *kf[3] = {k0, k1, k2};
+  const size_t e = xcd_block();
+  if (e >= nelems) return;  // the whole block
+  const int lane = (int)threadIdx.x;
+  int ec[3];
+  gf::box_elem_coords(topo, e, ec);
+  static_assert(N == 8, "the LDS halo serves N = 8");
+  const gf::BoxElemDeltas de = gf::box_elem_deltas(topo, 8, ec);
+  const gf::Halo8Col hc(de, lane & 7, lane >> 3);
+  const int64_t ebase = (int64_t)e * 512;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    gf::Halo8Load<double> L;
+    L.issue(kf[c], de, ebase, lane);
+    double av[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) av[k] = acc[c][ebase + lane + 64 * k];
+    __syncthreads();  // the previous component's sums have read LDS
+    L.store(hk, lane);
+    __syncthreads();
+    gf::static_for<0, 8>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      constexpr int NT = (k == 0 || k == 7) ? 8 : 4;  // terms that can be non-zero
+      int ad[8];
+      const int cnt = gf::halo8_order<k>(hc, de, ad);
+      double sk = 0.0;
+#pragma unroll
+      for (int m = 0; m < NT; m++) sk = sk + hk[ad[m]];
+      const double ks = cnt == 1 ? L.own[k] : sk * gf::box_inv(cnt);
+      u[c][ebase + lane + 64 * k] = 1.0 * av[k] + cl * ks;
+    });
+  }
+}
+
 template <int N, bool KP, int ACC, bool SMV, bool HL>
 void launch_stage3b_h(gf_mesh *m, int nb, double *const *uin, double *const *kp,
                       double *const *acc, double *const *kout, double bh, double ch, double nu) {
@@ -542,6 +591,15 @@ int launch_stage3b(gf_mesh *m, double *const *uin, double *const *kp, double *co
 template <int N>
 int launch_rk_out_box(gf_mesh *m, double *const *u, double *const *acc, double *const *kf,
                       double cl) {
+  if constexpr (N == 8) {
+    if (m->halo8) {
+      hipLaunchKernelGGL(k_rk_out_halo8<N>, dim3(xcd_grid(m->nelems)), dim3(64), 0, m->ctx->stream,
+                         u[0], u[1], u[2], acc[0], acc[1], acc[2], kf[0], kf[1], kf[2], cl,
+                         m->box, m->nelems);
+      GF_LAUNCH_CHECK(m->ctx);
+      return GF_OK;
+    }
+  }
   constexpr int NN = N * N * N;
   constexpr int CE = NN >= kThreads ? 1 : kThreads / NN;
   size_t nchunks = (m->nelems + CE - 1) / CE;
