@@ -279,19 +279,30 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
     const int64_t ebase = (int64_t)t.e * 512;
     double *hk = sl, *hu = sl + gf::kHalo8;
     const gf::Halo8Col hc(de, t.i, t.j);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      gf::Halo8Load<double> Lk, Lu;
-      Lk.issue(kp[c], de, ebase, lane);
+    // Two register sets: component c + 1's loads are issued before component
+    // c's values are stored to LDS and summed, so they are in flight meanwhile
+    // (not where uin's halo is loaded too: two such sets would spill).
+    constexpr bool PIPE = !GU;
+    gf::Halo8Load<double> LK[2], LU[2];
+    double AO[2][8];
+    auto issue = [&](int c, int b) {
+      LK[b].issue(kp[c], de, ebase, lane);
       if (GU) {
-        Lu.issue(uin[c], de, ebase, lane);
+        LU[b].issue(uin[c], de, ebase, lane);
       } else {
 #pragma unroll
-        for (int k = 0; k < 8; k++) Lu.own[k] = uin[c][col + (size_t)k * 64];
+        for (int k = 0; k < 8; k++) LU[b].own[k] = uin[c][col + (size_t)k * 64];
       }
-      double ao[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) ao[k] = ACC == 2 ? acc[c][col + (size_t)k * 64] : 0.0;
+      for (int k = 0; k < 8; k++) AO[b][k] = ACC == 2 ? acc[c][col + (size_t)k * 64] : 0.0;
+    };
+    if (PIPE) issue(0, 0);
+    gf::static_for<0, 3>([&](auto cc) {
+      constexpr int c = decltype(cc)::value, b = PIPE ? c & 1 : 0;
+      if constexpr (!PIPE) issue(c, b);
+      else if constexpr (c < 2) issue(c + 1, b ^ 1);
+      const gf::Halo8Load<double> &Lk = LK[b], &Lu = LU[b];
+      const double(&ao)[8] = AO[b];
       __syncthreads();  // the previous component's group sums have read LDS
       Lk.store(hk, lane);
       if (GU) Lu.store(hu, lane);
@@ -329,7 +340,7 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
         if (ACC && t.active) acc[c][col + (size_t)k * 64] = av;
         ru[c][k] = v;
       });
-    }
+    });
     __syncthreads();  // the element buffers below reuse the halo area
   } else {
   // Stage input (and accumulator) per node; all loads are issued before any
@@ -514,13 +525,21 @@ __global__ __launch_bounds__(64) void k_rk_out_halo8(
   const gf::BoxElemDeltas de = gf::box_elem_deltas(topo, 8, ec);
   const gf::Halo8Col hc(de, lane & 7, lane >> 3);
   const int64_t ebase = (int64_t)e * 512;
+  // two register sets: component c + 1's loads are in flight during
+  // component c's sums
+  gf::Halo8Load<double> LS[2];
+  double AV[2][8];
+  auto issue = [&](int c, int b) {
+    LS[b].issue(kf[c], de, ebase, lane);
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    gf::Halo8Load<double> L;
-    L.issue(kf[c], de, ebase, lane);
-    double av[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) av[k] = acc[c][ebase + lane + 64 * k];
+    for (int k = 0; k < 8; k++) AV[b][k] = acc[c][ebase + lane + 64 * k];
+  };
+  issue(0, 0);
+  gf::static_for<0, 3>([&](auto cc) {
+    constexpr int c = decltype(cc)::value, b = c & 1;
+    if constexpr (c < 2) issue(c + 1, b ^ 1);
+    const gf::Halo8Load<double> &L = LS[b];
+    const double(&av)[8] = AV[b];
     __syncthreads();  // the previous component's sums have read LDS
     L.store(hk, lane);
     __syncthreads();
@@ -535,7 +554,7 @@ __global__ __launch_bounds__(64) void k_rk_out_halo8(
       const double ks = cnt == 1 ? L.own[k] : sk * gf::box_inv(cnt);
       u[c][ebase + lane + 64 * k] = 1.0 * av[k] + cl * ks;
     });
-  }
+  });
 }
 
 template <int N, bool KP, int ACC, bool SMV, bool HL>
