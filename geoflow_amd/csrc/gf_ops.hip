@@ -515,10 +515,14 @@ __global__ void k_narrow_f32(float *__restrict__ o, const double *__restrict__ a
 // in ascending local id and accumulated from zero, the reference's
 // std::accumulate order (ggfx.hpp:60-76, 401-405), then written back to every
 // member.  Singleton groups are identities and are skipped.
+// The grid is a multiple of 8 and its blocks are renumbered so that each of
+// the 8 XCDs (each with its own L2) walks one contiguous range of groups:
+// groups of neighbouring indices share elements, and so cache lines.
 template <int OP, typename T>
 __global__ void k_dss(T *__restrict__ u, const int *__restrict__ goff,
                       const int *__restrict__ gmem, int ngroups) {
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += gridDim.x * blockDim.x) {
+  const int bx = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  for (int g = bx * blockDim.x + threadIdx.x; g < ngroups; g += gridDim.x * blockDim.x) {
     int b = goff[g], e = goff[g + 1];
     if (e - b <= 8) {
       // groups of at most 8 (boxes, the sphere): every member index and
@@ -690,7 +694,7 @@ int launch_mass_init(gf_mesh *m) {
 // Internal entry points used by other translation units.
 int gf_dss_t(gf_mesh *m, int op, double *u) {
   if (m->ngroups == 0) return GF_OK;
-  int nb = nblocks((size_t)m->ngroups);
+  const int nb = (nblocks((size_t)m->ngroups) + 7) / 8 * 8;  // k_dss: a multiple of 8
   hipStream_t s = m->ctx->stream;
   switch (op) {
     case GF_DSS_SUM: hipLaunchKernelGGL((k_dss<GF_DSS_SUM, double>), dim3(nb), dim3(kThreads), 0, s, u, m->goff, m->gmem, m->ngroups); break;
@@ -705,7 +709,7 @@ int gf_dss_t(gf_mesh *m, int op, double *u) {
 
 int gf_dss_f32(gf_mesh *m, int op, float *u) {
   if (m->ngroups == 0) return GF_OK;
-  int nb = nblocks((size_t)m->ngroups);
+  const int nb = (nblocks((size_t)m->ngroups) + 7) / 8 * 8;  // k_dss: a multiple of 8
   hipStream_t s = m->ctx->stream;
   if (op == GF_DSS_SUM)
     hipLaunchKernelGGL((k_dss<GF_DSS_SUM, float>), dim3(nb), dim3(kThreads), 0, s, u, m->goff, m->gmem, m->ngroups);

@@ -1139,6 +1139,8 @@ struct EmbAdvArgs {
   const T *kprev;       // smoothed K of the previous stage, or null
   T *ust;               // state, advanced by ch * kprev when kprev is set
   T *ucopy;             // receives uin when kprev is null (step-start copy)
+  int acc_from_uin;     // the accumulation of kprev starts from the stage's uin
+                        // value instead of ust (the step's first accumulation)
   double bh, ch, nu;
   const double *c[3];   // velocity (read when wform is set)
   const double *ndx[6]; // dXi_i/dX_j at [i*3+j]
@@ -1173,7 +1175,8 @@ __global__ __launch_bounds__(kThreads) void k_adv_stage_embed(EmbAdvArgs<T> a) {
     T v = a.uin[g];
     if (a.kprev) {
       const T kp = a.kprev[g];
-      a.ust[g] = a.ust[g] + (T)a.ch * kp;
+      const T base = a.acc_from_uin ? v : a.ust[g];
+      a.ust[g] = base + (T)a.ch * kp;
       v = v + (T)a.bh * kp;
     } else if (a.ucopy) {
       a.ucopy[g] = v;
@@ -1253,7 +1256,7 @@ template <typename T>
 __global__ void k_final_input_t(T *__restrict__ u3, const T *__restrict__ uin,
                                 const T *__restrict__ k, T bh, T *acc, const T *accsrc, T ch,
                                 size_t n) {
-  GRID_LOOP(g, n) {
+  XCD_GRID_LOOP(g, n) {
     T kv = k[g];
     u3[g] = uin[g] + bh * kv;
     if (acc) acc[g] = accsrc[g] + kv * ch;
@@ -1262,26 +1265,30 @@ __global__ void k_final_input_t(T *__restrict__ u3, const T *__restrict__ uin,
 template <typename T>
 __global__ void k_axpby_t(T *__restrict__ z, T a, const T *__restrict__ x, T b, const T *y,
                           size_t n) {
-  GRID_LOOP(g, n) z[g] = a * x[g] + b * y[g];
+  XCD_GRID_LOOP(g, n) z[g] = a * x[g] + b * y[g];
 }
 int dss_smooth(gf_mesh *m, double *u) { return gf_dss_t(m, GF_DSS_SMOOTH, u); }
 int dss_smooth(gf_mesh *m, float *u) { return gf_dss_f32(m, GF_DSS_SMOOTH, u); }
 
-// Pure-advection RK step on a 2-D embedded isotropic mesh with three velocity
+// Pure-advection RK steps on a 2-D embedded isotropic mesh with three velocity
 // components (mode 1, nstate 4): the arithmetic of rk_step_general with each
 // stage's RHS, stage input and state accumulation fused into one element
 // kernel.  Stage sequence (step_b, gexrk_stepper.ipp:251-341):
 //   s < ns-1:  K_s = F(uin + b_{s,s-1} h K_{s-1});  Smooth K_s;  ust += c_s h K_s
 //   last:      us = uin + b h K_{ns-2};  Smooth us;  K = F(us);  Smooth K;
 //              ust += c h K;  Smooth ust
-// where the ust accumulation of K_{s-1} is done by stage s's kernel.  The
-// Butcher tables of orders 1-4 have only sub-diagonal b, so the reference's
-// isum = sum_j b_{sj} h K_j equals b_{s,s-1} h K_{s-1} exactly.  T = float
-// is the mixed-precision form: the advected field and its stage fields in
-// FP32, the metric terms in FP64 (k_adv_stage_embed).
+// where the ust accumulation of K_{s-1} is done by stage s's kernel, the
+// first one from the step-start copy of uin (equal to ust at that point).
+// The Butcher tables of orders 1-4 have only sub-diagonal b, so the
+// reference's isum = sum_j b_{sj} h K_j equals b_{s,s-1} h K_{s-1} exactly.
+// (Forming the Smooths by gather in the reading kernels instead of the DSS
+// passes, from a per-node partner map, measured slower on the sphere: the
+// dependent partner loads cost each kernel more than the pass they save.)
+// T = float is the mixed-precision form: the advected field and its stage
+// fields in FP32, the metric terms in FP64 (k_adv_stage_embed).
 template <typename T>
-int rk_step_embed_adv_t(gf_mesh *m, double nu, int order, double dt, T *ust0,
-                        const double *const *vel) {
+int rk_steps_embed_adv_t(gf_mesh *m, double nu, int order, double dt, T *ust0,
+                         const double *const *vel, int nsteps) {
   const size_t n = m->ndof, bytes = n * sizeof(T), dbytes = n * sizeof(double);
   double alpha[4], beta[16], c[4];
   butcher(order, alpha, beta, c);
@@ -1306,44 +1313,52 @@ int rk_step_embed_adv_t(gf_mesh *m, double nu, int order, double dt, T *ust0,
   a.w[0] = w0;
   a.w[1] = w1;
   a.nelems = m->nelems;
-  for (int s = 0; s < ns - 1; s++) {
-    a.wform = s == 0;
-    a.uin = s == 0 ? ust0 : ucopy;
-    a.kprev = s == 0 ? nullptr : K[(s - 1) % 2];
-    a.ust = ust0;
-    a.ucopy = s == 0 ? ucopy : nullptr;
-    a.bh = s == 0 ? 0.0 : beta[s * 4 + (s - 1)] * h;
-    a.ch = s == 0 ? 0.0 : c[s - 1] * h;
-    a.kout = K[s % 2];
+  for (int step = 0; step < nsteps; step++) {
+    for (int s = 0; s < ns - 1; s++) {
+      a.wform = s == 0;
+      a.uin = s == 0 ? ust0 : ucopy;
+      a.acc_from_uin = s == 1;
+      a.kprev = s == 0 ? nullptr : K[(s - 1) % 2];
+      a.ust = ust0;
+      a.ucopy = s == 0 ? ucopy : nullptr;
+      a.bh = s == 0 ? 0.0 : beta[s * 4 + (s - 1)] * h;
+      a.ch = s == 0 ? 0.0 : c[s - 1] * h;
+      a.kout = K[s % 2];
+      RC(launch_adv_stage_embed_t(m, a));
+      RC(dss_smooth(m, K[s % 2]));
+    }
+    if (ns >= 2) {
+      // order 2: the step's first accumulation, from the step-start copy
+      hipLaunchKernelGGL(k_final_input_t<T>, dim3(xcd_grid(nblocks(n))), dim3(kThreads), 0,
+                         m->ctx->stream, us, ucopy, K[(ns - 2) % 2],
+                         (T)(beta[(ns - 1) * 4 + (ns - 2)] * h), ust0, ns == 2 ? ucopy : ust0,
+                         (T)(c[ns - 2] * h), n);
+      GF_LAUNCH_CHECK(m->ctx);
+    } else {
+      GF_HIP(m->ctx, hipMemcpyAsync(us, ust0, bytes, hipMemcpyDeviceToDevice, m->ctx->stream));
+    }
+    RC(dss_smooth(m, us));
+    a.uin = us;
+    a.acc_from_uin = 0;
+    a.kprev = nullptr;
+    a.ucopy = nullptr;
+    a.wform = ns == 1;
+    a.bh = a.ch = 0.0;
+    a.kout = K[(ns - 1) % 2];
     RC(launch_adv_stage_embed_t(m, a));
-    RC(dss_smooth(m, K[s % 2]));
-  }
-  if (ns >= 2) {
-    hipLaunchKernelGGL(k_final_input_t<T>, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, us,
-                       ucopy, K[(ns - 2) % 2], (T)(beta[(ns - 1) * 4 + (ns - 2)] * h), ust0, ust0,
-                       (T)(c[ns - 2] * h), n);
+    T *kl = K[(ns - 1) % 2];
+    RC(dss_smooth(m, kl));
+    hipLaunchKernelGGL(k_axpby_t<T>, dim3(xcd_grid(nblocks(n))), dim3(kThreads), 0,
+                       m->ctx->stream, ust0, (T)1, ust0, (T)(c[ns - 1] * h), kl, n);
     GF_LAUNCH_CHECK(m->ctx);
-  } else {
-    GF_HIP(m->ctx, hipMemcpyAsync(us, ust0, bytes, hipMemcpyDeviceToDevice, m->ctx->stream));
+    RC(dss_smooth(m, ust0));
   }
-  RC(dss_smooth(m, us));
-  a.uin = us;
-  a.kprev = nullptr;
-  a.ucopy = nullptr;
-  a.wform = ns == 1;
-  a.bh = a.ch = 0.0;
-  a.kout = K[(ns - 1) % 2];
-  RC(launch_adv_stage_embed_t(m, a));
-  T *kl = K[(ns - 1) % 2];
-  RC(dss_smooth(m, kl));
-  hipLaunchKernelGGL(k_axpby_t<T>, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, ust0, (T)1,
-                     ust0, (T)(c[ns - 1] * h), kl, n);
-  GF_LAUNCH_CHECK(m->ctx);
-  return dss_smooth(m, ust0);
+  return GF_OK;
 }
 
-int rk_step_embed_adv(gf_mesh *m, double nu, int order, double dt, double *const *ust) {
-  return rk_step_embed_adv_t<double>(m, nu, order, dt, ust[0], ust + 1);
+int rk_step_embed_adv(gf_mesh *m, double nu, int order, double dt, double *const *ust,
+                      int nsteps) {
+  return rk_steps_embed_adv_t<double>(m, nu, order, dt, ust[0], ust + 1, nsteps);
 }
 
 // General RK step composed from the operators, following step_b literally
@@ -1442,7 +1457,7 @@ int gf_burgers_step(gf_mesh *m, int mode, double nu, int order, int nstate, doub
   else if (fused)
     rc = rk_step_fused(m, nu, order, dt, u);
   else if (embed_adv)
-    rc = rk_step_embed_adv(m, nu, order, dt, u);
+    rc = rk_step_embed_adv(m, nu, order, dt, u, 1);
   else
     rc = rk_step_general(m, mode, nu, order, nstate, dt, u);
   if (rc) return rc;
@@ -1458,6 +1473,12 @@ int gf_burgers_steps(gf_mesh *m, int mode, double nu, int order, int nstate, dou
     for (int k = 0; k < nstate; k++)
       if (!u[k]) return GF_ERR_ARG;
     return rk_steps_box(m, nu, order, dt, nsteps, u);
+  }
+  if (mode == 1 && m->gtype == GF_GE_2DEMBEDDED && m->Niso && nstate == 4 && order >= 1 &&
+      order <= 4) {
+    for (int k = 0; k < nstate; k++)
+      if (!u[k]) return GF_ERR_ARG;
+    return rk_step_embed_adv(m, nu, order, dt, u, nsteps);
   }
   for (int s = 0; s < nsteps; s++) {
     int rc = gf_burgers_step(m, mode, nu, order, nstate, t + s * dt, dt, u, 0);
@@ -1480,7 +1501,7 @@ int gf_burgers_steps_mixed(gf_mesh *m, int mode, double nu, int order, int nstat
   if (!h32) return gf_set_err(m->ctx, GF_ERR_ALLOC, "workspace");
   hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, h32, u[0], 1.0,
                      n);
-  for (int s = 0; s < nsteps; s++) RC(rk_step_embed_adv_t<float>(m, nu, order, dt, h32, u + 1));
+  RC(rk_steps_embed_adv_t<float>(m, nu, order, dt, h32, u + 1, nsteps));
   hipLaunchKernelGGL(k_from_f32, dim3(nblocks(n)), dim3(kThreads), 0, m->ctx->stream, u[0], h32, n);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;

@@ -78,6 +78,12 @@ __device__ __forceinline__ size_t xcd_block() {
   return (size_t)(blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
 }
 
+// Grid-stride loop over [0, n) with the blocks renumbered by xcd_block (the
+// grid a multiple of kXCD): each XCD walks contiguous ranges.
+#define XCD_GRID_LOOP(g, n)                                                 \
+  for (size_t g = xcd_block() * blockDim.x + threadIdx.x; g < (n);         \
+       g += (size_t)gridDim.x * blockDim.x)
+
 // Thread/element mapping shared by the 3-D element kernels.
 template <int N>
 struct ColIdx {
