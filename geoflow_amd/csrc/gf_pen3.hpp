@@ -19,6 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "gf_dtab.hpp"
 
 namespace gf {
@@ -39,6 +41,39 @@ template <typename P>
 __device__ __forceinline__ P opaque_ptr(P p) {
   asm volatile("" : "+s"(p));
   return p;
+}
+
+// y = A x for the GLL matrix A = D (TR false) or D^T (TR true) at even N in
+// FP32, through the even/odd split of a centro-antisymmetric matrix
+// (A(N-1-l, N-1-m) = -A(l, m), which D and D^T are): with
+// e_m = x_m + x_{N-1-m}, o_m = x_m - x_{N-1-m} (m < N/2) and
+// P(l, m) = (A(l, m) + A(l, N-1-m)) / 2, Q(l, m) = (A(l, m) - A(l, N-1-m)) / 2,
+//   y_l = E_l + O_l,  y_{N-1-l} = O_l - E_l  (l < N/2),
+//   E_l = sum_m P(l, m) e_m,  O_l = sum_m Q(l, m) o_m,
+// N^2/2 multiply-adds and 2N adds instead of N^2 multiply-adds.  P and Q
+// are formed in FP64 from the table and rounded once.
+template <int N, bool TR>
+__device__ __forceinline__ void gll_apply_eo(const float (&x)[N], float (&y)[N]) {
+  static_assert(N % 2 == 0, "even N");
+  constexpr int H = N / 2;
+  auto A = [](int l, int m) { return TR ? GllD<N>::at(m, l) : GllD<N>::at(l, m); };
+  float e[H], o[H];
+#pragma unroll
+  for (int m = 0; m < H; m++) {
+    e[m] = x[m] + x[N - 1 - m];
+    o[m] = x[m] - x[N - 1 - m];
+  }
+#pragma unroll
+  for (int l = 0; l < H; l++) {
+    float E = 0.0f, O = 0.0f;
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+      E = fmaf((float)(0.5 * (A(l, m) + A(l, N - 1 - m))), e[m], E);
+      O = fmaf((float)(0.5 * (A(l, m) - A(l, N - 1 - m))), o[m], O);
+    }
+    y[l] = E + O;
+    y[N - 1 - l] = O - E;
+  }
 }
 
 template <int N>
@@ -84,6 +119,9 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
                                         ZCB &&zcb, POST &&post) {
   using S = Pen3<N>;
   (void)Dp;
+  // FP32 at even N: the contractions in even/odd form (gll_apply_eo); FP64
+  // keeps the reference's ascending-m sums
+  constexpr bool EO = std::is_same<T, float>::value && N % 2 == 0;
   // D(l,m) and D(m,l) as compile-time constants
   auto dR = [](int l, int m) { return (T)GllD<N>::at(l, m); };
   auto dC = [](int l, int m) { return (T)GllD<N>::at(m, l); };
@@ -95,12 +133,17 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
   // z-part on the column, in registers
   T dz[N];
   {
-    T gz[N];
+    T gz[N], uzv[N];
+    if constexpr (EO) gll_apply_eo<N, false>(ru, uzv);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       T uz = T(0);
+      if constexpr (EO) {
+        uz = uzv[k];
+      } else {
 #pragma unroll
-      for (int m = 0; m < N; m++) uz = fma(dR(k, m), ru[m], uz);
+        for (int m = 0; m < N; m++) uz = fma(dR(k, m), ru[m], uz);
+      }
       const T M = W[k] * cab;
       zcb(k, uz, M);
       T f = uz * g2;
@@ -108,26 +151,35 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
       gz[k] = M * f;
       __builtin_amdgcn_sched_barrier(0);
     }
+    if constexpr (EO) {
+      gll_apply_eo<N, true>(gz, dz);
+    } else {
 #pragma unroll
-    for (int k = 0; k < N; k++) {
-      T c = T(0);
+      for (int k = 0; k < N; k++) {
+        T c = T(0);
 #pragma unroll
-      for (int m = 0; m < N; m++) c = fma(dC(k, m), gz[m], c);
-      dz[k] = c;
-      __builtin_amdgcn_sched_barrier(0);
+        for (int m = 0; m < N; m++) c = fma(dC(k, m), gz[m], c);
+        dz[k] = c;
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
   __syncthreads();
   // x-part on the x-pencil (j=a, k=b) -> sd
   {
-    T xp[N], f[N];
+    T xp[N], f[N], uxv[N];
 #pragma unroll
     for (int m = 0; m < N; m++) xp[m] = su[S::at(m, a, b)];
+    if constexpr (EO) gll_apply_eo<N, false>(xp, uxv);
 #pragma unroll
     for (int i = 0; i < N; i++) {
       T ux = T(0);
+      if constexpr (EO) {
+        ux = uxv[i];
+      } else {
 #pragma unroll
-      for (int m = 0; m < N; m++) ux = fma(dR(i, m), xp[m], ux);
+        for (int m = 0; m < N; m++) ux = fma(dR(i, m), xp[m], ux);
+      }
       const T M = W[i] * cab;
       const T hx = xcb(i, ux, M);
       if (HOOK) sx[S::at(i, a, b)] = hx;
@@ -136,26 +188,38 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
       f[i] = M * g;
       __builtin_amdgcn_sched_barrier(0);
     }
+    if constexpr (EO) {
+      T sv[N];
+      gll_apply_eo<N, true>(f, sv);
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-      T s = T(0);
+      for (int i = 0; i < N; i++) sd[S::at(i, a, b)] = sv[i];
+    } else {
 #pragma unroll
-      for (int m = 0; m < N; m++) s = fma(dC(i, m), f[m], s);
-      sd[S::at(i, a, b)] = s;
-      __builtin_amdgcn_sched_barrier(0);
+      for (int i = 0; i < N; i++) {
+        T s = T(0);
+#pragma unroll
+        for (int m = 0; m < N; m++) s = fma(dC(i, m), f[m], s);
+        sd[S::at(i, a, b)] = s;
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
   // y-part on the y-pencil (i=a, k=b), kept in registers until sd is complete
   T dy[N], hy[HOOK ? N : 1];
   {
-    T yp[N], f[N];
+    T yp[N], f[N], uyv[N];
 #pragma unroll
     for (int m = 0; m < N; m++) yp[m] = su[S::at(a, m, b)];
+    if constexpr (EO) gll_apply_eo<N, false>(yp, uyv);
 #pragma unroll
     for (int j = 0; j < N; j++) {
       T uy = T(0);
+      if constexpr (EO) {
+        uy = uyv[j];
+      } else {
 #pragma unroll
-      for (int m = 0; m < N; m++) uy = fma(dR(j, m), yp[m], uy);
+        for (int m = 0; m < N; m++) uy = fma(dR(j, m), yp[m], uy);
+      }
       const T M = W[j] * cab;
       const T h = ycb(j, uy, M);
       if (HOOK) hy[j] = h;
@@ -164,13 +228,17 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
       f[j] = M * g;
       __builtin_amdgcn_sched_barrier(0);
     }
+    if constexpr (EO) {
+      gll_apply_eo<N, true>(f, dy);
+    } else {
 #pragma unroll
-    for (int j = 0; j < N; j++) {
-      T s = T(0);
+      for (int j = 0; j < N; j++) {
+        T s = T(0);
 #pragma unroll
-      for (int m = 0; m < N; m++) s = fma(dC(j, m), f[m], s);
-      dy[j] = s;
-      __builtin_amdgcn_sched_barrier(0);
+        for (int m = 0; m < N; m++) s = fma(dC(j, m), f[m], s);
+        dy[j] = s;
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
   __syncthreads();
