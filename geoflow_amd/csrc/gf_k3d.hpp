@@ -180,7 +180,8 @@ void k_burgers_stage3p(
     auto post = [&](int k, double L, double M, double hx, double hy) {
       if (!t.active) return;
       const size_t g = col + k * S::N2;
-      // z-derivative again (same operands and order as in the z pass)
+      // z-derivative again (ascending-m sum: registers are not spent keeping
+      // the z pass's values)
       double uz = 0.0;
 #pragma unroll
       for (int m2 = 0; m2 < N; m2++) uz = fma(gf::GllD<N>::at(k, m2), ru[c][m2], uz);
@@ -412,7 +413,8 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
     auto post = [&](int k, double L, double M, double hx, double hy) {
       if (!t.active) return;
       const size_t g = col + k * S::N2;
-      // z-derivative again (same operands and order as in the z pass)
+      // z-derivative again (ascending-m sum: registers are not spent keeping
+      // the z pass's values)
       double uz = 0.0;
 #pragma unroll
       for (int m2 = 0; m2 < N; m2++) uz = fma(gf::GllD<N>::at(k, m2), ru[c][m2], uz);

@@ -19,8 +19,6 @@
 
 #include <hip/hip_runtime.h>
 
-#include <type_traits>
-
 #include "gf_dtab.hpp"
 
 namespace gf {
@@ -43,21 +41,23 @@ __device__ __forceinline__ P opaque_ptr(P p) {
   return p;
 }
 
-// y = A x for the GLL matrix A = D (TR false) or D^T (TR true) at even N in
-// FP32, through the even/odd split of a centro-antisymmetric matrix
+// y = A x for the GLL matrix A = D (TR false) or D^T (TR true) at even N,
+// through the even/odd split of a centro-antisymmetric matrix
 // (A(N-1-l, N-1-m) = -A(l, m), which D and D^T are): with
 // e_m = x_m + x_{N-1-m}, o_m = x_m - x_{N-1-m} (m < N/2) and
 // P(l, m) = (A(l, m) + A(l, N-1-m)) / 2, Q(l, m) = (A(l, m) - A(l, N-1-m)) / 2,
 //   y_l = E_l + O_l,  y_{N-1-l} = O_l - E_l  (l < N/2),
 //   E_l = sum_m P(l, m) e_m,  O_l = sum_m Q(l, m) o_m,
 // N^2/2 multiply-adds and 2N adds instead of N^2 multiply-adds.  P and Q
-// are formed in FP64 from the table and rounded once.
-template <int N, bool TR>
-__device__ __forceinline__ void gll_apply_eo(const float (&x)[N], float (&y)[N]) {
+// are formed in FP64 from the table (and rounded once to T).  At even N the
+// table's D is centro-antisymmetric bit for bit, so y is D x up to the
+// rounding of the different operation order (relative 1e-16 in FP64).
+template <int N, bool TR, typename T>
+__device__ __forceinline__ void gll_apply_eo(const T (&x)[N], T (&y)[N]) {
   static_assert(N % 2 == 0, "even N");
   constexpr int H = N / 2;
   auto A = [](int l, int m) { return TR ? GllD<N>::at(m, l) : GllD<N>::at(l, m); };
-  float e[H], o[H];
+  T e[H], o[H];
 #pragma unroll
   for (int m = 0; m < H; m++) {
     e[m] = x[m] + x[N - 1 - m];
@@ -65,11 +65,11 @@ __device__ __forceinline__ void gll_apply_eo(const float (&x)[N], float (&y)[N])
   }
 #pragma unroll
   for (int l = 0; l < H; l++) {
-    float E = 0.0f, O = 0.0f;
+    T E = T(0), O = T(0);
 #pragma unroll
     for (int m = 0; m < H; m++) {
-      E = fmaf((float)(0.5 * (A(l, m) + A(l, N - 1 - m))), e[m], E);
-      O = fmaf((float)(0.5 * (A(l, m) - A(l, N - 1 - m))), o[m], O);
+      E = fma((T)(0.5 * (A(l, m) + A(l, N - 1 - m))), e[m], E);
+      O = fma((T)(0.5 * (A(l, m) - A(l, N - 1 - m))), o[m], O);
     }
     y[l] = E + O;
     y[N - 1 - l] = O - E;
@@ -93,9 +93,10 @@ struct Pen3 {
 
 // Computes, for the column (i=a, j=b) owned by the calling thread,
 //   post(k, Lu_k, M_k, hx_k, hy_k),   Lu = ((x-part + y-part) + z-part)
-// where each part is D_d^T (M * ((D_d u) * G_d) [* p]) summed in ascending m,
-// i.e. the order of GHelmholtz::reg_prod / GGrid::compute_grefdiv
-// (ghelmholtz.ipp:308-378, ggrid.ipp:1847-1927), and M_k is the lumped mass
+// where each part is D_d^T (M * ((D_d u) * G_d) [* p]), the operations of
+// GHelmholtz::reg_prod / GGrid::compute_grefdiv (ghelmholtz.ipp:308-378,
+// ggrid.ipp:1847-1927), each contraction summed in ascending m at odd N and
+// in even/odd form at even N (gll_apply_eo), and M_k is the lumped mass
 // W_i W_j W_k Jac (gmass.ipp:200-210; formed here as W_k ((W_j W_i) Jac), which
 // rounds differently from the stored mass by at most an ulp or two).
 //   ru    the thread's column of u (nodes (a,b,0..N-1))
@@ -119,9 +120,9 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
                                         ZCB &&zcb, POST &&post) {
   using S = Pen3<N>;
   (void)Dp;
-  // FP32 at even N: the contractions in even/odd form (gll_apply_eo); FP64
-  // keeps the reference's ascending-m sums
-  constexpr bool EO = std::is_same<T, float>::value && N % 2 == 0;
+  // even N: the contractions in even/odd form (gll_apply_eo); odd N: the
+  // reference's ascending-m sums
+  constexpr bool EO = N % 2 == 0;
   // D(l,m) and D(m,l) as compile-time constants
   auto dR = [](int l, int m) { return (T)GllD<N>::at(l, m); };
   auto dC = [](int l, int m) { return (T)GllD<N>::at(m, l); };
@@ -134,7 +135,7 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
   T dz[N];
   {
     T gz[N], uzv[N];
-    if constexpr (EO) gll_apply_eo<N, false>(ru, uzv);
+    if constexpr (EO) gll_apply_eo<N, false, T>(ru, uzv);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       T uz = T(0);
@@ -152,7 +153,7 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
       __builtin_amdgcn_sched_barrier(0);
     }
     if constexpr (EO) {
-      gll_apply_eo<N, true>(gz, dz);
+      gll_apply_eo<N, true, T>(gz, dz);
     } else {
 #pragma unroll
       for (int k = 0; k < N; k++) {
@@ -170,7 +171,7 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
     T xp[N], f[N], uxv[N];
 #pragma unroll
     for (int m = 0; m < N; m++) xp[m] = su[S::at(m, a, b)];
-    if constexpr (EO) gll_apply_eo<N, false>(xp, uxv);
+    if constexpr (EO) gll_apply_eo<N, false, T>(xp, uxv);
 #pragma unroll
     for (int i = 0; i < N; i++) {
       T ux = T(0);
@@ -190,7 +191,7 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
     }
     if constexpr (EO) {
       T sv[N];
-      gll_apply_eo<N, true>(f, sv);
+      gll_apply_eo<N, true, T>(f, sv);
 #pragma unroll
       for (int i = 0; i < N; i++) sd[S::at(i, a, b)] = sv[i];
     } else {
@@ -210,7 +211,7 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
     T yp[N], f[N], uyv[N];
 #pragma unroll
     for (int m = 0; m < N; m++) yp[m] = su[S::at(a, m, b)];
-    if constexpr (EO) gll_apply_eo<N, false>(yp, uyv);
+    if constexpr (EO) gll_apply_eo<N, false, T>(yp, uyv);
 #pragma unroll
     for (int j = 0; j < N; j++) {
       T uy = T(0);
@@ -229,7 +230,7 @@ __device__ __forceinline__ void pen_op3(const T (&ru)[N], T *su, T *sd, T *sx, c
       __builtin_amdgcn_sched_barrier(0);
     }
     if constexpr (EO) {
-      gll_apply_eo<N, true>(f, dy);
+      gll_apply_eo<N, true, T>(f, dy);
     } else {
 #pragma unroll
       for (int j = 0; j < N; j++) {
