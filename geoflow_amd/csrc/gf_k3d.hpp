@@ -173,18 +173,16 @@ void k_burgers_stage3p(
   const double rx = a.edx[t.e * 3], ry = a.edx[t.e * 3 + 1], rz = a.edx[t.e * 3 + 2];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    auto zcb = [&](int, double, double) {};
+    // the z pass's derivatives, kept for the advection term
+    double uzs[N];
+    auto zcb = [&](int k, double uz, double) { uzs[k] = uz; };
     auto xcb = [&](int, double ux, double) { return ux * rx; };
     auto ycb = [&](int, double uy, double) { return uy * ry; };
     double *kout = a.kout[c];
     auto post = [&](int k, double L, double M, double hx, double hy) {
       if (!t.active) return;
       const size_t g = col + k * S::N2;
-      // z-derivative again (ascending-m sum: registers are not spent keeping
-      // the z pass's values)
-      double uz = 0.0;
-#pragma unroll
-      for (int m2 = 0; m2 < N; m2++) uz = fma(gf::GllD<N>::at(k, m2), ru[c][m2], uz);
+      const double uz = uzs[k];
       double po = hx * ru[0][k];
       po += hy * ru[1][k];
       po += (uz * rz) * ru[2][k];
@@ -231,7 +229,7 @@ int launch_stage3p(gf_mesh *m, const StageArgs &a) {
 //            and is this stage's input
 // ---------------------------------------------------------------------------
 template <int N, bool KP, int ACC, bool SMV, bool HL>
-__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(N == 8 ? 4 : 1)))
 void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict__ ui1,
                        const double *__restrict__ ui2, const double *__restrict__ kp0,
                        const double *__restrict__ kp1, const double *__restrict__ kp2,
@@ -406,18 +404,16 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
   const double rx = edx[t.e * 3], ry = edx[t.e * 3 + 1], rz = edx[t.e * 3 + 2];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    auto zcb = [&](int, double, double) {};
+    // the z pass's derivatives, kept for the advection term
+    double uzs[N];
+    auto zcb = [&](int k, double uz, double) { uzs[k] = uz; };
     auto xcb = [&](int, double ux, double) { return ux * rx; };
     auto ycb = [&](int, double uy, double) { return uy * ry; };
     double *ko = kout[c];
     auto post = [&](int k, double L, double M, double hx, double hy) {
       if (!t.active) return;
       const size_t g = col + k * S::N2;
-      // z-derivative again (ascending-m sum: registers are not spent keeping
-      // the z pass's values)
-      double uz = 0.0;
-#pragma unroll
-      for (int m2 = 0; m2 < N; m2++) uz = fma(gf::GllD<N>::at(k, m2), ru[c][m2], uz);
+      const double uz = uzs[k];
       double po = hx * ru[0][k];
       po += hy * ru[1][k];
       po += (uz * rz) * ru[2][k];
