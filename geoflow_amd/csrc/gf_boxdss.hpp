@@ -367,23 +367,20 @@ __device__ __forceinline__ int64_t halo8_src(const BoxElemDeltas &de, int lane, 
          halo8_delta(de, 2, sz);
 }
 
-// Issues the 17 loads of field u for the element at ebase (8 own column
-// values of the lane, 9 halo values) into registers; store() writes them to
-// the LDS halo hl (kHalo8 values, zero slot included).
+// The partner values of field u for the element at ebase: 9 loads per lane
+// into registers (halo8_src), then their LDS slots in the halo hl (kHalo8
+// values) and the zero slot; the element's own values, hl[0, 512), are
+// copied separately (one 4 KB block in the element's layout).
 template <typename T>
-struct Halo8Load {
-  T own[8], hv[9];
+struct Halo8Rim {
+  T hv[9];
   int slot[9];
   __device__ __forceinline__ void issue(const T *__restrict__ u, const BoxElemDeltas &de,
                                         int64_t ebase, int lane) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) own[k] = u[ebase + lane + 64 * k];
-#pragma unroll
     for (int q = 0; q < 9; q++) hv[q] = u[ebase + halo8_src(de, lane, q, &slot[q])];
   }
   __device__ __forceinline__ void store(T *hl, int lane) const {
-#pragma unroll
-    for (int k = 0; k < 8; k++) hl[lane + 64 * k] = own[k];
 #pragma unroll
     for (int q = 0; q < 9; q++)
       if (slot[q] >= 0) hl[slot[q]] = hv[q];

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
 // column owner through LDS, so every node's advection is summed in the
 // reference order ((ux rx) u0 + (uy ry) u1) + (uz rz) u2 (gadvect.ipp:58-100).
 template <int N>
-__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(gf::Pen3<N>::THREADS) __attribute__((amdgpu_waves_per_eu(N <= 8 ? 3 : 1)))
 void k_burgers_stage3p(
     StageArgs a, gf::PenD<double> Dp) {
   using S = gf::Pen3<N>;
@@ -241,10 +241,15 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
                        size_t nelems) {
   using S = gf::Pen3<N>;
   // N = 8 with a previous stage: the three LDS element buffers share memory
-  // with the halos of the gather (gf::Halo8Load; the kernel is held to two
-  // waves per SIMD by its registers, so 16 KB of LDS per wave costs nothing)
+  // with the halos of the gather (lds_copy512, gf::Halo8Rim; the kernel is
+  // held to two waves per SIMD by its registers, so 16 KB of LDS per wave
+  // costs nothing)
   constexpr bool HALO = N == 8 && KP && HL;
-  constexpr int NSL = HALO ? 2 * gf::kHalo8 : 3 * S::EPB * S::SZ;
+  constexpr bool GU = SMV || ACC == 1;  // the stage reads uin's group sums
+  // halo path: K's halo, uin's halo or own values, and (without uin's halo)
+  // the accumulator's own values, one after the other
+  constexpr int NHL = GU ? 2 * gf::kHalo8 : gf::kHalo8 + 512 + (ACC == 2 ? 512 : 0);
+  constexpr int NSL = HALO && NHL > 3 * S::SZ ? NHL : 3 * S::EPB * S::SZ;
   static_assert(!HALO || 3 * S::SZ <= NSL, "element buffers fit the halo area");
   __shared__ double sl[NSL];
   double(*su)[S::SZ] = (double(*)[S::SZ])sl;
@@ -262,70 +267,70 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
     gf::box_elem_coords(topo, t.e, ec);
     de = gf::box_elem_deltas(topo, N, ec);
   }
-  // The inverse mass of the column, read once for all components.
+  // The inverse mass of the column, read once for all components (through
+  // LDS on the halo path).
   double im[N];
+  if constexpr (HALO) {
+    lds_copy512(imass + (size_t)t.e * 512, sl, (int)threadIdx.x);
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) im[k] = imass[col + (size_t)k * S::N2];
+    for (int k = 0; k < N; k++) im[k] = sl[threadIdx.x + 64 * k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; k++) im[k] = imass[col + (size_t)k * S::N2];
+  }
   double ru[3][N];
-  constexpr bool GU = SMV || ACC == 1;  // the stage reads uin's group sums
   if constexpr (HALO) {
     // Per component: every load of the previous stage derivative (and of
-    // uin's halo where its group sums are needed) is issued at once, the
-    // values go to LDS, and each node sums its group from there in GGFX
-    // order (halo8_order) -- the operations and order of the gather path
+    // uin's halo where its group sums are needed) is issued at once -- the
+    // element's own values copied straight to LDS, the partner values
+    // through registers -- and each node sums its group from LDS in GGFX
+    // order (halo8_order): the operations and order of the gather path
     // below, so the same result bit for bit.
     const int lane = (int)threadIdx.x;
     const int64_t ebase = (int64_t)t.e * 512;
-    double *hk = sl, *hu = sl + gf::kHalo8;
+    // K's halo; uin's halo (GU) or own values; the accumulator's own values
+    double *hk = sl, *hu = sl + gf::kHalo8, *ha = sl + gf::kHalo8 + 512;
     const gf::Halo8Col hc(de, t.i, t.j);
-    // Two register sets: component c + 1's loads are issued before component
-    // c's values are stored to LDS and summed, so they are in flight meanwhile
-    // (not where uin's halo is loaded too: two such sets would spill).
-    constexpr bool PIPE = !GU;
-    gf::Halo8Load<double> LK[2], LU[2];
-    double AO[2][8];
-    auto issue = [&](int c, int b) {
-      LK[b].issue(kp[c], de, ebase, lane);
-      if (GU) {
-        LU[b].issue(uin[c], de, ebase, lane);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 8; k++) LU[b].own[k] = uin[c][col + (size_t)k * 64];
-      }
-#pragma unroll
-      for (int k = 0; k < 8; k++) AO[b][k] = ACC == 2 ? acc[c][col + (size_t)k * 64] : 0.0;
-    };
-    if (PIPE) issue(0, 0);
     gf::static_for<0, 3>([&](auto cc) {
-      constexpr int c = decltype(cc)::value, b = PIPE ? c & 1 : 0;
-      if constexpr (!PIPE) issue(c, b);
-      else if constexpr (c < 2) issue(c + 1, b ^ 1);
-      const gf::Halo8Load<double> &Lk = LK[b], &Lu = LU[b];
-      const double(&ao)[8] = AO[b];
+      constexpr int c = decltype(cc)::value;
       __syncthreads();  // the previous component's group sums have read LDS
+      // the element's own values straight to LDS, the partner values
+      // through registers
+      lds_copy512(kp[c] + ebase, hk, lane);
+      lds_copy512(uin[c] + ebase, hu, lane);
+      if (ACC == 2 && !GU) lds_copy512(acc[c] + ebase, ha, lane);
+      gf::Halo8Rim<double> Lk, Lu;
+      Lk.issue(kp[c], de, ebase, lane);
+      if (GU) Lu.issue(uin[c], de, ebase, lane);
+      double ao[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) ao[k] = ACC == 2 && GU ? acc[c][col + (size_t)k * 64] : 0.0;
       Lk.store(hk, lane);
       if (GU) Lu.store(hu, lane);
       __syncthreads();
       gf::static_for<0, 8>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         constexpr int NT = (k == 0 || k == 7) ? 8 : 4;  // terms that can be non-zero
+        const int own = lane + 64 * k;
         int ad[8];
         const int cnt = gf::halo8_order<k>(hc, de, ad);
         double sk = 0.0;
 #pragma unroll
         for (int m = 0; m < NT; m++) sk = sk + hk[ad[m]];
         // GGFX Smooth: Sum / count (an exact multiply for counts 1, 2, 4, 8)
-        const double ks = cnt == 1 ? Lk.own[k] : sk * gf::box_inv(cnt);
-        double v = Lu.own[k] + bh * ks;
+        const double ks = cnt == 1 ? hk[own] : sk * gf::box_inv(cnt);
+        const double uo = hu[own];
+        double v = uo + bh * ks;
         double av = 0.0;
-        if (ACC == 2) av = ao[k] + ks * ch;
+        if (ACC == 2) av = (GU ? ao[k] : ha[own]) + ks * ch;
         if (ACC == 3) av = v;
         if (ACC == 1) {
           // the accumulator starts from Smooth(uin), so it stays continuous
           double su1 = 0.0;
 #pragma unroll
           for (int m = 0; m < NT; m++) su1 = su1 + hu[ad[m]];
-          const double us = cnt == 1 ? Lu.own[k] : su1 * gf::box_inv(cnt);
+          const double us = cnt == 1 ? uo : su1 * gf::box_inv(cnt);
           av = us + ks * ch;
         }
         if (SMV && cnt > 1) {
@@ -498,10 +503,10 @@ __global__ __launch_bounds__(kThreads) void k_rk_out_box(
   }
 }
 
-// Step output on an N = 8 box through the LDS halo (gf::Halo8Load): one
-// element per 64-thread block; per component the element's K and all its
-// partner values are loaded at once (17 loads per lane) with the
-// accumulator, and each node sums its group from LDS in canonical order
+// Step output on an N = 8 box through the LDS halo: one element per
+// 64-thread block; per component the element's K and accumulator are copied
+// to LDS (lds_copy512) while its partner values are loaded into
+// registers (gf::Halo8Rim), and each node sums its group from LDS in canonical order
 // (gf::halo8_order) -- k_rk_out_box's operations, the same result bit for
 // bit, with every K value fetched once instead of once per group copy.
 template <int N>
@@ -510,7 +515,9 @@ __global__ __launch_bounds__(64) void k_rk_out_halo8(
     const double *__restrict__ a0, const double *__restrict__ a1, const double *__restrict__ a2,
     const double *__restrict__ k0, const double *__restrict__ k1, const double *__restrict__ k2,
     double cl, gf::BoxTopo topo, size_t nelems) {
-  __shared__ double hk[gf::kHalo8];
+  // K's halo, then the accumulator's own values
+  __shared__ double sl[gf::kHalo8 + 512];
+  double *hk = sl, *ha = sl + gf::kHalo8;
   double *u[3] = {u0, u1, u2};
   const double *acc[3] = {a0, a1, a2};
   const double *kf[3] = {k0, k1, k2};
@@ -523,34 +530,27 @@ __global__ __launch_bounds__(64) void k_rk_out_halo8(
   const gf::BoxElemDeltas de = gf::box_elem_deltas(topo, 8, ec);
   const gf::Halo8Col hc(de, lane & 7, lane >> 3);
   const int64_t ebase = (int64_t)e * 512;
-  // two register sets: component c + 1's loads are in flight during
-  // component c's sums
-  gf::Halo8Load<double> LS[2];
-  double AV[2][8];
-  auto issue = [&](int c, int b) {
-    LS[b].issue(kf[c], de, ebase, lane);
-#pragma unroll
-    for (int k = 0; k < 8; k++) AV[b][k] = acc[c][ebase + lane + 64 * k];
-  };
-  issue(0, 0);
   gf::static_for<0, 3>([&](auto cc) {
-    constexpr int c = decltype(cc)::value, b = c & 1;
-    if constexpr (c < 2) issue(c + 1, b ^ 1);
-    const gf::Halo8Load<double> &L = LS[b];
-    const double(&av)[8] = AV[b];
+    constexpr int c = decltype(cc)::value;
     __syncthreads();  // the previous component's sums have read LDS
+    // own values straight to LDS, the partner values through registers
+    lds_copy512(kf[c] + ebase, hk, lane);
+    lds_copy512(acc[c] + ebase, ha, lane);
+    gf::Halo8Rim<double> L;
+    L.issue(kf[c], de, ebase, lane);
     L.store(hk, lane);
     __syncthreads();
     gf::static_for<0, 8>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
       constexpr int NT = (k == 0 || k == 7) ? 8 : 4;  // terms that can be non-zero
+      const int own = lane + 64 * k;
       int ad[8];
       const int cnt = gf::halo8_order<k>(hc, de, ad);
       double sk = 0.0;
 #pragma unroll
       for (int m = 0; m < NT; m++) sk = sk + hk[ad[m]];
-      const double ks = cnt == 1 ? L.own[k] : sk * gf::box_inv(cnt);
-      u[c][ebase + lane + 64 * k] = 1.0 * av[k] + cl * ks;
+      const double ks = cnt == 1 ? hk[own] : sk * gf::box_inv(cnt);
+      u[c][ebase + own] = 1.0 * ha[own] + cl * ks;
     });
   });
 }

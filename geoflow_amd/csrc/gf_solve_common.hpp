@@ -78,6 +78,20 @@ __device__ __forceinline__ size_t xcd_block() {
   return (size_t)(blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
 }
 
+// Copies the 512 doubles at g (an element, 4 KB) to LDS at l with four
+// 16-byte global-to-LDS loads per lane (global_load_lds_dwordx4: lane L's 16
+// bytes land at l + 16 L, so l gets g's layout), half the instructions of
+// per-column 8-byte loads and no registers holding the data in flight.  l
+// must be the same for the whole wave; the compiler waits for the copy before
+// the first LDS read after the next barrier.
+__device__ __forceinline__ void lds_copy512(const double *__restrict__ g, double *l, int lane) {
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    __builtin_amdgcn_global_load_lds((const void *)(g + q * 128 + 2 * lane),
+                                     (__attribute__((address_space(3))) void *)(l + q * 128), 16,
+                                     0, 0);
+}
+
 // Grid-stride loop over [0, n) with the blocks renumbered by xcd_block (the
 // grid a multiple of kXCD): each XCD walks contiguous ranges.
 #define XCD_GRID_LOOP(g, n)                                                 \

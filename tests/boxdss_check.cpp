@@ -4,8 +4,9 @@
 // the box's lattice groups are the mesh's GGFX groups (every case listed with
 // expect_ggfx = 1), the GGFX group in the same ascending order; the own slot
 // holds the node itself.  At p = 7 the LDS halo of the stage kernels is
-// simulated too: each element's 64 lanes fill a halo array with the global
-// indices their 17 loads read (Halo8Load), and for every node the halo
+// simulated too: each element's own values are copied to a halo array and
+// its 64 lanes fill the rest with the global indices their 9 partner loads
+// read (Halo8Rim), and for every node the halo
 // slots of halo8_order, read in order with the zero slot skipped, must give
 // box_group's members (and only the z faces may use the last four terms) on
 // every box where box_halo8_ok holds, and fail somewhere on the others.
@@ -101,8 +102,12 @@ int main() {
         const gf::BoxElemDeltas de = gf::box_elem_deltas(t, N, ec);
         const int64_t ebase = (int64_t)e * 512;
         for (double &h : hl) h = -1.0;
+        // the element's own values as the kernels' 16-byte global-to-LDS
+        // copy leaves them (the element's layout), then each lane's
+        // partner values
+        for (int q = 0; q < 512; q++) hl[q] = gid[ebase + q];
         for (int lane = 0; lane < 64; lane++) {
-          gf::Halo8Load<double> L;
+          gf::Halo8Rim<double> L;
           L.issue(gid.data(), de, ebase, lane);
           L.store(hl.data(), lane);
         }

@@ -7,8 +7,9 @@ builders (gf_setup.cpp) and checks, node by node over periodic, Dirichlet and
 degenerate (one element per direction) boxes at orders 1-7, that the members
 equal the GGFX group in ascending order with 64- and 32-bit indices, and that
 the reported own slot holds the node itself; at p = 7 it also fills the N = 8
-stage kernels' LDS halo (Halo8Load) lane by lane with global indices and checks
-that halo8_group's slots give every node's group in the same order.
+stage kernels' LDS halo with global indices (own values copied, partner
+values by Halo8Rim lane by lane) and checks that halo8_order's slots give
+every node's group in the same order.
 """
 import os
 import subprocess
