@@ -222,6 +222,22 @@ class BoxGrid {
     check(gf_grefderiv(mesh_, idir, dotrans ? 1 : 0, u.data(), du.data()), ctx_.handle(),
           "gf_grefderiv");
   }
+  // GGrid::compute_grefdiv (src/cdg/grid/ggrid.ipp:1847-1927): divu =
+  // sum_j D_j^T u_j (dotrans) or D_j u_j, reference space; entries of u may
+  // be null (skipped).
+  void grefdiv(const std::vector<const Field *> &u, bool dotrans, Field &divu) {
+    std::vector<const double *> up(dim_, nullptr);
+    for (size_t k = 0; k < u.size() && k < (size_t)dim_; k++) up[k] = u[k] ? u[k]->data() : nullptr;
+    check(gf_grefdiv(mesh_, dotrans ? 1 : 0, up.data(), divu.data()), ctx_.handle(),
+          "gf_grefdiv");
+  }
+  // GMTK::grad (src/cdg/blas/gmtk.ipp:2883-2892): du[k] = d u / d x_k, k < dim.
+  void grad(const Field &u, State &du) {
+    if ((int)du.size() < dim_) throw Error(GF_ERR_ARG, "grad: need dim output fields");
+    std::vector<double *> dp(du.size());
+    for (size_t k = 0; k < du.size(); k++) dp[k] = du[k]->data();
+    check(gf_grad(mesh_, u.data(), dp.data()), ctx_.handle(), "gf_grad");
+  }
   // GGrid::dealias (src/cdg/grid/ggrid.ipp:2424-2503) through the order-pq
   // GLL grid (the reference's qdealias_order).
   void dealias(const Field &v1, const Field &v2, Field &prod, int pq) {
@@ -301,6 +317,13 @@ class GHelmholtz {
     check(gf_helmholtz_apply(g_.mesh(), pscalar_, pfield_, qscalar_, qfield_, in.data(),
                              out.data()),
           g_.context().handle(), "gf_helmholtz_apply");
+  }
+  // Inverse of the assembled diagonal of p L + q M with scalar p and the
+  // current mass term (the Jacobi preconditioner GCG's precond = 1 uses; no
+  // reference counterpart: GCG ships no preconditioner).
+  void diag_inv(Field &dinv) {
+    check(gf_helmholtz_diag_inv(g_.mesh(), pscalar_, qscalar_, qfield_, dinv.data()),
+          g_.context().handle(), "gf_helmholtz_diag_inv");
   }
   BoxGrid &grid() { return g_; }
   double p_scalar() const { return pscalar_; }
@@ -384,6 +407,22 @@ class GCG {
   std::vector<double> resid_;
 };
 
+// GAdvect (src/cdg/sem/gadvect.hpp): po = M (u . grad p), unassembled, for
+// the velocity components u (apply with ivec = -2, gadvect.ipp:57-99).
+class GAdvect {
+ public:
+  explicit GAdvect(BoxGrid &g) : g_(g) {}
+  void apply(const Field &p, const State &u, State & /*utmp*/, Field &po) {
+    std::vector<const double *> up(u.size());
+    for (size_t k = 0; k < u.size(); k++) up[k] = u[k]->data();
+    check(gf_advect(g_.mesh(), p.data(), (int)u.size(), up.data(), po.data()),
+          g_.context().handle(), "gf_advect");
+  }
+
+ private:
+  BoxGrid &g_;
+};
+
 // GExRKStepper (src/cdg/pdes/gexrk_stepper.hpp:20-60) bound to the GBurgers
 // right-hand side (src/cdg/pdes/gburgers.ipp:202-257): mode 0 nonlinear,
 // 1 pure advection of u[0] by u[1..], 2 heat.  step() advances u in place by
@@ -404,6 +443,16 @@ class BurgersStepper {
     for (size_t k = 0; k < u.size(); k++) up[k] = u[k]->data();
     check(gf_burgers_steps(g_.mesh(), mode_, nu_, order_, (int)u.size(), t, dt, nsteps, up.data()),
           g_.context().handle(), "gf_burgers_steps");
+  }
+  // GBurgers::dudt_impl (src/cdg/pdes/gburgers.ipp:202-257): the right-hand
+  // side the steps integrate, dudt[k] for the evolved components.
+  void dudt(double /*t*/, const State &u, State &dudt) {
+    std::vector<const double *> up(u.size());
+    for (size_t k = 0; k < u.size(); k++) up[k] = u[k]->data();
+    std::vector<double *> dp(dudt.size());
+    for (size_t k = 0; k < dudt.size(); k++) dp[k] = dudt[k]->data();
+    check(gf_burgers_rhs(g_.mesh(), mode_, nu_, (int)u.size(), up.data(), dp.data()),
+          g_.context().handle(), "gf_burgers_rhs");
   }
 
  private:

@@ -115,6 +115,43 @@ int main(int argc, char **argv) {
     gd.dealias(v, w, pr, 8);
     save("dealias.bin", pr.download());
     save("integrate.bin", {gd.integrate(v, tmp)});
+
+    // GMTK::grad, GGrid::compute_grefdiv, GAdvect and GBurgers::dudt_impl on
+    // the same box; the Jacobi diagonal of the Dirichlet box's operator.
+    std::vector<Field> gr;
+    for (int c = 0; c < 3; c++) gr.emplace_back(ctx, nd);
+    State sg = {&gr[0], &gr[1], &gr[2]};
+    gd.grad(v, sg);
+    for (int c = 0; c < 3; c++) {
+      char name[32];
+      std::snprintf(name, sizeof name, "grad_%d.bin", c);
+      save(name, gr[c].download());
+    }
+    Field dvg = gd.field();
+    gd.grefdiv({&v, &w, nullptr}, true, dvg);
+    save("grefdiv.bin", dvg.download());
+    std::vector<double> hz(nd);
+    for (size_t i = 0; i < nd; i++) hz[i] = 0.2 * std::sin(2 * pi * gd.xyz(1)[i]);
+    Field wz(ctx, hz);
+    State vel = {&w, &v, &wz};
+    Field po = gd.field();
+    GAdvect adv(gd);
+    adv.apply(v, vel, tmp, po);
+    save("advect.bin", po.download());
+    std::vector<Field> rhs;
+    for (int c = 0; c < 3; c++) rhs.emplace_back(ctx, nd);
+    State sr = {&rhs[0], &rhs[1], &rhs[2]};
+    State su3 = {&v, &w, &wz};
+    BurgersStepper rk3(gd, 0, 0.05);
+    rk3.dudt(0.0, su3, sr);
+    for (int c = 0; c < 3; c++) {
+      char name[32];
+      std::snprintf(name, sizeof name, "dudt_%d.bin", c);
+      save(name, rhs[c].download());
+    }
+    Field dinv = g.field();
+    H.diag_inv(dinv);
+    save("diag_inv.bin", dinv.download());
     ctx.synchronize();
   } catch (const Error &e) {
     std::fprintf(stderr, "adapter_driver: %s\n", e.what());

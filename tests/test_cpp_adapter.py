@@ -8,7 +8,10 @@ files; here they are compared with the CPU oracle on the same inputs:
   * 10 identity-preconditioned CG iterations: 1e-11 relative;
   * 3 RK4 steps of nonlinear Burgers (nu=0.05, dt=1e-3): 1e-12 relative;
   * on a box whose GGFX groups are built on the device: GGrid::deriv and
-    GGrid::dealias (pq = 8) at 1e-12 relative, GGrid::integrate to 1e-13.
+    GGrid::dealias (pq = 8) at 1e-12 relative, GGrid::integrate to 1e-13;
+    GMTK::grad, GGrid::compute_grefdiv, GAdvect::apply and
+    GBurgers::dudt_impl at 1e-12 relative;
+  * the Jacobi diagonal of the Dirichlet box's operator at 1e-12 relative.
 """
 import os
 import subprocess
@@ -82,3 +85,14 @@ def test_adapter_matches_oracle(ctx, tmp_path):
     assert relerr(_load(tmp_path, "deriv_x.bin"), gd.deriv(v, 1)) <= 1e-12
     assert relerr(_load(tmp_path, "dealias.bin"), gd.dealias(v, w, 8)) <= 1e-12
     assert abs(_load(tmp_path, "integrate.bin")[0] - gd.integrate(v)) <= 1e-13
+    # GMTK::grad, GGrid::compute_grefdiv, GAdvect, GBurgers::dudt_impl; the
+    # Jacobi diagonal of the Dirichlet box's operator
+    wz = 0.2 * np.sin(2 * np.pi * Y)
+    for c in range(3):
+        assert relerr(_load(tmp_path, "grad_%d.bin" % c), gd.deriv(v, c + 1)) <= 1e-12
+    assert relerr(_load(tmp_path, "grefdiv.bin"), gd.grefdiv([v, w, None], True)) <= 1e-12
+    assert relerr(_load(tmp_path, "advect.bin"), gd.advect(v, [w, v, wz])) <= 1e-12
+    ref = gd.burgers_rhs([v, w, wz], 0.05)
+    for c in range(3):
+        assert relerr(_load(tmp_path, "dudt_%d.bin" % c), ref[c]) <= 1e-12
+    assert relerr(_load(tmp_path, "diag_inv.bin"), og.helmholtz_diag_inv(0.5, q)) <= 1e-12
