@@ -117,18 +117,81 @@ class DeviceVector {
 using Field = DeviceVector<double>;
 using State = std::vector<Field *>;
 
+// GGrid (src/cdg/grid/ggrid.hpp): a grid's device mesh and the operations of
+// GGrid that every grid type shares; BoxGrid and IcosGrid build it.
+class Grid {
+ public:
+  virtual ~Grid() {
+    if (mesh_) gf_mesh_destroy(mesh_);
+  }
+  Grid(const Grid &) = delete;
+  Grid &operator=(const Grid &) = delete;
+
+  Context &context() { return ctx_; }
+  gf_mesh *mesh() const { return mesh_; }
+  int dim() const { return dim_; }
+  size_t ndof() const { return ndof_; }
+  size_t nelems() const { return nelems_; }
+  double minnodedist() const { return minnodedist_; }
+  const std::vector<double> &xyz(int d) const { return xyz_[d]; }
+  Field field() { return Field(ctx_, ndof_); }
+
+  // GGrid::deriv (src/cdg/grid/ggrid.ipp:1163-1191): du = d u / dx_idir
+  // (idir 1..dim, 1..3 on embedded grids).
+  void deriv(const Field &u, int idir, State & /*utmp*/, Field &du) {
+    check(gf_deriv(mesh_, idir, u.data(), du.data()), ctx_.handle(), "gf_deriv");
+  }
+  // GGrid::compute_grefderiv (src/cdg/grid/ggrid.ipp:1954-2265): reference-
+  // space derivative along idir, D^T when dotrans.
+  void grefderiv(const Field &u, int idir, bool dotrans, Field &du) {
+    check(gf_grefderiv(mesh_, idir, dotrans ? 1 : 0, u.data(), du.data()), ctx_.handle(),
+          "gf_grefderiv");
+  }
+  // GGrid::compute_grefdiv (src/cdg/grid/ggrid.ipp:1847-1927): divu =
+  // sum_j D_j^T u_j (dotrans) or D_j u_j, reference space; entries of u may
+  // be null (skipped).
+  void grefdiv(const std::vector<const Field *> &u, bool dotrans, Field &divu) {
+    std::vector<const double *> up(dim_, nullptr);
+    for (size_t k = 0; k < u.size() && k < (size_t)dim_; k++) up[k] = u[k] ? u[k]->data() : nullptr;
+    check(gf_grefdiv(mesh_, dotrans ? 1 : 0, up.data(), divu.data()), ctx_.handle(),
+          "gf_grefdiv");
+  }
+  // GMTK::grad (src/cdg/blas/gmtk.ipp:2883-2892): du[k] = d u / d x_k for
+  // k < dim (3 components on embedded grids).
+  void grad(const Field &u, State &du) {
+    std::vector<double *> dp(du.size());
+    for (size_t k = 0; k < du.size(); k++) dp[k] = du[k]->data();
+    check(gf_grad(mesh_, u.data(), dp.data()), ctx_.handle(), "gf_grad");
+  }
+  // GGrid::integrate (src/cdg/grid/ggrid.ipp:1077-1148).
+  double integrate(const Field &u, State & /*utmp*/) {
+    double r = 0.0;
+    check(gf_integrate(mesh_, u.data(), &r), ctx_.handle(), "gf_integrate");
+    return r;
+  }
+
+ protected:
+  Grid(Context &ctx, int dim) : ctx_(ctx), dim_(dim) {}
+  Context &ctx_;
+  int dim_;
+  size_t ndof_ = 0, nelems_ = 0;
+  double minnodedist_ = 0;
+  std::array<std::vector<double>, 3> xyz_;
+  gf_mesh *mesh_ = nullptr;
+};
+
 // GGridBox for one rank: nodes, regular-element metrics, boundary mask and the
 // GGFX coincidence groups, built on the host; operator data uploaded to the
 // device as one mesh (src/cdg/grid/ggrid_box.ipp:254-463, 753-802, 862-978;
 // src/cdg/comm/ggfx.hpp:167-342).  Periodic directions use GF_BDY_PERIODIC.
-class BoxGrid {
+class BoxGrid : public Grid {
  public:
   BoxGrid(Context &ctx, int dim, std::array<int, 3> p, std::array<int, 3> ne,
           std::array<double, 3> P0 = {0, 0, 0}, std::array<double, 3> P1 = {1, 1, 1},
           std::array<int, 6> bdy = {GF_BDY_PERIODIC, GF_BDY_PERIODIC, GF_BDY_PERIODIC,
                                     GF_BDY_PERIODIC, GF_BDY_PERIODIC, GF_BDY_PERIODIC},
           double tol_fac = 0.001, bool device_maps = false)
-      : ctx_(ctx), dim_(dim) {
+      : Grid(ctx, dim) {
     if (dim != 2 && dim != 3) throw Error(GF_ERR_ARG, "BoxGrid: dim must be 2 or 3");
     size_t nn = 1, nel = 1;
     for (int d = 0; d < dim; d++) {
@@ -195,77 +258,87 @@ class BoxGrid {
       }
     check(gf_mesh_create(ctx.handle(), &md, &mesh_), ctx.handle(), "gf_mesh_create");
   }
-  ~BoxGrid() {
-    if (mesh_) gf_mesh_destroy(mesh_);
-  }
-  BoxGrid(const BoxGrid &) = delete;
-  BoxGrid &operator=(const BoxGrid &) = delete;
-
-  Context &context() { return ctx_; }
-  gf_mesh *mesh() const { return mesh_; }
-  int dim() const { return dim_; }
-  size_t ndof() const { return ndof_; }
-  size_t nelems() const { return nelems_; }
-  double minnodedist() const { return minnodedist_; }
-  const std::vector<double> &xyz(int d) const { return xyz_[d]; }
   const std::vector<double> &mask() const { return mask_; }
-  Field field() { return Field(ctx_, ndof_); }
 
-  // GGrid::deriv (src/cdg/grid/ggrid.ipp:1163-1191): du = d u / dx_idir
-  // (idir 1..dim).
-  void deriv(const Field &u, int idir, State & /*utmp*/, Field &du) {
-    check(gf_deriv(mesh_, idir, u.data(), du.data()), ctx_.handle(), "gf_deriv");
-  }
-  // GGrid::compute_grefderiv (src/cdg/grid/ggrid.ipp:1954-2265): reference-
-  // space derivative along idir, D^T when dotrans.
-  void grefderiv(const Field &u, int idir, bool dotrans, Field &du) {
-    check(gf_grefderiv(mesh_, idir, dotrans ? 1 : 0, u.data(), du.data()), ctx_.handle(),
-          "gf_grefderiv");
-  }
-  // GGrid::compute_grefdiv (src/cdg/grid/ggrid.ipp:1847-1927): divu =
-  // sum_j D_j^T u_j (dotrans) or D_j u_j, reference space; entries of u may
-  // be null (skipped).
-  void grefdiv(const std::vector<const Field *> &u, bool dotrans, Field &divu) {
-    std::vector<const double *> up(dim_, nullptr);
-    for (size_t k = 0; k < u.size() && k < (size_t)dim_; k++) up[k] = u[k] ? u[k]->data() : nullptr;
-    check(gf_grefdiv(mesh_, dotrans ? 1 : 0, up.data(), divu.data()), ctx_.handle(),
-          "gf_grefdiv");
-  }
-  // GMTK::grad (src/cdg/blas/gmtk.ipp:2883-2892): du[k] = d u / d x_k, k < dim.
-  void grad(const Field &u, State &du) {
-    if ((int)du.size() < dim_) throw Error(GF_ERR_ARG, "grad: need dim output fields");
-    std::vector<double *> dp(du.size());
-    for (size_t k = 0; k < du.size(); k++) dp[k] = du[k]->data();
-    check(gf_grad(mesh_, u.data(), dp.data()), ctx_.handle(), "gf_grad");
-  }
   // GGrid::dealias (src/cdg/grid/ggrid.ipp:2424-2503) through the order-pq
   // GLL grid (the reference's qdealias_order).
   void dealias(const Field &v1, const Field &v2, Field &prod, int pq) {
     check(gf_dealias(mesh_, pq, v1.data(), v2.data(), prod.data()), ctx_.handle(), "gf_dealias");
   }
-  // GGrid::integrate (src/cdg/grid/ggrid.ipp:1077-1148).
-  double integrate(const Field &u, State & /*utmp*/) {
-    double r = 0.0;
-    check(gf_integrate(mesh_, u.data(), &r), ctx_.handle(), "gf_integrate");
-    return r;
+ private:
+  std::array<std::vector<double>, 3> xyzp_, W_, D_;
+  std::vector<double> mask_, edx_, ejac_;
+  std::vector<int64_t> offs_, members_;
+};
+
+// GGridIcos (2-D) for one rank (src/cdg/grid/ggrid_icos.ipp:139-513): the
+// base icosahedron refined (refine 0: GICOS_BISECTION with ilevel
+// bisections; 1: GICOS_LAGRANGIAN with ilevel rows per base triangle), three
+// GE_2DEMBEDDED quads per triangle on the sphere of the given radius, the
+// embedded metrics of GElem_base (src/cdg/sem/gelem_base.cpp:877-955,
+// 1066-1170, 1249-1300) and the GGFX groups (at most 6 members,
+// GGridIcos::max_duplicates).
+class IcosGrid : public Grid {
+ public:
+  IcosGrid(Context &ctx, int p, int ilevel, double radius = 1.0, int refine = 0,
+           double tol_fac = 0.001)
+      : Grid(ctx, 2) {
+    size_t nel = 0;
+    check(gf_icos_grid(p, ilevel, refine, radius, &nel, nullptr, nullptr, nullptr, nullptr),
+          nullptr, "gf_icos_grid");
+    nelems_ = nel;
+    ndof_ = nel * (size_t)(p + 1) * (size_t)(p + 1);
+    for (int d = 0; d < 3; d++) xyz_[d].resize(ndof_);
+    for (auto &v : dxidx_) v.resize(ndof_);
+    jac_.resize(ndof_);
+    double *xp[3] = {xyz_[0].data(), xyz_[1].data(), xyz_[2].data()};
+    double *dp[6];
+    for (int k = 0; k < 6; k++) dp[k] = dxidx_[k].data();
+    check(gf_icos_grid(p, ilevel, refine, radius, &nel, xp, dp, jac_.data(), &minnodedist_),
+          nullptr, "gf_icos_grid");
+    const double *cx[3] = {xp[0], xp[1], xp[2]};
+    offs_.resize(ndof_ + 1);
+    size_t nm = 0;
+    const double tol = tol_fac * minnodedist_;
+    check(gf_ggfx_build(ndof_, 3, cx, tol, 6, offs_.data(), nullptr, &nm), nullptr,
+          "gf_ggfx_build");
+    members_.resize(nm);
+    check(gf_ggfx_build(ndof_, 3, cx, tol, 6, offs_.data(), members_.data(), &nm), nullptr,
+          "gf_ggfx_build");
+    W_.resize(p + 1);
+    D_.resize((size_t)(p + 1) * (p + 1));
+    std::vector<double> xi(p + 1);
+    check(gf_gll_basis(p, xi.data(), W_.data(), D_.data()), nullptr, "gf_gll_basis");
+    gf_mesh_desc md{};
+    md.dim = 2;
+    md.gtype = GF_GE_2DEMBEDDED;
+    md.nelems = nel;
+    md.N[0] = md.N[1] = p + 1;
+    md.N[2] = 1;
+    for (int d = 0; d < 2; d++) {
+      md.D[d] = D_.data();
+      md.W[d] = W_.data();
+    }
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 3; j++) md.node_dxidx[i * 3 + j] = dxidx_[i * 3 + j].data();
+    md.node_jac = jac_.data();
+    for (int d = 0; d < 3; d++) md.xyz[d] = xyz_[d].data();
+    md.ggfx_offs = offs_.data();
+    md.ggfx_members = members_.data();
+    check(gf_mesh_create(ctx.handle(), &md, &mesh_), ctx.handle(), "gf_mesh_create");
   }
 
  private:
-  Context &ctx_;
-  int dim_;
-  size_t ndof_ = 0, nelems_ = 0;
-  double minnodedist_ = 0;
-  std::array<std::vector<double>, 3> xyz_, xyzp_, W_, D_;
-  std::vector<double> mask_, edx_, ejac_;
+  std::array<std::vector<double>, 6> dxidx_;
+  std::vector<double> jac_, W_, D_;
   std::vector<int64_t> offs_, members_;
-  gf_mesh *mesh_ = nullptr;
 };
 
 // GMass (src/cdg/sem/gmass.hpp:37-44): out = M in, unassembled; inverse
 // applies 1/DSS-Smooth(M).
 class GMass {
  public:
-  explicit GMass(BoxGrid &g, bool inverse = false) : g_(g), inv_(inverse) {}
+  explicit GMass(Grid &g, bool inverse = false) : g_(g), inv_(inverse) {}
   void opVec_prod(const Field &in, State & /*utmp*/, Field &out) {
     gf_ctx *c = g_.context().handle();
     if (inv_)
@@ -275,7 +348,7 @@ class GMass {
   }
 
  private:
-  BoxGrid &g_;
+  Grid &g_;
   bool inv_;
 };
 
@@ -285,7 +358,7 @@ class GMass {
 // grid has nodes is the scalar p[0] or q[0] (ghelmholtz.ipp:184-186).
 class GHelmholtz {
  public:
-  explicit GHelmholtz(BoxGrid &g) : g_(g) {}
+  explicit GHelmholtz(Grid &g) : g_(g) {}
   void set_Lap_scalar(const Field &p) {
     if (p.size() < g_.ndof()) {
       pfield_ = nullptr;
@@ -325,14 +398,14 @@ class GHelmholtz {
     check(gf_helmholtz_diag_inv(g_.mesh(), pscalar_, qscalar_, qfield_, dinv.data()),
           g_.context().handle(), "gf_helmholtz_diag_inv");
   }
-  BoxGrid &grid() { return g_; }
+  Grid &grid() { return g_; }
   double p_scalar() const { return pscalar_; }
   const double *p_field() const { return pfield_; }
   const double *q_field() const { return qfield_; }
   double q_scalar() const { return qscalar_; }
 
  private:
-  BoxGrid &g_;
+  Grid &g_;
   double pscalar_ = 1.0;
   const double *pfield_ = nullptr;
   const double *qfield_ = nullptr;
@@ -344,7 +417,7 @@ class GHelmholtz {
 class GGFX {
  public:
   enum ReductionOp { Sum = GF_DSS_SUM, Smooth = GF_DSS_SMOOTH, Max = GF_DSS_MAX, Min = GF_DSS_MIN };
-  explicit GGFX(BoxGrid &g) : g_(g) {}
+  explicit GGFX(Grid &g) : g_(g) {}
   bool doOp(Field &u, ReductionOp op) {
     check(gf_dss(g_.mesh(), (int)op, u.data()), g_.context().handle(), "gf_dss");
     return true;
@@ -354,7 +427,7 @@ class GGFX {
   }
 
  private:
-  BoxGrid &g_;
+  Grid &g_;
 };
 
 // GCG (src/cdg/solvers/gcg.hpp:29-75) with LinSolverBase::Traits
@@ -371,7 +444,7 @@ class GCG {
     int precision = 0;    // 0 FP64, 1 FP32 recurrences with FP64 reliable updates
     double inner_tol = 0; // 0 = library default
   };
-  GCG(Traits &traits, BoxGrid &grid) : traits_(traits), g_(grid) {}
+  GCG(Traits &traits, Grid &grid) : traits_(traits), g_(grid) {}
   int solve(GHelmholtz &A, const Field &b, Field &x) { return run(A, b, nullptr, x); }
   int solve(GHelmholtz &A, const Field &b, const Field &xb, Field &x) {
     return run(A, b, xb.data(), x);
@@ -402,7 +475,7 @@ class GCG {
     return st;
   }
   Traits traits_;
-  BoxGrid &g_;
+  Grid &g_;
   gf_cg_result res_{};
   std::vector<double> resid_;
 };
@@ -411,7 +484,7 @@ class GCG {
 // the velocity components u (apply with ivec = -2, gadvect.ipp:57-99).
 class GAdvect {
  public:
-  explicit GAdvect(BoxGrid &g) : g_(g) {}
+  explicit GAdvect(Grid &g) : g_(g) {}
   void apply(const Field &p, const State &u, State & /*utmp*/, Field &po) {
     std::vector<const double *> up(u.size());
     for (size_t k = 0; k < u.size(); k++) up[k] = u[k]->data();
@@ -420,7 +493,7 @@ class GAdvect {
   }
 
  private:
-  BoxGrid &g_;
+  Grid &g_;
 };
 
 // GExRKStepper (src/cdg/pdes/gexrk_stepper.hpp:20-60) bound to the GBurgers
@@ -429,7 +502,7 @@ class GAdvect {
 // one explicit RK step of order 1..4 with GGFX Smooth after each stage.
 class BurgersStepper {
  public:
-  BurgersStepper(BoxGrid &g, int mode, double nu) : g_(g), mode_(mode), nu_(nu) {}
+  BurgersStepper(Grid &g, int mode, double nu) : g_(g), mode_(mode), nu_(nu) {}
   void setOrder(int order, int /*nstage*/ = 0) { order_ = order; }
   void step(double t, State &u, double dt, bool check_finite = false) {
     std::vector<double *> up(u.size());
@@ -456,7 +529,7 @@ class BurgersStepper {
   }
 
  private:
-  BoxGrid &g_;
+  Grid &g_;
   int mode_;
   double nu_;
   int order_ = 4;

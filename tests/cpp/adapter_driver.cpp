@@ -152,6 +152,33 @@ int main(int argc, char **argv) {
     Field dinv = g.field();
     H.diag_inv(dinv);
     save("diag_inv.bin", dinv.download());
+
+    // The icosahedral sphere (p = 4, one bisection): GMass, GHelmholtz and
+    // three RK4 steps of advection under solid-body rotation about z.
+    IcosGrid gi(ctx, 4, 1);
+    const size_t ni = gi.ndof();
+    std::vector<double> hb(ni), c0(ni), c1(ni), c2(ni, 0.0);
+    for (size_t i = 0; i < ni; i++) {
+      const double X = gi.xyz(0)[i], Y = gi.xyz(1)[i], Z = gi.xyz(2)[i];
+      hb[i] = X * Y + Z;
+      c0[i] = -Y;
+      c1[i] = X;
+    }
+    Field fb(ctx, hb), f0(ctx, c0), f1(ctx, c1), f2(ctx, c2);
+    GMass Mi(gi);
+    Field mb = gi.field();
+    Mi.opVec_prod(fb, tmp, mb);
+    save("icos_mass.bin", mb.download());
+    GHelmholtz Hi(gi);
+    Hi.set_Lap_scalar(0.7);
+    Hi.set_mass_scalar(1.3);
+    Field hi = gi.field();
+    Hi.opVec_prod(fb, tmp, hi);
+    save("icos_helm.bin", hi.download());
+    State si = {&fb, &f0, &f1, &f2};
+    BurgersStepper rki(gi, 1, 1e-3);
+    for (int s = 0; s < 3; s++) rki.step(s * 1e-3, si, 1e-3);
+    save("icos_u.bin", fb.download());
     ctx.synchronize();
   } catch (const Error &e) {
     std::fprintf(stderr, "adapter_driver: %s\n", e.what());

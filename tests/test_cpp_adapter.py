@@ -11,7 +11,9 @@ files; here they are compared with the CPU oracle on the same inputs:
     GGrid::dealias (pq = 8) at 1e-12 relative, GGrid::integrate to 1e-13;
     GMTK::grad, GGrid::compute_grefdiv, GAdvect::apply and
     GBurgers::dudt_impl at 1e-12 relative;
-  * the Jacobi diagonal of the Dirichlet box's operator at 1e-12 relative.
+  * the Jacobi diagonal of the Dirichlet box's operator at 1e-12 relative;
+  * on the icosahedral sphere (IcosGrid, p = 4, one bisection): mass,
+    Helmholtz and 3 RK4 steps of advection at 1e-12 relative.
 """
 import os
 import subprocess
@@ -96,3 +98,15 @@ def test_adapter_matches_oracle(ctx, tmp_path):
     for c in range(3):
         assert relerr(_load(tmp_path, "dudt_%d.bin" % c), ref[c]) <= 1e-12
     assert relerr(_load(tmp_path, "diag_inv.bin"), og.helmholtz_diag_inv(0.5, q)) <= 1e-12
+
+    # IcosGrid: mass, Helmholtz and RK4 advection on the sphere
+    oi = oracle.Grid.icos(4, 1, radius=1.0)
+    X, Y, Z = oi.get("x"), oi.get("y"), oi.get("z")
+    hb = X * Y + Z
+    assert relerr(_load(tmp_path, "icos_mass.bin"), oi.mass_apply(hb)) <= 1e-12
+    assert relerr(_load(tmp_path, "icos_helm.bin"),
+                  oi.helmholtz(hb, 0.7, np.full(oi.ndof, 1.3))) <= 1e-12
+    st = [hb.copy(), -Y, X.copy(), np.zeros(oi.ndof)]
+    for s in range(3):
+        oi.burgers_step(st, 1e-3, 1e-3, t=s * 1e-3, mode=1)
+    assert relerr(_load(tmp_path, "icos_u.bin"), st[0]) <= 1e-12
