@@ -33,6 +33,7 @@ GF_MESH_BOX_GATHER = 1
 GF_MESH_BOX_MASK = 2
 GF_MESH_CONST_D = 4
 GF_MESH_BOX_HALO = 8
+GF_MESH_IMASS_ELEM = 16
 GF_ERR_PRECOND = 3
 GF_ERR_ALLOC = 12
 GF_RESERVE_RK, GF_RESERVE_CG, GF_RESERVE_CG_MIXED, GF_RESERVE_ADJOINT = 1, 2, 4, 8

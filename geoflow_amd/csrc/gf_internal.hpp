@@ -51,6 +51,10 @@ struct gf_mesh_s {
   double *G[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double *xyz[3] = {nullptr, nullptr, nullptr};
   double *mask = nullptr, *mass = nullptr, *imass = nullptr, *imult = nullptr;
+  // The inverse mass of one element when it is the same, bit for bit, at
+  // every element (uniform periodic boxes): nn values the RK stage kernels
+  // read in place of imass (null otherwise).
+  double *imass_el = nullptr;
   float *mask32 = nullptr, *imult32 = nullptr;
   bool has_mask = false;
   // DSS: coincidence groups with more than one member (CSR, ascending members)

@@ -238,7 +238,7 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
                        const double *__restrict__ imass, const double *__restrict__ W,
                        const double *__restrict__ edx, const double *__restrict__ ejac,
                        gf::PenD<double> Dp, double bh, double ch, double nu, gf::BoxTopo topo,
-                       size_t nelems) {
+                       size_t nelems, size_t imstride) {
   using S = gf::Pen3<N>;
   // N = 8 with a previous stage: the three LDS element buffers share memory
   // with the halos of the gather (lds_copy512, gf::Halo8Rim; the kernel is
@@ -271,13 +271,13 @@ void k_burgers_stage3b(const double *__restrict__ ui0, const double *__restrict_
   // LDS on the halo path).
   double im[N];
   if constexpr (HALO) {
-    lds_copy512(imass + (size_t)t.e * 512, sl, (int)threadIdx.x);
+    lds_copy512(imass + t.e * imstride, sl, (int)threadIdx.x);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < N; k++) im[k] = sl[threadIdx.x + 64 * k];
   } else {
 #pragma unroll
-    for (int k = 0; k < N; k++) im[k] = imass[col + (size_t)k * S::N2];
+    for (int k = 0; k < N; k++) im[k] = imass[t.e * imstride + t.r + (size_t)k * S::N2];
   }
   double ru[3][N];
   if constexpr (HALO) {
@@ -563,8 +563,8 @@ void launch_stage3b_h(gf_mesh *m, int nb, double *const *uin, double *const *kp,
                      m->ctx->stream, uin[0], uin[1], uin[2], kp ? kp[0] : nullptr,
                      kp ? kp[1] : nullptr, kp ? kp[2] : nullptr, acc ? acc[0] : nullptr,
                      acc ? acc[1] : nullptr, acc ? acc[2] : nullptr, kout[0], kout[1], kout[2],
-                     m->imass, m->W[0], m->edx, m->ejac, pend_d(m), bh, ch, nu, m->box,
-                     m->nelems);
+                     m->imass_el ? m->imass_el : m->imass, m->W[0], m->edx, m->ejac, pend_d(m),
+                     bh, ch, nu, m->box, m->nelems, m->imass_el ? (size_t)0 : (size_t)S::NN);
 }
 
 // The N = 8 gather through the LDS halo where the box's partner offsets

@@ -1084,6 +1084,19 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     for (size_t q = 0; q < n; q++) imf[q] = (float)im[q];
     GF_HIP(ctx, hipMemcpy(m->imult32, imf.data(), n * sizeof(float), hipMemcpyHostToDevice));
   }
+  if (d->gtype == GF_GE_REGULAR && m->dim == 3 && m->Niso && m->nelems > 1) {
+    // inverse mass identical at every element (bit for bit): the RK stages
+    // read one element's values instead of the whole array
+    std::vector<double> im(n);
+    GF_HIP(ctx, hipMemcpy(im.data(), m->imass, n * sizeof(double), hipMemcpyDeviceToHost));
+    const size_t nn = m->nn;
+    bool same = true;
+    for (size_t q = nn; same && q < n; q++) same = im[q] == im[q % nn];
+    if (same) {
+      GF_HIP(ctx, hipMalloc((void **)&m->imass_el, nn * sizeof(double)));
+      GF_HIP(ctx, hipMemcpy(m->imass_el, im.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
 #undef UP
   *out = m;
   return GF_OK;
@@ -1092,7 +1105,8 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
 int gf_mesh_flags(const gf_mesh *m) {
   if (!m) return 0;
   return (m->has_box ? GF_MESH_BOX_GATHER : 0) | (m->has_box_mask ? GF_MESH_BOX_MASK : 0) |
-         (m->dconst ? GF_MESH_CONST_D : 0) | (m->halo8 ? GF_MESH_BOX_HALO : 0);
+         (m->dconst ? GF_MESH_CONST_D : 0) | (m->halo8 ? GF_MESH_BOX_HALO : 0) |
+         (m->imass_el ? GF_MESH_IMASS_ELEM : 0);
 }
 
 int gf_mesh_destroy(gf_mesh *m) {
@@ -1114,7 +1128,8 @@ int gf_mesh_destroy(gf_mesh *m) {
     F(m->pDC32[k]);
   }
   F(m->D32); F(m->W32); F(m->edx); F(m->ejac); F(m->edx32); F(m->ejac32); F(m->njac);
-  F(m->mask); F(m->mass); F(m->imass); F(m->imult); F(m->mask32); F(m->imult32);
+  F(m->mask); F(m->mass); F(m->imass); F(m->imass_el); F(m->imult); F(m->mask32);
+  F(m->imult32);
   F(m->goff); F(m->gmem); F(m->red); F(m->scal);
   for (int k = 0; k < 6; k++) F(m->G32[k]);
   F(m->mass32);

@@ -244,9 +244,12 @@ size_t gf_mesh_ndof(const gf_mesh *mesh);
  *                     pencil kernels (D as compile-time constants) are used;
  * GF_MESH_BOX_HALO    p = 7 structured box whose RK stages gather the previous
  *                     stage derivative through an LDS halo of each element
- *                     (same results as the per-node gather). */
+ *                     (same results as the per-node gather);
+ * GF_MESH_IMASS_ELEM  3-D regular mesh whose inverse mass is the same at every
+ *                     element bit for bit (uniform periodic boxes), so the RK
+ *                     stages read one element's values. */
 enum { GF_MESH_BOX_GATHER = 1, GF_MESH_BOX_MASK = 2, GF_MESH_CONST_D = 4,
-       GF_MESH_BOX_HALO = 8 };
+       GF_MESH_BOX_HALO = 8, GF_MESH_IMASS_ELEM = 16 };
 int gf_mesh_flags(const gf_mesh *mesh);
 /* Copy a device-resident mesh array to host: "mass", "imass", "imult",
  * "mask" (ndof doubles each).  Synchronous. */

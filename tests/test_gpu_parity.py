@@ -328,12 +328,20 @@ def test_box_multistep_p7_against_oracle(ctx, ne):
     oracle (GExRKStepper::step_b, gexrk_stepper.ipp:251-341) at 1e-12.  The
     4x3x3 and 4x2x2 boxes gather the previous stage derivative through the
     LDS halo (partner offsets ordered x < y < z); the periodic 2x1x3 box,
-    whose y partners lie within the element, takes the per-node gather."""
+    whose y partners lie within the element, takes the per-node gather.  The
+    4x2x2 box's inverse mass is the same at every element, so its stages read
+    one element's values (GF_MESH_IMASS_ELEM)."""
     print("box multistep p7 vs oracle", ne, flush=True)
     c = dict(dim=3, p=7, ne=ne, bdy=[6] * 6)
     og, bg, m = make(ctx, c)
     assert m.flags & 1
     assert bool(m.flags & _capi.GF_MESH_BOX_HALO) == (ne != [2, 1, 3])
+    if ne == [4, 2, 2]:
+        # element sizes 1/4, 1/2, 1/2: every element's inverse mass is the
+        # same, and the stages read one element's table
+        assert m.flags & _capi.GF_MESH_IMASS_ELEM
+        im = m.get("imass").reshape(-1, 512)
+        assert np.array_equal(im, np.broadcast_to(im[0], im.shape))
     us = workloads.fourier_field_3d(*bg.xyz, seed=21, kmax=3)
     ref = [u.copy() for u in us]
     nu, dt = 0.02, 2e-4
