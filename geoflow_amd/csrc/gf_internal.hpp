@@ -55,6 +55,14 @@ struct gf_mesh_s {
   // every element (uniform periodic boxes): nn values the RK stage kernels
   // read in place of imass (null otherwise).
   double *imass_el = nullptr;
+  // Structured-box CG with the Jacobi preconditioner: the Jacobi diagonal
+  // inverse of element dref (the box's centre) in FP64 and FP32, and per
+  // element whether its values equal those bit for bit (set per solve); the
+  // CG kernels then read the table for such elements.
+  double *dtab = nullptr;
+  float *dtab32 = nullptr;
+  unsigned char *deq = nullptr;
+  size_t dref = 0;
   float *mask32 = nullptr, *imult32 = nullptr;
   bool has_mask = false;
   // DSS: coincidence groups with more than one member (CSR, ascending members)

@@ -55,7 +55,8 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
     gf::PenD<T> Dp, const T *__restrict__ W, const T *__restrict__ edx,
     const T *__restrict__ ejac, const T *__restrict__ qf, T qs, const T *__restrict__ mask, double pd,
     const double *__restrict__ scal, double *__restrict__ part, size_t nelems, gf::BoxTopo topo,
-    int tmask, const int64_t *__restrict__ seg_eoff, int nseg, const double *__restrict__ sscal) {
+    int tmask, const int64_t *__restrict__ seg_eoff, int nseg, const double *__restrict__ sscal,
+    const T *__restrict__ dtab, const unsigned char *__restrict__ deq) {
   using S = gf::Pen3<N>;
   __shared__ T su[S::EPB][S::SZ], sd[S::EPB][S::SZ];
   __shared__ double sred[S::THREADS > 64 ? S::THREADS : 64];
@@ -77,13 +78,15 @@ __global__ __launch_bounds__(gf::Pen3<N>::THREADS) void k_cg_apply3p(
   const size_t col = t.e * S::NN + t.r;
   const T jac = ejac[t.e];
   const T f0 = edx[t.e * 3], f1 = edx[t.e * 3 + 1], f2 = edx[t.e * 3 + 2];
-  // all loads of the column first, then the stores of w
+  // all loads of the column first, then the stores of w; the Jacobi values
+  // from the table where the element's equal it
+  const T *dsrc = dinv ? ((dtab && deq[t.e]) ? dtab + t.r : dinv + col) : nullptr;
   T ru[N], dv[N], wv[N];
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const size_t g = col + k * S::N2;
     ru[k] = r[g];
-    dv[k] = dinv ? dinv[g] : T(1);
+    dv[k] = dsrc ? dsrc[k * S::N2] : T(1);
     wv[k] = first ? T(0) : w[g];
   }
 #pragma unroll
@@ -657,7 +660,7 @@ int launch_cg_apply3p(gf_mesh *m, const CGApply3Args &a) {
                      (const T *)a.r, (const T *)a.dinv, (T *)a.w, (T *)a.q, Dp, (const T *)a.W,
                      (const T *)a.edx, (const T *)a.ejac, (const T *)a.qf, (T)a.qs, (const T *)a.mask, a.p,
                      (const double *)a.scal, a.part, a.nelems, a.topo, a.tmask, a.seg_eoff, a.nseg,
-                     (const double *)a.sscal);
+                     (const double *)a.sscal, (const T *)a.dtab, a.deq);
   GF_LAUNCH_CHECK(m->ctx);
   return GF_OK;
 }
@@ -842,7 +845,8 @@ template <typename T, int NORM, int N, typename I>
 __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
     T *__restrict__ x, const T *__restrict__ w, T *__restrict__ r, const T *__restrict__ qv,
     const T *__restrict__ dinv, const double *__restrict__ mass, gf::BoxTopo topo, double *scal,
-    double *__restrict__ part, size_t nelems) {
+    double *__restrict__ part, size_t nelems, const T *__restrict__ dtab,
+    const unsigned char *__restrict__ deq) {
   using VT = Vec16<T>;
   using V = typename VT::V;
   constexpr int R = VT::R;
@@ -878,7 +882,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_update_boxv(
     const V wo = *(const V *)(w + g0);
     const V ro = *(const V *)(r + g0);
     V dv;
-    if (dinv) dv = *(const V *)(dinv + g0);
+    if (dinv)  // the Jacobi values from the table where the element's equal it
+      dv = (dtab && deq[e]) ? *(const V *)(dtab + (g0 - (I)(e * NN))) : *(const V *)(dinv + g0);
     const V q0 = *(const V *)(qv + g0);
     V qy = {}, qz = {}, qyz = {};
     if (py) qy = *(const V *)(qv + g0 + dy);
@@ -955,11 +960,11 @@ int launch_update_box_n(gf_mesh *m, CGVecs<T> &v, double *part) {
     if (m->ndof < ((size_t)1 << 31))
       hipLaunchKernelGGL((k_cg_update_boxv<T, NORM, N, int32_t>), dim3(kUpdBlocks), dim3(kThreads),
                          0, m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal,
-                         part, m->nelems);
+                         part, m->nelems, v.dtab, v.deq);
     else
       hipLaunchKernelGGL((k_cg_update_boxv<T, NORM, N, int64_t>), dim3(kUpdBlocks), dim3(kThreads),
                          0, m->ctx->stream, v.x, v.w, v.r, v.q, v.dinv, m->mass, m->box, m->scal,
-                         part, m->nelems);
+                         part, m->nelems, v.dtab, v.deq);
     return GF_OK;
   }
   // 32-bit node indices whenever every index of the mesh fits

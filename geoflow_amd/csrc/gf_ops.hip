@@ -1084,6 +1084,15 @@ int gf_mesh_create(gf_ctx *ctx, const gf_mesh_desc *d, gf_mesh **out) {
     for (size_t q = 0; q < n; q++) imf[q] = (float)im[q];
     GF_HIP(ctx, hipMemcpy(m->imult32, imf.data(), n * sizeof(float), hipMemcpyHostToDevice));
   }
+  if (m->has_box && m->dconst) {
+    // Jacobi-diagonal table of the box CG (filled per solve)
+    GF_HIP(ctx, hipMalloc((void **)&m->dtab, m->nn * sizeof(double)));
+    GF_HIP(ctx, hipMalloc((void **)&m->dtab32, m->nn * sizeof(float)));
+    GF_HIP(ctx, hipMalloc((void **)&m->deq, m->nelems));
+    m->dref = (size_t)(m->box.ne[0] / 2) +
+              (size_t)m->box.ne[0] * ((size_t)(m->box.ne[1] / 2) +
+                                      (size_t)m->box.ne[1] * (size_t)(m->box.ne[2] / 2));
+  }
   if (d->gtype == GF_GE_REGULAR && m->dim == 3 && m->Niso && m->nelems > 1) {
     // inverse mass identical at every element (bit for bit): the RK stages
     // read one element's values instead of the whole array
@@ -1128,7 +1137,8 @@ int gf_mesh_destroy(gf_mesh *m) {
     F(m->pDC32[k]);
   }
   F(m->D32); F(m->W32); F(m->edx); F(m->ejac); F(m->edx32); F(m->ejac32); F(m->njac);
-  F(m->mask); F(m->mass); F(m->imass); F(m->imass_el); F(m->imult); F(m->mask32);
+  F(m->mask); F(m->mass); F(m->imass); F(m->imass_el); F(m->dtab); F(m->dtab32); F(m->deq);
+  F(m->imult); F(m->mask32);
   F(m->imult32);
   F(m->goff); F(m->gmem); F(m->red); F(m->scal);
   for (int k = 0; k < 6; k++) F(m->G32[k]);

@@ -1624,6 +1624,8 @@ int cg_loop(gf_mesh *m, CGVecs<T> &v, int normtype, double *d_resid, int *iters)
   a3.sscal = nullptr;
   a3.r = v.r;
   a3.dinv = v.dinv;
+  a3.dtab = v.dtab;
+  a3.deq = v.deq;
   a3.w = v.w;
   a3.q = v.q;
   a3.D = v.D;
@@ -1770,6 +1772,8 @@ int cg_loop_seg(gf_mesh *m, CGVecs<T> &v, int normtype) {
   CGApply3Args a3;
   a3.r = v.r;
   a3.dinv = v.dinv;
+  a3.dtab = nullptr;
+  a3.deq = nullptr;
   a3.w = v.w;
   a3.q = v.q;
   a3.D = v.D;
@@ -2041,6 +2045,28 @@ int cg_solve_seg(gf_mesh *m, const gf_cg_params *prm, const double *b, const dou
   return status;
 }
 
+// Jacobi table of a structured box: dtab = the Jacobi diagonal inverse of
+// element dref (FP64 and FP32) and deq[e] = whether element e's values equal
+// it bit for bit (uniform elements with complete groups do).
+__global__ __launch_bounds__(kThreads) void k_dinv_table(const double *__restrict__ dinv,
+                                                         size_t nn, size_t dref, size_t nelems,
+                                                         double *__restrict__ dtab,
+                                                         float *__restrict__ dtab32,
+                                                         unsigned char *__restrict__ deq) {
+  const double *ref = dinv + dref * nn;
+  for (size_t e = blockIdx.x; e < nelems; e += gridDim.x) {
+    int diff = 0;
+    for (size_t q = threadIdx.x; q < nn; q += blockDim.x) diff |= dinv[e * nn + q] != ref[q];
+    const int any = __syncthreads_or(diff);
+    if (threadIdx.x == 0) deq[e] = any ? 0 : 1;
+  }
+  if (blockIdx.x == 0)
+    for (size_t q = threadIdx.x; q < nn; q += blockDim.x) {
+      dtab[q] = ref[q];
+      dtab32[q] = (float)ref[q];
+    }
+}
+
 int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const double *xb,
                   double *x, gf_cg_result *res, const CGWork &wk) {
   hipStream_t s = m->ctx->stream;
@@ -2059,6 +2085,13 @@ int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const do
   // ghelmholtz.ipp:184-186, 370-373)
   const double qs = prm->q_field ? 0.0 : prm->q_scalar;
   if (prm->precond) RC(gf_helmholtz_diag_inv(m, prm->p_scalar, qs, prm->q_field, dinv));
+  const bool dtabv = prm->precond && m->deq;
+  if (dtabv) {
+    hipLaunchKernelGGL(k_dinv_table, dim3((unsigned)std::min<size_t>(m->nelems, 65536)),
+                       dim3(kThreads), 0, s, dinv, m->nn, m->dref, m->nelems, m->dtab, m->dtab32,
+                       m->deq);
+    GF_LAUNCH_CHECK(m->ctx);
+  }
   if (bbv) {  // x = x*mask + xb (gcg.ipp:371-375)
     RC(gf_mul_t(m, x, m->mask, x));
     RC(gf_axpy_t(m, 1.0, xb, x));
@@ -2113,6 +2146,8 @@ int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const do
     v.w = w;
     v.q = q;
     v.dinv = dinv;
+    v.dtab = dtabv ? m->dtab : nullptr;
+    v.deq = dtabv ? m->deq : nullptr;
     v.mask = mask;
     v.imult = m->imult;
     v.qf = prm->q_field;
@@ -2152,6 +2187,8 @@ int cg_solve_impl(gf_mesh *m, const gf_cg_params *prm, const double *b, const do
     v.w = w32;
     v.q = q32;
     v.dinv = dinv32;
+    v.dtab = dtabv ? m->dtab32 : nullptr;
+    v.deq = dtabv ? m->deq : nullptr;
     v.mask = bbv ? m->mask32 : nullptr;
     v.imult = m->imult32;
     v.qf = qf32;
@@ -2454,6 +2491,8 @@ extern "C" int gf_time_kernel(gf_mesh *m, const char *which, int reps, double *a
     }
     ca3.r = buf[0];
     ca3.dinv = buf[1];
+    ca3.dtab = nullptr;
+    ca3.deq = nullptr;
     ca3.w = buf[2];
     ca3.q = buf[3];
     ca3.D = f32 ? (const void *)m->D32 : (const void *)m->D[0];

@@ -122,6 +122,10 @@ struct ColIdx {
 
 struct CGApply3Args {
   const void *r, *dinv;
+  // Jacobi table: elements with deq[e] read dtab (one element's values) in
+  // place of dinv; null: dinv everywhere
+  const void *dtab;
+  const unsigned char *deq;
   void *w, *q;
   const void *D, *W, *edx, *ejac, *qf, *mask;
   const void *DR[3], *DC[3];  // pencil copies of D (same precision as T)
@@ -221,6 +225,8 @@ template <typename T>
 struct CGVecs {
   T *x, *r, *w, *q;
   const T *dinv, *mask, *qf;
+  const T *dtab;               // Jacobi table (CGApply3Args), or null
+  const unsigned char *deq;
   const T *imult;  // 1/multiplicity in the recurrence precision (exact in FP64)
   const void *D, *W, *edx, *ejac;
   const void *G[6];  // T node metrics and lumped mass (meshes with node metrics)

@@ -353,14 +353,20 @@ def test_box_multistep_p7_against_oracle(ctx, ne):
             assert relerr(host(dus[k]), ref[k]) <= TOL, (n, k)
 
 
-@pytest.mark.parametrize("bdy", [[0] * 6, [6] * 6], ids=["dirichlet", "periodic"])
-def test_cg_gather_path_matches_csr_path(ctx, bdy):
+@pytest.mark.parametrize("bdy,p,ne", [([0] * 6, 5, [3, 2, 3]), ([6] * 6, 5, [3, 2, 3]),
+                                      ([0] * 6, 7, [4, 4, 4]), ([6] * 6, 7, [4, 4, 4])],
+                         ids=["dirichlet", "periodic", "dirichlet_p7_4cube", "periodic_p7_4cube"])
+def test_cg_gather_path_matches_csr_path(ctx, bdy, p, ne):
     """Structured-box CG (gather assembly in the update, mask from the box
     faces) against the CSR DSS path on the same problem: the assembly is the
     same sum in the same order, only the partial sums of the dot products are
     grouped differently, so 20 FP64 iterations agree to rounding and both
-    mixed-precision solves converge to the same solution."""
-    bg = ga.BoxGrid(3, 5, [3, 2, 3], bdy=bdy)
+    mixed-precision solves converge to the same solution.  On the 4x4x4
+    p = 7 boxes (element sizes powers of two) the Jacobi diagonal of every
+    element with complete groups equals the centre element's bit for bit, so
+    the structured path's apply and update read those from its table."""
+    print("cg gather vs csr", bdy[0], p, ne, flush=True)
+    bg = ga.BoxGrid(3, p, ne, bdy=bdy)
     f = workloads.unique_normal(bg, seed=11)
     outs = []
     for structured in (True, False):
