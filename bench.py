@@ -477,8 +477,9 @@ def main():
                          "(for profiler counter passes) and print their average times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(see DESIGN.md for how the path would shard)" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU; --gpus %d is not supported (the kernels work on "
+              "one GPU's share of the elements; several GPUs would split the element "
+              "ranges and exchange the shared faces)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -8,7 +8,7 @@ launch; an ensemble with orders {4, 6, 8} needs three launches per kernel,
 whatever the number of members.
 
 The step is implicit in the viscous term (the reference has no implicit
-stepper; this is the build's addition for the ensemble workload):
+stepper; this implicit form is an addition for the ensemble workload):
     M u*   = M u^n - dt M (u^n . grad) u^n           (GAdvect, explicit)
     (M + nu dt L) u^{n+1} = M u*                     (Helmholtz, Jacobi PCG)
 per velocity component, the solve in FP64 or with FP32 inner iterations and

@@ -416,7 +416,7 @@ def _diag_grid(kind):
 @pytest.mark.parametrize("kind", ["box2_dirichlet", "box2_periodic", "box3_periodic", "terrain2",
                                   "terrain3", "sphere"])
 def test_jacobi_diagonal_by_unit_vectors(kind, pfield):
-    """or_helmholtz_diag_inv (the build's Jacobi preconditioner) against the
+    """or_helmholtz_diag_inv (the Jacobi preconditioner, no reference counterpart) against the
     assembled diagonal found by brute force: apply (p L + q M) to the
     continuous unit field of each node group, assemble (GGFX Sum) and read
     the group's own entry.  On the terrain and sphere meshes the off-diagonal
